@@ -1,0 +1,472 @@
+// C ABI of the curvami GPU library: argument checks, host orchestration,
+// status codes. Reference interfaces replaced are cited in include/curvami_gpu.h.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+
+#include "cvm_internal.h"
+
+struct cvm_context {
+    cvm::Context c;
+};
+
+namespace cvm {
+
+Context::~Context() {
+    for (Domain* d : doms) delete d;
+    doms.clear();
+    for (size_t i = 0; i < timing.ev0.size(); ++i) {
+        (void)hipEventDestroy(timing.ev0[i]);
+        (void)hipEventDestroy(timing.ev1[i]);
+    }
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+// ---- per-kernel event timing on the context's stream ------------------------
+static int kernel_id(Timing& t, const char* name) {
+    for (size_t i = 0; i < t.names.size(); ++i)
+        if (t.names[i] == name) return static_cast<int>(i);
+    t.names.emplace_back(name);
+    t.total_ms.push_back(0.0);
+    t.count.push_back(0);
+    return static_cast<int>(t.names.size()) - 1;
+}
+
+static void timing_collect(Context& c) {
+    Timing& t = c.timing;
+    if (t.pending.empty()) return;
+    CVM_HIP(hipStreamSynchronize(c.stream));
+    for (size_t i = 0; i < t.pending.size(); ++i) {
+        float ms = 0.0f;
+        CVM_HIP(hipEventElapsedTime(&ms, t.ev0[i], t.ev1[i]));
+        t.total_ms[t.pending[i]] += ms;
+        t.count[t.pending[i]] += 1;
+    }
+    t.pending.clear();
+}
+
+int timing_begin(Context& c, const char* name) {
+    Timing& t = c.timing;
+    if (!t.on) return -1;
+    const int id = kernel_id(t, name);
+    const size_t slot = t.pending.size();
+    if (slot >= 4096) timing_collect(c);
+    const size_t s2 = t.pending.size();
+    while (t.ev0.size() <= s2) {
+        hipEvent_t a, b;
+        CVM_HIP(hipEventCreate(&a));
+        CVM_HIP(hipEventCreate(&b));
+        t.ev0.push_back(a);
+        t.ev1.push_back(b);
+    }
+    t.pending.push_back(id);
+    CVM_HIP(hipEventRecord(t.ev0[s2], c.stream));
+    return static_cast<int>(s2);
+}
+
+void timing_end(Context& c, int slot) {
+    if (slot < 0) return;
+    CVM_HIP(hipEventRecord(c.timing.ev1[slot], c.stream));
+}
+
+static Domain& domain(Context& c, int id) {
+    if (id < 0 || id >= static_cast<int>(c.doms.size())) throw Error(CVM_E_INVALID, "bad domain id");
+    return *c.doms[id];
+}
+
+static void laminate_abd(const double* plies, int nply, double* A, double* B, double* D) {
+    // Constitutive::computeABD (plate/Constitutive.h:10-50)
+    for (int k = 0; k < 9; ++k) { A[k] = 0.0; B[k] = 0.0; D[k] = 0.0; }
+    double h = 0.0;
+    for (int i = 0; i < nply; ++i) h += plies[6 * i + 5];
+    double z0 = -h / 2.0;
+    for (int i = 0; i < nply; ++i) {
+        const double E1 = plies[6 * i], E2 = plies[6 * i + 1], nu12 = plies[6 * i + 2], G12 = plies[6 * i + 3];
+        const double ang = plies[6 * i + 4], th = plies[6 * i + 5];
+        const double nu21 = (E2 / E1) * nu12;
+        const double den = 1.0 - nu12 * nu21;
+        const double Q00 = E1 / den, Q11 = E2 / den, Q01 = (nu12 * E2) / den, Q22 = G12;
+        const double t = ang * M_PI / 180.0;
+        const double m = std::cos(t), n = std::sin(t);
+        const double m2 = m * m, n2 = n * n, m4 = m2 * m2, n4 = n2 * n2;
+        double Qb[9];
+        Qb[0] = Q00 * m4 + 2.0 * (Q01 + 2.0 * Q22) * m2 * n2 + Q11 * n4;
+        Qb[4] = Q00 * n4 + 2.0 * (Q01 + 2.0 * Q22) * m2 * n2 + Q11 * m4;
+        Qb[1] = Qb[3] = (Q00 + Q11 - 4.0 * Q22) * m2 * n2 + Q01 * (m4 + n4);
+        Qb[2] = Qb[6] = (Q00 - Q01 - 2.0 * Q22) * n * m * m2 + (Q01 - Q11 + 2.0 * Q22) * n2 * n * m;
+        Qb[5] = Qb[7] = (Q00 - Q01 - 2.0 * Q22) * m * n * n2 + (Q01 - Q11 + 2.0 * Q22) * m2 * m * n;
+        Qb[8] = (Q00 + Q11 - 2.0 * Q01 - 2.0 * Q22) * m2 * n2 + Q22 * (m4 + n4);
+        const double z1 = z0 + th;
+        for (int k = 0; k < 9; ++k) {
+            A[k] += Qb[k] * (z1 - z0);
+            B[k] += 0.5 * Qb[k] * (z1 * z1 - z0 * z0);
+            D[k] += (1.0 / 3.0) * Qb[k] * (z1 * z1 * z1 - z0 * z0 * z0);
+        }
+        z0 = z1;
+    }
+}
+
+}  // namespace cvm
+
+using cvm::Context;
+using cvm::Error;
+
+#define CVM_TRY(ctx)                                              \
+    if (!(ctx)) return CVM_E_INVALID;                             \
+    try {
+#define CVM_CATCH(ctx)                                            \
+    }                                                             \
+    catch (const cvm::Error& e) {                                 \
+        (ctx)->c.err = e.what();                                  \
+        return e.code;                                            \
+    }                                                             \
+    catch (const std::bad_alloc&) {                               \
+        (ctx)->c.err = "host allocation failed";                  \
+        return CVM_E_NO_MEMORY;                                   \
+    }                                                             \
+    catch (const std::exception& e) {                             \
+        (ctx)->c.err = e.what();                                  \
+        return CVM_E_INVALID;                                     \
+    }                                                             \
+    return CVM_OK;
+
+extern "C" {
+
+int cvm_create(int device, cvm_context** out) {
+    if (!out) return CVM_E_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CVM_E_HIP;
+    if (device < 0 || device >= ndev) return CVM_E_INVALID;
+    cvm_context* ctx = new (std::nothrow) cvm_context();
+    if (!ctx) return CVM_E_NO_MEMORY;
+    ctx->c.device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return CVM_E_HIP;
+    }
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+        ctx->c.lds_limit = static_cast<size_t>(lds);
+    *out = ctx;
+    return CVM_OK;
+}
+
+void cvm_destroy(cvm_context* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->c.device);
+    if (ctx->c.stream) (void)hipStreamSynchronize(ctx->c.stream);
+    delete ctx;
+}
+
+const char* cvm_last_error(const cvm_context* ctx) { return ctx ? ctx->c.err.c_str() : "null context"; }
+
+int cvm_set_nodes(cvm_context* ctx, int nn, const double* X, const double* Y) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    if (nn < 0 || (nn > 0 && (!X || !Y))) throw Error(CVM_E_INVALID, "bad node arrays");
+    CVM_HIP(hipSetDevice(c.device));
+    c.nn = nn;
+    c.hX.assign(X, X + nn);
+    c.hY.assign(Y, Y + nn);
+    c.X.upload(c.hX, c.stream);
+    c.Y.upload(c.hY, c.stream);
+    c.plan.ready = false;
+    for (cvm::Domain* d : c.doms) { d->npairs = -1; d->have_sf = false; }
+    CVM_HIP(hipStreamSynchronize(c.stream));
+    CVM_CATCH(ctx)
+}
+
+int cvm_set_laminate(cvm_context* ctx, int nply, const double* plies) {
+    CVM_TRY(ctx)
+    if (nply <= 0 || !plies) throw Error(CVM_E_INVALID, "need at least one ply");
+    cvm::laminate_abd(plies, nply, ctx->c.A, ctx->c.B, ctx->c.D);
+    ctx->c.have_mat = true;
+    CVM_CATCH(ctx)
+}
+
+int cvm_get_abd(const cvm_context* ctx, double* A, double* B, double* D) {
+    if (!ctx) return CVM_E_INVALID;
+    for (int k = 0; k < 9; ++k) {
+        if (A) A[k] = ctx->c.A[k];
+        if (B) B[k] = ctx->c.B[k];
+        if (D) D[k] = ctx->c.D[k];
+    }
+    return CVM_OK;
+}
+
+int cvm_add_crease(cvm_context* ctx, int nv, const double* vx, const double* vy, int extend_start, int extend_end,
+                   int is_closed) {
+    CVM_TRY(ctx)
+    if (nv < 0 || (nv > 0 && (!vx || !vy))) throw Error(CVM_E_INVALID, "bad crease arrays");
+    cvm::Crease cr;
+    cr.vx.assign(vx, vx + nv);
+    cr.vy.assign(vy, vy + nv);
+    cr.ext_s = extend_start ? 1 : 0;
+    cr.ext_e = extend_end ? 1 : 0;
+    cr.closed = is_closed ? 1 : 0;
+    ctx->c.creases.push_back(cr);
+    ctx->c.creases_dirty = true;
+    for (cvm::Domain* d : ctx->c.doms) d->have_sf = false;
+    CVM_CATCH(ctx)
+}
+
+int cvm_add_domain(cvm_context* ctx, int kind, int ng, const double* gx, const double* gy, const double* w,
+                   const double* nx, const double* ny, const double* tx, const double* ty, int* domain_id) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    if (kind < CVM_DOMAIN_FACE || kind > CVM_DOMAIN_FOLD_RIGHT) throw Error(CVM_E_INVALID, "bad domain kind");
+    if (ng < 0 || (ng > 0 && (!gx || !gy || !w))) throw Error(CVM_E_INVALID, "bad Gauss-point arrays");
+    CVM_HIP(hipSetDevice(c.device));
+    auto* d = new cvm::Domain();
+    d->kind = kind;
+    d->ng = ng;
+    d->hx.assign(gx, gx + ng);
+    d->hy.assign(gy, gy + ng);
+    d->hw.assign(w, w + ng);
+    d->hnx = nx ? std::vector<double>(nx, nx + ng) : std::vector<double>(ng, 0.0);
+    d->hny = ny ? std::vector<double>(ny, ny + ng) : std::vector<double>(ng, 0.0);
+    d->htx = tx ? std::vector<double>(tx, tx + ng) : std::vector<double>(ng, 0.0);
+    d->hty = ty ? std::vector<double>(ty, ty + ng) : std::vector<double>(ng, 0.0);
+    try {
+        d->gx.upload(d->hx, c.stream);
+        d->gy.upload(d->hy, c.stream);
+        d->d_w.upload(d->hw, c.stream);
+        d->d_tx.upload(d->htx, c.stream);
+        d->d_ty.upload(d->hty, c.stream);
+        CVM_HIP(hipStreamSynchronize(c.stream));
+    } catch (...) {
+        delete d;
+        throw;
+    }
+    c.doms.push_back(d);
+    if (kind == CVM_DOMAIN_FACE) c.plan.ready = false;
+    if (domain_id) *domain_id = static_cast<int>(c.doms.size()) - 1;
+    CVM_CATCH(ctx)
+}
+
+int cvm_add_fold(cvm_context* ctx, int left, int right, double Kt, double Ang0, int* fold_id) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    cvm::Domain& L = cvm::domain(c, left);
+    cvm::Domain& R = cvm::domain(c, right);
+    if (L.ng != R.ng) throw Error(CVM_E_INVALID, "fold left/right domains differ in size");
+    cvm::Fold f;
+    f.left = left; f.right = right; f.Kt = Kt; f.Ang0 = Ang0;
+    c.folds.push_back(std::move(f));
+    c.plan.ready = false;
+    if (fold_id) *fold_id = static_cast<int>(c.folds.size()) - 1;
+    CVM_CATCH(ctx)
+}
+
+int cvm_set_fold_gp_state(cvm_context* ctx, int fold_id, int n, const double* theta0, const char* active) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    if (fold_id < 0 || fold_id >= static_cast<int>(c.folds.size())) throw Error(CVM_E_INVALID, "bad fold id");
+    cvm::Fold& f = c.folds[fold_id];
+    if ((theta0 || active) && n != c.doms[f.left]->ng) throw Error(CVM_E_INVALID, "state size differs from crease GP count");
+    const bool active_changed = (active != nullptr) != !f.active.empty() ||
+                                (active && !std::equal(active, active + n, f.active.begin()));
+    f.theta0.assign(theta0, theta0 + (theta0 ? n : 0));
+    f.active.assign(active, active + (active ? n : 0));
+    if (theta0) f.d_theta0.upload(f.theta0, c.stream);
+    CVM_HIP(hipStreamSynchronize(c.stream));
+    if (active_changed) c.plan.ready = false;   // the fold pattern depends on activity
+    CVM_CATCH(ctx)
+}
+
+int cvm_neighbours(cvm_context* ctx, int domain, double rho, int64_t* npairs) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    CVM_HIP(hipSetDevice(c.device));
+    cvm::Domain& d = cvm::domain(c, domain);
+    cvm::launch_frnn(c, d, rho);
+    if (npairs) *npairs = d.npairs;
+    if (d.kind == CVM_DOMAIN_FACE || d.kind == CVM_DOMAIN_FOLD_LEFT || d.kind == CVM_DOMAIN_FOLD_RIGHT)
+        c.plan.ready = false;
+    CVM_CATCH(ctx)
+}
+
+int cvm_get_neighbours(cvm_context* ctx, int domain, int* off, int* ids, double* s) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    cvm::Domain& d = cvm::domain(c, domain);
+    if (d.npairs < 0) throw Error(CVM_E_STATE, "neighbours not computed");
+    if (off) std::copy(d.h_off.begin(), d.h_off.end(), off);
+    if (ids) std::copy(d.h_ids.begin(), d.h_ids.end(), ids);
+    if (s && d.npairs) {
+        d.s.download(s, d.npairs, c.stream);
+        CVM_HIP(hipStreamSynchronize(c.stream));
+    }
+    CVM_CATCH(ctx)
+}
+
+int cvm_shape_functions(cvm_context* ctx, int domain, int order, int kernel, int* nbad, int* first_bad) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    CVM_HIP(hipSetDevice(c.device));
+    cvm::Domain& d = cvm::domain(c, domain);
+    int code = 0, nb = 0, fb = 0;
+    cvm::launch_rkpm(c, d, d.rho, order, kernel, &code, &nb, &fb);
+    if (nbad) *nbad = nb;
+    if (first_bad) *first_bad = fb;
+    if (code == CVM_E_SINGULAR_MOMENT) {
+        const int nfirst = d.h_off[fb + 1] - d.h_off[fb];
+        throw Error(code, "RKPM: singular moment matrix at " + std::to_string(nb) + " of " + std::to_string(d.ng) +
+                              " Gauss points (first is index " + std::to_string(fb) + " with " +
+                              std::to_string(nfirst) +
+                              " neighbours). The support is rank-deficient for the requested basis order - "
+                              "typically too few visible nodes, which happens where a crease cuts most of a "
+                              "support away. Increase the support radius, refine the mesh there, or lower the order.");
+    }
+    if (code == CVM_E_PIVOT_COLLAPSE) {
+        const int nfirst = d.h_off[fb + 1] - d.h_off[fb];
+        throw Error(code, "RKPM: moment-matrix pivot collapsed during block inversion at " + std::to_string(nb) +
+                              " of " + std::to_string(d.ng) + " Gauss points (first is index " + std::to_string(fb) +
+                              " with " + std::to_string(nfirst) +
+                              " neighbours). Same cause and remedies as a singular moment matrix.");
+    }
+    CVM_CATCH(ctx)
+}
+
+int cvm_get_shape_functions(cvm_context* ctx, int domain, double* psi, double* dpx, double* dpy, double* dxx,
+                            double* dyy, double* dxy) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    cvm::Domain& d = cvm::domain(c, domain);
+    if (!d.have_sf) throw Error(CVM_E_STATE, "shape functions not computed");
+    double* outs[6] = {psi, dpx, dpy, dxx, dyy, dxy};
+    for (int k = 0; k < 6; ++k)
+        if (outs[k] && d.npairs)
+            CVM_HIP(hipMemcpyAsync(outs[k], d.sf.p + k * d.npairs, d.npairs * sizeof(double), hipMemcpyDeviceToHost,
+                                   c.stream));
+    CVM_HIP(hipStreamSynchronize(c.stream));
+    CVM_CATCH(ctx)
+}
+
+int cvm_init_assembly(cvm_context* ctx, int64_t* nnz_face, int64_t* nnz_union) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    CVM_HIP(hipSetDevice(c.device));
+    if (!c.have_mat) throw Error(CVM_E_STATE, "laminate not set");
+    cvm::build_plan(c);
+    if (nnz_face) *nnz_face = static_cast<int64_t>(c.plan.face_inner.size());
+    if (nnz_union) *nnz_union = static_cast<int64_t>(c.plan.uni_inner.size());
+    CVM_CATCH(ctx)
+}
+
+int cvm_get_pattern(cvm_context* ctx, int with_folds, int* outer, int* inner) {
+    CVM_TRY(ctx)
+    const cvm::Plan& P = ctx->c.plan;
+    if (!P.ready) throw Error(CVM_E_STATE, "assembly plan not initialised");
+    const auto& o = with_folds ? P.uni_outer : P.face_outer;
+    const auto& i = with_folds ? P.uni_inner : P.face_inner;
+    if (outer) std::copy(o.begin(), o.end(), outer);
+    if (inner) std::copy(i.begin(), i.end(), inner);
+    CVM_CATCH(ctx)
+}
+
+int cvm_csr_mapping(cvm_context* ctx, int gp, int* out, int cap, int* n) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    const cvm::Plan& P = c.plan;
+    if (!P.ready) throw Error(CVM_E_STATE, "assembly plan not initialised");
+    cvm::Domain* fd = nullptr;
+    for (cvm::Domain* d : c.doms) if (d->kind == CVM_DOMAIN_FACE) fd = d;
+    if (gp < 0 || gp >= fd->ng) throw Error(CVM_E_INVALID, "bad GP index");
+    const int b = fd->h_off[gp], m = fd->h_off[gp + 1] - b;
+    const int cnt = m * (m + 1) / 2;
+    if (n) *n = cnt;
+    if (!out) return CVM_OK;
+    if (cap < cnt) throw Error(CVM_E_INVALID, "output too small");
+    int mo = 0;
+    for (int i = 0; i < m; ++i)
+        for (int j = i; j < m; ++j) {
+            const int ni = fd->h_ids[b + i], nj = fd->h_ids[b + j];
+            const int lo = std::min(ni, nj), hi = std::max(ni, nj);
+            const auto& ch = P.face_rows[hi];
+            auto it = std::lower_bound(ch.begin(), ch.end(), lo);
+            out[mo++] = (it == ch.end() || *it != lo) ? -1 : static_cast<int>(it - ch.begin());
+        }
+    CVM_CATCH(ctx)
+}
+
+int cvm_assemble(cvm_context* ctx, const double* x, int flags, double* F_int, double* K_values) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    CVM_HIP(hipSetDevice(c.device));
+    cvm::Plan& P = c.plan;
+    if (!P.ready) throw Error(CVM_E_STATE, "assembly plan not initialised (cvm_init_assembly)");
+    if (!x) throw Error(CVM_E_INVALID, "x is null");
+    for (cvm::Domain* d : c.doms)
+        if ((d->kind == CVM_DOMAIN_FACE) && !d->have_sf) throw Error(CVM_E_STATE, "face shape functions not computed");
+    const bool dev = (flags & CVM_DEVICE_PTRS) != 0;
+    const bool tangent = (flags & CVM_TANGENT) != 0;
+    const int64_t nnz = (flags & CVM_ELASTIC_FOLDS) ? static_cast<int64_t>(P.uni_inner.size())
+                                                    : static_cast<int64_t>(P.face_inner.size());
+    const double* xd = x;
+    if (!dev) {
+        c.d_x.upload(x, 3 * static_cast<size_t>(c.nn), c.stream);
+        xd = c.d_x.p;
+    }
+    double* Fd = F_int;
+    double* Kd = K_values;
+    if (!dev) {
+        c.d_F.alloc(std::max<size_t>(3 * static_cast<size_t>(c.nn), 1));
+        Fd = c.d_F.p;
+        if (tangent) {
+            c.d_K.alloc(std::max<int64_t>(nnz, 1));
+            Kd = c.d_K.p;
+        }
+    } else if (!Fd || (tangent && !Kd)) {
+        throw Error(CVM_E_INVALID, "device output pointer is null");
+    }
+    cvm::launch_assembly(c, flags, xd, Fd, Kd);
+    if (!dev) {
+        if (F_int) CVM_HIP(hipMemcpyAsync(F_int, Fd, 3 * sizeof(double) * c.nn, hipMemcpyDeviceToHost, c.stream));
+        if (tangent && K_values && nnz)
+            CVM_HIP(hipMemcpyAsync(K_values, Kd, sizeof(double) * nnz, hipMemcpyDeviceToHost, c.stream));
+        CVM_HIP(hipStreamSynchronize(c.stream));
+    }
+    CVM_CATCH(ctx)
+}
+
+int cvm_synchronize(cvm_context* ctx) {
+    CVM_TRY(ctx)
+    CVM_HIP(hipStreamSynchronize(ctx->c.stream));
+    CVM_CATCH(ctx)
+}
+
+int cvm_set_profiling(cvm_context* ctx, int on) {
+    CVM_TRY(ctx)
+    cvm::timing_collect(ctx->c);
+    ctx->c.timing.on = on != 0;
+    CVM_CATCH(ctx)
+}
+
+int cvm_kernel_count(cvm_context* ctx) {
+    if (!ctx) return 0;
+    try { cvm::timing_collect(ctx->c); } catch (...) {}
+    return static_cast<int>(ctx->c.timing.names.size());
+}
+
+int cvm_kernel_stats(cvm_context* ctx, int i, char* name, int name_cap, double* total_ms, int64_t* launches) {
+    CVM_TRY(ctx)
+    cvm::Timing& t = ctx->c.timing;
+    cvm::timing_collect(ctx->c);
+    if (i < 0 || i >= static_cast<int>(t.names.size())) throw Error(CVM_E_INVALID, "bad kernel index");
+    if (name && name_cap > 0) std::snprintf(name, name_cap, "%s", t.names[i].c_str());
+    if (total_ms) *total_ms = t.total_ms[i];
+    if (launches) *launches = t.count[i];
+    CVM_CATCH(ctx)
+}
+
+int cvm_reset_profiling(cvm_context* ctx) {
+    CVM_TRY(ctx)
+    cvm::timing_collect(ctx->c);
+    for (auto& v : ctx->c.timing.total_ms) v = 0.0;
+    for (auto& v : ctx->c.timing.count) v = 0;
+    CVM_CATCH(ctx)
+}
+
+}  // extern "C"

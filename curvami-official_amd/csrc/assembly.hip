@@ -1,0 +1,524 @@
+// Kirchhoff-Love residual and block-upper tangent on gfx950 (FP64), with a
+// deterministic scatter into the CSC of K.
+//
+// Reference: NonLinearAssembler::updateStiffness (plate/Assembly.h:158-345),
+// Kinematics::compute (plate/Kinematics.h:61-120), Constitutive::evaluate
+// (plate/Constitutive.h:57-70), Constraints::evaluateElasticFold
+// (plate/Constraints.h:341-436), Meshless::assembleSystem (plate/Meshless.h:113-127).
+//
+// Element block, regrouped. With Bm_i = Psi_i F0^T (Psi_i = [[px,0],[0,py],[py,px]]),
+// Bb_i = beta_i b^T and Db_i = -g_i b^T (g_i = F0 C0^-1 grad Psi_i), the pair block
+// of Assembly.h:290-300 equals
+//   K_IJ = W [ F0 (Psi_i^T A Psi_j) F0^T + x b^T + b y^T + z b b^T + geo I ],
+//   x = F0 Psi_i^T (B beta_j) - Tu_i g_j,  y = F0 Psi_j^T (B beta_i) - Tu_j g_i,
+//   z = beta_i^T D beta_j + (b.v) gradPsi_i^T C0^-1 gradPsi_j,  geo = gradPsi_i^T N gradPsi_j,
+// (A, B, D symmetric), which needs 27 numbers per neighbour and ~130 flop per
+// pair instead of the 3x3 products of the reference.
+//
+// Scatter: a workgroup owns a tile of consecutive face GPs. Its GPs are taken
+// in order; the pairs of one GP hit distinct slots, so they are added to the
+// tile's LDS accumulator in parallel without atomics, then the tile's touched
+// blocks go to a scratch row. k_reduce_K sums, per K block, its scratch rows in
+// ascending tile order and then its fold contributions in crease-GP order.
+#include <algorithm>
+
+#include "cvm_internal.h"
+
+namespace cvm {
+
+constexpr int TILE_THREADS = 256;
+constexpr int NBMAX = 64;        // neighbours per GP
+constexpr int NBS = 27;          // doubles per neighbour record in LDS
+
+struct MatC {
+    double A[9], B[9], D[9];
+};
+
+// neighbour record layout
+enum {
+    R_PX = 0, R_PY, R_AP00, R_AP01, R_AP10, R_AP11, R_AP20, R_AP21,
+    R_BB0, R_BB1, R_BB2, R_DB0, R_DB1, R_DB2, R_BE0, R_BE1, R_BE2,
+    R_G0, R_G1, R_G2, R_TU, R_NP0, R_NP1, R_CP0, R_CP1
+};
+// GP record in LDS
+enum { G_W = 0, G_A1 = 1, G_A2 = 4, G_B = 7, G_BDV = 10, G_N = 11 };
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(TILE_THREADS) void k_tile(
+    const int* __restrict__ tile_gp, const int* __restrict__ tile_uoff, const int* __restrict__ tile_unodes,
+    const int* __restrict__ tile_toff, const uint16_t* __restrict__ tpairs,
+    const int* __restrict__ off, const uint8_t* __restrict__ uloc,
+    const double* __restrict__ sf, int64_t npairs, const double* __restrict__ w,
+    const double* __restrict__ xu, MatC mat, int max_touched, int lut_dim, int with_tangent,
+    double* __restrict__ scr, double* __restrict__ fscr) {
+    extern __shared__ double smem[];
+    const int t = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int g0 = tile_gp[t], g1 = tile_gp[t + 1];
+    const int ub = tile_uoff[t], U = tile_uoff[t + 1] - ub;
+    const int tb = tile_toff[t], T = tile_toff[t + 1] - tb;
+
+    double* acc = smem;                                   // max_touched * 9
+    double* nb = acc + static_cast<size_t>(max_touched) * 9;   // NBMAX * NBS
+    double* uc = nb + NBMAX * NBS;                        // lut_dim * 3
+    double* fa = uc + lut_dim * 3;                        // lut_dim * 3
+    double* gd = fa + lut_dim * 3;                        // 16
+    uint16_t* lut = reinterpret_cast<uint16_t*>(gd + 16); // lut_dim * lut_dim
+
+    for (int k = tid; k < U; k += TILE_THREADS) {
+        const int v = tile_unodes[ub + k];
+        uc[3 * k] = xu[3 * v]; uc[3 * k + 1] = xu[3 * v + 1]; uc[3 * k + 2] = xu[3 * v + 2];
+        fa[3 * k] = 0.0; fa[3 * k + 1] = 0.0; fa[3 * k + 2] = 0.0;
+    }
+    if (with_tangent) {
+        for (int k = tid; k < T; k += TILE_THREADS) {
+            const uint16_t pr = tpairs[tb + k];
+            lut[(pr >> 8) * lut_dim + (pr & 0xff)] = static_cast<uint16_t>(k);
+        }
+        for (int k = tid; k < 9 * T; k += TILE_THREADS) acc[k] = 0.0;
+    }
+    __syncthreads();
+
+    const double* A = mat.A;
+    const double* B = mat.B;
+    const double* D = mat.D;
+    for (int g = g0; g < g1; ++g) {
+        const int b = off[g], n = off[g + 1] - b;
+        if (tid < 64) {
+            // ---- kinematics (Kinematics.h:75-113) -------------------------------
+            const int lane = tid;
+            const bool act = lane < n;
+            double px = 0, py = 0, hxx = 0, hyy = 0, hxy = 0;
+            int ul = 0;
+            double u0 = 0, u1 = 0, u2 = 0;
+            if (act) {
+                const int64_t p = b + lane;
+                px = sf[npairs + p]; py = sf[2 * npairs + p];
+                hxx = sf[3 * npairs + p]; hyy = sf[4 * npairs + p]; hxy = sf[5 * npairs + p];
+                ul = uloc[p];
+                u0 = uc[3 * ul]; u1 = uc[3 * ul + 1]; u2 = uc[3 * ul + 2];
+            }
+            const double a1x = 1.0 + wave_sum(u0 * px), a1y = wave_sum(u1 * px), a1z = wave_sum(u2 * px);
+            const double a2x = wave_sum(u0 * py), a2y = 1.0 + wave_sum(u1 * py), a2z = wave_sum(u2 * py);
+            const double rxx0 = wave_sum(u0 * hxx), rxx1 = wave_sum(u1 * hxx), rxx2 = wave_sum(u2 * hxx);
+            const double ryy0 = wave_sum(u0 * hyy), ryy1 = wave_sum(u1 * hyy), ryy2 = wave_sum(u2 * hyy);
+            const double rxy0 = wave_sum(u0 * hxy), rxy1 = wave_sum(u1 * hxy), rxy2 = wave_sum(u2 * hxy);
+            const double C00 = a1x * a1x + a1y * a1y + a1z * a1z;
+            const double C11 = a2x * a2x + a2y * a2y + a2z * a2z;
+            const double C01 = a1x * a2x + a1y * a2y + a1z * a2z;
+            const double idet = 1.0 / (C00 * C11 - C01 * C01);
+            const double Ci00 = C11 * idet, Ci01 = -C01 * idet, Ci11 = C00 * idet;
+            const double nx = a1y * a2z - a1z * a2y, ny = a1z * a2x - a1x * a2z, nz = a1x * a2y - a1y * a2x;
+            const double J = sqrt(nx * nx + ny * ny + nz * nz);
+            const double bx = nx / J, by = ny / J, bz = nz / J;
+            // strains (Voigt, engineering shear) and resultants (Constitutive.h:57-70)
+            const double e0[3] = {0.5 * (C00 - 1.0), 0.5 * (C11 - 1.0), C01};
+            const double e1[3] = {-(rxx0 * bx + rxx1 * by + rxx2 * bz), -(ryy0 * bx + ryy1 * by + ryy2 * bz),
+                                  -2.0 * (rxy0 * bx + rxy1 * by + rxy2 * bz)};
+            double N[3], M[3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                N[r] = A[3 * r] * e0[0] + A[3 * r + 1] * e0[1] + A[3 * r + 2] * e0[2]
+                     + B[3 * r] * e1[0] + B[3 * r + 1] * e1[1] + B[3 * r + 2] * e1[2];
+                M[r] = B[r] * e0[0] + B[3 + r] * e0[1] + B[6 + r] * e0[2]
+                     + D[3 * r] * e1[0] + D[3 * r + 1] * e1[1] + D[3 * r + 2] * e1[2];
+            }
+            const double W = w[g];
+            // V = F0 C0^-1 (Assembly.h:217), v_curv, p_XY, b.v (:221-224)
+            const double V0x = a1x * Ci00 + a2x * Ci01, V0y = a1y * Ci00 + a2y * Ci01, V0z = a1z * Ci00 + a2z * Ci01;
+            const double V1x = a1x * Ci01 + a2x * Ci11, V1y = a1y * Ci01 + a2y * Ci11, V1z = a1z * Ci01 + a2z * Ci11;
+            const double vcx = rxx0 * M[0] + 2.0 * rxy0 * M[2] + ryy0 * M[1];
+            const double vcy = rxx1 * M[0] + 2.0 * rxy1 * M[2] + ryy1 * M[1];
+            const double vcz = rxx2 * M[0] + 2.0 * rxy2 * M[2] + ryy2 * M[1];
+            const double ft0 = a1x * vcx + a1y * vcy + a1z * vcz, ft1 = a2x * vcx + a2y * vcy + a2z * vcz;
+            const double pX = Ci00 * ft0 + Ci01 * ft1, pY = Ci01 * ft0 + Ci11 * ft1;
+            const double bdv = bx * vcx + by * vcy + bz * vcz;
+            if (act) {
+                // per-neighbour quantities (Assembly.h:233-268)
+                const double gI0 = V0x * px + V1x * py, gI1 = V0y * px + V1y * py, gI2 = V0z * px + V1z * py;
+                const double SXX = rxx0 * gI0 + rxx1 * gI1 + rxx2 * gI2;
+                const double SYY = ryy0 * gI0 + ryy1 * gI1 + ryy2 * gI2;
+                const double SXY = rxy0 * gI0 + rxy1 * gI1 + rxy2 * gI2;
+                const double be0 = -hxx + SXX, be1 = -hyy + SYY, be2 = 2.0 * (-hxy + SXY);
+                // f_local = (Bm^T N + Bb^T M) W (:249-250)
+                const double ca = px * N[0] + py * N[2], cb = py * N[1] + px * N[2];
+                const double cm = be0 * M[0] + be1 * M[1] + be2 * M[2];
+                fa[3 * ul] += W * (a1x * ca + a2x * cb + bx * cm);
+                fa[3 * ul + 1] += W * (a1y * ca + a2y * cb + by * cm);
+                fa[3 * ul + 2] += W * (a1z * ca + a2z * cb + bz * cm);
+                if (with_tangent) {
+                    double* r = nb + lane * NBS;
+                    r[R_PX] = px; r[R_PY] = py;
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        r[R_AP00 + 2 * q] = A[3 * q] * px + A[3 * q + 2] * py;
+                        r[R_AP00 + 2 * q + 1] = A[3 * q + 1] * py + A[3 * q + 2] * px;
+                        r[R_BB0 + q] = B[3 * q] * be0 + B[3 * q + 1] * be1 + B[3 * q + 2] * be2;
+                        r[R_DB0 + q] = D[3 * q] * be0 + D[3 * q + 1] * be1 + D[3 * q + 2] * be2;
+                    }
+                    r[R_BE0] = be0; r[R_BE1] = be1; r[R_BE2] = be2;
+                    r[R_G0] = gI0; r[R_G1] = gI1; r[R_G2] = gI2;
+                    r[R_TU] = (px * pX + py * pY) - (hxx * M[0] + 2.0 * hxy * M[2] + hyy * M[1]);
+                    r[R_NP0] = N[0] * px + N[2] * py; r[R_NP1] = N[2] * px + N[1] * py;
+                    r[R_CP0] = Ci00 * px + Ci01 * py; r[R_CP1] = Ci01 * px + Ci11 * py;
+                    r[NBS - 1] = static_cast<double>(ul);
+                }
+            }
+            if (lane == 0 && with_tangent) {
+                gd[G_W] = W;
+                gd[G_A1] = a1x; gd[G_A1 + 1] = a1y; gd[G_A1 + 2] = a1z;
+                gd[G_A2] = a2x; gd[G_A2 + 1] = a2y; gd[G_A2 + 2] = a2z;
+                gd[G_B] = bx; gd[G_B + 1] = by; gd[G_B + 2] = bz;
+                gd[G_BDV] = bdv;
+                gd[G_N] = static_cast<double>(n);
+            }
+        }
+        if (!with_tangent) continue;
+        __syncthreads();
+        // ---- pair blocks (Assembly.h:282-324), i <= j -----------------------------
+        const int npr = n * (n + 1) / 2;
+        const double W = gd[G_W];
+        const double a1[3] = {gd[G_A1], gd[G_A1 + 1], gd[G_A1 + 2]};
+        const double a2[3] = {gd[G_A2], gd[G_A2 + 1], gd[G_A2 + 2]};
+        const double bv[3] = {gd[G_B], gd[G_B + 1], gd[G_B + 2]};
+        const double bdv = gd[G_BDV];
+        const double two_n1 = 2.0 * n + 1.0;
+        for (int p = tid; p < npr; p += TILE_THREADS) {
+            // row i of the upper triangle: rowstart(i) = i*n - i*(i-1)/2
+            int i = static_cast<int>((two_n1 - sqrt(two_n1 * two_n1 - 8.0 * p)) * 0.5);
+            if (i < 0) i = 0;
+            if (i > n - 1) i = n - 1;
+            while (i > 0 && i * n - i * (i - 1) / 2 > p) --i;
+            while (i + 1 < n && (i + 1) * n - (i + 1) * i / 2 <= p) ++i;
+            const int j = i + (p - (i * n - i * (i - 1) / 2));
+            const double* ri = nb + i * NBS;
+            const double* rj = nb + j * NBS;
+            const double pxi = ri[R_PX], pyi = ri[R_PY], pxj = rj[R_PX], pyj = rj[R_PY];
+            const double S00 = pxi * rj[R_AP00] + pyi * rj[R_AP20];
+            const double S01 = pxi * rj[R_AP01] + pyi * rj[R_AP21];
+            const double S10 = pyi * rj[R_AP10] + pxi * rj[R_AP20];
+            const double S11 = pyi * rj[R_AP11] + pxi * rj[R_AP21];
+            const double m0 = pxi * rj[R_BB0] + pyi * rj[R_BB2], m1 = pyi * rj[R_BB1] + pxi * rj[R_BB2];
+            const double q0 = pxj * ri[R_BB0] + pyj * ri[R_BB2], q1 = pyj * ri[R_BB1] + pxj * ri[R_BB2];
+            const double z = ri[R_BE0] * rj[R_DB0] + ri[R_BE1] * rj[R_DB1] + ri[R_BE2] * rj[R_DB2]
+                           + bdv * (pxi * rj[R_CP0] + pyi * rj[R_CP1]);
+            const double geo = pxi * rj[R_NP0] + pyi * rj[R_NP1];
+            const double tui = ri[R_TU], tuj = rj[R_TU];
+            double xv[3], yv[3], T0[3], T1[3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                xv[r] = a1[r] * m0 + a2[r] * m1 - tui * rj[R_G0 + r];
+                yv[r] = a1[r] * q0 + a2[r] * q1 - tuj * ri[R_G0 + r];
+                T0[r] = a1[r] * S00 + a2[r] * S10;
+                T1[r] = a1[r] * S01 + a2[r] * S11;
+            }
+            const int ua = static_cast<int>(ri[NBS - 1]), ubb = static_cast<int>(rj[NBS - 1]);
+            const bool flip = ua > ubb;   // union sorted by node id: ua > ub <=> ni > nj
+            const int lo = flip ? ubb : ua, hi = flip ? ua : ubb;
+            double kv[3][3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc)
+                    kv[r][cc] = W * (T0[r] * a1[cc] + T1[r] * a2[cc] + xv[r] * bv[cc] + bv[r] * yv[cc]
+                                     + z * bv[r] * bv[cc] + (r == cc ? geo : 0.0));
+            double* dst = acc + 9 * static_cast<int>(lut[lo * lut_dim + hi]);
+            // canonical (lo, hi) block, stored column-major: dst[3c + r] = block(r, c)
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+                for (int r = 0; r < 3; ++r) dst[3 * cc + r] += flip ? kv[cc][r] : kv[r][cc];
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    for (int k = tid; k < 3 * U; k += TILE_THREADS) fscr[3 * static_cast<int64_t>(ub) + k] = fa[k];
+    if (with_tangent)
+        for (int k = tid; k < 9 * T; k += TILE_THREADS) scr[9 * static_cast<int64_t>(tb) + k] = acc[k];
+}
+
+// ---------------------------------------------------------------------------
+// Elastic hinge per crease GP (Constraints.h:71-119, :341-436). Writes the
+// GP's W*Kt and W*moment, and per node of L u R the aggregated gradient
+// G(node) = sum over its occurrences of dtheta/du (3 dofs).
+// ---------------------------------------------------------------------------
+struct FoldDev {
+    const double* lsf; const double* rsf;
+    const int* loff; const int* roff;
+    const int* lids; const int* rids;
+    int64_t lnp, rnp;
+    const double* w;       // left-domain weights
+    const double* tx; const double* ty;
+    const double* theta0;  // may be null
+    double Kt, Ang0;
+};
+
+__device__ __forceinline__ void cross3(const double* a, const double* b, double* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+__device__ void normal_derivs(const double* a1, const double* a2, const double* b, double nn, double out[3][6]) {
+    double dn[3][6];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        double e[3] = {0, 0, 0};
+        e[i] = 1.0;
+        double c1[3], c2[3];
+        cross3(e, a2, c1);
+        cross3(a1, e, c2);
+        dn[0][i] = c1[0]; dn[1][i] = c1[1]; dn[2][i] = c1[2];
+        dn[0][i + 3] = c2[0]; dn[1][i + 3] = c2[1]; dn[2][i + 3] = c2[2];
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) acc += (((r == k) ? 1.0 : 0.0) - b[r] * b[k]) / nn * dn[k][c];
+            out[r][c] = acc;
+        }
+}
+
+__device__ double fold_angle_grad(const double* F, double tx, double ty, double* grad) {
+    const double* a1L = F; const double* a2L = F + 3; const double* a1R = F + 6; const double* a2R = F + 9;
+    double nL[3], nR[3];
+    cross3(a1L, a2L, nL);
+    cross3(a1R, a2R, nR);
+    const double normL = sqrt(nL[0] * nL[0] + nL[1] * nL[1] + nL[2] * nL[2]);
+    const double normR = sqrt(nR[0] * nR[0] + nR[1] * nR[1] + nR[2] * nR[2]);
+    const double bL[3] = {nL[0] / normL, nL[1] / normL, nL[2] / normL};
+    const double bR[3] = {nR[0] / normR, nR[1] / normR, nR[2] / normR};
+    const double aT[3] = {a1L[0] * tx + a2L[0] * ty, a1L[1] * tx + a2L[1] * ty, a1L[2] * tx + a2L[2] * ty};
+    const double naT = sqrt(aT[0] * aT[0] + aT[1] * aT[1] + aT[2] * aT[2]);
+    const double t[3] = {aT[0] / naT, aT[1] / naT, aT[2] / naT};
+    const double c = bL[0] * bR[0] + bL[1] * bR[1] + bL[2] * bR[2];
+    double bLxbR[3];
+    cross3(bL, bR, bLxbR);
+    const double s = bLxbR[0] * t[0] + bLxbR[1] * t[1] + bLxbR[2] * t[2];
+    const double theta = atan2(-s, c);
+    double dbL[3][6], dbR[3][6];
+    normal_derivs(a1L, a2L, bL, normL, dbL);
+    normal_derivs(a1R, a2R, bR, normR, dbR);
+    const double daT[3][6] = {{tx, 0, 0, ty, 0, 0}, {0, tx, 0, 0, ty, 0}, {0, 0, tx, 0, 0, ty}};
+    double dt[3][6];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int cc = 0; cc < 6; ++cc) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) acc += (((r == k) ? 1.0 : 0.0) - t[r] * t[k]) / naT * daT[k][cc];
+            dt[r][cc] = acc;
+        }
+    double bRxt[3], txbL[3];
+    cross3(bR, t, bRxt);
+    cross3(t, bL, txbL);
+    const double denom = c * c + s * s;
+    const double idn = (denom > 1e-14) ? 1.0 / denom : 0.0;
+#pragma unroll
+    for (int cc = 0; cc < 6; ++cc) {
+        const double dcL = bR[0] * dbL[0][cc] + bR[1] * dbL[1][cc] + bR[2] * dbL[2][cc];
+        const double dcR = bL[0] * dbR[0][cc] + bL[1] * dbR[1][cc] + bL[2] * dbR[2][cc];
+        const double dsL = (bRxt[0] * dbL[0][cc] + bRxt[1] * dbL[1][cc] + bRxt[2] * dbL[2][cc])
+                         + (bLxbR[0] * dt[0][cc] + bLxbR[1] * dt[1][cc] + bLxbR[2] * dt[2][cc]);
+        const double dsR = txbL[0] * dbR[0][cc] + txbL[1] * dbR[1][cc] + txbL[2] * dbR[2][cc];
+        grad[cc] = -(c * dsL - s * dcL) * idn;
+        grad[6 + cc] = -(c * dsR - s * dcR) * idn;
+    }
+    return theta;
+}
+
+__global__ void k_fold(int nfgp, const FoldDev* __restrict__ folds, const int* __restrict__ fgp_fold,
+                       const int* __restrict__ fgp_local, const int* __restrict__ fgp_noff,
+                       const int* __restrict__ lpo, const int* __restrict__ rpo,
+                       const int* __restrict__ lpos, const int* __restrict__ rpos,
+                       const double* __restrict__ xu, double* __restrict__ fgp_rec, double* __restrict__ Gnd) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nfgp) return;
+    const FoldDev fd = folds[fgp_fold[q]];
+    const int i = fgp_local[q];
+    double F[12];
+    // F0 of the left and right GPs (Kinematics.h:75-91)
+    for (int side = 0; side < 2; ++side) {
+        const double* sf = side ? fd.rsf : fd.lsf;
+        const int* of = side ? fd.roff : fd.loff;
+        const int* id = side ? fd.rids : fd.lids;
+        const int64_t np = side ? fd.rnp : fd.lnp;
+        double a1[3] = {1.0, 0.0, 0.0}, a2[3] = {0.0, 1.0, 0.0};
+        for (int k = of[i]; k < of[i + 1]; ++k) {
+            const int v = id[k];
+            const double px = sf[np + k], py = sf[2 * np + k];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) { a1[d] += xu[3 * v + d] * px; a2[d] += xu[3 * v + d] * py; }
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { F[6 * side + d] = a1[d]; F[6 * side + 3 + d] = a2[d]; }
+    }
+    double grad[12];
+    const double theta = fold_angle_grad(F, fd.tx[i], fd.ty[i], grad);
+    const double th0 = fd.theta0 ? fd.theta0[i] : fd.Ang0;
+    const double W = fd.w[i];
+    fgp_rec[2 * q] = W * fd.Kt;
+    fgp_rec[2 * q + 1] = W * (fd.Kt * (theta - th0));
+    const int nb0 = fgp_noff[q], nb1 = fgp_noff[q + 1];
+    for (int k = nb0; k < nb1; ++k) { Gnd[3 * k] = 0.0; Gnd[3 * k + 1] = 0.0; Gnd[3 * k + 2] = 0.0; }
+    for (int side = 0; side < 2; ++side) {
+        const double* sf = side ? fd.rsf : fd.lsf;
+        const int* of = side ? fd.roff : fd.loff;
+        const int64_t np = side ? fd.rnp : fd.lnp;
+        const int* pos = side ? (rpos + rpo[q]) : (lpos + lpo[q]);
+        const int blk = 6 * side;
+        for (int k = of[i]; k < of[i + 1]; ++k) {
+            const double dX = sf[np + k], dY = sf[2 * np + k];
+            double* G = Gnd + 3 * static_cast<int64_t>(nb0 + pos[k - of[i]]);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) G[d] += grad[blk + d] * dX + grad[blk + d + 3] * dY;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Segmented reductions (one thread per K block / per node).
+// ---------------------------------------------------------------------------
+__global__ void k_reduce_K(int64_t nblocks, const int64_t* __restrict__ kc_off, const int* __restrict__ kc_idx,
+                           const double* __restrict__ scr, const int64_t* __restrict__ kf_off,
+                           const int* __restrict__ kf_idx, const int* __restrict__ nd_gp,
+                           const double* __restrict__ fgp_rec, const double* __restrict__ Gnd, int folds,
+                           const int64_t* __restrict__ base, const int* __restrict__ stride,
+                           double* __restrict__ K) {
+    const int64_t blk = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (blk >= nblocks) return;
+    const int64_t bs = base[blk];
+    if (bs < 0) return;
+    double v[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) v[k] = 0.0;
+    for (int64_t a = kc_off[blk]; a < kc_off[blk + 1]; ++a) {
+        const double* s = scr + 9 * static_cast<int64_t>(kc_idx[a]);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) v[k] += s[k];
+    }
+    if (folds) {
+        double f[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) f[k] = 0.0;
+        for (int64_t a = kf_off[blk]; a < kf_off[blk + 1]; ++a) {
+            const int lo = kf_idx[2 * a], hi = kf_idx[2 * a + 1];
+            const double wk = fgp_rec[2 * nd_gp[lo]];
+            const double* Gl = Gnd + 3 * static_cast<int64_t>(lo);
+            const double* Gh = Gnd + 3 * static_cast<int64_t>(hi);
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int r = 0; r < 3; ++r) f[3 * c + r] += Gl[r] * Gh[c] * wk;
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) v[k] += f[k];
+    }
+    const int st = stride[blk];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) K[bs + static_cast<int64_t>(c) * st + r] = v[3 * c + r];
+}
+
+__global__ void k_reduce_F(int nn, const int* __restrict__ fc_off, const int* __restrict__ fc_idx,
+                           const double* __restrict__ fscr, const int* __restrict__ ff_off,
+                           const int* __restrict__ ff_idx, const int* __restrict__ nd_gp,
+                           const double* __restrict__ fgp_rec, const double* __restrict__ Gnd, int folds,
+                           double* __restrict__ F) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    for (int a = fc_off[i]; a < fc_off[i + 1]; ++a) {
+        const double* s = fscr + 3 * static_cast<int64_t>(fc_idx[a]);
+        f0 += s[0]; f1 += s[1]; f2 += s[2];
+    }
+    if (folds) {
+        for (int a = ff_off[i]; a < ff_off[i + 1]; ++a) {
+            const int k = ff_idx[a];
+            const double wm = fgp_rec[2 * nd_gp[k] + 1];
+            f0 += Gnd[3 * k] * wm; f1 += Gnd[3 * k + 1] * wm; f2 += Gnd[3 * k + 2] * wm;
+        }
+    }
+    F[3 * i] = f0; F[3 * i + 1] = f1; F[3 * i + 2] = f2;
+}
+
+size_t tile_lds_bytes(int max_touched, int lut_dim) {
+    return sizeof(double) * (static_cast<size_t>(max_touched) * 9 + NBMAX * NBS + 6 * lut_dim + 16)
+         + sizeof(uint16_t) * static_cast<size_t>(lut_dim) * lut_dim;
+}
+
+void launch_assembly(Context& c, int flags, const double* xdev, double* Fdev, double* Kdev) {
+    Plan& P = c.plan;
+    if (!P.ready) throw Error(CVM_E_STATE, "assembly plan not initialised (cvm_init_assembly)");
+    Domain* fdp = nullptr;
+    for (Domain* d : c.doms) if (d->kind == CVM_DOMAIN_FACE) fdp = d;
+    Domain& fd = *fdp;
+    const bool tangent = (flags & CVM_TANGENT) != 0;
+    const bool folds = (flags & CVM_ELASTIC_FOLDS) != 0 && P.nfgp > 0;
+    const int lut_dim = P.lut_dim;
+    MatC m;
+    std::memcpy(m.A, c.A, sizeof(m.A));
+    std::memcpy(m.B, c.B, sizeof(m.B));
+    std::memcpy(m.D, c.D, sizeof(m.D));
+    c.d_fscr.alloc(std::max<int64_t>(3 * P.nfscratch, 1));
+    if (tangent) c.d_scr.alloc(std::max<int64_t>(9 * P.nscratch, 1));
+    const size_t lds = tile_lds_bytes(tangent ? P.max_touched : 0, lut_dim);
+    int slot = timing_begin(c, tangent ? "tile_residual_tangent" : "tile_residual");
+    hipLaunchKernelGGL(k_tile, dim3(P.ntiles), dim3(TILE_THREADS), lds, c.stream, P.d_tile_gp.p, P.d_tile_uoff.p,
+                       P.d_tile_unodes.p, P.d_tile_toff.p, P.d_tile_tpairs.p, fd.off.p, P.d_pair_uloc.p, fd.sf.p,
+                       static_cast<int64_t>(fd.npairs), fd.d_w.p, xdev, m, tangent ? P.max_touched : 0, lut_dim,
+                       tangent ? 1 : 0, c.d_scr.p, c.d_fscr.p);
+    CVM_HIP(hipGetLastError());
+    timing_end(c, slot);
+    if (folds) {
+        std::vector<FoldDev> fdv(c.folds.size());
+        for (size_t f = 0; f < c.folds.size(); ++f) {
+            const Fold& fo = c.folds[f];
+            Domain& L = *c.doms[fo.left];
+            Domain& R = *c.doms[fo.right];
+            FoldDev& v = fdv[f];
+            v.lsf = L.sf.p; v.rsf = R.sf.p; v.loff = L.off.p; v.roff = R.off.p; v.lids = L.ids.p; v.rids = R.ids.p;
+            v.lnp = L.npairs; v.rnp = R.npairs; v.w = L.d_w.p; v.tx = L.d_tx.p; v.ty = L.d_ty.p;
+            v.theta0 = fo.theta0.empty() ? nullptr : fo.d_theta0.p;
+            v.Kt = fo.Kt; v.Ang0 = fo.Ang0;
+        }
+        c.d_folddev.upload(reinterpret_cast<const unsigned char*>(fdv.data()), fdv.size() * sizeof(FoldDev), c.stream);
+        c.d_fold_gp.alloc(2 * P.nfgp);
+        c.d_fold_nd.alloc(3 * std::max<int64_t>(P.nfnodes, 1));
+        slot = timing_begin(c, "fold_hinge");
+        hipLaunchKernelGGL(k_fold, dim3((P.nfgp + 63) / 64), dim3(64), 0, c.stream, P.nfgp,
+                           reinterpret_cast<const FoldDev*>(c.d_folddev.p), P.d_fgp_fold.p, P.d_fgp_local.p,
+                           P.d_fgp_noff.p, P.d_fgp_lpo.p, P.d_fgp_rpo.p, P.d_fgp_lpos.p, P.d_fgp_rpos.p, xdev,
+                           c.d_fold_gp.p, c.d_fold_nd.p);
+        CVM_HIP(hipGetLastError());
+        timing_end(c, slot);
+    }
+    slot = timing_begin(c, "reduce_F");
+    hipLaunchKernelGGL(k_reduce_F, dim3((c.nn + 255) / 256), dim3(256), 0, c.stream, c.nn, P.d_fc_off.p, P.d_fc_idx.p,
+                       c.d_fscr.p, P.d_ff_off.p, P.d_ff_idx.p, P.d_fnd_gp.p, c.d_fold_gp.p, c.d_fold_nd.p,
+                       folds ? 1 : 0, Fdev);
+    CVM_HIP(hipGetLastError());
+    timing_end(c, slot);
+    if (tangent) {
+        const bool uni = (flags & CVM_ELASTIC_FOLDS) != 0;
+        slot = timing_begin(c, "reduce_K");
+        hipLaunchKernelGGL(k_reduce_K, dim3(static_cast<unsigned>((P.nblocks + 255) / 256)), dim3(256), 0, c.stream,
+                           P.nblocks, P.d_kc_off.p, P.d_kc_idx.p, c.d_scr.p, P.d_kf_off.p, P.d_kf_idx.p, P.d_fnd_gp.p,
+                           c.d_fold_gp.p, c.d_fold_nd.p, folds ? 1 : 0, uni ? P.d_blk_base_u.p : P.d_blk_base_f.p,
+                           uni ? P.d_blk_stride_u.p : P.d_blk_stride_f.p, Kdev);
+        CVM_HIP(hipGetLastError());
+        timing_end(c, slot);
+    }
+}
+
+}  // namespace cvm

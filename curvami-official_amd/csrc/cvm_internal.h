@@ -1,0 +1,214 @@
+// Internal definitions of the curvami GPU library (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/curvami_gpu.h"
+
+namespace cvm {
+
+// Error carrying a C-ABI status code; converted to a status at the ABI.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define CVM_HIP(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            throw ::cvm::Error(CVM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Owning device allocation.
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        if (count == n && p) return;
+        release();
+        if (count == 0) return;
+        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            throw Error(CVM_E_NO_MEMORY, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+        }
+        n = count;
+    }
+    void upload(const T* h, size_t count, hipStream_t s) {
+        alloc(count);
+        if (count) CVM_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
+    void download(T* h, size_t count, hipStream_t s) const {
+        if (count) CVM_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+struct Crease {
+    std::vector<double> vx, vy;
+    int ext_s = 0, ext_e = 0, closed = 0;
+};
+
+// Shape-function cache of one domain, device resident, pair order = FRNN order
+// (GP-major, neighbours in the search order of GlobalFRNN::execute).
+struct Domain {
+    int kind = CVM_DOMAIN_FACE;
+    int ng = 0;
+    std::vector<double> hx, hy, hw, hnx, hny, htx, hty;   // host copies of the GP data
+    DBuf<double> gx, gy, d_w, d_tx, d_ty;
+    // neighbour lists
+    double rho = 0.0;
+    int64_t npairs = -1;
+    DBuf<int> off;            // ng + 1
+    DBuf<int> ids;            // npairs
+    DBuf<double> s;           // npairs
+    std::vector<int> h_off, h_ids;     // host mirrors (needed for the pattern plan)
+    // shape functions: psi, dpx, dpy, dxx, dyy, dxy (SoA, npairs each)
+    DBuf<double> sf;          // 6 * npairs
+    int order = 0, kernel = 0;
+    bool have_sf = false;
+    // per-GP crease indicator data: 6 doubles per (crease, GP)
+    DBuf<double> ind;
+};
+
+struct Fold {
+    int left = -1, right = -1;
+    double Kt = 0.0, Ang0 = 0.0;
+    std::vector<double> theta0;   // per crease GP (empty: Ang0)
+    std::vector<char> active;     // per crease GP (empty: all active)
+    DBuf<double> d_theta0;
+};
+
+// Assembly plan: tiles of face GPs, block pattern, contributor lists.
+struct Plan {
+    bool ready = false;
+    int nn = 0;
+    // block-upper pattern at node level: CSC of blocks (union of face and fold pairs)
+    std::vector<int> col_ptr;       // nn + 1 (blocks per column node)
+    std::vector<int> col_rows;      // nblocks: row node of each block (ascending in a column)
+    std::vector<char> blk_face;     // 1 if block is in the face pattern
+    int64_t nblocks = 0;
+    // face-only pattern (CSC of scalars) and union pattern
+    std::vector<int> face_outer, face_inner, uni_outer, uni_inner;
+    // per block: base index and column stride in both patterns (-1: absent)
+    DBuf<int64_t> d_blk_base_u, d_blk_base_f;
+    DBuf<int> d_blk_stride_u, d_blk_stride_f;
+    // tiles
+    int ntiles = 0;
+    int tile_max_gps = 16, union_max = 64;
+    DBuf<int> d_tile_gp;        // ntiles + 1
+    DBuf<int> d_tile_uoff;      // ntiles + 1
+    DBuf<int> d_tile_unodes;    // union node ids
+    DBuf<int> d_tile_toff;      // ntiles + 1 (touched pairs)
+    DBuf<uint16_t> d_tile_tpairs;   // (ua << 8) | ub
+    DBuf<uint8_t> d_pair_uloc;  // per face pair: union-local index
+    int64_t nscratch = 0;       // total touched pairs
+    int64_t nfscratch = 0;      // total union entries
+    // K contributors: per block, scratch indices (ascending tile order)
+    DBuf<int64_t> d_kc_off;
+    DBuf<int> d_kc_idx;
+    // F contributors: per node, fscratch indices
+    DBuf<int> d_fc_off;
+    DBuf<int> d_fc_idx;
+    // folds: crease GPs aggregated per node
+    int nfgp = 0;                       // total crease GPs over folds
+    DBuf<int> d_fgp_fold;               // fold of each crease GP
+    DBuf<int> d_fgp_local;              // index within its fold
+    DBuf<int> d_fgp_noff;               // per crease GP: offsets into node set (nfgp + 1)
+    DBuf<int> d_fgp_nodes;              // node ids of L u R (sorted unique)
+    DBuf<int> d_fgp_lpos;               // per L neighbour entry: position in node set (by crease GP)
+    DBuf<int> d_fgp_rpos;               // per R neighbour entry
+    DBuf<int> d_fgp_lpo, d_fgp_rpo;     // per crease GP: offsets into lpos / rpos
+    DBuf<int> d_fnd_gp;                 // node-set entry -> crease GP
+    int64_t nfnodes = 0;                // node-set entries over all crease GPs
+    DBuf<int64_t> d_kf_off;             // per block: fold contributor offsets
+    DBuf<int> d_kf_idx;                 // per contribution: node-set entries of lo and hi
+    DBuf<int> d_ff_off;                 // per node: fold contributions (nodeset entry indices)
+    DBuf<int> d_ff_idx;
+    int max_touched = 0, max_union = 0, lut_dim = 0;
+    int face_ng = 0;
+    int64_t face_npairs = 0;
+    std::vector<std::vector<int>> face_rows;   // face pattern rows per column (csrMapping)
+};
+
+struct Timing {
+    bool on = false;
+    std::vector<std::string> names;
+    std::vector<hipEvent_t> ev0, ev1;
+    std::vector<int> pending;   // kernel id per event pair recorded since last read
+    std::vector<double> total_ms;
+    std::vector<int64_t> count;
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // nodes
+    int nn = 0;
+    std::vector<double> hX, hY;
+    DBuf<double> X, Y;
+    // material
+    double A[9] = {0}, B[9] = {0}, D[9] = {0};
+    bool have_mat = false;
+    // creases (host) + device packed vertices
+    std::vector<Crease> creases;
+    DBuf<double> d_cv;          // packed vertices x,y
+    DBuf<int> d_cinfo;          // per crease: offset, count, ext_s, ext_e, closed
+    DBuf<int8_t> d_node_sign;   // per (crease, node): +1 / -1
+    bool creases_dirty = true;
+    std::vector<Domain*> doms;
+    std::vector<Fold> folds;
+    Plan plan;
+    // assembly buffers
+    DBuf<double> d_x;           // 3 nn displacements
+    DBuf<double> d_F;           // 3 nn
+    DBuf<double> d_K;           // nnz (union)
+    DBuf<double> d_scr;         // 9 * nscratch
+    DBuf<double> d_fscr;        // 3 * nfscratch
+    DBuf<double> d_fold_gp;     // per crease GP: W*Kt, W*moment
+    DBuf<double> d_fold_nd;     // per (crease GP, node of L u R): aggregated dtheta/du
+    DBuf<unsigned char> d_folddev;
+    size_t lds_limit = 65536;
+    Timing timing;
+    int tile_gps = 16;
+    int union_max = 64;
+    ~Context();
+};
+
+// kernel launchers (implemented in the .hip files)
+void launch_indicator_gp(Context& c, Domain& d);
+void launch_node_signs(Context& c);
+void launch_frnn(Context& c, Domain& d, double rho);
+void launch_rkpm(Context& c, Domain& d, double rho, int order, int kernel, int* code, int* nbad, int* first);
+void launch_assembly(Context& c, int flags, const double* xdev, double* Fdev, double* Kdev);
+size_t tile_lds_bytes(int max_touched, int lut_dim);
+
+// timing helpers
+int timing_begin(Context& c, const char* name);
+void timing_end(Context& c, int slot);
+
+void build_plan(Context& c);
+
+}  // namespace cvm

@@ -1,0 +1,272 @@
+// Fixed-radius neighbour search and crease indicator on gfx950.
+//
+// Neighbour lists must equal the reference's bit for bit (sets and order), so
+// the distance test and the cell assignment use the reference's operations
+// without fused multiply-add: floor((x - min) / rho) with IEEE division, and
+// dist2 = dx*dx + dy*dy (core/include/curvami/FRNN.h:12-16, :63-72).
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "cvm_internal.h"
+
+namespace cvm {
+
+// ---------------------------------------------------------------------------
+// Crease indicator (core/include/curvami/Enrichment.h:23-92), evaluated once
+// per point. The reference evaluates it per (pair, crease) inside the RKPM
+// solve (GlobalRKPM.h:91-92); the value depends on the point only.
+// ---------------------------------------------------------------------------
+struct IndOut { double f, gx, gy, hxx, hyy, hxy; };
+
+__device__ __forceinline__ IndOut indicator(double Px, double Py, const double* vx, const double* vy,
+                                            int N, int ext_s, int ext_e, int closed) {
+#pragma clang fp contract(off)
+    IndOut o{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (N < 2) return o;
+    const double inf = __builtin_inf();
+    double d = inf;
+    double Cbx = 0.0, Cby = 0.0, tb = 0.0, nbx = 0.0, nby = 0.0;
+    int ib = 0;
+    for (int i = 0; i < N - 1; ++i) {
+        const double Ax = vx[i], Ay = vy[i];
+        const double ux = vx[i + 1] - Ax, uy = vy[i + 1] - Ay;
+        const double vsq = ux * ux + uy * uy;
+        if (vsq == 0.0) continue;
+        const double wx = Px - Ax, wy = Py - Ay;
+        const double t = (wx * ux + wy * uy) / vsq;
+        const double tmin = (i == 0 && ext_s) ? -inf : 0.0;
+        const double tmax = (i == N - 2 && ext_e) ? inf : 1.0;
+        const double tc = fmax(tmin, fmin(tmax, t));
+        const double Cx = Ax + tc * ux, Cy = Ay + tc * uy;
+        const double ex = Px - Cx, ey = Py - Cy;
+        const double dsq = ex * ex + ey * ey;
+        if (dsq < d * d) {
+            d = sqrt(dsq);
+            Cbx = Cx; Cby = Cy; ib = i; tb = tc;
+            const double sv = sqrt(vsq);
+            nbx = -uy / sv; nby = ux / sv;
+        }
+    }
+    if (!(d > 1e-12)) return o;                       // on the crease: f = 0
+    const double ux = (Px - Cbx) / d, uy = (Py - Cby) / d;
+    const double dp = ux * nbx + uy * nby;
+    const bool sc = (ib == 0 && tb == 0.0 && !ext_s && !closed);
+    const bool ec = (ib == N - 2 && tb == 1.0 && !ext_e && !closed);
+    if (sc || ec) {
+        o.f = dp;
+        o.gx = (nbx - o.f * ux) / d;
+        o.gy = (nby - o.f * uy) / d;
+        const double d2 = d * d;
+        o.hxx = (3 * o.f * ux * ux - 2 * nbx * ux - o.f) / d2;
+        o.hyy = (3 * o.f * uy * uy - 2 * nby * uy - o.f) / d2;
+        o.hxy = (3 * o.f * ux * uy - (nbx * uy + ux * nby)) / d2;
+    } else {
+        o.f = (dp >= 0.0) ? 1.0 : -1.0;
+    }
+    return o;
+}
+
+// Per-GP indicator data, layout ind[(c * 6 + k) * ng + g] (lane-coalesced).
+__global__ void k_indicator_gp(const double* __restrict__ gx, const double* __restrict__ gy, int ng,
+                               const double* __restrict__ cv, const int* __restrict__ cinfo, int nc,
+                               double* __restrict__ ind) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    const double px = gx[g], py = gy[g];
+    for (int c = 0; c < nc; ++c) {
+        const int off = cinfo[5 * c], nv = cinfo[5 * c + 1];
+        const IndOut o = indicator(px, py, cv + 2 * off, cv + 2 * off + nv, nv, cinfo[5 * c + 2],
+                                   cinfo[5 * c + 3], cinfo[5 * c + 4]);
+        double* dst = ind + static_cast<size_t>(c) * 6 * ng + g;
+        dst[0] = o.f; dst[ng] = o.gx; dst[2 * ng] = o.gy;
+        dst[3 * ng] = o.hxx; dst[4 * ng] = o.hyy; dst[5 * ng] = o.hxy;
+    }
+}
+
+// Node side of every crease: +1 when f >= 0 (a node on the crease counts as
+// the + side), -1 otherwise (GlobalRKPM.h:93).
+__global__ void k_node_sign(const double* __restrict__ X, const double* __restrict__ Y, int nn,
+                            const double* __restrict__ cv, const int* __restrict__ cinfo, int nc,
+                            int8_t* __restrict__ sign) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    for (int c = 0; c < nc; ++c) {
+        const int off = cinfo[5 * c], nv = cinfo[5 * c + 1];
+        const IndOut o = indicator(X[i], Y[i], cv + 2 * off, cv + 2 * off + nv, nv, cinfo[5 * c + 2],
+                                   cinfo[5 * c + 3], cinfo[5 * c + 4]);
+        sign[static_cast<size_t>(c) * nn + i] = (o.f >= 0.0) ? 1 : -1;
+    }
+}
+
+static void upload_creases(Context& c) {
+    if (!c.creases_dirty) return;
+    std::vector<double> cv;
+    std::vector<int> info;
+    for (const auto& cr : c.creases) {
+        const int off = static_cast<int>(cv.size() / 2);
+        const int nv = static_cast<int>(cr.vx.size());
+        // per crease: nv x values then nv y values at pair offset `off`
+        std::vector<double> blk(2 * nv);
+        for (int i = 0; i < nv; ++i) { blk[i] = cr.vx[i]; blk[nv + i] = cr.vy[i]; }
+        cv.insert(cv.end(), blk.begin(), blk.end());
+        info.insert(info.end(), {off, nv, cr.ext_s, cr.ext_e, cr.closed});
+    }
+    if (cv.empty()) cv.push_back(0.0);
+    if (info.empty()) info.assign(5, 0);
+    c.d_cv.upload(cv, c.stream);
+    c.d_cinfo.upload(info, c.stream);
+    c.creases_dirty = false;
+}
+
+void launch_node_signs(Context& c) {
+    upload_creases(c);
+    const int nc = static_cast<int>(c.creases.size());
+    c.d_node_sign.alloc(std::max<size_t>(1, static_cast<size_t>(nc) * c.nn));
+    if (nc == 0 || c.nn == 0) return;
+    const int slot = timing_begin(c, "node_sign");
+    hipLaunchKernelGGL(k_node_sign, dim3((c.nn + 255) / 256), dim3(256), 0, c.stream, c.X.p, c.Y.p, c.nn,
+                       c.d_cv.p, c.d_cinfo.p, nc, c.d_node_sign.p);
+    CVM_HIP(hipGetLastError());
+    timing_end(c, slot);
+}
+
+void launch_indicator_gp(Context& c, Domain& d) {
+    upload_creases(c);
+    const int nc = static_cast<int>(c.creases.size());
+    d.ind.alloc(std::max<size_t>(1, static_cast<size_t>(nc) * 6 * d.ng));
+    if (nc == 0 || d.ng == 0) return;
+    const int slot = timing_begin(c, "indicator_gp");
+    hipLaunchKernelGGL(k_indicator_gp, dim3((d.ng + 255) / 256), dim3(256), 0, c.stream, d.gx.p, d.gy.p, d.ng,
+                       c.d_cv.p, c.d_cinfo.p, nc, d.ind.p);
+    CVM_HIP(hipGetLastError());
+    timing_end(c, slot);
+}
+
+// ---------------------------------------------------------------------------
+// Fixed-radius neighbours (FRNN.h:23-78). Nodes are binned by the key
+// y*cols + x of their rho-sized cell, in ascending node index within a cell
+// (the insertion order of the reference's per-cell vectors). A query scans
+// the keys (qy+dy)*cols + (qx+dx), dx outer and dy inner, exactly as the
+// reference does, including the wrap of an out-of-range x into the adjacent
+// row's key; keys outside [0, nkeys) are empty, like a failed map lookup.
+// ---------------------------------------------------------------------------
+struct Grid {
+    double minx, miny, cell;
+    int cols, nkeys;
+};
+
+template <bool FILL>
+__global__ void k_frnn(const double* __restrict__ X, const double* __restrict__ Y,
+                       const double* __restrict__ gx, const double* __restrict__ gy, int ng,
+                       const int* __restrict__ cstart, const int* __restrict__ cnodes, Grid gr, double rho2,
+                       double rho, int* __restrict__ count, const int* __restrict__ off,
+                       int* __restrict__ ids, double* __restrict__ s) {
+#pragma clang fp contract(off)
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ng) return;
+    const double px = gx[q], py = gy[q];
+    const int qx = static_cast<int>(floor((px - gr.minx) / gr.cell));
+    const int qy = static_cast<int>(floor((py - gr.miny) / gr.cell));
+    int n = 0;
+    int o = FILL ? off[q] : 0;
+    for (int dx = -1; dx <= 1; ++dx) {
+        for (int dy = -1; dy <= 1; ++dy) {
+            const long long key = static_cast<long long>(qy + dy) * gr.cols + (qx + dx);
+            if (key < 0 || key >= gr.nkeys) continue;
+            const int b = cstart[key], e = cstart[key + 1];
+            for (int k = b; k < e; ++k) {
+                const int ni = cnodes[k];
+                const double ex = X[ni] - px, ey = Y[ni] - py;
+                const double d2 = ex * ex + ey * ey;
+                if (d2 <= rho2) {
+                    if (FILL) {
+                        ids[o] = ni;
+                        s[o] = sqrt(d2) / rho;
+                        ++o;
+                    } else {
+                        ++n;
+                    }
+                }
+            }
+        }
+    }
+    if (!FILL) count[q] = n;
+}
+
+__global__ void k_set_zero_int(int* p) { p[0] = 0; }
+
+void launch_frnn(Context& c, Domain& d, double rho) {
+    if (c.nn == 0) throw Error(CVM_E_STATE, "nodes not set");
+    if (!(rho > 0.0)) throw Error(CVM_E_INVALID, "rho must be > 0");
+    d.rho = rho;
+    d.off.alloc(d.ng + 1);
+    d.h_off.assign(d.ng + 1, 0);
+    if (d.ng == 0) {
+        d.npairs = 0;
+        CVM_HIP(hipMemsetAsync(d.off.p, 0, sizeof(int), c.stream));
+        return;
+    }
+    // Host binning of the nodes (O(nn)); bounds and cols as FRNN.h:30-42.
+    double mnx = c.hX[0], mny = c.hY[0], mxx = c.hX[0], mxy = c.hY[0];
+    for (int i = 0; i < c.nn; ++i) {
+        mnx = std::min(mnx, c.hX[i]); mny = std::min(mny, c.hY[i]);
+        mxx = std::max(mxx, c.hX[i]); mxy = std::max(mxy, c.hY[i]);
+    }
+    Grid gr;
+    gr.minx = mnx; gr.miny = mny; gr.cell = rho;
+    gr.cols = static_cast<int>(std::ceil((mxx - mnx) / rho)) + 1;
+    std::vector<long long> key(c.nn);
+    long long maxkey = 0;
+    for (int i = 0; i < c.nn; ++i) {
+        const int x = static_cast<int>(std::floor((c.hX[i] - mnx) / rho));
+        const int y = static_cast<int>(std::floor((c.hY[i] - mny) / rho));
+        key[i] = static_cast<long long>(y) * gr.cols + x;
+        maxkey = std::max(maxkey, key[i]);
+    }
+    if (maxkey + 1 > (1LL << 30)) throw Error(CVM_E_INVALID, "neighbour grid too large for this rho");
+    gr.nkeys = static_cast<int>(maxkey + 1);
+    std::vector<int> cstart(gr.nkeys + 1, 0), cnodes(c.nn);
+    for (int i = 0; i < c.nn; ++i) cstart[key[i] + 1]++;
+    for (int k = 0; k < gr.nkeys; ++k) cstart[k + 1] += cstart[k];
+    {
+        std::vector<int> fillp(cstart.begin(), cstart.end() - 1);
+        for (int i = 0; i < c.nn; ++i) cnodes[fillp[key[i]]++] = i;   // ascending index per cell
+    }
+    DBuf<int> d_cstart, d_cnodes, d_count;
+    d_cstart.upload(cstart, c.stream);
+    d_cnodes.upload(cnodes, c.stream);
+    d_count.alloc(d.ng);
+    const double rho2 = rho * rho;
+    const dim3 grid((d.ng + 255) / 256), blk(256);
+    int slot = timing_begin(c, "frnn_count");
+    hipLaunchKernelGGL((k_frnn<false>), grid, blk, 0, c.stream, c.X.p, c.Y.p, d.gx.p, d.gy.p, d.ng,
+                       d_cstart.p, d_cnodes.p, gr, rho2, rho, d_count.p, nullptr, nullptr, nullptr);
+    CVM_HIP(hipGetLastError());
+    timing_end(c, slot);
+    // off = [0, inclusive_scan(count)]
+    hipLaunchKernelGGL(k_set_zero_int, dim3(1), dim3(1), 0, c.stream, d.off.p);
+    size_t tmp_bytes = 0;
+    CVM_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, d_count.p, d.off.p + 1, d.ng, c.stream));
+    DBuf<unsigned char> tmp;
+    tmp.alloc(std::max<size_t>(tmp_bytes, 1));
+    CVM_HIP(hipcub::DeviceScan::InclusiveSum(tmp.p, tmp_bytes, d_count.p, d.off.p + 1, d.ng, c.stream));
+    CVM_HIP(hipMemcpyAsync(d.h_off.data(), d.off.p, (d.ng + 1) * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    CVM_HIP(hipStreamSynchronize(c.stream));
+    d.npairs = d.h_off[d.ng];
+    d.ids.alloc(std::max<int64_t>(d.npairs, 1));
+    d.s.alloc(std::max<int64_t>(d.npairs, 1));
+    slot = timing_begin(c, "frnn_fill");
+    hipLaunchKernelGGL((k_frnn<true>), grid, blk, 0, c.stream, c.X.p, c.Y.p, d.gx.p, d.gy.p, d.ng,
+                       d_cstart.p, d_cnodes.p, gr, rho2, rho, nullptr, d.off.p, d.ids.p, d.s.p);
+    CVM_HIP(hipGetLastError());
+    timing_end(c, slot);
+    d.h_ids.resize(d.npairs);
+    if (d.npairs) CVM_HIP(hipMemcpyAsync(d.h_ids.data(), d.ids.p, d.npairs * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    CVM_HIP(hipStreamSynchronize(c.stream));
+    d.have_sf = false;
+}
+
+}  // namespace cvm

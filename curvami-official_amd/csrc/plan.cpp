@@ -1,0 +1,347 @@
+// Assembly plan: the block-upper sparsity pattern of K (face pairs united with
+// the elastic-fold pairs), the tiling of the face Gauss points, and the
+// precomputed slot maps that make the device scatter deterministic.
+//
+// Pattern semantics follow NonLinearAssembler::initSparsityPattern
+// (plate/Assembly.h:59-153): node block (lo, hi) with lo <= hi is present when
+// some face GP has both nodes as neighbours; column 3*hi + c of the CSC holds,
+// for each such lo in ascending order, the three rows 3*lo + r. The fold blocks
+// are the upper-filtered triplets of Constraints::evaluateElasticFold
+// (plate/Constraints.h:420-433) over every active crease GP, and
+// Meshless::assembleSystem adds them as K + K_fold, i.e. over the union
+// (plate/Meshless.h:118-126).
+//
+// Device scatter (no colliding atomics, run-to-run bit-identical):
+//  * tile t = a run of consecutive face GPs whose neighbour union has at most
+//    `union_max` nodes; every local pair of every GP of the tile is mapped to a
+//    "touched" slot of the tile (pairs of union nodes that some GP of the tile
+//    couples). The tile kernel accumulates its GPs in order into those slots in
+//    LDS and writes them to a scratch row;
+//  * for every K block, the list of scratch rows that hold a contribution,
+//    ascending in tile (hence GP) order, is the segment the reduction kernel
+//    sums, followed by the fold contributions in crease-GP order.
+#include <algorithm>
+#include <numeric>
+
+#include "cvm_internal.h"
+
+namespace cvm {
+
+namespace {
+
+Domain* face_domain(Context& c) {
+    Domain* f = nullptr;
+    for (Domain* d : c.doms)
+        if (d->kind == CVM_DOMAIN_FACE) {
+            if (f) throw Error(CVM_E_INVALID, "more than one face domain");
+            f = d;
+        }
+    if (!f) throw Error(CVM_E_STATE, "no face domain");
+    return f;
+}
+
+
+}  // namespace
+
+void build_plan(Context& c) {
+    Plan& P = c.plan;
+    P = Plan();   // release old buffers
+    Domain& fd = *face_domain(c);
+    if (fd.npairs < 0 || !fd.have_sf) throw Error(CVM_E_STATE, "face shape functions not computed");
+    const int nn = c.nn;
+    P.nn = nn;
+    P.tile_max_gps = c.tile_gps;
+    P.union_max = c.union_max;
+    const int ng = fd.ng;
+    const std::vector<int>& off = fd.h_off;
+    const std::vector<int>& ids = fd.h_ids;
+    int nmax = 0;
+    for (int g = 0; g < ng; ++g) nmax = std::max(nmax, off[g + 1] - off[g]);
+    if (nmax > 64) throw Error(CVM_E_INVALID, "a Gauss point has more than 64 neighbours");
+    if (P.union_max > 255) P.union_max = 255;
+    if (P.union_max < nmax) P.union_max = nmax;
+
+    // ---- tiles and touched pairs (tile size shrinks until the LDS fits) -----
+    std::vector<int> tile_gp, tile_uoff, unodes, tile_toff;
+    std::vector<uint8_t> uloc(ids.size());
+    std::vector<uint16_t> tpairs;
+    std::vector<int> stamp(nn, -1), where(nn, -1);
+    for (int tg = P.tile_max_gps;; tg = std::max(1, tg / 2)) {
+        tile_gp.assign(1, 0); tile_uoff.assign(1, 0); unodes.clear(); tile_toff.assign(1, 0); tpairs.clear();
+        std::fill(stamp.begin(), stamp.end(), -1);
+        int t = 0, g = 0;
+        std::vector<int> cur;
+        std::vector<uint8_t> mark;
+        int max_touched = 0, max_union = 0;
+        while (g < ng) {
+            cur.clear();
+            const int g0 = g;
+            while (g < ng && g - g0 < tg) {
+                int add = 0;
+                for (int k = off[g]; k < off[g + 1]; ++k)
+                    if (stamp[ids[k]] != t) { stamp[ids[k]] = t; ++add; cur.push_back(ids[k]); }
+                if (static_cast<int>(cur.size()) > P.union_max && g > g0) {
+                    for (int k = 0; k < add; ++k) { stamp[cur.back()] = -1; cur.pop_back(); }
+                    break;
+                }
+                ++g;
+            }
+            std::sort(cur.begin(), cur.end());
+            for (size_t k = 0; k < cur.size(); ++k) where[cur[k]] = static_cast<int>(k);
+            for (int q = g0; q < g; ++q)
+                for (int k = off[q]; k < off[q + 1]; ++k) uloc[k] = static_cast<uint8_t>(where[ids[k]]);
+            unodes.insert(unodes.end(), cur.begin(), cur.end());
+            tile_gp.push_back(g);
+            tile_uoff.push_back(static_cast<int>(unodes.size()));
+            // touched pairs of the tile, in (ua, ub) order
+            const int U = static_cast<int>(cur.size());
+            mark.assign(static_cast<size_t>(U) * U, 0);
+            for (int q = g0; q < g; ++q) {
+                const int b = off[q], n = off[q + 1] - b;
+                for (int i = 0; i < n; ++i)
+                    for (int j = i; j < n; ++j) {
+                        int a = uloc[b + i], bb = uloc[b + j];
+                        if (a > bb) std::swap(a, bb);
+                        mark[a * U + bb] = 1;
+                    }
+            }
+            const size_t t0 = tpairs.size();
+            for (int a = 0; a < U; ++a)
+                for (int bb = a; bb < U; ++bb)
+                    if (mark[a * U + bb]) tpairs.push_back(static_cast<uint16_t>((a << 8) | bb));
+            tile_toff.push_back(static_cast<int>(tpairs.size()));
+            max_touched = std::max(max_touched, static_cast<int>(tpairs.size() - t0));
+            max_union = std::max(max_union, U);
+            ++t;
+        }
+        P.ntiles = t;
+        P.max_touched = max_touched;
+        P.max_union = max_union;
+        P.lut_dim = std::max(max_union, 1);
+        P.tile_max_gps = tg;
+        if (tile_lds_bytes(max_touched, P.lut_dim) <= c.lds_limit) break;
+        if (tg == 1) throw Error(CVM_E_INVALID, "assembly tile of one Gauss point exceeds the LDS limit");
+    }
+    // face blocks from the touched pairs
+    std::vector<std::vector<int>> cols(nn);     // face + fold rows per column
+    for (int t = 0; t < P.ntiles; ++t) {
+        const int* un = unodes.data() + tile_uoff[t];
+        for (int s2 = tile_toff[t]; s2 < tile_toff[t + 1]; ++s2) {
+            const int a = tpairs[s2] >> 8, bb = tpairs[s2] & 0xff;
+            cols[un[bb]].push_back(un[a]);
+        }
+    }
+    P.nscratch = static_cast<int64_t>(tpairs.size());
+    P.nfscratch = static_cast<int64_t>(unodes.size());
+
+    // face pattern (per column: sorted unique rows)
+    std::vector<std::vector<int>> face_rows(nn);
+    for (int j = 0; j < nn; ++j) {
+        auto& v = cols[j];
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        face_rows[j] = v;
+    }
+    P.face_rows = face_rows;
+
+    // ---- fold crease GPs -----------------------------------------------------
+    struct FGP { int fold, local, left_b, left_n, right_b, right_n; std::vector<int> nodes; };
+    std::vector<FGP> fgps;
+    for (size_t f = 0; f < c.folds.size(); ++f) {
+        const Fold& fo = c.folds[f];
+        Domain& L = *c.doms[fo.left];
+        Domain& R = *c.doms[fo.right];
+        if (L.npairs < 0 || R.npairs < 0 || !L.have_sf || !R.have_sf)
+            throw Error(CVM_E_STATE, "fold domains need shape functions before the assembly plan");
+        for (int i = 0; i < L.ng; ++i) {
+            if (!fo.active.empty() && !fo.active[i]) continue;
+            FGP q;
+            q.fold = static_cast<int>(f); q.local = i;
+            q.left_b = L.h_off[i]; q.left_n = L.h_off[i + 1] - L.h_off[i];
+            q.right_b = R.h_off[i]; q.right_n = R.h_off[i + 1] - R.h_off[i];
+            for (int k = 0; k < q.left_n; ++k) q.nodes.push_back(L.h_ids[q.left_b + k]);
+            for (int k = 0; k < q.right_n; ++k) q.nodes.push_back(R.h_ids[q.right_b + k]);
+            std::sort(q.nodes.begin(), q.nodes.end());
+            q.nodes.erase(std::unique(q.nodes.begin(), q.nodes.end()), q.nodes.end());
+            for (size_t a = 0; a < q.nodes.size(); ++a)
+                for (size_t bb = a; bb < q.nodes.size(); ++bb) cols[q.nodes[bb]].push_back(q.nodes[a]);
+            fgps.push_back(std::move(q));
+        }
+    }
+    P.nfgp = static_cast<int>(fgps.size());
+
+    // union pattern
+    P.col_ptr.assign(nn + 1, 0);
+    for (int j = 0; j < nn; ++j) {
+        auto& v = cols[j];
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        P.col_ptr[j + 1] = P.col_ptr[j] + static_cast<int>(v.size());
+    }
+    P.nblocks = P.col_ptr[nn];
+    P.col_rows.resize(P.nblocks);
+    P.blk_face.assign(P.nblocks, 0);
+    std::vector<int64_t> base_u(P.nblocks), base_f(P.nblocks, -1);
+    std::vector<int> stride_u(P.nblocks), stride_f(P.nblocks, 0);
+    P.face_outer.assign(3 * nn + 1, 0);
+    P.uni_outer.assign(3 * nn + 1, 0);
+    {
+        int64_t fo = 0, uo = 0;
+        for (int j = 0; j < nn; ++j) {
+            const auto& ur = cols[j];
+            const auto& fr = face_rows[j];
+            size_t fpos = 0;
+            for (size_t k = 0; k < ur.size(); ++k) {
+                const int64_t blk = P.col_ptr[j] + static_cast<int64_t>(k);
+                P.col_rows[blk] = ur[k];
+                base_u[blk] = uo + 3 * static_cast<int64_t>(k);
+                stride_u[blk] = 3 * static_cast<int>(ur.size());
+                while (fpos < fr.size() && fr[fpos] < ur[k]) ++fpos;
+                if (fpos < fr.size() && fr[fpos] == ur[k]) {
+                    P.blk_face[blk] = 1;
+                    base_f[blk] = fo + 3 * static_cast<int64_t>(fpos);
+                    stride_f[blk] = 3 * static_cast<int>(fr.size());
+                }
+            }
+            for (int cc = 0; cc < 3; ++cc) {
+                for (int r : fr)
+                    for (int rr = 0; rr < 3; ++rr) P.face_inner.push_back(3 * r + rr);
+                fo += 3 * static_cast<int64_t>(fr.size());
+                P.face_outer[3 * j + cc + 1] = static_cast<int>(fo);
+                for (int r : ur)
+                    for (int rr = 0; rr < 3; ++rr) P.uni_inner.push_back(3 * r + rr);
+                uo += 3 * static_cast<int64_t>(ur.size());
+                P.uni_outer[3 * j + cc + 1] = static_cast<int>(uo);
+            }
+        }
+        if (uo > 0x7fffffffLL) throw Error(CVM_E_INVALID, "K has more than 2^31 stored values (int32 CSC)");
+    }
+    auto block_of = [&](int lo, int hi) -> int64_t {
+        const int* b = P.col_rows.data() + P.col_ptr[hi];
+        const int* e = P.col_rows.data() + P.col_ptr[hi + 1];
+        const int* it = std::lower_bound(b, e, lo);
+        if (it == e || *it != lo) return -1;
+        return static_cast<int64_t>(it - P.col_rows.data());
+    };
+
+    // ---- K contributors: face tiles then folds ---------------------------------
+    std::vector<int64_t> kc_cnt(P.nblocks + 1, 0);
+    std::vector<int64_t> tblk(tpairs.size());
+    for (int t = 0; t < P.ntiles; ++t) {
+        const int* un = unodes.data() + tile_uoff[t];
+        for (int s = tile_toff[t]; s < tile_toff[t + 1]; ++s) {
+            const int a = tpairs[s] >> 8, bb = tpairs[s] & 0xff;
+            const int64_t blk = block_of(un[a], un[bb]);
+            if (blk < 0) throw Error(CVM_E_PATTERN_MISS,
+                "Assembly: Gauss-point neighbour missing from the sparsity pattern; CSR mapping is inconsistent.");
+            tblk[s] = blk;
+            kc_cnt[blk + 1]++;
+        }
+    }
+    for (int64_t b = 0; b < P.nblocks; ++b) kc_cnt[b + 1] += kc_cnt[b];
+    std::vector<int> kc_idx(kc_cnt[P.nblocks]);
+    {
+        std::vector<int64_t> fillp(kc_cnt.begin(), kc_cnt.end() - 1);
+        for (size_t s = 0; s < tpairs.size(); ++s) kc_idx[fillp[tblk[s]]++] = static_cast<int>(s);
+    }
+    if (P.nscratch > 0x7fffffffLL) throw Error(CVM_E_INVALID, "assembly scratch exceeds 2^31 rows");
+
+    // F contributors per node
+    std::vector<int> fc_cnt(nn + 1, 0);
+    for (int v : unodes) fc_cnt[v + 1]++;
+    for (int i = 0; i < nn; ++i) fc_cnt[i + 1] += fc_cnt[i];
+    std::vector<int> fc_idx(fc_cnt[nn]);
+    {
+        std::vector<int> fillp(fc_cnt.begin(), fc_cnt.end() - 1);
+        for (size_t k = 0; k < unodes.size(); ++k) fc_idx[fillp[unodes[k]]++] = static_cast<int>(k);
+    }
+
+    // fold contributors
+    std::vector<int> fgp_fold, fgp_local, fgp_noff{0}, fgp_nodes, fgp_lpos, fgp_rpos;
+    std::vector<int64_t> kf_cnt(P.nblocks + 1, 0);
+    std::vector<int> ff_cnt(nn + 1, 0);
+    for (const FGP& q : fgps) {
+        fgp_fold.push_back(q.fold); fgp_local.push_back(q.local);
+        const int base = fgp_noff.back();
+        for (int v : q.nodes) { fgp_nodes.push_back(v); ff_cnt[v + 1]++; }
+        fgp_noff.push_back(base + static_cast<int>(q.nodes.size()));
+        const Fold& fo = c.folds[q.fold];
+        Domain& L = *c.doms[fo.left];
+        Domain& R = *c.doms[fo.right];
+        for (int k = 0; k < q.left_n; ++k)
+            fgp_lpos.push_back(static_cast<int>(std::lower_bound(q.nodes.begin(), q.nodes.end(), L.h_ids[q.left_b + k]) - q.nodes.begin()));
+        for (int k = 0; k < q.right_n; ++k)
+            fgp_rpos.push_back(static_cast<int>(std::lower_bound(q.nodes.begin(), q.nodes.end(), R.h_ids[q.right_b + k]) - q.nodes.begin()));
+        for (size_t a = 0; a < q.nodes.size(); ++a)
+            for (size_t bb = a; bb < q.nodes.size(); ++bb) kf_cnt[block_of(q.nodes[a], q.nodes[bb]) + 1]++;
+    }
+    for (int64_t b = 0; b < P.nblocks; ++b) kf_cnt[b + 1] += kf_cnt[b];
+    std::vector<int> kf_idx(2 * kf_cnt[P.nblocks]);   // pairs: (global nodeset index lo, hi)
+    {
+        std::vector<int64_t> fillp(kf_cnt.begin(), kf_cnt.end() - 1);
+        for (size_t qi = 0; qi < fgps.size(); ++qi) {
+            const FGP& q = fgps[qi];
+            const int base = fgp_noff[qi];
+            for (size_t a = 0; a < q.nodes.size(); ++a)
+                for (size_t bb = a; bb < q.nodes.size(); ++bb) {
+                    const int64_t blk = block_of(q.nodes[a], q.nodes[bb]);
+                    const int64_t at = fillp[blk]++;
+                    kf_idx[2 * at] = base + static_cast<int>(a);
+                    kf_idx[2 * at + 1] = base + static_cast<int>(bb);
+                }
+        }
+    }
+    for (int i = 0; i < nn; ++i) ff_cnt[i + 1] += ff_cnt[i];
+    std::vector<int> ff_idx(ff_cnt[nn]);
+    {
+        std::vector<int> fillp(ff_cnt.begin(), ff_cnt.end() - 1);
+        for (size_t k = 0; k < fgp_nodes.size(); ++k) ff_idx[fillp[fgp_nodes[k]]++] = static_cast<int>(k);
+    }
+    // nodeset entry -> crease GP
+    std::vector<int> nd_gp(fgp_nodes.size());
+    for (size_t qi = 0; qi < fgps.size(); ++qi)
+        for (int k = fgp_noff[qi]; k < fgp_noff[qi + 1]; ++k) nd_gp[k] = static_cast<int>(qi);
+
+    // ---- upload -----------------------------------------------------------------
+    hipStream_t s = c.stream;
+    P.d_blk_base_u.upload(base_u, s);
+    P.d_blk_base_f.upload(base_f, s);
+    P.d_blk_stride_u.upload(stride_u, s);
+    P.d_blk_stride_f.upload(stride_f, s);
+    P.d_tile_gp.upload(tile_gp, s);
+    P.d_tile_uoff.upload(tile_uoff, s);
+    P.d_tile_unodes.upload(unodes, s);
+    P.d_tile_toff.upload(tile_toff, s);
+    P.d_tile_tpairs.upload(tpairs, s);
+    P.d_pair_uloc.upload(uloc, s);
+    P.d_kc_off.upload(kc_cnt, s);
+    P.d_kc_idx.upload(kc_idx, s);
+    P.d_fc_off.upload(fc_cnt, s);
+    P.d_fc_idx.upload(fc_idx, s);
+    if (P.nfgp) {
+        P.d_fgp_fold.upload(fgp_fold, s);
+        P.d_fgp_local.upload(fgp_local, s);
+        P.d_fgp_noff.upload(fgp_noff, s);
+        P.d_fgp_nodes.upload(fgp_nodes, s);
+        P.d_fgp_lpos.upload(fgp_lpos, s);
+        P.d_fgp_rpos.upload(fgp_rpos, s);
+        P.d_fnd_gp.upload(nd_gp, s);
+    }
+    P.d_kf_off.upload(kf_cnt, s);
+    P.d_kf_idx.upload(kf_idx.empty() ? std::vector<int>{0} : kf_idx, s);
+    P.d_ff_off.upload(ff_cnt, s);
+    P.d_ff_idx.upload(ff_idx.empty() ? std::vector<int>{0} : ff_idx, s);
+    if (!fgps.empty()) {
+        std::vector<int> lpo{0}, rpo{0};
+        for (const FGP& q : fgps) { lpo.push_back(lpo.back() + q.left_n); rpo.push_back(rpo.back() + q.right_n); }
+        P.d_fgp_lpo.upload(lpo, s);
+        P.d_fgp_rpo.upload(rpo, s);
+    }
+    P.nfnodes = static_cast<int64_t>(fgp_nodes.size());
+    P.face_ng = ng;
+    P.face_npairs = fd.npairs;
+    CVM_HIP(hipStreamSynchronize(s));
+    P.ready = true;
+}
+
+}  // namespace cvm

@@ -1,0 +1,355 @@
+// RKPM shape functions up to second derivatives on gfx950 (FP64).
+//
+// Reference algorithm: core/include/curvami/GlobalRKPM.h:19-397. Same
+// quantities, reorganised for the device:
+//  * one Gauss point per lane; the neighbour loop accumulates the six
+//    window-weighted monomial moments mu_j[a,b] = sum_I phi_I^(j) xi^a eta^b
+//    (j = phi, phi_x, phi_y, phi_xx, phi_yy, phi_xy; a + b <= 2p) in
+//    registers. Every entry of the reference's moment matrices M, M_x, ...,
+//    M_xy (GlobalRKPM.h:161-189) is a fixed combination of these
+//    (product rule on x^a y^b), so the 6 x 21 packed entries (order 2) cost
+//    90 accumulators and ~110 flop per pair instead of ~1.6 kflop;
+//  * instead of the explicit 3x3 inverse + Schur bordering of the full
+//    inverse and its derivatives (:203-345), M = L D L^T is factored once and
+//    only the first column c = M^-1 e0 and its derivatives are solved:
+//    c_x = -M^-1 M_x c, c_xx = -M^-1 (M_xx c + 2 M_x c_x), ... (row 0 of M^-1
+//    is all the final assembly :363-397 reads). The LDL^T pivots D_k (k >= 3)
+//    are exactly the reference's Schur complements q, so its pivot-collapse
+//    test (:303-305) is applied to them; the 3x3 leading block is inverted by
+//    its adjugate and checked with the reference's residual test (:214-220).
+//  * the crease visibility (:86-112) uses per-GP indicator values and per-node
+//    side signs computed once (the reference recomputes both per pair).
+#include <algorithm>
+
+#include "cvm_internal.h"
+
+namespace cvm {
+
+template <int P>
+struct RK {
+    static constexpr int NP = (P + 1) * (P + 2) / 2;
+    static constexpr int DM = 2 * P;                              // max monomial degree
+    static constexpr int NMON = (DM + 1) * (DM + 2) / 2;
+};
+
+// basis term k: degree ascending, then p_y ascending (GlobalRKPM.h:137-150)
+__host__ __device__ constexpr int deg_of(int k) {
+    int d = 0;
+    while ((d + 1) * (d + 2) / 2 <= k) ++d;
+    return d;
+}
+__host__ __device__ constexpr int bex(int k) { return deg_of(k) - (k - deg_of(k) * (deg_of(k) + 1) / 2); }
+__host__ __device__ constexpr int bey(int k) { return k - deg_of(k) * (deg_of(k) + 1) / 2; }
+__host__ __device__ constexpr int midx(int a, int b) { return (a + b) * (a + b + 1) / 2 + b; }
+
+template <int KER>
+__device__ __forceinline__ void window(double s, double& w, double& dw, double& d2w) {
+    if (KER == CVM_KERNEL_CUBIC_BSPLINE) {
+        if (s >= 1.0) { w = 0.0; dw = 0.0; d2w = 0.0; return; }
+        if (s <= 0.5) {
+            w = 2.0 / 3.0 - 4.0 * s * s + 4.0 * s * s * s;
+            dw = (s < 1e-14) ? 0.0 : -8.0 * s + 12.0 * s * s;
+            d2w = -8.0 + 24.0 * s;
+        } else {
+            w = 4.0 / 3.0 - 4.0 * s + 4.0 * s * s - 4.0 / 3.0 * s * s * s;
+            dw = -4.0 + 8.0 * s - 4.0 * s * s;
+            d2w = 8.0 - 8.0 * s;
+        }
+        return;
+    }
+    // quartic spline, GlobalRKPM.h:15-17
+    if (s >= 1.0) { w = 0.0; dw = 0.0; d2w = 0.0; return; }
+    const double s2 = s * s;
+    w = 1.0 - 6.0 * s2 + 8.0 * s2 * s - 3.0 * s2 * s2;
+    dw = (s < 1e-14) ? 0.0 : -12.0 * s + 24.0 * s * s - 12.0 * s * s * s;
+    d2w = -12.0 + 48.0 * s - 36.0 * s * s;
+}
+
+struct Jet { double v, x, y, xx, yy, xy; };
+
+// Window jet of one pair with crease visibility (GlobalRKPM.h:59-112).
+template <int KER>
+__device__ __forceinline__ Jet pair_jet(double xi, double eta, double si, int I, int g, int ng, int nn,
+                                        const double* __restrict__ ind, const int8_t* __restrict__ nsign,
+                                        int nc, double rho) {
+    double w, d, dd;
+    window<KER>(si, w, d, dd);
+    const double dx = (si == 0.0) ? 0.0 : xi / si;
+    const double dy = (si == 0.0) ? 0.0 : eta / si;
+    const double si3 = si * si * si;
+    // xi^2/s^3 and eta^2/s^3 as in GlobalRKPM.h:79-80
+    const double d2x = (si == 0.0) ? dd : (xi * xi) / si3;
+    const double d2y = (si == 0.0) ? dd : (eta * eta) / si3;
+    Jet j;
+    j.v = w; j.x = d * dx; j.y = d * dy;
+    j.xx = dd * dx * dx + d * d2x;
+    j.yy = dd * dy * dy + d * d2y;
+    j.xy = (si == 0.0) ? 0.0 : dd * dx * dy + d * (-(xi * eta) / si3);
+    for (int c = 0; c < nc; ++c) {
+        const double sn = static_cast<double>(nsign[static_cast<size_t>(c) * nn + I]);
+        const double* ig = ind + static_cast<size_t>(c) * 6 * ng + g;
+        const double f = ig[0];
+        const double W = 0.5 * (1.0 + sn * f);
+        const double hs = 0.5 * sn;
+        const double dWx = hs * ig[ng] * rho, dWy = hs * ig[2 * ng] * rho;
+        const double ddWxx = hs * ig[3 * ng] * rho * rho;
+        const double ddWyy = hs * ig[4 * ng] * rho * rho;
+        const double ddWxy = hs * ig[5 * ng] * rho * rho;
+        const Jet o = j;
+        j.v = o.v * W;
+        j.x = o.x * W + o.v * dWx;
+        j.y = o.y * W + o.v * dWy;
+        j.xx = o.xx * W + 2.0 * o.x * dWx + o.v * ddWxx;
+        j.yy = o.yy * W + 2.0 * o.y * dWy + o.v * ddWyy;
+        j.xy = o.xy * W + o.x * dWy + o.y * dWx + o.v * ddWxy;
+    }
+    return j;
+}
+
+template <int P, int KER>
+__global__ __launch_bounds__(128) void k_rkpm(
+    const double* __restrict__ X, const double* __restrict__ Y, int nn,
+    const double* __restrict__ gx, const double* __restrict__ gy, int ng,
+    const int* __restrict__ off, const int* __restrict__ ids, const double* __restrict__ sarr,
+    const double* __restrict__ ind, const int8_t* __restrict__ nsign, int nc, double rho,
+    double* __restrict__ sf, int64_t npairs, int* __restrict__ status /* n1, first1, n2, first2 */) {
+    using R = RK<P>;
+    constexpr int NP = R::NP, NMON = R::NMON, DM = R::DM;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    const double px = gx[g], py = gy[g];
+    const int b = off[g], e = off[g + 1];
+
+    // ---- moments of monomials -------------------------------------------
+    double mu[6][NMON];
+#pragma unroll
+    for (int j = 0; j < 6; ++j)
+#pragma unroll
+        for (int m = 0; m < NMON; ++m) mu[j][m] = 0.0;
+    for (int k = b; k < e; ++k) {
+        const int I = ids[k];
+        const double xi = (px - X[I]) / rho;
+        const double eta = (py - Y[I]) / rho;
+        const Jet jt = pair_jet<KER>(xi, eta, sarr[k], I, g, ng, nn, ind, nsign, nc, rho);
+        double xp[DM + 1], yp[DM + 1];
+        xp[0] = 1.0; yp[0] = 1.0;
+#pragma unroll
+        for (int q = 1; q <= DM; ++q) { xp[q] = xp[q - 1] * xi; yp[q] = yp[q - 1] * eta; }
+#pragma unroll
+        for (int d = 0; d <= DM; ++d)
+#pragma unroll
+            for (int bb = 0; bb <= d; ++bb) {
+                const int m = midx(d - bb, bb);
+                const double mon = xp[d - bb] * yp[bb];
+                mu[0][m] += jt.v * mon;
+                mu[1][m] += jt.x * mon;
+                mu[2][m] += jt.y * mon;
+                mu[3][m] += jt.xx * mon;
+                mu[4][m] += jt.yy * mon;
+                mu[5][m] += jt.xy * mon;
+            }
+    }
+    auto MU = [&](int j, int a, int bb) -> double { return (a < 0 || bb < 0) ? 0.0 : mu[j][midx(a, bb)]; };
+    // Entries of M and its derivatives from the monomial moments.
+    auto Mv = [&](int i, int j) { return MU(0, bex(i) + bex(j), bey(i) + bey(j)); };
+    auto Mxv = [&](int i, int j) {
+        const int a = bex(i) + bex(j), c = bey(i) + bey(j);
+        return MU(1, a, c) + a * MU(0, a - 1, c);
+    };
+    auto Myv = [&](int i, int j) {
+        const int a = bex(i) + bex(j), c = bey(i) + bey(j);
+        return MU(2, a, c) + c * MU(0, a, c - 1);
+    };
+    auto Mxxv = [&](int i, int j) {
+        const int a = bex(i) + bex(j), c = bey(i) + bey(j);
+        return MU(3, a, c) + 2 * a * MU(1, a - 1, c) + a * (a - 1) * MU(0, a - 2, c);
+    };
+    auto Myyv = [&](int i, int j) {
+        const int a = bex(i) + bex(j), c = bey(i) + bey(j);
+        return MU(4, a, c) + 2 * c * MU(2, a, c - 1) + c * (c - 1) * MU(0, a, c - 2);
+    };
+    auto Mxyv = [&](int i, int j) {
+        const int a = bex(i) + bex(j), c = bey(i) + bey(j);
+        return MU(5, a, c) + a * MU(2, a - 1, c) + c * MU(1, a, c - 1) + a * c * MU(0, a - 1, c - 1);
+    };
+
+    // ---- singularity tests (GlobalRKPM.h:214-220 and :303-305) ----------
+    int flag = 0;
+    {
+        double m[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) m[r][c] = Mv(r, c);
+        auto cof = [&](int i, int j) {
+            const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            return m[i1][j1] * m[i2][j2] - m[i1][j2] * m[i2][j1];
+        };
+        const double c0 = cof(0, 0), c1 = cof(1, 0), c2 = cof(2, 0);
+        const double det = c0 * m[0][0] + c1 * m[1][0] + c2 * m[2][0];
+        const double id = 1.0 / det;
+        double inv[3][3];
+        inv[0][0] = c0 * id; inv[0][1] = c1 * id; inv[0][2] = c2 * id;
+        inv[1][0] = cof(0, 1) * id; inv[1][1] = cof(1, 1) * id; inv[1][2] = cof(2, 1) * id;
+        inv[2][0] = cof(0, 2) * id; inv[2][1] = cof(1, 2) * id; inv[2][2] = cof(2, 2) * id;
+        bool finite = true;
+        double rmax = 0.0;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                finite = finite && isfinite(inv[r][c]);
+                const double pr = m[r][0] * inv[0][c] + m[r][1] * inv[1][c] + m[r][2] * inv[2][c];
+                rmax = fmax(rmax, fabs(pr - (r == c ? 1.0 : 0.0)));
+            }
+        if (!finite || !(rmax <= 1e-8)) flag |= 1;
+    }
+
+    // ---- L D L^T of M ------------------------------------------------------
+    double L[NP][NP], Dg[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        double dk = Mv(k, k);
+#pragma unroll
+        for (int j = 0; j < k; ++j) dk -= L[k][j] * L[k][j] * Dg[j];
+        Dg[k] = dk;
+        if (k >= 3 && !(fabs(dk) > 1e-12 * fmax(fabs(Mv(k, k)), 1.0))) flag |= 2;
+        const double idk = 1.0 / dk;
+#pragma unroll
+        for (int i = k + 1; i < NP; ++i) {
+            double v = Mv(i, k);
+#pragma unroll
+            for (int j = 0; j < k; ++j) v -= L[i][j] * L[k][j] * Dg[j];
+            L[i][k] = v * idk;
+        }
+    }
+    auto solve = [&](double r[NP]) {
+#pragma unroll
+        for (int i = 0; i < NP; ++i)
+#pragma unroll
+            for (int j = 0; j < i; ++j) r[i] -= L[i][j] * r[j];
+#pragma unroll
+        for (int i = 0; i < NP; ++i) r[i] /= Dg[i];
+#pragma unroll
+        for (int i = NP - 1; i >= 0; --i)
+#pragma unroll
+            for (int j = i + 1; j < NP; ++j) r[i] -= L[j][i] * r[j];
+    };
+    double c[NP], cx[NP], cy[NP], cxx[NP], cyy[NP], cxy[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) c[i] = (i == 0) ? 1.0 : 0.0;
+    solve(c);
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+        double sx = 0.0, sy = 0.0;
+#pragma unroll
+        for (int j = 0; j < NP; ++j) { sx += Mxv(i, j) * c[j]; sy += Myv(i, j) * c[j]; }
+        cx[i] = -sx; cy[i] = -sy;
+    }
+    solve(cx);
+    solve(cy);
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+        double sxx = 0.0, syy = 0.0, sxy = 0.0;
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+            sxx += Mxxv(i, j) * c[j] + 2.0 * Mxv(i, j) * cx[j];
+            syy += Myyv(i, j) * c[j] + 2.0 * Myv(i, j) * cy[j];
+            sxy += Mxyv(i, j) * c[j] + Mxv(i, j) * cy[j] + Myv(i, j) * cx[j];
+        }
+        cxx[i] = -sxx; cyy[i] = -syy; cxy[i] = -sxy;
+    }
+    solve(cxx);
+    solve(cyy);
+    solve(cxy);
+
+    if (flag) {
+        if (flag & 1) { atomicAdd(&status[0], 1); atomicMin(&status[1], g); }
+        if (flag & 2) { atomicAdd(&status[2], 1); atomicMin(&status[3], g); }
+    }
+
+    // ---- Psi, grad Psi, Hessian of Psi (GlobalRKPM.h:363-397) ------------
+    for (int k = b; k < e; ++k) {
+        const int I = ids[k];
+        const double xi = (px - X[I]) / rho;
+        const double eta = (py - Y[I]) / rho;
+        const Jet jt = pair_jet<KER>(xi, eta, sarr[k], I, g, ng, nn, ind, nsign, nc, rho);
+        double xp[P + 1], yp[P + 1];
+        xp[0] = 1.0; yp[0] = 1.0;
+#pragma unroll
+        for (int q = 1; q <= P; ++q) { xp[q] = xp[q - 1] * xi; yp[q] = yp[q - 1] * eta; }
+        double C = 0, Cx = 0, Cy = 0, Cxx = 0, Cyy = 0, Cxy = 0;
+#pragma unroll
+        for (int t = 0; t < NP; ++t) {
+            const int ax = bex(t), ay = bey(t);
+            const double p = xp[ax] * yp[ay];
+            const double pxv = ax > 0 ? ax * xp[ax > 0 ? ax - 1 : 0] * yp[ay] : 0.0;
+            const double pyv = ay > 0 ? ay * xp[ax] * yp[ay > 0 ? ay - 1 : 0] : 0.0;
+            const double pxx = ax > 1 ? ax * (ax - 1) * xp[ax > 1 ? ax - 2 : 0] * yp[ay] : 0.0;
+            const double pyy = ay > 1 ? ay * (ay - 1) * xp[ax] * yp[ay > 1 ? ay - 2 : 0] : 0.0;
+            const double pxy = (ax > 0 && ay > 0) ? ax * ay * xp[ax > 0 ? ax - 1 : 0] * yp[ay > 0 ? ay - 1 : 0] : 0.0;
+            C += c[t] * p;
+            Cx += cx[t] * p + c[t] * pxv;
+            Cy += cy[t] * p + c[t] * pyv;
+            Cxx += cxx[t] * p + 2.0 * cx[t] * pxv + c[t] * pxx;
+            Cyy += cyy[t] * p + 2.0 * cy[t] * pyv + c[t] * pyy;
+            Cxy += cxy[t] * p + cx[t] * pyv + cy[t] * pxv + c[t] * pxy;
+        }
+        const double r2 = rho * rho;
+        sf[k] = C * jt.v;
+        sf[npairs + k] = (Cx * jt.v + C * jt.x) / rho;
+        sf[2 * npairs + k] = (Cy * jt.v + C * jt.y) / rho;
+        sf[3 * npairs + k] = (Cxx * jt.v + 2.0 * Cx * jt.x + C * jt.xx) / r2;
+        sf[4 * npairs + k] = (Cyy * jt.v + 2.0 * Cy * jt.y + C * jt.yy) / r2;
+        sf[5 * npairs + k] = (Cxy * jt.v + Cx * jt.y + Cy * jt.x + C * jt.xy) / r2;
+    }
+}
+
+__global__ void k_status_init(int* s) {
+    s[0] = 0; s[1] = 0x7fffffff; s[2] = 0; s[3] = 0x7fffffff;
+}
+
+template <int P, int KER>
+static void run(Context& c, Domain& d, double rho, int* st) {
+    const int nc = static_cast<int>(c.creases.size());
+    hipLaunchKernelGGL((k_rkpm<P, KER>), dim3((d.ng + 127) / 128), dim3(128), 0, c.stream, c.X.p, c.Y.p, c.nn,
+                       d.gx.p, d.gy.p, d.ng, d.off.p, d.ids.p, d.s.p, d.ind.p, c.d_node_sign.p, nc, rho,
+                       d.sf.p, static_cast<int64_t>(d.npairs), st);
+}
+
+void launch_rkpm(Context& c, Domain& d, double rho, int order, int kernel, int* code, int* nbad, int* first) {
+    if (d.npairs < 0) throw Error(CVM_E_STATE, "neighbours not computed for this domain");
+    if (order < 1 || order > 3) throw Error(CVM_E_INVALID, "order must be 1, 2 or 3");
+    if (kernel != CVM_KERNEL_QUARTIC && kernel != CVM_KERNEL_CUBIC_BSPLINE)
+        throw Error(CVM_E_INVALID, "unknown window kernel");
+    launch_node_signs(c);
+    launch_indicator_gp(c, d);
+    d.sf.alloc(std::max<int64_t>(6 * d.npairs, 1));
+    DBuf<int> st;
+    st.alloc(4);
+    hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, c.stream, st.p);
+    *code = CVM_OK; *nbad = 0; *first = 0;
+    if (d.ng > 0) {
+        const int slot = timing_begin(c, "rkpm");
+        if (kernel == CVM_KERNEL_QUARTIC) {
+            if (order == 1) run<1, CVM_KERNEL_QUARTIC>(c, d, rho, st.p);
+            else if (order == 2) run<2, CVM_KERNEL_QUARTIC>(c, d, rho, st.p);
+            else run<3, CVM_KERNEL_QUARTIC>(c, d, rho, st.p);
+        } else {
+            if (order == 1) run<1, CVM_KERNEL_CUBIC_BSPLINE>(c, d, rho, st.p);
+            else if (order == 2) run<2, CVM_KERNEL_CUBIC_BSPLINE>(c, d, rho, st.p);
+            else run<3, CVM_KERNEL_CUBIC_BSPLINE>(c, d, rho, st.p);
+        }
+        CVM_HIP(hipGetLastError());
+        timing_end(c, slot);
+    }
+    int hs[4];
+    CVM_HIP(hipMemcpyAsync(hs, st.p, sizeof(hs), hipMemcpyDeviceToHost, c.stream));
+    CVM_HIP(hipStreamSynchronize(c.stream));
+    d.order = order; d.kernel = kernel;
+    if (hs[0] > 0) { *code = CVM_E_SINGULAR_MOMENT; *nbad = hs[0]; *first = hs[1]; }
+    else if (hs[2] > 0) { *code = CVM_E_PIVOT_COLLAPSE; *nbad = hs[2]; *first = hs[3]; }
+    d.have_sf = (*code == CVM_OK);
+}
+
+}  // namespace cvm

@@ -1,0 +1,264 @@
+"""ctypes binding of libcurvami_gpu.so (the C ABI declared in include/curvami_gpu.h).
+
+`Plate` mirrors the reference's Meshless + NonLinearAssembler surface
+(plate/Meshless.h:11-127, plate/Assembly.h:18-54): domains, folds, neighbour
+search, shape functions, pattern, assembleSystem. Errors raise RuntimeError
+with the library's message (the reference throws std::runtime_error with the
+same text, e.g. core/include/curvami/GlobalRKPM.h:246-255).
+
+There is no CPU fallback: if the library or a GPU is missing, loading fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcurvami_gpu.so")
+
+_d = C.POINTER(C.c_double)
+_i = C.POINTER(C.c_int)
+_i64 = C.POINTER(C.c_int64)
+
+# status codes (cvm_status)
+OK, E_INVALID, E_HIP, E_SINGULAR_MOMENT, E_PIVOT_COLLAPSE, E_PATTERN_MISS, E_NO_MEMORY, E_STATE, E_BUDGET = range(9)
+FACE, EDGE, POINT, FOLD_LEFT, FOLD_RIGHT = range(5)
+RESIDUAL, TANGENT, ELASTIC_FOLDS, DEVICE_PTRS = 1, 2, 4, 8
+
+# every exported symbol of include/curvami_gpu.h with its signature
+SIGNATURES = {
+    "cvm_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "cvm_destroy": (None, [C.c_void_p]),
+    "cvm_last_error": (C.c_char_p, [C.c_void_p]),
+    "cvm_set_nodes": (C.c_int, [C.c_void_p, C.c_int, _d, _d]),
+    "cvm_set_laminate": (C.c_int, [C.c_void_p, C.c_int, _d]),
+    "cvm_get_abd": (C.c_int, [C.c_void_p, _d, _d, _d]),
+    "cvm_add_crease": (C.c_int, [C.c_void_p, C.c_int, _d, _d, C.c_int, C.c_int, C.c_int]),
+    "cvm_add_domain": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _d, _d, _d, _d, _d, _d, _d, _i]),
+    "cvm_add_fold": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, _i]),
+    "cvm_set_fold_gp_state": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _d, C.c_char_p]),
+    "cvm_neighbours": (C.c_int, [C.c_void_p, C.c_int, C.c_double, _i64]),
+    "cvm_get_neighbours": (C.c_int, [C.c_void_p, C.c_int, _i, _i, _d]),
+    "cvm_shape_functions": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _i, _i]),
+    "cvm_get_shape_functions": (C.c_int, [C.c_void_p, C.c_int, _d, _d, _d, _d, _d, _d]),
+    "cvm_init_assembly": (C.c_int, [C.c_void_p, _i64, _i64]),
+    "cvm_get_pattern": (C.c_int, [C.c_void_p, C.c_int, _i, _i]),
+    "cvm_csr_mapping": (C.c_int, [C.c_void_p, C.c_int, _i, C.c_int, _i]),
+    "cvm_assemble": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "cvm_synchronize": (C.c_int, [C.c_void_p]),
+    "cvm_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "cvm_kernel_count": (C.c_int, [C.c_void_p]),
+    "cvm_kernel_stats": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int, _d, _i64]),
+    "cvm_reset_profiling": (C.c_int, [C.c_void_p]),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build() "
+                           "(there is no CPU fallback for the hot path)")
+    L = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+class CvmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(_d)
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Plate:
+    """Device-resident creased plate: nodes, laminate, creases, domains, folds."""
+
+    def __init__(self, device: int = 0):
+        self._L = load_library()
+        h = C.c_void_p()
+        st = self._L.cvm_create(device, C.byref(h))
+        if st != OK:
+            raise CvmError(st, f"cvm_create failed with status {st} (no HIP device?)")
+        self._h = h
+        self.nn = 0
+        self.domain_sizes = []
+        self.npairs = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.cvm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != OK:
+            raise CvmError(st, self._L.cvm_last_error(self._h).decode(errors="replace"))
+
+    # ---- setup ----------------------------------------------------------------
+    def set_nodes(self, X, Y):
+        X, Y = _f64(X), _f64(Y)
+        self.nn = X.size
+        self._check(self._L.cvm_set_nodes(self._h, X.size, _dp(X), _dp(Y)))
+
+    def set_plies(self, plies):
+        plies = _f64(plies)
+        self._check(self._L.cvm_set_laminate(self._h, plies.shape[0], _dp(plies)))
+
+    def abd(self):
+        A, B, D = np.empty(9), np.empty(9), np.empty(9)
+        self._check(self._L.cvm_get_abd(self._h, _dp(A), _dp(B), _dp(D)))
+        return A.reshape(3, 3), B.reshape(3, 3), D.reshape(3, 3)
+
+    def add_crease(self, vx, vy, ext_s=True, ext_e=True, closed=False):
+        vx, vy = _f64(vx), _f64(vy)
+        self._check(self._L.cvm_add_crease(self._h, vx.size, _dp(vx), _dp(vy), int(ext_s), int(ext_e), int(closed)))
+
+    def add_domain(self, gps, kind=FACE) -> int:
+        a = [_f64(x) for x in (gps.gx, gps.gy, gps.w, gps.nx, gps.ny, gps.tx, gps.ty)]
+        did = C.c_int(-1)
+        self._check(self._L.cvm_add_domain(self._h, kind, a[0].size, *[_dp(x) for x in a], C.byref(did)))
+        self.domain_sizes.append(a[0].size)
+        return did.value
+
+    def add_fold(self, left, right, Kt, Ang0=0.0) -> int:
+        fid = C.c_int(-1)
+        self._check(self._L.cvm_add_fold(self._h, left, right, Kt, Ang0, C.byref(fid)))
+        return fid.value
+
+    def set_fold_gp_state(self, fold, theta0=None, active=None):
+        n = self.domain_sizes[0] if theta0 is None and active is None else \
+            (len(theta0) if theta0 is not None else len(active))
+        th = _f64(theta0)
+        ac = None if active is None else np.ascontiguousarray(active, dtype=np.int8).tobytes()
+        self._check(self._L.cvm_set_fold_gp_state(self._h, fold, n, _dp(th), ac))
+
+    # ---- hot path -------------------------------------------------------------
+    def neighbours(self, dom, rho):
+        n = C.c_int64(0)
+        self._check(self._L.cvm_neighbours(self._h, dom, rho, C.byref(n)))
+        self.npairs[dom] = n.value
+        return n.value
+
+    def get_neighbours(self, dom):
+        ng = self.domain_sizes[dom]
+        npairs = self.npairs[dom]
+        off = np.empty(ng + 1, np.int32); ids = np.empty(max(npairs, 1), np.int32); s = np.empty(max(npairs, 1))
+        self._check(self._L.cvm_get_neighbours(self._h, dom, off.ctypes.data_as(_i), ids.ctypes.data_as(_i), _dp(s)))
+        return off, ids[:npairs], s[:npairs]
+
+    def shape_functions(self, dom, order=2, kernel=0):
+        nb, fb = C.c_int(0), C.c_int(0)
+        st = self._L.cvm_shape_functions(self._h, dom, order, kernel, C.byref(nb), C.byref(fb))
+        if st in (E_SINGULAR_MOMENT, E_PIVOT_COLLAPSE):
+            err = CvmError(st, self._L.cvm_last_error(self._h).decode())
+            err.nbad, err.first = nb.value, fb.value
+            raise err
+        self._check(st)
+
+    def get_shape_functions(self, dom):
+        n = max(self.npairs[dom], 1)
+        arrs = [np.empty(n) for _ in range(6)]
+        self._check(self._L.cvm_get_shape_functions(self._h, dom, *[_dp(a) for a in arrs]))
+        return [a[:self.npairs[dom]] for a in arrs]
+
+    def init_assembly(self):
+        nf, nu = C.c_int64(0), C.c_int64(0)
+        self._check(self._L.cvm_init_assembly(self._h, C.byref(nf), C.byref(nu)))
+        self.nnz_face, self.nnz_union = nf.value, nu.value
+        return self.nnz_face, self.nnz_union
+
+    def pattern(self, with_folds=True):
+        nnz = self.nnz_union if with_folds else self.nnz_face
+        outer = np.empty(3 * self.nn + 1, np.int32); inner = np.empty(max(nnz, 1), np.int32)
+        self._check(self._L.cvm_get_pattern(self._h, int(with_folds), outer.ctypes.data_as(_i), inner.ctypes.data_as(_i)))
+        return outer, inner[:nnz]
+
+    def csr_mapping(self, gp):
+        n = C.c_int(0)
+        self._check(self._L.cvm_csr_mapping(self._h, gp, None, 0, C.byref(n)))
+        out = np.empty(max(n.value, 1), np.int32)
+        self._check(self._L.cvm_csr_mapping(self._h, gp, out.ctypes.data_as(_i), n.value, C.byref(n)))
+        return out[:n.value]
+
+    def assemble(self, x, elastic_folds=True, tangent=True):
+        """Meshless::assembleSystem: returns (F_int, K_values) in the pattern
+        returned by pattern(with_folds=elastic_folds)."""
+        x = _f64(x)
+        F = np.empty(3 * self.nn)
+        nnz = self.nnz_union if elastic_folds else self.nnz_face
+        K = np.empty(max(nnz, 1)) if tangent else None
+        flags = RESIDUAL | (TANGENT if tangent else 0) | (ELASTIC_FOLDS if elastic_folds else 0)
+        self._check(self._L.cvm_assemble(self._h, x.ctypes.data, flags, F.ctypes.data,
+                                         K.ctypes.data if K is not None else None))
+        return F, (K[:nnz] if tangent else None)
+
+    def assemble_device(self, x_ptr, F_ptr, K_ptr, flags):
+        """Device-pointer variant (inputs already resident in HBM)."""
+        self._check(self._L.cvm_assemble(self._h, C.c_void_p(x_ptr), flags | DEVICE_PTRS,
+                                         C.c_void_p(F_ptr), C.c_void_p(K_ptr) if K_ptr else None))
+
+    def synchronize(self):
+        self._check(self._L.cvm_synchronize(self._h))
+
+    # ---- measurement ----------------------------------------------------------
+    def set_profiling(self, on=True):
+        self._check(self._L.cvm_set_profiling(self._h, int(on)))
+
+    def reset_profiling(self):
+        self._check(self._L.cvm_reset_profiling(self._h))
+
+    def kernel_stats(self):
+        out = {}
+        for i in range(self._L.cvm_kernel_count(self._h)):
+            name = C.create_string_buffer(128)
+            ms = C.c_double(0); cnt = C.c_int64(0)
+            self._check(self._L.cvm_kernel_stats(self._h, i, name, 128, C.byref(ms), C.byref(cnt)))
+            out[name.value.decode()] = (ms.value, cnt.value)
+        return out
+
+
+def plate_from_sheet(sh, device: int = 0, compute: bool = True, order=None, kernel=None):
+    """Build the device plate for a workloads.Sheet: face domain 0, fold domains
+    (left, right) per fold; optionally neighbours + shape functions on every
+    domain and the assembly plan."""
+    p = Plate(device)
+    p.set_nodes(sh.X, sh.Y)
+    p.set_plies(sh.plies)
+    for cr in sh.creases:
+        p.add_crease(cr.vx, cr.vy, cr.ext_s, cr.ext_e, cr.closed)
+    p.face_dom = p.add_domain(sh.face, FACE)
+    p.fold_doms = []
+    for f in sh.folds:
+        l = p.add_domain(f.left, FOLD_LEFT)
+        r = p.add_domain(f.right, FOLD_RIGHT)
+        p.add_fold(l, r, f.Kt, f.Ang0)
+        p.fold_doms.append((l, r))
+    if compute:
+        for d in range(len(p.domain_sizes)):
+            p.neighbours(d, sh.rho)
+            p.shape_functions(d, sh.order if order is None else order, sh.kernel if kernel is None else kernel)
+        p.init_assembly()
+    return p

@@ -1,0 +1,157 @@
+/*
+ * curvami_gpu.h -- C ABI of the MI355X (gfx950) implementation of Curvami's
+ * per-Gauss-point hot path: fixed-radius neighbour search, RKPM shape
+ * functions up to second derivatives with crease-cut visibility, and the
+ * Kirchhoff-Love residual / block-upper CSC tangent with the elastic hinge.
+ *
+ * Conventions
+ *  - Plain C types only; every entry point returns a cvm_status (never throws).
+ *  - Host pointers unless a flag says otherwise; outputs are caller-allocated.
+ *  - One HIP stream per context; a context is not thread-safe (as the
+ *    reference's assembler, plate/Assembly.h:18-45, is not re-entrant).
+ *    Many contexts may coexist.
+ *  - Reference citations are relative to the Curvami source tree.
+ */
+#ifndef CURVAMI_GPU_H
+#define CURVAMI_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CVM_API __attribute__((visibility("default")))
+
+typedef struct cvm_context cvm_context;
+
+/* Status codes. SINGULAR_MOMENT and PIVOT_COLLAPSE are the two runtime_errors
+ * of GlobalRKPM::solve (core/include/curvami/GlobalRKPM.h:241-256, :347-359);
+ * PATTERN_MISS is the CSR-map throw of initSparsityPattern (plate/Assembly.h:148-152). */
+typedef enum {
+    CVM_OK = 0,
+    CVM_E_INVALID = 1,
+    CVM_E_HIP = 2,
+    CVM_E_SINGULAR_MOMENT = 3,
+    CVM_E_PIVOT_COLLAPSE = 4,
+    CVM_E_PATTERN_MISS = 5,
+    CVM_E_NO_MEMORY = 6,
+    CVM_E_STATE = 7,
+    CVM_E_BUDGET = 8
+} cvm_status;
+
+/* Integration-domain kinds (plate/Meshless.h:14-17: faces, edges, points, fold left/right). */
+typedef enum {
+    CVM_DOMAIN_FACE = 0,
+    CVM_DOMAIN_EDGE = 1,
+    CVM_DOMAIN_POINT = 2,
+    CVM_DOMAIN_FOLD_LEFT = 3,
+    CVM_DOMAIN_FOLD_RIGHT = 4
+} cvm_domain_kind;
+
+/* Window functions: the reference's quartic spline (GlobalRKPM.h:15-17) and an
+ * added cubic B-spline. */
+typedef enum { CVM_KERNEL_QUARTIC = 0, CVM_KERNEL_CUBIC_BSPLINE = 1 } cvm_kernel;
+
+/* cvm_assemble flags. */
+enum {
+    CVM_RESIDUAL = 1,        /* compute F_int */
+    CVM_TANGENT = 2,         /* compute K values */
+    CVM_ELASTIC_FOLDS = 4,   /* add elastic hinges (Meshless.h:118-126, i.e. flagRig == false) */
+    CVM_DEVICE_PTRS = 8      /* x, F_int and K_values are device pointers */
+};
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Create a context on HIP device `device`. No reference counterpart: the
+ * reference keeps this state in Meshless (plate/Meshless.h:11-27). */
+CVM_API int cvm_create(int device, cvm_context** out);
+CVM_API void cvm_destroy(cvm_context* ctx);
+/* Text of the last error (the reference's exception message where one exists). */
+CVM_API const char* cvm_last_error(const cvm_context* ctx);
+
+/* Node reference coordinates (Node::X_ref, core/include/curvami/MeshfreeData.h:7-14). */
+CVM_API int cvm_set_nodes(cvm_context* ctx, int nn, const double* X, const double* Y);
+
+/* Laminate from plies, 6 doubles per ply: E1, E2, nu12, G12, angle_deg, thickness
+ * (Ply, plate/PlateData.h:11-15); computes A, B, D as Constitutive::computeABD
+ * (plate/Constitutive.h:10-50). */
+CVM_API int cvm_set_laminate(cvm_context* ctx, int nply, const double* plies);
+/* Read back A, B, D (row-major 3x3 each). */
+CVM_API int cvm_get_abd(const cvm_context* ctx, double* A, double* B, double* D);
+
+/* Crease polyline (Enrichment::FoldLine, core/include/curvami/Enrichment.h:10-15). */
+CVM_API int cvm_add_crease(cvm_context* ctx, int nv, const double* vx, const double* vy,
+                           int extend_start, int extend_end, int is_closed);
+
+/* Integration domain: Gauss points (GaussPointData::X_ref, weight, normal,
+ * tangent; MeshfreeData.h:17-35). Normal/tangent may be NULL (zero). */
+CVM_API int cvm_add_domain(cvm_context* ctx, int kind, int ng, const double* gx, const double* gy,
+                           const double* w, const double* nx, const double* ny,
+                           const double* tx, const double* ty, int* domain_id);
+
+/* Elastic fold between two domains of equal size (Fold, plate/PlateData.h:72-94;
+ * Meshless::addFold, plate/Meshless.h:30-46). */
+CVM_API int cvm_add_fold(cvm_context* ctx, int left_domain, int right_domain, double Kt, double Ang0,
+                         int* fold_id);
+/* Per crease-GP rest angle and activity (progressive creasing; Fold::gp_theta0,
+ * gp_active, PlateData.h:90-93). NULL clears. */
+CVM_API int cvm_set_fold_gp_state(cvm_context* ctx, int fold_id, int n, const double* theta0,
+                                  const char* active);
+
+/* ---- hot path ----------------------------------------------------------- */
+
+/* Fixed-radius neighbour search of one domain against the nodes.
+ * Replaces GlobalFRNN::execute (core/include/curvami/FRNN.h:23-78): same
+ * neighbour sets and order, s = sqrt(dist2) / rho. */
+CVM_API int cvm_neighbours(cvm_context* ctx, int domain, double rho, int64_t* npairs);
+/* Copy the lists out: off[ng+1], ids[npairs], s[npairs] (any may be NULL). */
+CVM_API int cvm_get_neighbours(cvm_context* ctx, int domain, int* off, int* ids, double* s);
+
+/* RKPM shape functions of one domain, cut by every crease added.
+ * Replaces GlobalRKPM::solve (core/include/curvami/GlobalRKPM.h:19-397).
+ * Returns CVM_E_SINGULAR_MOMENT / CVM_E_PIVOT_COLLAPSE with *nbad and
+ * *first_bad set as the reference's messages report them. */
+CVM_API int cvm_shape_functions(cvm_context* ctx, int domain, int order, int kernel,
+                                int* nbad, int* first_bad);
+/* Copy Psi, dPsi/dX, dPsi/dY, d2Psi/dX2, d2Psi/dY2, d2Psi/dXdY (npairs each,
+ * FRNN order; GaussPointData::Psi/dPsi/ddPsi). Any pointer may be NULL. */
+CVM_API int cvm_get_shape_functions(cvm_context* ctx, int domain, double* psi, double* dpx, double* dpy,
+                                    double* dxx, double* dyy, double* dxy);
+
+/* Sparsity pattern and assembly plan over the face domain and the folds.
+ * Replaces NonLinearAssembler::initSparsityPattern (plate/Assembly.h:59-153).
+ * nnz_face: values of the face pattern; nnz_union: with the fold blocks added
+ * (the pattern of Meshless::assembleSystem with elastic folds, Meshless.h:118-126). */
+CVM_API int cvm_init_assembly(cvm_context* ctx, int64_t* nnz_face, int64_t* nnz_union);
+/* Eigen-style CSC of the block-upper K: outer[3nn+1], inner[nnz]. */
+CVM_API int cvm_get_pattern(cvm_context* ctx, int with_folds, int* outer, int* inner);
+/* NonLinearAssembler::csrMapping (Assembly.h:54): one position per local pair
+ * (i <= j) of face GP `gp`; returns the count in *n. */
+CVM_API int cvm_csr_mapping(cvm_context* ctx, int gp, int* out, int cap, int* n);
+
+/* Residual and tangent at displacement x (3 DOFs per node).
+ * Replaces NonLinearAssembler::updateStiffness (plate/Assembly.h:158-345)
+ * together with Meshless::assembleSystem / assembleSystemInternalOnly
+ * (plate/Meshless.h:102-127) and Constraints::evaluateElasticFold
+ * (plate/Constraints.h:341-436). F_int[3nn] is overwritten; K_values[nnz]
+ * follows cvm_get_pattern(with_folds = flags & CVM_ELASTIC_FOLDS). */
+CVM_API int cvm_assemble(cvm_context* ctx, const double* x, int flags, double* F_int, double* K_values);
+
+/* Wait for the context's stream. */
+CVM_API int cvm_synchronize(cvm_context* ctx);
+
+/* ---- measurement -------------------------------------------------------- */
+
+/* Per-kernel HIP-event timing on the context's stream (on/off). */
+CVM_API int cvm_set_profiling(cvm_context* ctx, int on);
+/* Number of timed kernels; name, total ms and launch count of kernel i. */
+CVM_API int cvm_kernel_count(cvm_context* ctx);
+CVM_API int cvm_kernel_stats(cvm_context* ctx, int i, char* name, int name_cap, double* total_ms,
+                             int64_t* launches);
+CVM_API int cvm_reset_profiling(cvm_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CURVAMI_GPU_H */
