@@ -178,18 +178,15 @@ def main():
     assert np.all(np.isfinite(F))
 
     # roofline of the dominant kernel (algorithmic work of one launch)
-    rk_total = stats.get("rkpm", (0.0, 1))
+    rk_face = stats.get("rkpm_face", (0.0, 1))
     tile = stats.get("tile_residual_tangent", (0.0, 1))
-    rk_ms_step = rk_total[0] / args.steps
+    rk_face_ms = rk_face[0] / max(rk_face[1], 1)
     tile_ms = tile[0] / max(tile[1], 1)
     F_sf = float(np.sum(sf_flops(nper)))
     F_asm = float(np.sum(asm_flops(nper)))
     B_asm = float(np.sum(44.0 * nper + 8.0)) + 48.0 * sh.nn + 8.0 * nnz_union
-    # face-domain rkpm launch time = its share of the per-step rkpm total by pair count
-    npairs_all = sum(p.npairs[d] for d in range(ndom))
-    rk_face_ms = rk_ms_step * p.npairs[0] / npairs_all
     cands = {
-        "rkpm": (F_sf / (rk_face_ms * 1e-3) / 1e12 if rk_face_ms > 0 else 0.0, rk_face_ms),
+        "rkpm_face": (F_sf / (rk_face_ms * 1e-3) / 1e12 if rk_face_ms > 0 else 0.0, rk_face_ms),
         "tile_residual_tangent": (F_asm / (tile_ms * 1e-3) / 1e12 if tile_ms > 0 else 0.0, tile_ms),
     }
     dom_name = max(cands, key=lambda k: cands[k][1])

@@ -26,22 +26,22 @@
 
 namespace cvm {
 
-constexpr int TILE_THREADS = 256;
-constexpr int NBMAX = 64;        // neighbours per GP
-constexpr int NBS = 27;          // doubles per neighbour record in LDS
+constexpr int TILE_THREADS = 1024;   // 16 waves: one per GP of the tile in the preparation phase
+constexpr int TILE_GPS = 16;         // GPs per tile (presence masks are 16-bit)
+constexpr int UMAX = 64;             // union nodes per tile (one lane per neighbour)
+constexpr int RS = 18;               // doubles per neighbour record
+constexpr int GS = 20;               // doubles per GP record
 
 struct MatC {
     double A[9], B[9], D[9];
 };
 
-// neighbour record layout
-enum {
-    R_PX = 0, R_PY, R_AP00, R_AP01, R_AP10, R_AP11, R_AP20, R_AP21,
-    R_BB0, R_BB1, R_BB2, R_DB0, R_DB1, R_DB2, R_BE0, R_BE1, R_BE2,
-    R_G0, R_G1, R_G2, R_TU, R_NP0, R_NP1, R_CP0, R_CP1
-};
-// GP record in LDS
-enum { G_W = 0, G_A1 = 1, G_A2 = 4, G_B = 7, G_BDV = 10, G_N = 11 };
+// neighbour record: grad Psi, beta (bending B-row factor), Tu, g = F0 C0^-1 grad Psi,
+// B beta, D beta, and the residual contribution f_local
+enum { R_PX = 0, R_PY, R_BE0, R_BE1, R_BE2, R_TU, R_G0, R_G1, R_G2, R_BB0, R_BB1, R_BB2,
+       R_DB0, R_DB1, R_DB2, R_F0, R_F1, R_F2 };
+// GP record: weight, F0 columns, normal, b.v_curv, membrane resultant N, C0^-1
+enum { G_W = 0, G_A1 = 1, G_A2 = 4, G_B = 7, G_BDV = 10, G_N = 11, G_CI = 14 };
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -49,49 +49,49 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Per tile: (1) wave w prepares GP w of the tile (kinematics, resultants and the
+// per-neighbour records, Assembly.h:210-268); (2) every thread owns touched
+// blocks (lo, hi) of the tile and sums the GPs that couple them, in ascending GP
+// order, from the records of the two nodes: no LDS accumulation, no atomics on
+// values; (3) the tile's residual per union node, again in GP order.
 __global__ __launch_bounds__(TILE_THREADS) void k_tile(
     const int* __restrict__ tile_gp, const int* __restrict__ tile_uoff, const int* __restrict__ tile_unodes,
     const int* __restrict__ tile_toff, const uint16_t* __restrict__ tpairs,
     const int* __restrict__ off, const uint8_t* __restrict__ uloc,
     const double* __restrict__ sf, int64_t npairs, const double* __restrict__ w,
-    const double* __restrict__ xu, MatC mat, int max_touched, int lut_dim, int with_tangent,
+    const double* __restrict__ xu, MatC mat, int rec_cap, int with_tangent,
     double* __restrict__ scr, double* __restrict__ fscr) {
     extern __shared__ double smem[];
     const int t = blockIdx.x;
     const int tid = threadIdx.x;
-    const int g0 = tile_gp[t], g1 = tile_gp[t + 1];
+    const int g0 = tile_gp[t], ngp = tile_gp[t + 1] - g0;
     const int ub = tile_uoff[t], U = tile_uoff[t + 1] - ub;
     const int tb = tile_toff[t], T = tile_toff[t + 1] - tb;
+    const int pb = off[g0];
 
-    double* acc = smem;                                   // max_touched * 9
-    double* nb = acc + static_cast<size_t>(max_touched) * 9;   // NBMAX * NBS
-    double* uc = nb + NBMAX * NBS;                        // lut_dim * 3
-    double* fa = uc + lut_dim * 3;                        // lut_dim * 3
-    double* gd = fa + lut_dim * 3;                        // 16
-    uint16_t* lut = reinterpret_cast<uint16_t*>(gd + 16); // lut_dim * lut_dim
+    double* rec = smem;                                         // rec_cap * RS
+    double* gs = rec + static_cast<size_t>(rec_cap) * RS;       // TILE_GPS * GS
+    double* uc = gs + TILE_GPS * GS;                            // UMAX * 3
+    uint32_t* pm = reinterpret_cast<uint32_t*>(uc + UMAX * 3);  // UMAX presence masks
+    uint8_t* loc = reinterpret_cast<uint8_t*>(pm + UMAX);       // TILE_GPS * UMAX
 
-    for (int k = tid; k < U; k += TILE_THREADS) {
-        const int v = tile_unodes[ub + k];
-        uc[3 * k] = xu[3 * v]; uc[3 * k + 1] = xu[3 * v + 1]; uc[3 * k + 2] = xu[3 * v + 2];
-        fa[3 * k] = 0.0; fa[3 * k + 1] = 0.0; fa[3 * k + 2] = 0.0;
+    if (tid < U) {
+        const int v = tile_unodes[ub + tid];
+        uc[3 * tid] = xu[3 * v]; uc[3 * tid + 1] = xu[3 * v + 1]; uc[3 * tid + 2] = xu[3 * v + 2];
+        pm[tid] = 0u;
     }
-    if (with_tangent) {
-        for (int k = tid; k < T; k += TILE_THREADS) {
-            const uint16_t pr = tpairs[tb + k];
-            lut[(pr >> 8) * lut_dim + (pr & 0xff)] = static_cast<uint16_t>(k);
-        }
-        for (int k = tid; k < 9 * T; k += TILE_THREADS) acc[k] = 0.0;
-    }
+    loc[tid] = 0xff;   // TILE_GPS * UMAX == TILE_THREADS
     __syncthreads();
 
     const double* A = mat.A;
     const double* B = mat.B;
     const double* D = mat.D;
-    for (int g = g0; g < g1; ++g) {
-        const int b = off[g], n = off[g + 1] - b;
-        if (tid < 64) {
-            // ---- kinematics (Kinematics.h:75-113) -------------------------------
-            const int lane = tid;
+    {
+        const int wv = tid >> 6, lane = tid & 63;
+        if (wv < ngp) {
+            const int g = g0 + wv;
+            const int b = off[g], n = off[g + 1] - b;
+            const int rb = b - pb;
             const bool act = lane < n;
             double px = 0, py = 0, hxx = 0, hyy = 0, hxy = 0;
             int ul = 0;
@@ -103,6 +103,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
                 ul = uloc[p];
                 u0 = uc[3 * ul]; u1 = uc[3 * ul + 1]; u2 = uc[3 * ul + 2];
             }
+            // ---- kinematics (Kinematics.h:75-113) ------------------------------------
             const double a1x = 1.0 + wave_sum(u0 * px), a1y = wave_sum(u1 * px), a1z = wave_sum(u2 * px);
             const double a2x = wave_sum(u0 * py), a2y = 1.0 + wave_sum(u1 * py), a2z = wave_sum(u2 * py);
             const double rxx0 = wave_sum(u0 * hxx), rxx1 = wave_sum(u1 * hxx), rxx2 = wave_sum(u2 * hxx);
@@ -145,101 +146,107 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
                 const double SYY = ryy0 * gI0 + ryy1 * gI1 + ryy2 * gI2;
                 const double SXY = rxy0 * gI0 + rxy1 * gI1 + rxy2 * gI2;
                 const double be0 = -hxx + SXX, be1 = -hyy + SYY, be2 = 2.0 * (-hxy + SXY);
+                double* r = rec + static_cast<size_t>(rb + lane) * RS;
                 // f_local = (Bm^T N + Bb^T M) W (:249-250)
                 const double ca = px * N[0] + py * N[2], cb = py * N[1] + px * N[2];
                 const double cm = be0 * M[0] + be1 * M[1] + be2 * M[2];
-                fa[3 * ul] += W * (a1x * ca + a2x * cb + bx * cm);
-                fa[3 * ul + 1] += W * (a1y * ca + a2y * cb + by * cm);
-                fa[3 * ul + 2] += W * (a1z * ca + a2z * cb + bz * cm);
-                if (with_tangent) {
-                    double* r = nb + lane * NBS;
-                    r[R_PX] = px; r[R_PY] = py;
+                r[R_F0] = W * (a1x * ca + a2x * cb + bx * cm);
+                r[R_F1] = W * (a1y * ca + a2y * cb + by * cm);
+                r[R_F2] = W * (a1z * ca + a2z * cb + bz * cm);
+                r[R_PX] = px; r[R_PY] = py;
+                r[R_BE0] = be0; r[R_BE1] = be1; r[R_BE2] = be2;
+                r[R_TU] = (px * pX + py * pY) - (hxx * M[0] + 2.0 * hxy * M[2] + hyy * M[1]);
+                r[R_G0] = gI0; r[R_G1] = gI1; r[R_G2] = gI2;
 #pragma unroll
-                    for (int q = 0; q < 3; ++q) {
-                        r[R_AP00 + 2 * q] = A[3 * q] * px + A[3 * q + 2] * py;
-                        r[R_AP00 + 2 * q + 1] = A[3 * q + 1] * py + A[3 * q + 2] * px;
-                        r[R_BB0 + q] = B[3 * q] * be0 + B[3 * q + 1] * be1 + B[3 * q + 2] * be2;
-                        r[R_DB0 + q] = D[3 * q] * be0 + D[3 * q + 1] * be1 + D[3 * q + 2] * be2;
-                    }
-                    r[R_BE0] = be0; r[R_BE1] = be1; r[R_BE2] = be2;
-                    r[R_G0] = gI0; r[R_G1] = gI1; r[R_G2] = gI2;
-                    r[R_TU] = (px * pX + py * pY) - (hxx * M[0] + 2.0 * hxy * M[2] + hyy * M[1]);
-                    r[R_NP0] = N[0] * px + N[2] * py; r[R_NP1] = N[2] * px + N[1] * py;
-                    r[R_CP0] = Ci00 * px + Ci01 * py; r[R_CP1] = Ci01 * px + Ci11 * py;
-                    r[NBS - 1] = static_cast<double>(ul);
+                for (int q = 0; q < 3; ++q) {
+                    r[R_BB0 + q] = B[3 * q] * be0 + B[3 * q + 1] * be1 + B[3 * q + 2] * be2;
+                    r[R_DB0 + q] = D[3 * q] * be0 + D[3 * q + 1] * be1 + D[3 * q + 2] * be2;
                 }
+                loc[wv * UMAX + ul] = static_cast<uint8_t>(lane);
+                atomicOr(&pm[ul], 1u << wv);
             }
-            if (lane == 0 && with_tangent) {
-                gd[G_W] = W;
-                gd[G_A1] = a1x; gd[G_A1 + 1] = a1y; gd[G_A1 + 2] = a1z;
-                gd[G_A2] = a2x; gd[G_A2 + 1] = a2y; gd[G_A2 + 2] = a2z;
-                gd[G_B] = bx; gd[G_B + 1] = by; gd[G_B + 2] = bz;
-                gd[G_BDV] = bdv;
-                gd[G_N] = static_cast<double>(n);
+            if (lane == 0) {
+                double* G = gs + wv * GS;
+                G[G_W] = W;
+                G[G_A1] = a1x; G[G_A1 + 1] = a1y; G[G_A1 + 2] = a1z;
+                G[G_A2] = a2x; G[G_A2 + 1] = a2y; G[G_A2 + 2] = a2z;
+                G[G_B] = bx; G[G_B + 1] = by; G[G_B + 2] = bz;
+                G[G_BDV] = bdv;
+                G[G_N] = N[0]; G[G_N + 1] = N[1]; G[G_N + 2] = N[2];
+                G[G_CI] = Ci00; G[G_CI + 1] = Ci01; G[G_CI + 2] = Ci11;
             }
         }
-        if (!with_tangent) continue;
-        __syncthreads();
-        // ---- pair blocks (Assembly.h:282-324), i <= j -----------------------------
-        const int npr = n * (n + 1) / 2;
-        const double W = gd[G_W];
-        const double a1[3] = {gd[G_A1], gd[G_A1 + 1], gd[G_A1 + 2]};
-        const double a2[3] = {gd[G_A2], gd[G_A2 + 1], gd[G_A2 + 2]};
-        const double bv[3] = {gd[G_B], gd[G_B + 1], gd[G_B + 2]};
-        const double bdv = gd[G_BDV];
-        const double two_n1 = 2.0 * n + 1.0;
-        for (int p = tid; p < npr; p += TILE_THREADS) {
-            // row i of the upper triangle: rowstart(i) = i*n - i*(i-1)/2
-            int i = static_cast<int>((two_n1 - sqrt(two_n1 * two_n1 - 8.0 * p)) * 0.5);
-            if (i < 0) i = 0;
-            if (i > n - 1) i = n - 1;
-            while (i > 0 && i * n - i * (i - 1) / 2 > p) --i;
-            while (i + 1 < n && (i + 1) * n - (i + 1) * i / 2 <= p) ++i;
-            const int j = i + (p - (i * n - i * (i - 1) / 2));
-            const double* ri = nb + i * NBS;
-            const double* rj = nb + j * NBS;
-            const double pxi = ri[R_PX], pyi = ri[R_PY], pxj = rj[R_PX], pyj = rj[R_PY];
-            const double S00 = pxi * rj[R_AP00] + pyi * rj[R_AP20];
-            const double S01 = pxi * rj[R_AP01] + pyi * rj[R_AP21];
-            const double S10 = pyi * rj[R_AP10] + pxi * rj[R_AP20];
-            const double S11 = pyi * rj[R_AP11] + pxi * rj[R_AP21];
-            const double m0 = pxi * rj[R_BB0] + pyi * rj[R_BB2], m1 = pyi * rj[R_BB1] + pxi * rj[R_BB2];
-            const double q0 = pxj * ri[R_BB0] + pyj * ri[R_BB2], q1 = pyj * ri[R_BB1] + pxj * ri[R_BB2];
-            const double z = ri[R_BE0] * rj[R_DB0] + ri[R_BE1] * rj[R_DB1] + ri[R_BE2] * rj[R_DB2]
-                           + bdv * (pxi * rj[R_CP0] + pyi * rj[R_CP1]);
-            const double geo = pxi * rj[R_NP0] + pyi * rj[R_NP1];
-            const double tui = ri[R_TU], tuj = rj[R_TU];
-            double xv[3], yv[3], T0[3], T1[3];
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                xv[r] = a1[r] * m0 + a2[r] * m1 - tui * rj[R_G0 + r];
-                yv[r] = a1[r] * q0 + a2[r] * q1 - tuj * ri[R_G0 + r];
-                T0[r] = a1[r] * S00 + a2[r] * S10;
-                T1[r] = a1[r] * S01 + a2[r] * S11;
-            }
-            const int ua = static_cast<int>(ri[NBS - 1]), ubb = static_cast<int>(rj[NBS - 1]);
-            const bool flip = ua > ubb;   // union sorted by node id: ua > ub <=> ni > nj
-            const int lo = flip ? ubb : ua, hi = flip ? ua : ubb;
-            double kv[3][3];
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-#pragma unroll
-                for (int cc = 0; cc < 3; ++cc)
-                    kv[r][cc] = W * (T0[r] * a1[cc] + T1[r] * a2[cc] + xv[r] * bv[cc] + bv[r] * yv[cc]
-                                     + z * bv[r] * bv[cc] + (r == cc ? geo : 0.0));
-            double* dst = acc + 9 * static_cast<int>(lut[lo * lut_dim + hi]);
-            // canonical (lo, hi) block, stored column-major: dst[3c + r] = block(r, c)
-#pragma unroll
-            for (int cc = 0; cc < 3; ++cc)
-#pragma unroll
-                for (int r = 0; r < 3; ++r) dst[3 * cc + r] += flip ? kv[cc][r] : kv[r][cc];
-        }
-        __syncthreads();
     }
     __syncthreads();
-    for (int k = tid; k < 3 * U; k += TILE_THREADS) fscr[3 * static_cast<int64_t>(ub) + k] = fa[k];
-    if (with_tangent)
-        for (int k = tid; k < 9 * T; k += TILE_THREADS) scr[9 * static_cast<int64_t>(tb) + k] = acc[k];
+
+    if (with_tangent) {
+        // ---- pair blocks (Assembly.h:282-324) for the canonical (lo, hi) orientation:
+        // K_IJ with I the neighbour entry of lo and J that of hi (K_JI = K_IJ^T).
+        for (int sl = tid; sl < T; sl += TILE_THREADS) {
+            const uint16_t pr = tpairs[tb + sl];
+            const int ua = pr >> 8, ubb = pr & 0xff;
+            uint32_t mask = pm[ua] & pm[ubb];
+            double acc[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) acc[k] = 0.0;
+            while (mask) {
+                const int wv = __ffs(mask) - 1;
+                mask &= mask - 1;
+                const int base = off[g0 + wv] - pb;
+                const double* ri = rec + static_cast<size_t>(base + loc[wv * UMAX + ua]) * RS;
+                const double* rj = rec + static_cast<size_t>(base + loc[wv * UMAX + ubb]) * RS;
+                const double* G = gs + wv * GS;
+                const double W = G[G_W];
+                const double a1[3] = {G[G_A1], G[G_A1 + 1], G[G_A1 + 2]};
+                const double a2[3] = {G[G_A2], G[G_A2 + 1], G[G_A2 + 2]};
+                const double bv[3] = {G[G_B], G[G_B + 1], G[G_B + 2]};
+                const double pxi = ri[R_PX], pyi = ri[R_PY], pxj = rj[R_PX], pyj = rj[R_PY];
+                // A Psi_j (3x2) and S = Psi_i^T A Psi_j
+                const double ap00 = A[0] * pxj + A[2] * pyj, ap01 = A[1] * pyj + A[2] * pxj;
+                const double ap10 = A[3] * pxj + A[5] * pyj, ap11 = A[4] * pyj + A[5] * pxj;
+                const double ap20 = A[6] * pxj + A[8] * pyj, ap21 = A[7] * pyj + A[8] * pxj;
+                const double S00 = pxi * ap00 + pyi * ap20, S01 = pxi * ap01 + pyi * ap21;
+                const double S10 = pyi * ap10 + pxi * ap20, S11 = pyi * ap11 + pxi * ap21;
+                const double m0 = pxi * rj[R_BB0] + pyi * rj[R_BB2], m1 = pyi * rj[R_BB1] + pxi * rj[R_BB2];
+                const double q0 = pxj * ri[R_BB0] + pyj * ri[R_BB2], q1 = pyj * ri[R_BB1] + pxj * ri[R_BB2];
+                const double geo = pxi * (G[G_N] * pxj + G[G_N + 2] * pyj) + pyi * (G[G_N + 2] * pxj + G[G_N + 1] * pyj);
+                const double cp0 = G[G_CI] * pxj + G[G_CI + 1] * pyj, cp1 = G[G_CI + 1] * pxj + G[G_CI + 2] * pyj;
+                const double z = ri[R_BE0] * rj[R_DB0] + ri[R_BE1] * rj[R_DB1] + ri[R_BE2] * rj[R_DB2]
+                               + G[G_BDV] * (pxi * cp0 + pyi * cp1);
+                const double tui = ri[R_TU], tuj = rj[R_TU];
+                double xv[3], yv[3], T0[3], T1[3];
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    xv[r] = W * (a1[r] * m0 + a2[r] * m1 - tui * rj[R_G0 + r]);
+                    yv[r] = W * (a1[r] * q0 + a2[r] * q1 - tuj * ri[R_G0 + r] + z * bv[r]);
+                    T0[r] = W * (a1[r] * S00 + a2[r] * S10);
+                    T1[r] = W * (a1[r] * S01 + a2[r] * S11);
+                }
+                const double wg = W * geo;
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int r = 0; r < 3; ++r)
+                        acc[3 * c + r] += T0[r] * a1[c] + T1[r] * a2[c] + xv[r] * bv[c] + bv[r] * yv[c]
+                                          + (r == c ? wg : 0.0);
+            }
+            double* dst = scr + 9 * static_cast<int64_t>(tb + sl);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) dst[k] = acc[k];
+        }
+    }
+    // ---- residual per union node, GPs in order ---------------------------------------
+    if (tid < U) {
+        double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+        for (int wv = 0; wv < ngp; ++wv) {
+            const int l = loc[wv * UMAX + tid];
+            if (l == 0xff) continue;
+            const double* r = rec + static_cast<size_t>(off[g0 + wv] - pb + l) * RS;
+            f0 += r[R_F0]; f1 += r[R_F1]; f2 += r[R_F2];
+        }
+        double* dst = fscr + 3 * static_cast<int64_t>(ub + tid);
+        dst[0] = f0; dst[1] = f1; dst[2] = f2;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -336,11 +343,20 @@ __device__ double fold_angle_grad(const double* F, double tx, double ty, double*
     return theta;
 }
 
+// Per crease GP record: [W Kt, W moment, H (12 x 12, symmetrised FD Hessian of
+// theta, only with the exact tangent)]. Per (crease GP, node of L u R) record:
+// G (aggregated dtheta/du, 3) and the summed shape derivatives of the node's
+// left and right occurrences (dX_L, dY_L, dX_R, dY_R), which give the node's
+// 12-vector map B(node, d) used by the Hessian term.
+constexpr int FGP_REC = 2 + 144;
+constexpr int FND_REC = 7;
+
 __global__ void k_fold(int nfgp, const FoldDev* __restrict__ folds, const int* __restrict__ fgp_fold,
                        const int* __restrict__ fgp_local, const int* __restrict__ fgp_noff,
                        const int* __restrict__ lpo, const int* __restrict__ rpo,
                        const int* __restrict__ lpos, const int* __restrict__ rpos,
-                       const double* __restrict__ xu, double* __restrict__ fgp_rec, double* __restrict__ Gnd) {
+                       const double* __restrict__ xu, int exact, double* __restrict__ fgp_rec,
+                       double* __restrict__ Gnd) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nfgp) return;
     const FoldDev fd = folds[fgp_fold[q]];
@@ -363,13 +379,40 @@ __global__ void k_fold(int nfgp, const FoldDev* __restrict__ folds, const int* _
         for (int d = 0; d < 3; ++d) { F[6 * side + d] = a1[d]; F[6 * side + 3 + d] = a2[d]; }
     }
     double grad[12];
-    const double theta = fold_angle_grad(F, fd.tx[i], fd.ty[i], grad);
+    const double tx = fd.tx[i], ty = fd.ty[i];
+    const double theta = fold_angle_grad(F, tx, ty, grad);
     const double th0 = fd.theta0 ? fd.theta0[i] : fd.Ang0;
     const double W = fd.w[i];
-    fgp_rec[2 * q] = W * fd.Kt;
-    fgp_rec[2 * q + 1] = W * (fd.Kt * (theta - th0));
+    double* rec = fgp_rec + static_cast<int64_t>(FGP_REC) * q;
+    const double Wm = W * (fd.Kt * (theta - th0));
+    rec[0] = W * fd.Kt;
+    rec[1] = Wm;
+    if (exact && fabs(Wm) > 0.0) {
+        // Constraints.h:402-416: central differences of the analytic gradient,
+        // step 1e-6 max(1, |F|_inf), then symmetrised
+        double fmax = 0.0;
+        for (int k = 0; k < 12; ++k) fmax = fmax > fabs(F[k]) ? fmax : fabs(F[k]);
+        const double hs = 1e-6 * (fmax > 1.0 ? fmax : 1.0);
+        double* H = rec + 2;
+        for (int k = 0; k < 12; ++k) {
+            double Fp[12], Fm[12], gp[12], gm[12];
+            for (int r = 0; r < 12; ++r) { Fp[r] = F[r]; Fm[r] = F[r]; }
+            Fp[k] += hs; Fm[k] -= hs;
+            fold_angle_grad(Fp, tx, ty, gp);
+            fold_angle_grad(Fm, tx, ty, gm);
+            for (int r = 0; r < 12; ++r) H[12 * r + k] = (gp[r] - gm[r]) / (2.0 * hs);
+        }
+        for (int r = 0; r < 12; ++r)
+            for (int k = r + 1; k < 12; ++k) {
+                const double m = 0.5 * (H[12 * r + k] + H[12 * k + r]);
+                H[12 * r + k] = m; H[12 * k + r] = m;
+            }
+    } else {
+        for (int k = 0; k < 144; ++k) rec[2 + k] = 0.0;
+    }
     const int nb0 = fgp_noff[q], nb1 = fgp_noff[q + 1];
-    for (int k = nb0; k < nb1; ++k) { Gnd[3 * k] = 0.0; Gnd[3 * k + 1] = 0.0; Gnd[3 * k + 2] = 0.0; }
+    for (int k = nb0; k < nb1; ++k)
+        for (int e = 0; e < FND_REC; ++e) Gnd[static_cast<int64_t>(FND_REC) * k + e] = 0.0;
     for (int side = 0; side < 2; ++side) {
         const double* sf = side ? fd.rsf : fd.lsf;
         const int* of = side ? fd.roff : fd.loff;
@@ -378,9 +421,11 @@ __global__ void k_fold(int nfgp, const FoldDev* __restrict__ folds, const int* _
         const int blk = 6 * side;
         for (int k = of[i]; k < of[i + 1]; ++k) {
             const double dX = sf[np + k], dY = sf[2 * np + k];
-            double* G = Gnd + 3 * static_cast<int64_t>(nb0 + pos[k - of[i]]);
+            double* G = Gnd + static_cast<int64_t>(FND_REC) * (nb0 + pos[k - of[i]]);
 #pragma unroll
             for (int d = 0; d < 3; ++d) G[d] += grad[blk + d] * dX + grad[blk + d + 3] * dY;
+            G[3 + 2 * side] += dX;
+            G[4 + 2 * side] += dY;
         }
     }
 }
@@ -392,7 +437,7 @@ __global__ void k_reduce_K(int64_t nblocks, const int64_t* __restrict__ kc_off, 
                            const double* __restrict__ scr, const int64_t* __restrict__ kf_off,
                            const int* __restrict__ kf_idx, const int* __restrict__ nd_gp,
                            const double* __restrict__ fgp_rec, const double* __restrict__ Gnd, int folds,
-                           const int64_t* __restrict__ base, const int* __restrict__ stride,
+                           int exact, const int64_t* __restrict__ base, const int* __restrict__ stride,
                            double* __restrict__ K) {
     const int64_t blk = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (blk >= nblocks) return;
@@ -412,13 +457,32 @@ __global__ void k_reduce_K(int64_t nblocks, const int64_t* __restrict__ kc_off, 
         for (int k = 0; k < 9; ++k) f[k] = 0.0;
         for (int64_t a = kf_off[blk]; a < kf_off[blk + 1]; ++a) {
             const int lo = kf_idx[2 * a], hi = kf_idx[2 * a + 1];
-            const double wk = fgp_rec[2 * nd_gp[lo]];
-            const double* Gl = Gnd + 3 * static_cast<int64_t>(lo);
-            const double* Gh = Gnd + 3 * static_cast<int64_t>(hi);
+            const double* rec = fgp_rec + static_cast<int64_t>(FGP_REC) * nd_gp[lo];
+            const double wk = rec[0];
+            const double* Gl = Gnd + static_cast<int64_t>(FND_REC) * lo;
+            const double* Gh = Gnd + static_cast<int64_t>(FND_REC) * hi;
 #pragma unroll
             for (int c = 0; c < 3; ++c)
 #pragma unroll
                 for (int r = 0; r < 3; ++r) f[3 * c + r] += Gl[r] * Gh[c] * wk;
+            if (exact) {
+                // W moment * B(lo, r)^T H B(hi, c); B(node, d) has dX_L, dY_L, dX_R, dY_R
+                // at positions d, d + 3, 6 + d, 9 + d of the 12-vector
+                const double wm = rec[1];
+                const double* H = rec + 2;
+                const double bl[4] = {Gl[3], Gl[4], Gl[5], Gl[6]};
+                const double bh[4] = {Gh[3], Gh[4], Gh[5], Gh[6]};
+                for (int c = 0; c < 3; ++c) {
+                    const int qc[4] = {c, c + 3, 6 + c, 9 + c};
+                    for (int r = 0; r < 3; ++r) {
+                        const int pr[4] = {r, r + 3, 6 + r, 9 + r};
+                        double acc = 0.0;
+                        for (int u = 0; u < 4; ++u)
+                            for (int v = 0; v < 4; ++v) acc += bl[u] * H[12 * pr[u] + qc[v]] * bh[v];
+                        f[3 * c + r] += wm * acc;
+                    }
+                }
+            }
         }
 #pragma unroll
         for (int k = 0; k < 9; ++k) v[k] += f[k];
@@ -445,16 +509,17 @@ __global__ void k_reduce_F(int nn, const int* __restrict__ fc_off, const int* __
     if (folds) {
         for (int a = ff_off[i]; a < ff_off[i + 1]; ++a) {
             const int k = ff_idx[a];
-            const double wm = fgp_rec[2 * nd_gp[k] + 1];
-            f0 += Gnd[3 * k] * wm; f1 += Gnd[3 * k + 1] * wm; f2 += Gnd[3 * k + 2] * wm;
+            const double wm = fgp_rec[static_cast<int64_t>(FGP_REC) * nd_gp[k] + 1];
+            const double* G = Gnd + static_cast<int64_t>(FND_REC) * k;
+            f0 += G[0] * wm; f1 += G[1] * wm; f2 += G[2] * wm;
         }
     }
     F[3 * i] = f0; F[3 * i + 1] = f1; F[3 * i + 2] = f2;
 }
 
-size_t tile_lds_bytes(int max_touched, int lut_dim) {
-    return sizeof(double) * (static_cast<size_t>(max_touched) * 9 + NBMAX * NBS + 6 * lut_dim + 16)
-         + sizeof(uint16_t) * static_cast<size_t>(lut_dim) * lut_dim;
+size_t tile_lds_bytes(int rec_cap) {
+    return sizeof(double) * (static_cast<size_t>(rec_cap) * RS + TILE_GPS * GS + UMAX * 3)
+         + sizeof(uint32_t) * UMAX + TILE_GPS * UMAX;
 }
 
 void launch_assembly(Context& c, int flags, const double* xdev, double* Fdev, double* Kdev) {
@@ -465,19 +530,19 @@ void launch_assembly(Context& c, int flags, const double* xdev, double* Fdev, do
     Domain& fd = *fdp;
     const bool tangent = (flags & CVM_TANGENT) != 0;
     const bool folds = (flags & CVM_ELASTIC_FOLDS) != 0 && P.nfgp > 0;
-    const int lut_dim = P.lut_dim;
+    const bool exact = folds && tangent && (flags & CVM_EXACT_HINGE) != 0;
     MatC m;
     std::memcpy(m.A, c.A, sizeof(m.A));
     std::memcpy(m.B, c.B, sizeof(m.B));
     std::memcpy(m.D, c.D, sizeof(m.D));
     c.d_fscr.alloc(std::max<int64_t>(3 * P.nfscratch, 1));
     if (tangent) c.d_scr.alloc(std::max<int64_t>(9 * P.nscratch, 1));
-    const size_t lds = tile_lds_bytes(tangent ? P.max_touched : 0, lut_dim);
+    const size_t lds = tile_lds_bytes(P.rec_cap);
     int slot = timing_begin(c, tangent ? "tile_residual_tangent" : "tile_residual");
     hipLaunchKernelGGL(k_tile, dim3(P.ntiles), dim3(TILE_THREADS), lds, c.stream, P.d_tile_gp.p, P.d_tile_uoff.p,
                        P.d_tile_unodes.p, P.d_tile_toff.p, P.d_tile_tpairs.p, fd.off.p, P.d_pair_uloc.p, fd.sf.p,
-                       static_cast<int64_t>(fd.npairs), fd.d_w.p, xdev, m, tangent ? P.max_touched : 0, lut_dim,
-                       tangent ? 1 : 0, c.d_scr.p, c.d_fscr.p);
+                       static_cast<int64_t>(fd.npairs), fd.d_w.p, xdev, m, P.rec_cap, tangent ? 1 : 0, c.d_scr.p,
+                       c.d_fscr.p);
     CVM_HIP(hipGetLastError());
     timing_end(c, slot);
     if (folds) {
@@ -493,13 +558,13 @@ void launch_assembly(Context& c, int flags, const double* xdev, double* Fdev, do
             v.Kt = fo.Kt; v.Ang0 = fo.Ang0;
         }
         c.d_folddev.upload(reinterpret_cast<const unsigned char*>(fdv.data()), fdv.size() * sizeof(FoldDev), c.stream);
-        c.d_fold_gp.alloc(2 * P.nfgp);
-        c.d_fold_nd.alloc(3 * std::max<int64_t>(P.nfnodes, 1));
+        c.d_fold_gp.alloc(static_cast<size_t>(FGP_REC) * P.nfgp);
+        c.d_fold_nd.alloc(static_cast<size_t>(FND_REC) * std::max<int64_t>(P.nfnodes, 1));
         slot = timing_begin(c, "fold_hinge");
         hipLaunchKernelGGL(k_fold, dim3((P.nfgp + 63) / 64), dim3(64), 0, c.stream, P.nfgp,
                            reinterpret_cast<const FoldDev*>(c.d_folddev.p), P.d_fgp_fold.p, P.d_fgp_local.p,
                            P.d_fgp_noff.p, P.d_fgp_lpo.p, P.d_fgp_rpo.p, P.d_fgp_lpos.p, P.d_fgp_rpos.p, xdev,
-                           c.d_fold_gp.p, c.d_fold_nd.p);
+                           exact ? 1 : 0, c.d_fold_gp.p, c.d_fold_nd.p);
         CVM_HIP(hipGetLastError());
         timing_end(c, slot);
     }
@@ -514,7 +579,8 @@ void launch_assembly(Context& c, int flags, const double* xdev, double* Fdev, do
         slot = timing_begin(c, "reduce_K");
         hipLaunchKernelGGL(k_reduce_K, dim3(static_cast<unsigned>((P.nblocks + 255) / 256)), dim3(256), 0, c.stream,
                            P.nblocks, P.d_kc_off.p, P.d_kc_idx.p, c.d_scr.p, P.d_kf_off.p, P.d_kf_idx.p, P.d_fnd_gp.p,
-                           c.d_fold_gp.p, c.d_fold_nd.p, folds ? 1 : 0, uni ? P.d_blk_base_u.p : P.d_blk_base_f.p,
+                           c.d_fold_gp.p, c.d_fold_nd.p, folds ? 1 : 0, exact ? 1 : 0,
+                           uni ? P.d_blk_base_u.p : P.d_blk_base_f.p,
                            uni ? P.d_blk_stride_u.p : P.d_blk_stride_f.p, Kdev);
         CVM_HIP(hipGetLastError());
         timing_end(c, slot);

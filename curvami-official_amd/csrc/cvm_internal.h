@@ -146,7 +146,7 @@ struct Plan {
     DBuf<int> d_kf_idx;                 // per contribution: node-set entries of lo and hi
     DBuf<int> d_ff_off;                 // per node: fold contributions (nodeset entry indices)
     DBuf<int> d_ff_idx;
-    int max_touched = 0, max_union = 0, lut_dim = 0;
+    int max_touched = 0, max_union = 0, rec_cap = 0;
     int face_ng = 0;
     int64_t face_npairs = 0;
     std::vector<std::vector<int>> face_rows;   // face pattern rows per column (csrMapping)
@@ -203,7 +203,9 @@ void launch_node_signs(Context& c);
 void launch_frnn(Context& c, Domain& d, double rho);
 void launch_rkpm(Context& c, Domain& d, double rho, int order, int kernel, int* code, int* nbad, int* first);
 void launch_assembly(Context& c, int flags, const double* xdev, double* Fdev, double* Kdev);
-size_t tile_lds_bytes(int max_touched, int lut_dim);
+size_t tile_lds_bytes(int rec_cap);
+constexpr int kTileGps = 16;   // GPs per assembly tile
+constexpr int kTileUnion = 64; // union nodes per assembly tile
 
 // timing helpers
 int timing_begin(Context& c, const char* name);

@@ -58,21 +58,21 @@ void build_plan(Context& c) {
     int nmax = 0;
     for (int g = 0; g < ng; ++g) nmax = std::max(nmax, off[g + 1] - off[g]);
     if (nmax > 64) throw Error(CVM_E_INVALID, "a Gauss point has more than 64 neighbours");
-    if (P.union_max > 255) P.union_max = 255;
-    if (P.union_max < nmax) P.union_max = nmax;
+    P.union_max = kTileUnion;
 
     // ---- tiles and touched pairs (tile size shrinks until the LDS fits) -----
     std::vector<int> tile_gp, tile_uoff, unodes, tile_toff;
     std::vector<uint8_t> uloc(ids.size());
     std::vector<uint16_t> tpairs;
     std::vector<int> stamp(nn, -1), where(nn, -1);
-    for (int tg = P.tile_max_gps;; tg = std::max(1, tg / 2)) {
+    for (int tg = std::min(P.tile_max_gps, kTileGps);; tg = std::max(1, tg / 2)) {
         tile_gp.assign(1, 0); tile_uoff.assign(1, 0); unodes.clear(); tile_toff.assign(1, 0); tpairs.clear();
         std::fill(stamp.begin(), stamp.end(), -1);
         int t = 0, g = 0;
         std::vector<int> cur;
-        std::vector<uint8_t> mark;
-        int max_touched = 0, max_union = 0;
+        std::vector<uint32_t> pres;
+        std::vector<std::pair<uint32_t, uint16_t>> tlist;
+        int max_touched = 0, max_union = 0, rec_cap = 0;
         while (g < ng) {
             cur.clear();
             const int g0 = g;
@@ -93,33 +93,38 @@ void build_plan(Context& c) {
             unodes.insert(unodes.end(), cur.begin(), cur.end());
             tile_gp.push_back(g);
             tile_uoff.push_back(static_cast<int>(unodes.size()));
-            // touched pairs of the tile, in (ua, ub) order
+            // touched pairs of the tile: (ua, ub) coupled by at least one GP; sorted by
+            // the number of coupling GPs (descending), then by the presence mask, so that
+            // the lanes of a wave walk equally long, mostly identical GP sequences
             const int U = static_cast<int>(cur.size());
-            mark.assign(static_cast<size_t>(U) * U, 0);
-            for (int q = g0; q < g; ++q) {
-                const int b = off[q], n = off[q + 1] - b;
-                for (int i = 0; i < n; ++i)
-                    for (int j = i; j < n; ++j) {
-                        int a = uloc[b + i], bb = uloc[b + j];
-                        if (a > bb) std::swap(a, bb);
-                        mark[a * U + bb] = 1;
-                    }
-            }
-            const size_t t0 = tpairs.size();
+            pres.assign(U, 0u);
+            for (int q = g0; q < g; ++q)
+                for (int k = off[q]; k < off[q + 1]; ++k) pres[uloc[k]] |= 1u << (q - g0);
+            tlist.clear();
             for (int a = 0; a < U; ++a)
-                for (int bb = a; bb < U; ++bb)
-                    if (mark[a * U + bb]) tpairs.push_back(static_cast<uint16_t>((a << 8) | bb));
+                for (int bb = a; bb < U; ++bb) {
+                    const uint32_t m = pres[a] & pres[bb];
+                    if (m) tlist.push_back({m, static_cast<uint16_t>((a << 8) | bb)});
+                }
+            std::stable_sort(tlist.begin(), tlist.end(), [](const auto& x, const auto& y) {
+                const int cx = __builtin_popcount(x.first), cy = __builtin_popcount(y.first);
+                if (cx != cy) return cx > cy;
+                return x.first < y.first;
+            });
+            const size_t t0 = tpairs.size();
+            for (const auto& e : tlist) tpairs.push_back(e.second);
             tile_toff.push_back(static_cast<int>(tpairs.size()));
             max_touched = std::max(max_touched, static_cast<int>(tpairs.size() - t0));
             max_union = std::max(max_union, U);
+            rec_cap = std::max(rec_cap, off[g] - off[g0]);
             ++t;
         }
         P.ntiles = t;
         P.max_touched = max_touched;
         P.max_union = max_union;
-        P.lut_dim = std::max(max_union, 1);
+        P.rec_cap = std::max(rec_cap, 1);
         P.tile_max_gps = tg;
-        if (tile_lds_bytes(max_touched, P.lut_dim) <= c.lds_limit) break;
+        if (tile_lds_bytes(P.rec_cap) <= c.lds_limit) break;
         if (tg == 1) throw Error(CVM_E_INVALID, "assembly tile of one Gauss point exceeds the LDS limit");
     }
     // face blocks from the touched pairs

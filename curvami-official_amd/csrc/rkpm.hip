@@ -330,7 +330,7 @@ void launch_rkpm(Context& c, Domain& d, double rho, int order, int kernel, int* 
     hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, c.stream, st.p);
     *code = CVM_OK; *nbad = 0; *first = 0;
     if (d.ng > 0) {
-        const int slot = timing_begin(c, "rkpm");
+        const int slot = timing_begin(c, d.kind == CVM_DOMAIN_FACE ? "rkpm_face" : "rkpm_line");
         if (kernel == CVM_KERNEL_QUARTIC) {
             if (order == 1) run<1, CVM_KERNEL_QUARTIC>(c, d, rho, st.p);
             else if (order == 2) run<2, CVM_KERNEL_QUARTIC>(c, d, rho, st.p);

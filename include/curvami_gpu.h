@@ -58,7 +58,9 @@ enum {
     CVM_RESIDUAL = 1,        /* compute F_int */
     CVM_TANGENT = 2,         /* compute K values */
     CVM_ELASTIC_FOLDS = 4,   /* add elastic hinges (Meshless.h:118-126, i.e. flagRig == false) */
-    CVM_DEVICE_PTRS = 8      /* x, F_int and K_values are device pointers */
+    CVM_DEVICE_PTRS = 8,     /* x, F_int and K_values are device pointers */
+    CVM_EXACT_HINGE = 16     /* add the FD dihedral-angle Hessian term to the hinge tangent
+                                (Meshless::fold_exact_tangent, Meshless.h:48-50; Constraints.h:394-416) */
 };
 
 /* ---- context ------------------------------------------------------------ */

@@ -158,3 +158,22 @@ def test_pivot_collapse_reported_like_reference():
     assert ei.value.code == G.E_PIVOT_COLLAPSE
     assert (ei.value.nbad, ei.value.first) == (nb, fb)
     assert "pivot collapsed" in str(ei.value)
+
+
+@pytest.mark.parametrize("case", ["c1", "arc"])
+def test_exact_hinge_tangent(case, request):
+    """fold_exact_tangent: the FD dihedral-angle Hessian term (Constraints.h:394-416)."""
+    sh, orc, gpu = request.getfixturevalue(case)
+    if case == "c1":
+        x = W.rigid_fold_displacement(sh, 0.5, seed=3, sigma=1e-3)
+    else:
+        x = W.displacement(sh.nn, 2, 2e-2)
+    _, _, Kr, Fr = orc.assemble(x, flag_rig=False, exact_tangent=True)
+    _, _, Kr0, _ = orc.assemble(x, flag_rig=False, exact_tangent=False)
+    Fg, Kg = gpu.assemble(x, elastic_folds=True, exact_hinge=True)
+    hess_part = np.abs(Kr - Kr0).max()
+    assert hess_part > 0.0
+    ek = np.abs(Kg - Kr).max()
+    print(f"{case}: Hessian term max {hess_part:.3e}, |dK| {ek:.3e} (max|K| {np.abs(Kr).max():.3e})")
+    assert ek <= 1e-12 * np.abs(Kr).max() + 1e-6 * hess_part
+    assert np.abs(Fg - Fr).max() <= 1e-10 * np.abs(Fr).max()
