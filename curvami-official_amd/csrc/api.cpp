@@ -172,6 +172,7 @@ int cvm_set_nodes(cvm_context* ctx, int nn, const double* X, const double* Y) {
     c.X.upload(c.hX, c.stream);
     c.Y.upload(c.hY, c.stream);
     c.plan.ready = false;
+    c.node_sign_creases = -1;
     for (cvm::Domain* d : c.doms) { d->npairs = -1; d->have_sf = false; }
     CVM_HIP(hipStreamSynchronize(c.stream));
     CVM_CATCH(ctx)
@@ -207,7 +208,8 @@ int cvm_add_crease(cvm_context* ctx, int nv, const double* vx, const double* vy,
     cr.closed = is_closed ? 1 : 0;
     ctx->c.creases.push_back(cr);
     ctx->c.creases_dirty = true;
-    for (cvm::Domain* d : ctx->c.doms) d->have_sf = false;
+    ctx->c.node_sign_creases = -1;
+    for (cvm::Domain* d : ctx->c.doms) { d->have_sf = false; d->ind_creases = -1; }
     CVM_CATCH(ctx)
 }
 

@@ -56,7 +56,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // values; (3) the tile's residual per union node, again in GP order.
 __global__ __launch_bounds__(TILE_THREADS) void k_tile(
     const int* __restrict__ tile_gp, const int* __restrict__ tile_uoff, const int* __restrict__ tile_unodes,
-    const int* __restrict__ tile_toff, const uint16_t* __restrict__ tpairs,
+    const int* __restrict__ tile_toff, const uint16_t* __restrict__ tpairs, const int* __restrict__ slot_dst,
     const int* __restrict__ off, const uint8_t* __restrict__ uloc,
     const double* __restrict__ sf, int64_t npairs, const double* __restrict__ w,
     const double* __restrict__ xu, MatC mat, int rec_cap, int with_tangent,
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
                         acc[3 * c + r] += T0[r] * a1[c] + T1[r] * a2[c] + xv[r] * bv[c] + bv[r] * yv[c]
                                           + (r == c ? wg : 0.0);
             }
-            double* dst = scr + 9 * static_cast<int64_t>(tb + sl);
+            double* dst = scr + 9 * static_cast<int64_t>(slot_dst[tb + sl]);
 #pragma unroll
             for (int k = 0; k < 9; ++k) dst[k] = acc[k];
         }
@@ -433,7 +433,7 @@ __global__ void k_fold(int nfgp, const FoldDev* __restrict__ folds, const int* _
 // ---------------------------------------------------------------------------
 // Segmented reductions (one thread per K block / per node).
 // ---------------------------------------------------------------------------
-__global__ void k_reduce_K(int64_t nblocks, const int64_t* __restrict__ kc_off, const int* __restrict__ kc_idx,
+__global__ void k_reduce_K(int64_t nblocks, const int64_t* __restrict__ kc_off,
                            const double* __restrict__ scr, const int64_t* __restrict__ kf_off,
                            const int* __restrict__ kf_idx, const int* __restrict__ nd_gp,
                            const double* __restrict__ fgp_rec, const double* __restrict__ Gnd, int folds,
@@ -447,7 +447,7 @@ __global__ void k_reduce_K(int64_t nblocks, const int64_t* __restrict__ kc_off, 
 #pragma unroll
     for (int k = 0; k < 9; ++k) v[k] = 0.0;
     for (int64_t a = kc_off[blk]; a < kc_off[blk + 1]; ++a) {
-        const double* s = scr + 9 * static_cast<int64_t>(kc_idx[a]);
+        const double* s = scr + 9 * a;
 #pragma unroll
         for (int k = 0; k < 9; ++k) v[k] += s[k];
     }
@@ -540,7 +540,7 @@ void launch_assembly(Context& c, int flags, const double* xdev, double* Fdev, do
     const size_t lds = tile_lds_bytes(P.rec_cap);
     int slot = timing_begin(c, tangent ? "tile_residual_tangent" : "tile_residual");
     hipLaunchKernelGGL(k_tile, dim3(P.ntiles), dim3(TILE_THREADS), lds, c.stream, P.d_tile_gp.p, P.d_tile_uoff.p,
-                       P.d_tile_unodes.p, P.d_tile_toff.p, P.d_tile_tpairs.p, fd.off.p, P.d_pair_uloc.p, fd.sf.p,
+                       P.d_tile_unodes.p, P.d_tile_toff.p, P.d_tile_tpairs.p, P.d_slot_dst.p, fd.off.p, P.d_pair_uloc.p, fd.sf.p,
                        static_cast<int64_t>(fd.npairs), fd.d_w.p, xdev, m, P.rec_cap, tangent ? 1 : 0, c.d_scr.p,
                        c.d_fscr.p);
     CVM_HIP(hipGetLastError());
@@ -578,7 +578,7 @@ void launch_assembly(Context& c, int flags, const double* xdev, double* Fdev, do
         const bool uni = (flags & CVM_ELASTIC_FOLDS) != 0;
         slot = timing_begin(c, "reduce_K");
         hipLaunchKernelGGL(k_reduce_K, dim3(static_cast<unsigned>((P.nblocks + 255) / 256)), dim3(256), 0, c.stream,
-                           P.nblocks, P.d_kc_off.p, P.d_kc_idx.p, c.d_scr.p, P.d_kf_off.p, P.d_kf_idx.p, P.d_fnd_gp.p,
+                           P.nblocks, P.d_kc_off.p, c.d_scr.p, P.d_kf_off.p, P.d_kf_idx.p, P.d_fnd_gp.p,
                            c.d_fold_gp.p, c.d_fold_nd.p, folds ? 1 : 0, exact ? 1 : 0,
                            uni ? P.d_blk_base_u.p : P.d_blk_base_f.p,
                            uni ? P.d_blk_stride_u.p : P.d_blk_stride_f.p, Kdev);

@@ -88,8 +88,9 @@ struct Domain {
     DBuf<double> sf;          // 6 * npairs
     int order = 0, kernel = 0;
     bool have_sf = false;
-    // per-GP crease indicator data: 6 doubles per (crease, GP)
+    // per-GP crease indicator data: 6 doubles per (crease, GP); valid for `ind_creases` creases
     DBuf<double> ind;
+    int ind_creases = -1;
 };
 
 struct Fold {
@@ -125,9 +126,10 @@ struct Plan {
     DBuf<uint8_t> d_pair_uloc;  // per face pair: union-local index
     int64_t nscratch = 0;       // total touched pairs
     int64_t nfscratch = 0;      // total union entries
-    // K contributors: per block, scratch indices (ascending tile order)
+    // K contributors: per block, a contiguous segment of scratch rows (ascending
+    // tile order); slot_dst maps a tile's touched slot to its row
     DBuf<int64_t> d_kc_off;
-    DBuf<int> d_kc_idx;
+    DBuf<int> d_slot_dst;
     // F contributors: per node, fscratch indices
     DBuf<int> d_fc_off;
     DBuf<int> d_fc_idx;
@@ -178,6 +180,7 @@ struct Context {
     DBuf<int> d_cinfo;          // per crease: offset, count, ext_s, ext_e, closed
     DBuf<int8_t> d_node_sign;   // per (crease, node): +1 / -1
     bool creases_dirty = true;
+    int node_sign_creases = -1;   // crease count the node signs were computed for (-1: stale)
     std::vector<Domain*> doms;
     std::vector<Fold> folds;
     Plan plan;

@@ -124,6 +124,8 @@ static void upload_creases(Context& c) {
 void launch_node_signs(Context& c) {
     upload_creases(c);
     const int nc = static_cast<int>(c.creases.size());
+    if (c.node_sign_creases == nc) return;        // unchanged nodes and creases
+    c.node_sign_creases = nc;
     c.d_node_sign.alloc(std::max<size_t>(1, static_cast<size_t>(nc) * c.nn));
     if (nc == 0 || c.nn == 0) return;
     const int slot = timing_begin(c, "node_sign");
@@ -136,6 +138,8 @@ void launch_node_signs(Context& c) {
 void launch_indicator_gp(Context& c, Domain& d) {
     upload_creases(c);
     const int nc = static_cast<int>(c.creases.size());
+    if (d.ind_creases == nc) return;              // unchanged GPs and creases
+    d.ind_creases = nc;
     d.ind.alloc(std::max<size_t>(1, static_cast<size_t>(nc) * 6 * d.ng));
     if (nc == 0 || d.ng == 0) return;
     const int slot = timing_begin(c, "indicator_gp");

@@ -1,0 +1,45 @@
+// HBM calibration probe for the profiler's FETCH_SIZE / WRITE_SIZE counters:
+// copies `mib` MiB (default 1024) from one device buffer to another with a plain
+// vector-load/vector-store kernel, `reps` times. Each dispatch moves exactly
+// mib MiB in and mib MiB out, so the counters' per-dispatch readings divided by
+// that size give the scale factor used to turn counter values into bytes.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+
+__global__ void k_copy(const double2* __restrict__ a, double2* __restrict__ b, size_t n) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
+}
+
+int main(int argc, char** argv) {
+    const size_t mib = argc > 1 ? std::strtoull(argv[1], nullptr, 10) : 1024;
+    const int reps = argc > 2 ? std::atoi(argv[2]) : 5;
+    const size_t bytes = mib << 20, n = bytes / sizeof(double2);
+    double2 *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+        std::fprintf(stderr, "hipMalloc failed\n");
+        return 1;
+    }
+    if (hipMemset(a, 0, bytes) != hipSuccess) return 1;
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    float best = 1e30f;
+    for (int r = 0; r < reps; ++r) {
+        hipEventRecord(e0);
+        hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, 0, a, b, n);
+        hipEventRecord(e1);
+        hipEventSynchronize(e1);
+        float ms = 0;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) best = ms;
+    }
+    if (hipGetLastError() != hipSuccess) return 1;
+    std::printf("hbm_probe: %zu MiB copy, best %.3f ms, %.1f GB/s (read + write)\n", mib, best,
+                2.0 * bytes / (best * 1e-3) / 1e9);
+    hipFree(a);
+    hipFree(b);
+    return 0;
+}

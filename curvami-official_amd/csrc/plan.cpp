@@ -244,10 +244,13 @@ void build_plan(Context& c) {
         }
     }
     for (int64_t b = 0; b < P.nblocks; ++b) kc_cnt[b + 1] += kc_cnt[b];
-    std::vector<int> kc_idx(kc_cnt[P.nblocks]);
+    // block-major scratch: the tile kernel writes touched slot s straight to row
+    // slot_dst[s], the position of its contribution in its block's segment; the
+    // segments are in ascending tile order, i.e. GP order
+    std::vector<int> slot_dst(tpairs.size());
     {
         std::vector<int64_t> fillp(kc_cnt.begin(), kc_cnt.end() - 1);
-        for (size_t s = 0; s < tpairs.size(); ++s) kc_idx[fillp[tblk[s]]++] = static_cast<int>(s);
+        for (size_t s = 0; s < tpairs.size(); ++s) slot_dst[s] = static_cast<int>(fillp[tblk[s]]++);
     }
     if (P.nscratch > 0x7fffffffLL) throw Error(CVM_E_INVALID, "assembly scratch exceeds 2^31 rows");
 
@@ -320,7 +323,7 @@ void build_plan(Context& c) {
     P.d_tile_tpairs.upload(tpairs, s);
     P.d_pair_uloc.upload(uloc, s);
     P.d_kc_off.upload(kc_cnt, s);
-    P.d_kc_idx.upload(kc_idx, s);
+    P.d_slot_dst.upload(slot_dst, s);
     P.d_fc_off.upload(fc_cnt, s);
     P.d_fc_idx.upload(fc_idx, s);
     if (P.nfgp) {

@@ -30,16 +30,14 @@ struct RK {
     static constexpr int NP = (P + 1) * (P + 2) / 2;
     static constexpr int DM = 2 * P;                              // max monomial degree
     static constexpr int NMON = (DM + 1) * (DM + 2) / 2;
+    static constexpr int NCJ = 6 * NP;                            // c and its 5 derivatives
 };
 
-// basis term k: degree ascending, then p_y ascending (GlobalRKPM.h:137-150)
-__host__ __device__ constexpr int deg_of(int k) {
-    int d = 0;
-    while ((d + 1) * (d + 2) / 2 <= k) ++d;
-    return d;
-}
-__host__ __device__ constexpr int bex(int k) { return deg_of(k) - (k - deg_of(k) * (deg_of(k) + 1) / 2); }
-__host__ __device__ constexpr int bey(int k) { return k - deg_of(k) * (deg_of(k) + 1) / 2; }
+// Basis term k (degree ascending, then p_y ascending, GlobalRKPM.h:137-150)
+// for orders up to 3: exponents of x and y.
+__constant__ constexpr int kBEX[10] = {0, 1, 0, 2, 1, 0, 3, 2, 1, 0};
+__constant__ constexpr int kBEY[10] = {0, 0, 1, 0, 1, 2, 0, 1, 2, 3};
+template <int K> struct BE { static constexpr int x = kBEX[K], y = kBEY[K]; };
 __host__ __device__ constexpr int midx(int a, int b) { return (a + b) * (a + b + 1) / 2 + b; }
 
 template <int KER>
@@ -106,195 +104,235 @@ __device__ __forceinline__ Jet pair_jet(double xi, double eta, double si, int I,
     return j;
 }
 
+// Compile-time loop helper: f(Int<0>), ..., f(Int<N-1>).
+template <int V> struct Int { static constexpr int value = V; };
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(Int<B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+constexpr int RKPM_THREADS = 64;
+
 template <int P, int KER>
-__global__ __launch_bounds__(128) void k_rkpm(
+__global__ __launch_bounds__(RKPM_THREADS) void k_rkpm(
     const double* __restrict__ X, const double* __restrict__ Y, int nn,
     const double* __restrict__ gx, const double* __restrict__ gy, int ng,
     const int* __restrict__ off, const int* __restrict__ ids, const double* __restrict__ sarr,
     const double* __restrict__ ind, const int8_t* __restrict__ nsign, int nc, double rho,
     double* __restrict__ sf, int64_t npairs, int* __restrict__ status /* n1, first1, n2, first2 */) {
     using R = RK<P>;
-    constexpr int NP = R::NP, NMON = R::NMON, DM = R::DM;
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ng) return;
-    const double px = gx[g], py = gy[g];
-    const int b = off[g], e = off[g + 1];
+    constexpr int NP = R::NP, NMON = R::NMON, DM = R::DM, NCJ = R::NCJ;
+    __shared__ double cj_s[RKPM_THREADS * NCJ];
+    __shared__ int off_s[RKPM_THREADS + 1];
+    const int g0 = blockIdx.x * RKPM_THREADS;
+    const int nblk = min(RKPM_THREADS, ng - g0);
+    const int g = g0 + threadIdx.x;
+    // block's pair range: off_s[0..nblk] (nblk + 1 entries, nblk <= blockDim)
+    if (threadIdx.x < nblk) off_s[threadIdx.x] = off[g0 + threadIdx.x];
+    if (threadIdx.x == 0) off_s[nblk] = off[g0 + nblk];
 
-    // ---- moments of monomials -------------------------------------------
-    double mu[6][NMON];
+    if (threadIdx.x < nblk) {
+        const double px = gx[g], py = gy[g];
+        const int b = off[g], e = off[g + 1];
+        // ---- moments of monomials ---------------------------------------------
+        double mu[6][NMON];
+        static_for<0, NMON>([&](auto m) {
 #pragma unroll
-    for (int j = 0; j < 6; ++j)
+            for (int j = 0; j < 6; ++j) mu[j][m.value] = 0.0;
+        });
+        for (int k = b; k < e; ++k) {
+            const int I = ids[k];
+            const double xi = (px - X[I]) / rho;
+            const double eta = (py - Y[I]) / rho;
+            const Jet jt = pair_jet<KER>(xi, eta, sarr[k], I, g, ng, nn, ind, nsign, nc, rho);
+            double xp[DM + 1], yp[DM + 1];
+            xp[0] = 1.0; yp[0] = 1.0;
 #pragma unroll
-        for (int m = 0; m < NMON; ++m) mu[j][m] = 0.0;
-    for (int k = b; k < e; ++k) {
-        const int I = ids[k];
-        const double xi = (px - X[I]) / rho;
-        const double eta = (py - Y[I]) / rho;
-        const Jet jt = pair_jet<KER>(xi, eta, sarr[k], I, g, ng, nn, ind, nsign, nc, rho);
-        double xp[DM + 1], yp[DM + 1];
-        xp[0] = 1.0; yp[0] = 1.0;
+            for (int q = 1; q <= DM; ++q) { xp[q] = xp[q - 1] * xi; yp[q] = yp[q - 1] * eta; }
 #pragma unroll
-        for (int q = 1; q <= DM; ++q) { xp[q] = xp[q - 1] * xi; yp[q] = yp[q - 1] * eta; }
+            for (int d = 0; d <= DM; ++d)
 #pragma unroll
-        for (int d = 0; d <= DM; ++d)
-#pragma unroll
-            for (int bb = 0; bb <= d; ++bb) {
-                const int m = midx(d - bb, bb);
-                const double mon = xp[d - bb] * yp[bb];
-                mu[0][m] += jt.v * mon;
-                mu[1][m] += jt.x * mon;
-                mu[2][m] += jt.y * mon;
-                mu[3][m] += jt.xx * mon;
-                mu[4][m] += jt.yy * mon;
-                mu[5][m] += jt.xy * mon;
-            }
-    }
-    auto MU = [&](int j, int a, int bb) -> double { return (a < 0 || bb < 0) ? 0.0 : mu[j][midx(a, bb)]; };
-    // Entries of M and its derivatives from the monomial moments.
-    auto Mv = [&](int i, int j) { return MU(0, bex(i) + bex(j), bey(i) + bey(j)); };
-    auto Mxv = [&](int i, int j) {
-        const int a = bex(i) + bex(j), c = bey(i) + bey(j);
-        return MU(1, a, c) + a * MU(0, a - 1, c);
-    };
-    auto Myv = [&](int i, int j) {
-        const int a = bex(i) + bex(j), c = bey(i) + bey(j);
-        return MU(2, a, c) + c * MU(0, a, c - 1);
-    };
-    auto Mxxv = [&](int i, int j) {
-        const int a = bex(i) + bex(j), c = bey(i) + bey(j);
-        return MU(3, a, c) + 2 * a * MU(1, a - 1, c) + a * (a - 1) * MU(0, a - 2, c);
-    };
-    auto Myyv = [&](int i, int j) {
-        const int a = bex(i) + bex(j), c = bey(i) + bey(j);
-        return MU(4, a, c) + 2 * c * MU(2, a, c - 1) + c * (c - 1) * MU(0, a, c - 2);
-    };
-    auto Mxyv = [&](int i, int j) {
-        const int a = bex(i) + bex(j), c = bey(i) + bey(j);
-        return MU(5, a, c) + a * MU(2, a - 1, c) + c * MU(1, a, c - 1) + a * c * MU(0, a - 1, c - 1);
-    };
-
-    // ---- singularity tests (GlobalRKPM.h:214-220 and :303-305) ----------
-    int flag = 0;
-    {
-        double m[3][3];
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) m[r][c] = Mv(r, c);
-        auto cof = [&](int i, int j) {
-            const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
-            return m[i1][j1] * m[i2][j2] - m[i1][j2] * m[i2][j1];
+                for (int bb = 0; bb <= d; ++bb) {
+                    const int m = midx(d - bb, bb);
+                    const double mon = xp[d - bb] * yp[bb];
+                    mu[0][m] += jt.v * mon;
+                    mu[1][m] += jt.x * mon;
+                    mu[2][m] += jt.y * mon;
+                    mu[3][m] += jt.xx * mon;
+                    mu[4][m] += jt.yy * mon;
+                    mu[5][m] += jt.xy * mon;
+                }
+        }
+        // Entries of M and its derivatives from the monomial moments (product rule
+        // on x^a y^b), all indices compile-time constants.
+        auto MU = [&](int j, int a, int c) -> double { return (a < 0 || c < 0) ? 0.0 : mu[j][midx(a, c)]; };
+        auto Mv = [&](auto I, auto J) { return MU(0, BE<I.value>::x + BE<J.value>::x, BE<I.value>::y + BE<J.value>::y); };
+        auto Mxv = [&](auto I, auto J) {
+            constexpr int a = BE<I.value>::x + BE<J.value>::x, c = BE<I.value>::y + BE<J.value>::y;
+            return MU(1, a, c) + a * MU(0, a - 1, c);
         };
-        const double c0 = cof(0, 0), c1 = cof(1, 0), c2 = cof(2, 0);
-        const double det = c0 * m[0][0] + c1 * m[1][0] + c2 * m[2][0];
-        const double id = 1.0 / det;
-        double inv[3][3];
-        inv[0][0] = c0 * id; inv[0][1] = c1 * id; inv[0][2] = c2 * id;
-        inv[1][0] = cof(0, 1) * id; inv[1][1] = cof(1, 1) * id; inv[1][2] = cof(2, 1) * id;
-        inv[2][0] = cof(0, 2) * id; inv[2][1] = cof(1, 2) * id; inv[2][2] = cof(2, 2) * id;
-        bool finite = true;
-        double rmax = 0.0;
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                finite = finite && isfinite(inv[r][c]);
-                const double pr = m[r][0] * inv[0][c] + m[r][1] * inv[1][c] + m[r][2] * inv[2][c];
-                rmax = fmax(rmax, fabs(pr - (r == c ? 1.0 : 0.0)));
-            }
-        if (!finite || !(rmax <= 1e-8)) flag |= 1;
-    }
+        auto Myv = [&](auto I, auto J) {
+            constexpr int a = BE<I.value>::x + BE<J.value>::x, c = BE<I.value>::y + BE<J.value>::y;
+            return MU(2, a, c) + c * MU(0, a, c - 1);
+        };
+        auto Mxxv = [&](auto I, auto J) {
+            constexpr int a = BE<I.value>::x + BE<J.value>::x, c = BE<I.value>::y + BE<J.value>::y;
+            return MU(3, a, c) + 2 * a * MU(1, a - 1, c) + a * (a - 1) * MU(0, a - 2, c);
+        };
+        auto Myyv = [&](auto I, auto J) {
+            constexpr int a = BE<I.value>::x + BE<J.value>::x, c = BE<I.value>::y + BE<J.value>::y;
+            return MU(4, a, c) + 2 * c * MU(2, a, c - 1) + c * (c - 1) * MU(0, a, c - 2);
+        };
+        auto Mxyv = [&](auto I, auto J) {
+            constexpr int a = BE<I.value>::x + BE<J.value>::x, c = BE<I.value>::y + BE<J.value>::y;
+            return MU(5, a, c) + a * MU(2, a - 1, c) + c * MU(1, a, c - 1) + a * c * MU(0, a - 1, c - 1);
+        };
 
-    // ---- L D L^T of M ------------------------------------------------------
-    double L[NP][NP], Dg[NP];
+        // ---- singularity tests (GlobalRKPM.h:214-220 and :303-305) --------------
+        int flag = 0;
+        {
+            double m[3][3];
+            static_for<0, 3>([&](auto r) { static_for<0, 3>([&](auto c) { m[r.value][c.value] = Mv(r, c); }); });
+            auto cof = [&](int i, int j) {
+                const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+                return m[i1][j1] * m[i2][j2] - m[i1][j2] * m[i2][j1];
+            };
+            const double c0 = cof(0, 0), c1 = cof(1, 0), c2 = cof(2, 0);
+            const double det = c0 * m[0][0] + c1 * m[1][0] + c2 * m[2][0];
+            const double id = 1.0 / det;
+            double inv[3][3];
+            inv[0][0] = c0 * id; inv[0][1] = c1 * id; inv[0][2] = c2 * id;
+            inv[1][0] = cof(0, 1) * id; inv[1][1] = cof(1, 1) * id; inv[1][2] = cof(2, 1) * id;
+            inv[2][0] = cof(0, 2) * id; inv[2][1] = cof(1, 2) * id; inv[2][2] = cof(2, 2) * id;
+            bool finite = true;
+            double rmax = 0.0;
 #pragma unroll
-    for (int k = 0; k < NP; ++k) {
-        double dk = Mv(k, k);
+            for (int r = 0; r < 3; ++r)
 #pragma unroll
-        for (int j = 0; j < k; ++j) dk -= L[k][j] * L[k][j] * Dg[j];
-        Dg[k] = dk;
-        if (k >= 3 && !(fabs(dk) > 1e-12 * fmax(fabs(Mv(k, k)), 1.0))) flag |= 2;
-        const double idk = 1.0 / dk;
+                for (int c = 0; c < 3; ++c) {
+                    finite = finite && isfinite(inv[r][c]);
+                    const double pr = m[r][0] * inv[0][c] + m[r][1] * inv[1][c] + m[r][2] * inv[2][c];
+                    rmax = fmax(rmax, fabs(pr - (r == c ? 1.0 : 0.0)));
+                }
+            if (!finite || !(rmax <= 1e-8)) flag |= 1;
+        }
+
+        // ---- L D L^T of M, then c = M^-1 e0 and its derivatives -----------------
+        double L[NP][NP], Dg[NP];
+        static_for<0, NP>([&](auto K) {
+            constexpr int k = K.value;
+            double dk = Mv(K, K);
 #pragma unroll
-        for (int i = k + 1; i < NP; ++i) {
-            double v = Mv(i, k);
+            for (int j = 0; j < k; ++j) dk -= L[k][j] * L[k][j] * Dg[j];
+            Dg[k] = dk;
+            if (k >= 3 && !(fabs(dk) > 1e-12 * fmax(fabs(Mv(K, K)), 1.0))) flag |= 2;
+            const double idk = 1.0 / dk;
+            static_for<k + 1, NP>([&](auto I) {
+                constexpr int i = I.value;
+                double v = Mv(I, K);
 #pragma unroll
-            for (int j = 0; j < k; ++j) v -= L[i][j] * L[k][j] * Dg[j];
-            L[i][k] = v * idk;
+                for (int j = 0; j < k; ++j) v -= L[i][j] * L[k][j] * Dg[j];
+                L[i][k] = v * idk;
+            });
+        });
+        auto solve = [&](double (&r)[NP]) {
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+#pragma unroll
+                for (int j = 0; j < i; ++j) r[i] -= L[i][j] * r[j];
+#pragma unroll
+            for (int i = 0; i < NP; ++i) r[i] /= Dg[i];
+#pragma unroll
+            for (int i = NP - 1; i >= 0; --i)
+#pragma unroll
+                for (int j = i + 1; j < NP; ++j) r[i] -= L[j][i] * r[j];
+        };
+        double c[NP], cx[NP], cy[NP], txx[NP], tyy[NP], txy[NP];
+#pragma unroll
+        for (int i = 0; i < NP; ++i) c[i] = (i == 0) ? 1.0 : 0.0;
+        solve(c);
+        // second-order moment products with c first, so that mu_3..5 die early
+        static_for<0, NP>([&](auto I) {
+            double sxx = 0.0, syy = 0.0, sxy = 0.0, sx = 0.0, sy = 0.0;
+            static_for<0, NP>([&](auto J) {
+                sxx += Mxxv(I, J) * c[J.value];
+                syy += Myyv(I, J) * c[J.value];
+                sxy += Mxyv(I, J) * c[J.value];
+                sx += Mxv(I, J) * c[J.value];
+                sy += Myv(I, J) * c[J.value];
+            });
+            txx[I.value] = sxx; tyy[I.value] = syy; txy[I.value] = sxy;
+            cx[I.value] = -sx; cy[I.value] = -sy;
+        });
+        solve(cx);
+        solve(cy);
+        static_for<0, NP>([&](auto I) {
+            double sxx = 0.0, syy = 0.0, sxy = 0.0;
+            static_for<0, NP>([&](auto J) {
+                sxx += 2.0 * Mxv(I, J) * cx[J.value];
+                syy += 2.0 * Myv(I, J) * cy[J.value];
+                sxy += Mxv(I, J) * cy[J.value] + Myv(I, J) * cx[J.value];
+            });
+            txx[I.value] = -(txx[I.value] + sxx);
+            tyy[I.value] = -(tyy[I.value] + syy);
+            txy[I.value] = -(txy[I.value] + sxy);
+        });
+        solve(txx);
+        solve(tyy);
+        solve(txy);
+        double* cj = cj_s + threadIdx.x * NCJ;
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            cj[i] = c[i]; cj[NP + i] = cx[i]; cj[2 * NP + i] = cy[i];
+            cj[3 * NP + i] = txx[i]; cj[4 * NP + i] = tyy[i]; cj[5 * NP + i] = txy[i];
+        }
+        if (flag) {
+            if (flag & 1) { atomicAdd(&status[0], 1); atomicMin(&status[1], g); }
+            if (flag & 2) { atomicAdd(&status[2], 1); atomicMin(&status[3], g); }
         }
     }
-    auto solve = [&](double r[NP]) {
-#pragma unroll
-        for (int i = 0; i < NP; ++i)
-#pragma unroll
-            for (int j = 0; j < i; ++j) r[i] -= L[i][j] * r[j];
-#pragma unroll
-        for (int i = 0; i < NP; ++i) r[i] /= Dg[i];
-#pragma unroll
-        for (int i = NP - 1; i >= 0; --i)
-#pragma unroll
-            for (int j = i + 1; j < NP; ++j) r[i] -= L[j][i] * r[j];
-    };
-    double c[NP], cx[NP], cy[NP], cxx[NP], cyy[NP], cxy[NP];
-#pragma unroll
-    for (int i = 0; i < NP; ++i) c[i] = (i == 0) ? 1.0 : 0.0;
-    solve(c);
-#pragma unroll
-    for (int i = 0; i < NP; ++i) {
-        double sx = 0.0, sy = 0.0;
-#pragma unroll
-        for (int j = 0; j < NP; ++j) { sx += Mxv(i, j) * c[j]; sy += Myv(i, j) * c[j]; }
-        cx[i] = -sx; cy[i] = -sy;
-    }
-    solve(cx);
-    solve(cy);
-#pragma unroll
-    for (int i = 0; i < NP; ++i) {
-        double sxx = 0.0, syy = 0.0, sxy = 0.0;
-#pragma unroll
-        for (int j = 0; j < NP; ++j) {
-            sxx += Mxxv(i, j) * c[j] + 2.0 * Mxv(i, j) * cx[j];
-            syy += Myyv(i, j) * c[j] + 2.0 * Myv(i, j) * cy[j];
-            sxy += Mxyv(i, j) * c[j] + Mxv(i, j) * cy[j] + Myv(i, j) * cx[j];
+    __syncthreads();
+
+    // ---- Psi, grad Psi, Hessian of Psi (GlobalRKPM.h:363-397), one lane per pair
+    // over the block's contiguous pair range (coalesced stores) ---------------------
+    const int p0 = off_s[0], p1 = off_s[nblk];
+    for (int k = p0 + threadIdx.x; k < p1; k += RKPM_THREADS) {
+        int lo = 0, hi = nblk;                 // GP of pair k: off_s[lo] <= k < off_s[lo + 1]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (off_s[mid] <= k) lo = mid; else hi = mid;
         }
-        cxx[i] = -sxx; cyy[i] = -syy; cxy[i] = -sxy;
-    }
-    solve(cxx);
-    solve(cyy);
-    solve(cxy);
-
-    if (flag) {
-        if (flag & 1) { atomicAdd(&status[0], 1); atomicMin(&status[1], g); }
-        if (flag & 2) { atomicAdd(&status[2], 1); atomicMin(&status[3], g); }
-    }
-
-    // ---- Psi, grad Psi, Hessian of Psi (GlobalRKPM.h:363-397) ------------
-    for (int k = b; k < e; ++k) {
+        const int gg = g0 + lo;
+        const double* cj = cj_s + lo * NCJ;
         const int I = ids[k];
-        const double xi = (px - X[I]) / rho;
-        const double eta = (py - Y[I]) / rho;
-        const Jet jt = pair_jet<KER>(xi, eta, sarr[k], I, g, ng, nn, ind, nsign, nc, rho);
+        const double xi = (gx[gg] - X[I]) / rho;
+        const double eta = (gy[gg] - Y[I]) / rho;
+        const Jet jt = pair_jet<KER>(xi, eta, sarr[k], I, gg, ng, nn, ind, nsign, nc, rho);
         double xp[P + 1], yp[P + 1];
         xp[0] = 1.0; yp[0] = 1.0;
 #pragma unroll
         for (int q = 1; q <= P; ++q) { xp[q] = xp[q - 1] * xi; yp[q] = yp[q - 1] * eta; }
         double C = 0, Cx = 0, Cy = 0, Cxx = 0, Cyy = 0, Cxy = 0;
-#pragma unroll
-        for (int t = 0; t < NP; ++t) {
-            const int ax = bex(t), ay = bey(t);
+        static_for<0, NP>([&](auto T) {
+            constexpr int t = T.value, ax = BE<t>::x, ay = BE<t>::y;
             const double p = xp[ax] * yp[ay];
             const double pxv = ax > 0 ? ax * xp[ax > 0 ? ax - 1 : 0] * yp[ay] : 0.0;
             const double pyv = ay > 0 ? ay * xp[ax] * yp[ay > 0 ? ay - 1 : 0] : 0.0;
             const double pxx = ax > 1 ? ax * (ax - 1) * xp[ax > 1 ? ax - 2 : 0] * yp[ay] : 0.0;
             const double pyy = ay > 1 ? ay * (ay - 1) * xp[ax] * yp[ay > 1 ? ay - 2 : 0] : 0.0;
             const double pxy = (ax > 0 && ay > 0) ? ax * ay * xp[ax > 0 ? ax - 1 : 0] * yp[ay > 0 ? ay - 1 : 0] : 0.0;
-            C += c[t] * p;
-            Cx += cx[t] * p + c[t] * pxv;
-            Cy += cy[t] * p + c[t] * pyv;
-            Cxx += cxx[t] * p + 2.0 * cx[t] * pxv + c[t] * pxx;
-            Cyy += cyy[t] * p + 2.0 * cy[t] * pyv + c[t] * pyy;
-            Cxy += cxy[t] * p + cx[t] * pyv + cy[t] * pxv + c[t] * pxy;
-        }
+            const double v = cj[t], vx = cj[NP + t], vy = cj[2 * NP + t];
+            const double vxx = cj[3 * NP + t], vyy = cj[4 * NP + t], vxy = cj[5 * NP + t];
+            C += v * p;
+            Cx += vx * p + v * pxv;
+            Cy += vy * p + v * pyv;
+            Cxx += vxx * p + 2.0 * vx * pxv + v * pxx;
+            Cyy += vyy * p + 2.0 * vy * pyv + v * pyy;
+            Cxy += vxy * p + vx * pyv + vy * pxv + v * pxy;
+        });
         const double r2 = rho * rho;
         sf[k] = C * jt.v;
         sf[npairs + k] = (Cx * jt.v + C * jt.x) / rho;
@@ -312,7 +350,8 @@ __global__ void k_status_init(int* s) {
 template <int P, int KER>
 static void run(Context& c, Domain& d, double rho, int* st) {
     const int nc = static_cast<int>(c.creases.size());
-    hipLaunchKernelGGL((k_rkpm<P, KER>), dim3((d.ng + 127) / 128), dim3(128), 0, c.stream, c.X.p, c.Y.p, c.nn,
+    hipLaunchKernelGGL((k_rkpm<P, KER>), dim3((d.ng + RKPM_THREADS - 1) / RKPM_THREADS), dim3(RKPM_THREADS), 0,
+                       c.stream, c.X.p, c.Y.p, c.nn,
                        d.gx.p, d.gy.p, d.ng, d.off.p, d.ids.p, d.s.p, d.ind.p, c.d_node_sign.p, nc, rho,
                        d.sf.p, static_cast<int64_t>(d.npairs), st);
 }
