@@ -26,21 +26,20 @@
 
 namespace cvm {
 
-constexpr int TILE_THREADS = 1024;   // 16 waves: one per GP of the tile in the preparation phase
+constexpr int TILE_THREADS = 512;    // 8 waves; wave w prepares GPs w and w + 8 of the tile
 constexpr int TILE_GPS = 16;         // GPs per tile (presence masks are 16-bit)
 constexpr int UMAX = 64;             // union nodes per tile (one lane per neighbour)
-constexpr int RS = 18;               // doubles per neighbour record
-constexpr int GS = 20;               // doubles per GP record
+constexpr int RS = 9;                // doubles per neighbour record (SoA planes)
+constexpr int GS = 17;               // doubles per GP record (SoA over the tile's GPs)
 
 struct MatC {
     double A[9], B[9], D[9];
 };
 
-// neighbour record: grad Psi, beta (bending B-row factor), Tu, g = F0 C0^-1 grad Psi,
-// B beta, D beta, and the residual contribution f_local
-enum { R_PX = 0, R_PY, R_BE0, R_BE1, R_BE2, R_TU, R_G0, R_G1, R_G2, R_BB0, R_BB1, R_BB2,
-       R_DB0, R_DB1, R_DB2, R_F0, R_F1, R_F2 };
-// GP record: weight, F0 columns, normal, b.v_curv, membrane resultant N, C0^-1
+// neighbour record planes: grad Psi, beta (bending B-row factor,
+// Assembly.h:240-247), Tu (:265-267) and the residual contribution f_local (:249)
+enum { R_PX = 0, R_PY, R_BE0, R_BE1, R_BE2, R_TU, R_F0, R_F1, R_F2 };
+// GP record planes: weight, F0 columns, normal, b.v_curv, membrane resultant N, C0^-1
 enum { G_W = 0, G_A1 = 1, G_A2 = 4, G_B = 7, G_BDV = 10, G_N = 11, G_CI = 14 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -54,6 +53,10 @@ __device__ __forceinline__ double wave_sum(double v) {
 // blocks (lo, hi) of the tile and sums the GPs that couple them, in ascending GP
 // order, from the records of the two nodes: no LDS accumulation, no atomics on
 // values; (3) the tile's residual per union node, again in GP order.
+// The records hold only grad Psi, beta and Tu: g = F0 C0^-1 grad Psi and
+// D beta (and B beta when B != 0) are recomputed per pair from the GP's
+// F0, C0^-1 and the laminate, which keeps the LDS traffic per pair low.
+template <bool HAS_B>
 __global__ __launch_bounds__(TILE_THREADS) void k_tile(
     const int* __restrict__ tile_gp, const int* __restrict__ tile_uoff, const int* __restrict__ tile_unodes,
     const int* __restrict__ tile_toff, const uint16_t* __restrict__ tpairs, const int* __restrict__ slot_dst,
@@ -69,26 +72,28 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
     const int tb = tile_toff[t], T = tile_toff[t + 1] - tb;
     const int pb = off[g0];
 
-    double* rec = smem;                                         // rec_cap * RS
-    double* gs = rec + static_cast<size_t>(rec_cap) * RS;       // TILE_GPS * GS
+    double* rec = smem;                                         // RS planes of rec_cap
+    double* gs = rec + static_cast<size_t>(rec_cap) * RS;       // GS planes of TILE_GPS
     double* uc = gs + TILE_GPS * GS;                            // UMAX * 3
     uint32_t* pm = reinterpret_cast<uint32_t*>(uc + UMAX * 3);  // UMAX presence masks
     uint8_t* loc = reinterpret_cast<uint8_t*>(pm + UMAX);       // TILE_GPS * UMAX
+    auto R = [&](int f, int r) -> double& { return rec[static_cast<size_t>(f) * rec_cap + r]; };
+    auto Gp = [&](int f, int wv) -> double& { return gs[f * TILE_GPS + wv]; };
 
     if (tid < U) {
         const int v = tile_unodes[ub + tid];
         uc[3 * tid] = xu[3 * v]; uc[3 * tid + 1] = xu[3 * v + 1]; uc[3 * tid + 2] = xu[3 * v + 2];
         pm[tid] = 0u;
     }
-    loc[tid] = 0xff;   // TILE_GPS * UMAX == TILE_THREADS
+    for (int k = tid; k < TILE_GPS * UMAX; k += TILE_THREADS) loc[k] = 0xff;
     __syncthreads();
 
     const double* A = mat.A;
     const double* B = mat.B;
     const double* D = mat.D;
-    {
-        const int wv = tid >> 6, lane = tid & 63;
-        if (wv < ngp) {
+    for (int wv = tid >> 6; wv < ngp; wv += TILE_THREADS / 64) {
+        const int lane = tid & 63;
+        {
             const int g = g0 + wv;
             const int b = off[g], n = off[g + 1] - b;
             const int rb = b - pb;
@@ -124,10 +129,12 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
             double N[3], M[3];
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
-                N[r] = A[3 * r] * e0[0] + A[3 * r + 1] * e0[1] + A[3 * r + 2] * e0[2]
-                     + B[3 * r] * e1[0] + B[3 * r + 1] * e1[1] + B[3 * r + 2] * e1[2];
-                M[r] = B[r] * e0[0] + B[3 + r] * e0[1] + B[6 + r] * e0[2]
-                     + D[3 * r] * e1[0] + D[3 * r + 1] * e1[1] + D[3 * r + 2] * e1[2];
+                N[r] = A[3 * r] * e0[0] + A[3 * r + 1] * e0[1] + A[3 * r + 2] * e0[2];
+                M[r] = D[3 * r] * e1[0] + D[3 * r + 1] * e1[1] + D[3 * r + 2] * e1[2];
+                if (HAS_B) {
+                    N[r] += B[3 * r] * e1[0] + B[3 * r + 1] * e1[1] + B[3 * r + 2] * e1[2];
+                    M[r] += B[r] * e0[0] + B[3 + r] * e0[1] + B[6 + r] * e0[2];
+                }
             }
             const double W = w[g];
             // V = F0 C0^-1 (Assembly.h:217), v_curv, p_XY, b.v (:221-224)
@@ -146,34 +153,27 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
                 const double SYY = ryy0 * gI0 + ryy1 * gI1 + ryy2 * gI2;
                 const double SXY = rxy0 * gI0 + rxy1 * gI1 + rxy2 * gI2;
                 const double be0 = -hxx + SXX, be1 = -hyy + SYY, be2 = 2.0 * (-hxy + SXY);
-                double* r = rec + static_cast<size_t>(rb + lane) * RS;
+                const int r = rb + lane;
                 // f_local = (Bm^T N + Bb^T M) W (:249-250)
                 const double ca = px * N[0] + py * N[2], cb = py * N[1] + px * N[2];
                 const double cm = be0 * M[0] + be1 * M[1] + be2 * M[2];
-                r[R_F0] = W * (a1x * ca + a2x * cb + bx * cm);
-                r[R_F1] = W * (a1y * ca + a2y * cb + by * cm);
-                r[R_F2] = W * (a1z * ca + a2z * cb + bz * cm);
-                r[R_PX] = px; r[R_PY] = py;
-                r[R_BE0] = be0; r[R_BE1] = be1; r[R_BE2] = be2;
-                r[R_TU] = (px * pX + py * pY) - (hxx * M[0] + 2.0 * hxy * M[2] + hyy * M[1]);
-                r[R_G0] = gI0; r[R_G1] = gI1; r[R_G2] = gI2;
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    r[R_BB0 + q] = B[3 * q] * be0 + B[3 * q + 1] * be1 + B[3 * q + 2] * be2;
-                    r[R_DB0 + q] = D[3 * q] * be0 + D[3 * q + 1] * be1 + D[3 * q + 2] * be2;
-                }
+                R(R_F0, r) = W * (a1x * ca + a2x * cb + bx * cm);
+                R(R_F1, r) = W * (a1y * ca + a2y * cb + by * cm);
+                R(R_F2, r) = W * (a1z * ca + a2z * cb + bz * cm);
+                R(R_PX, r) = px; R(R_PY, r) = py;
+                R(R_BE0, r) = be0; R(R_BE1, r) = be1; R(R_BE2, r) = be2;
+                R(R_TU, r) = (px * pX + py * pY) - (hxx * M[0] + 2.0 * hxy * M[2] + hyy * M[1]);
                 loc[wv * UMAX + ul] = static_cast<uint8_t>(lane);
                 atomicOr(&pm[ul], 1u << wv);
             }
             if (lane == 0) {
-                double* G = gs + wv * GS;
-                G[G_W] = W;
-                G[G_A1] = a1x; G[G_A1 + 1] = a1y; G[G_A1 + 2] = a1z;
-                G[G_A2] = a2x; G[G_A2 + 1] = a2y; G[G_A2 + 2] = a2z;
-                G[G_B] = bx; G[G_B + 1] = by; G[G_B + 2] = bz;
-                G[G_BDV] = bdv;
-                G[G_N] = N[0]; G[G_N + 1] = N[1]; G[G_N + 2] = N[2];
-                G[G_CI] = Ci00; G[G_CI + 1] = Ci01; G[G_CI + 2] = Ci11;
+                Gp(G_W, wv) = W;
+                Gp(G_A1, wv) = a1x; Gp(G_A1 + 1, wv) = a1y; Gp(G_A1 + 2, wv) = a1z;
+                Gp(G_A2, wv) = a2x; Gp(G_A2 + 1, wv) = a2y; Gp(G_A2 + 2, wv) = a2z;
+                Gp(G_B, wv) = bx; Gp(G_B + 1, wv) = by; Gp(G_B + 2, wv) = bz;
+                Gp(G_BDV, wv) = bdv;
+                Gp(G_N, wv) = N[0]; Gp(G_N + 1, wv) = N[1]; Gp(G_N + 2, wv) = N[2];
+                Gp(G_CI, wv) = Ci00; Gp(G_CI + 1, wv) = Ci01; Gp(G_CI + 2, wv) = Ci11;
             }
         }
     }
@@ -189,36 +189,54 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
             double acc[9];
 #pragma unroll
             for (int k = 0; k < 9; ++k) acc[k] = 0.0;
-            while (mask) {
-                const int wv = __ffs(mask) - 1;
-                mask &= mask - 1;
+            // Two coupling GPs per iteration (ascending GP order is kept: the first
+            // GP's contribution is added before the second's).
+            auto pair_block = [&](int wv, double (&kb)[9]) {
                 const int base = off[g0 + wv] - pb;
-                const double* ri = rec + static_cast<size_t>(base + loc[wv * UMAX + ua]) * RS;
-                const double* rj = rec + static_cast<size_t>(base + loc[wv * UMAX + ubb]) * RS;
-                const double* G = gs + wv * GS;
-                const double W = G[G_W];
-                const double a1[3] = {G[G_A1], G[G_A1 + 1], G[G_A1 + 2]};
-                const double a2[3] = {G[G_A2], G[G_A2 + 1], G[G_A2 + 2]};
-                const double bv[3] = {G[G_B], G[G_B + 1], G[G_B + 2]};
-                const double pxi = ri[R_PX], pyi = ri[R_PY], pxj = rj[R_PX], pyj = rj[R_PY];
+                const int ri = base + loc[wv * UMAX + ua];
+                const int rj = base + loc[wv * UMAX + ubb];
+                const double W = Gp(G_W, wv);
+                const double a1[3] = {Gp(G_A1, wv), Gp(G_A1 + 1, wv), Gp(G_A1 + 2, wv)};
+                const double a2[3] = {Gp(G_A2, wv), Gp(G_A2 + 1, wv), Gp(G_A2 + 2, wv)};
+                const double bv[3] = {Gp(G_B, wv), Gp(G_B + 1, wv), Gp(G_B + 2, wv)};
+                const double ci00 = Gp(G_CI, wv), ci01 = Gp(G_CI + 1, wv), ci11 = Gp(G_CI + 2, wv);
+                const double pxi = R(R_PX, ri), pyi = R(R_PY, ri), pxj = R(R_PX, rj), pyj = R(R_PY, rj);
+                const double bi0 = R(R_BE0, ri), bi1 = R(R_BE1, ri), bi2 = R(R_BE2, ri);
+                const double bj0 = R(R_BE0, rj), bj1 = R(R_BE1, rj), bj2 = R(R_BE2, rj);
+                const double tui = R(R_TU, ri), tuj = R(R_TU, rj);
                 // A Psi_j (3x2) and S = Psi_i^T A Psi_j
                 const double ap00 = A[0] * pxj + A[2] * pyj, ap01 = A[1] * pyj + A[2] * pxj;
                 const double ap10 = A[3] * pxj + A[5] * pyj, ap11 = A[4] * pyj + A[5] * pxj;
                 const double ap20 = A[6] * pxj + A[8] * pyj, ap21 = A[7] * pyj + A[8] * pxj;
                 const double S00 = pxi * ap00 + pyi * ap20, S01 = pxi * ap01 + pyi * ap21;
                 const double S10 = pyi * ap10 + pxi * ap20, S11 = pyi * ap11 + pxi * ap21;
-                const double m0 = pxi * rj[R_BB0] + pyi * rj[R_BB2], m1 = pyi * rj[R_BB1] + pxi * rj[R_BB2];
-                const double q0 = pxj * ri[R_BB0] + pyj * ri[R_BB2], q1 = pyj * ri[R_BB1] + pxj * ri[R_BB2];
-                const double geo = pxi * (G[G_N] * pxj + G[G_N + 2] * pyj) + pyi * (G[G_N + 2] * pxj + G[G_N + 1] * pyj);
-                const double cp0 = G[G_CI] * pxj + G[G_CI + 1] * pyj, cp1 = G[G_CI + 1] * pxj + G[G_CI + 2] * pyj;
-                const double z = ri[R_BE0] * rj[R_DB0] + ri[R_BE1] * rj[R_DB1] + ri[R_BE2] * rj[R_DB2]
-                               + G[G_BDV] * (pxi * cp0 + pyi * cp1);
-                const double tui = ri[R_TU], tuj = rj[R_TU];
+                // g = F0 C0^-1 grad Psi for both nodes
+                const double ci0 = ci00 * pxi + ci01 * pyi, ci1 = ci01 * pxi + ci11 * pyi;
+                const double cj0 = ci00 * pxj + ci01 * pyj, cj1 = ci01 * pxj + ci11 * pyj;
+                // z = beta_i . D beta_j + (b.v) grad Psi_i^T C0^-1 grad Psi_j
+                const double db0 = D[0] * bj0 + D[1] * bj1 + D[2] * bj2;
+                const double db1 = D[3] * bj0 + D[4] * bj1 + D[5] * bj2;
+                const double db2 = D[6] * bj0 + D[7] * bj1 + D[8] * bj2;
+                const double z = bi0 * db0 + bi1 * db1 + bi2 * db2 + Gp(G_BDV, wv) * (pxi * cj0 + pyi * cj1);
+                const double geo = pxi * (Gp(G_N, wv) * pxj + Gp(G_N + 2, wv) * pyj)
+                                 + pyi * (Gp(G_N + 2, wv) * pxj + Gp(G_N + 1, wv) * pyj);
+                double m0 = 0.0, m1 = 0.0, q0 = 0.0, q1 = 0.0;
+                if (HAS_B) {
+                    const double bbj0 = B[0] * bj0 + B[1] * bj1 + B[2] * bj2;
+                    const double bbj1 = B[3] * bj0 + B[4] * bj1 + B[5] * bj2;
+                    const double bbj2 = B[6] * bj0 + B[7] * bj1 + B[8] * bj2;
+                    const double bbi0 = B[0] * bi0 + B[1] * bi1 + B[2] * bi2;
+                    const double bbi1 = B[3] * bi0 + B[4] * bi1 + B[5] * bi2;
+                    const double bbi2 = B[6] * bi0 + B[7] * bi1 + B[8] * bi2;
+                    m0 = pxi * bbj0 + pyi * bbj2; m1 = pyi * bbj1 + pxi * bbj2;
+                    q0 = pxj * bbi0 + pyj * bbi2; q1 = pyj * bbi1 + pxj * bbi2;
+                }
                 double xv[3], yv[3], T0[3], T1[3];
 #pragma unroll
                 for (int r = 0; r < 3; ++r) {
-                    xv[r] = W * (a1[r] * m0 + a2[r] * m1 - tui * rj[R_G0 + r]);
-                    yv[r] = W * (a1[r] * q0 + a2[r] * q1 - tuj * ri[R_G0 + r] + z * bv[r]);
+                    const double gi = a1[r] * ci0 + a2[r] * ci1, gj = a1[r] * cj0 + a2[r] * cj1;
+                    xv[r] = W * (a1[r] * m0 + a2[r] * m1 - tui * gj);
+                    yv[r] = W * (a1[r] * q0 + a2[r] * q1 - tuj * gi + z * bv[r]);
                     T0[r] = W * (a1[r] * S00 + a2[r] * S10);
                     T1[r] = W * (a1[r] * S01 + a2[r] * S11);
                 }
@@ -227,8 +245,25 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
                 for (int c = 0; c < 3; ++c)
 #pragma unroll
                     for (int r = 0; r < 3; ++r)
-                        acc[3 * c + r] += T0[r] * a1[c] + T1[r] * a2[c] + xv[r] * bv[c] + bv[r] * yv[c]
-                                          + (r == c ? wg : 0.0);
+                        kb[3 * c + r] = T0[r] * a1[c] + T1[r] * a2[c] + xv[r] * bv[c] + bv[r] * yv[c]
+                                        + (r == c ? wg : 0.0);
+            };
+            while (mask) {
+                const int w0 = __ffs(mask) - 1;
+                mask &= mask - 1;
+                double k0[9];
+                pair_block(w0, k0);
+                if (mask) {
+                    const int w1 = __ffs(mask) - 1;
+                    mask &= mask - 1;
+                    double k1[9];
+                    pair_block(w1, k1);
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) acc[k] = (acc[k] + k0[k]) + k1[k];
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) acc[k] += k0[k];
+                }
             }
             double* dst = scr + 9 * static_cast<int64_t>(slot_dst[tb + sl]);
 #pragma unroll
@@ -241,8 +276,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
         for (int wv = 0; wv < ngp; ++wv) {
             const int l = loc[wv * UMAX + tid];
             if (l == 0xff) continue;
-            const double* r = rec + static_cast<size_t>(off[g0 + wv] - pb + l) * RS;
-            f0 += r[R_F0]; f1 += r[R_F1]; f2 += r[R_F2];
+            const int r = off[g0 + wv] - pb + l;
+            f0 += R(R_F0, r); f1 += R(R_F1, r); f2 += R(R_F2, r);
         }
         double* dst = fscr + 3 * static_cast<int64_t>(ub + tid);
         dst[0] = f0; dst[1] = f1; dst[2] = f2;
@@ -539,7 +574,10 @@ void launch_assembly(Context& c, int flags, const double* xdev, double* Fdev, do
     if (tangent) c.d_scr.alloc(std::max<int64_t>(9 * P.nscratch, 1));
     const size_t lds = tile_lds_bytes(P.rec_cap);
     int slot = timing_begin(c, tangent ? "tile_residual_tangent" : "tile_residual");
-    hipLaunchKernelGGL(k_tile, dim3(P.ntiles), dim3(TILE_THREADS), lds, c.stream, P.d_tile_gp.p, P.d_tile_uoff.p,
+    const bool has_b = m.B[0] != 0.0 || m.B[1] != 0.0 || m.B[2] != 0.0 || m.B[3] != 0.0 || m.B[4] != 0.0 ||
+                       m.B[5] != 0.0 || m.B[6] != 0.0 || m.B[7] != 0.0 || m.B[8] != 0.0;
+    hipLaunchKernelGGL(has_b ? k_tile<true> : k_tile<false>, dim3(P.ntiles), dim3(TILE_THREADS), lds, c.stream,
+                       P.d_tile_gp.p, P.d_tile_uoff.p,
                        P.d_tile_unodes.p, P.d_tile_toff.p, P.d_tile_tpairs.p, P.d_slot_dst.p, fd.off.p, P.d_pair_uloc.p, fd.sf.p,
                        static_cast<int64_t>(fd.npairs), fd.d_w.p, xdev, m, P.rec_cap, tangent ? 1 : 0, c.d_scr.p,
                        c.d_fscr.p);
