@@ -29,16 +29,18 @@ namespace cvm {
 constexpr int TILE_THREADS = 512;    // 8 waves; wave w prepares GPs w and w + 8 of the tile
 constexpr int TILE_GPS = 16;         // GPs per tile (presence masks are 16-bit)
 constexpr int UMAX = 64;             // union nodes per tile (one lane per neighbour)
-constexpr int RS = 9;                // doubles per neighbour record (SoA planes)
-constexpr int GS = 17;               // doubles per GP record (SoA over the tile's GPs)
+constexpr int RS = 9;                // neighbour record planes (3 FP64 residual + 6 tangent-type)
+constexpr int GS = 17;               // GP record planes (tangent type, SoA over the tile's GPs)
 
 struct MatC {
     double A[9], B[9], D[9];
 };
 
 // neighbour record planes: grad Psi, beta (bending B-row factor,
-// Assembly.h:240-247), Tu (:265-267) and the residual contribution f_local (:249)
-enum { R_PX = 0, R_PY, R_BE0, R_BE1, R_BE2, R_TU, R_F0, R_F1, R_F2 };
+// Assembly.h:240-247), Tu (:265-267) in the tangent type TT, and the residual
+// contribution f_local (:249) in FP64
+enum { R_PX = 0, R_PY, R_BE0, R_BE1, R_BE2, R_TU };
+enum { R_F0 = 0, R_F1, R_F2 };
 // GP record planes: weight, F0 columns, normal, b.v_curv, membrane resultant N, C0^-1
 enum { G_W = 0, G_A1 = 1, G_A2 = 4, G_B = 7, G_BDV = 10, G_N = 11, G_CI = 14 };
 
@@ -56,14 +58,17 @@ __device__ __forceinline__ double wave_sum(double v) {
 // The records hold only grad Psi, beta and Tu: g = F0 C0^-1 grad Psi and
 // D beta (and B beta when B != 0) are recomputed per pair from the GP's
 // F0, C0^-1 and the laminate, which keeps the LDS traffic per pair low.
-template <bool HAS_B>
+// TT = double: the FP64 tangent; TT = float: the mixed-precision tangent (FP32
+// pair blocks and FP32 accumulation), with kinematics, resultants and the
+// residual still in FP64.
+template <bool HAS_B, typename TT>
 __global__ __launch_bounds__(TILE_THREADS) void k_tile(
     const int* __restrict__ tile_gp, const int* __restrict__ tile_uoff, const int* __restrict__ tile_unodes,
     const int* __restrict__ tile_toff, const uint16_t* __restrict__ tpairs, const int* __restrict__ slot_dst,
     const int* __restrict__ off, const uint8_t* __restrict__ uloc,
     const double* __restrict__ sf, int64_t npairs, const double* __restrict__ w,
     const double* __restrict__ xu, MatC mat, int rec_cap, int with_tangent,
-    double* __restrict__ scr, double* __restrict__ fscr) {
+    TT* __restrict__ scr, double* __restrict__ fscr) {
     extern __shared__ double smem[];
     const int t = blockIdx.x;
     const int tid = threadIdx.x;
@@ -72,13 +77,15 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
     const int tb = tile_toff[t], T = tile_toff[t + 1] - tb;
     const int pb = off[g0];
 
-    double* rec = smem;                                         // RS planes of rec_cap
-    double* gs = rec + static_cast<size_t>(rec_cap) * RS;       // GS planes of TILE_GPS
-    double* uc = gs + TILE_GPS * GS;                            // UMAX * 3
-    uint32_t* pm = reinterpret_cast<uint32_t*>(uc + UMAX * 3);  // UMAX presence masks
+    double* recf = smem;                                        // 3 FP64 planes of rec_cap
+    double* uc = recf + static_cast<size_t>(rec_cap) * 3;       // UMAX * 3
+    TT* rect = reinterpret_cast<TT*>(uc + UMAX * 3);            // 6 TT planes of rec_cap
+    TT* gs = rect + static_cast<size_t>(rec_cap) * 6;           // GS planes of TILE_GPS
+    uint32_t* pm = reinterpret_cast<uint32_t*>(gs + TILE_GPS * GS);  // UMAX presence masks
     uint8_t* loc = reinterpret_cast<uint8_t*>(pm + UMAX);       // TILE_GPS * UMAX
-    auto R = [&](int f, int r) -> double& { return rec[static_cast<size_t>(f) * rec_cap + r]; };
-    auto Gp = [&](int f, int wv) -> double& { return gs[f * TILE_GPS + wv]; };
+    auto RF = [&](int f, int r) -> double& { return recf[static_cast<size_t>(f) * rec_cap + r]; };
+    auto R = [&](int f, int r) -> TT& { return rect[static_cast<size_t>(f) * rec_cap + r]; };
+    auto Gp = [&](int f, int wv) -> TT& { return gs[f * TILE_GPS + wv]; };
 
     if (tid < U) {
         const int v = tile_unodes[ub + tid];
@@ -157,106 +164,109 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
                 // f_local = (Bm^T N + Bb^T M) W (:249-250)
                 const double ca = px * N[0] + py * N[2], cb = py * N[1] + px * N[2];
                 const double cm = be0 * M[0] + be1 * M[1] + be2 * M[2];
-                R(R_F0, r) = W * (a1x * ca + a2x * cb + bx * cm);
-                R(R_F1, r) = W * (a1y * ca + a2y * cb + by * cm);
-                R(R_F2, r) = W * (a1z * ca + a2z * cb + bz * cm);
-                R(R_PX, r) = px; R(R_PY, r) = py;
-                R(R_BE0, r) = be0; R(R_BE1, r) = be1; R(R_BE2, r) = be2;
-                R(R_TU, r) = (px * pX + py * pY) - (hxx * M[0] + 2.0 * hxy * M[2] + hyy * M[1]);
+                RF(R_F0, r) = W * (a1x * ca + a2x * cb + bx * cm);
+                RF(R_F1, r) = W * (a1y * ca + a2y * cb + by * cm);
+                RF(R_F2, r) = W * (a1z * ca + a2z * cb + bz * cm);
+                R(R_PX, r) = static_cast<TT>(px); R(R_PY, r) = static_cast<TT>(py);
+                R(R_BE0, r) = static_cast<TT>(be0); R(R_BE1, r) = static_cast<TT>(be1);
+                R(R_BE2, r) = static_cast<TT>(be2);
+                R(R_TU, r) = static_cast<TT>((px * pX + py * pY) - (hxx * M[0] + 2.0 * hxy * M[2] + hyy * M[1]));
                 loc[wv * UMAX + ul] = static_cast<uint8_t>(lane);
                 atomicOr(&pm[ul], 1u << wv);
             }
             if (lane == 0) {
-                Gp(G_W, wv) = W;
-                Gp(G_A1, wv) = a1x; Gp(G_A1 + 1, wv) = a1y; Gp(G_A1 + 2, wv) = a1z;
-                Gp(G_A2, wv) = a2x; Gp(G_A2 + 1, wv) = a2y; Gp(G_A2 + 2, wv) = a2z;
-                Gp(G_B, wv) = bx; Gp(G_B + 1, wv) = by; Gp(G_B + 2, wv) = bz;
-                Gp(G_BDV, wv) = bdv;
-                Gp(G_N, wv) = N[0]; Gp(G_N + 1, wv) = N[1]; Gp(G_N + 2, wv) = N[2];
-                Gp(G_CI, wv) = Ci00; Gp(G_CI + 1, wv) = Ci01; Gp(G_CI + 2, wv) = Ci11;
+                const double gv[GS] = {W, a1x, a1y, a1z, a2x, a2y, a2z, bx, by, bz, bdv, N[0], N[1], N[2],
+                                       Ci00, Ci01, Ci11};
+#pragma unroll
+                for (int f = 0; f < GS; ++f) Gp(f, wv) = static_cast<TT>(gv[f]);
             }
         }
     }
     __syncthreads();
 
     if (with_tangent) {
+        TT Ac[9], Bc[9], Dc[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            Ac[k] = static_cast<TT>(A[k]); Bc[k] = static_cast<TT>(B[k]); Dc[k] = static_cast<TT>(D[k]);
+        }
         // ---- pair blocks (Assembly.h:282-324) for the canonical (lo, hi) orientation:
         // K_IJ with I the neighbour entry of lo and J that of hi (K_JI = K_IJ^T).
         for (int sl = tid; sl < T; sl += TILE_THREADS) {
             const uint16_t pr = tpairs[tb + sl];
             const int ua = pr >> 8, ubb = pr & 0xff;
             uint32_t mask = pm[ua] & pm[ubb];
-            double acc[9];
+            TT acc[9];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) acc[k] = 0.0;
+            for (int k = 0; k < 9; ++k) acc[k] = TT(0);
             // Two coupling GPs per iteration (ascending GP order is kept: the first
             // GP's contribution is added before the second's).
-            auto pair_block = [&](int wv, double (&kb)[9]) {
+            auto pair_block = [&](int wv, TT (&kb)[9]) {
                 const int base = off[g0 + wv] - pb;
                 const int ri = base + loc[wv * UMAX + ua];
                 const int rj = base + loc[wv * UMAX + ubb];
-                const double W = Gp(G_W, wv);
-                const double a1[3] = {Gp(G_A1, wv), Gp(G_A1 + 1, wv), Gp(G_A1 + 2, wv)};
-                const double a2[3] = {Gp(G_A2, wv), Gp(G_A2 + 1, wv), Gp(G_A2 + 2, wv)};
-                const double bv[3] = {Gp(G_B, wv), Gp(G_B + 1, wv), Gp(G_B + 2, wv)};
-                const double ci00 = Gp(G_CI, wv), ci01 = Gp(G_CI + 1, wv), ci11 = Gp(G_CI + 2, wv);
-                const double pxi = R(R_PX, ri), pyi = R(R_PY, ri), pxj = R(R_PX, rj), pyj = R(R_PY, rj);
-                const double bi0 = R(R_BE0, ri), bi1 = R(R_BE1, ri), bi2 = R(R_BE2, ri);
-                const double bj0 = R(R_BE0, rj), bj1 = R(R_BE1, rj), bj2 = R(R_BE2, rj);
-                const double tui = R(R_TU, ri), tuj = R(R_TU, rj);
+                const TT W = Gp(G_W, wv);
+                const TT a1[3] = {Gp(G_A1, wv), Gp(G_A1 + 1, wv), Gp(G_A1 + 2, wv)};
+                const TT a2[3] = {Gp(G_A2, wv), Gp(G_A2 + 1, wv), Gp(G_A2 + 2, wv)};
+                const TT bv[3] = {Gp(G_B, wv), Gp(G_B + 1, wv), Gp(G_B + 2, wv)};
+                const TT ci00 = Gp(G_CI, wv), ci01 = Gp(G_CI + 1, wv), ci11 = Gp(G_CI + 2, wv);
+                const TT pxi = R(R_PX, ri), pyi = R(R_PY, ri), pxj = R(R_PX, rj), pyj = R(R_PY, rj);
+                const TT bi0 = R(R_BE0, ri), bi1 = R(R_BE1, ri), bi2 = R(R_BE2, ri);
+                const TT bj0 = R(R_BE0, rj), bj1 = R(R_BE1, rj), bj2 = R(R_BE2, rj);
+                const TT tui = R(R_TU, ri), tuj = R(R_TU, rj);
                 // A Psi_j (3x2) and S = Psi_i^T A Psi_j
-                const double ap00 = A[0] * pxj + A[2] * pyj, ap01 = A[1] * pyj + A[2] * pxj;
-                const double ap10 = A[3] * pxj + A[5] * pyj, ap11 = A[4] * pyj + A[5] * pxj;
-                const double ap20 = A[6] * pxj + A[8] * pyj, ap21 = A[7] * pyj + A[8] * pxj;
-                const double S00 = pxi * ap00 + pyi * ap20, S01 = pxi * ap01 + pyi * ap21;
-                const double S10 = pyi * ap10 + pxi * ap20, S11 = pyi * ap11 + pxi * ap21;
+                const TT ap00 = Ac[0] * pxj + Ac[2] * pyj, ap01 = Ac[1] * pyj + Ac[2] * pxj;
+                const TT ap10 = Ac[3] * pxj + Ac[5] * pyj, ap11 = Ac[4] * pyj + Ac[5] * pxj;
+                const TT ap20 = Ac[6] * pxj + Ac[8] * pyj, ap21 = Ac[7] * pyj + Ac[8] * pxj;
+                const TT S00 = pxi * ap00 + pyi * ap20, S01 = pxi * ap01 + pyi * ap21;
+                const TT S10 = pyi * ap10 + pxi * ap20, S11 = pyi * ap11 + pxi * ap21;
                 // g = F0 C0^-1 grad Psi for both nodes
-                const double ci0 = ci00 * pxi + ci01 * pyi, ci1 = ci01 * pxi + ci11 * pyi;
-                const double cj0 = ci00 * pxj + ci01 * pyj, cj1 = ci01 * pxj + ci11 * pyj;
+                const TT ci0 = ci00 * pxi + ci01 * pyi, ci1 = ci01 * pxi + ci11 * pyi;
+                const TT cj0 = ci00 * pxj + ci01 * pyj, cj1 = ci01 * pxj + ci11 * pyj;
                 // z = beta_i . D beta_j + (b.v) grad Psi_i^T C0^-1 grad Psi_j
-                const double db0 = D[0] * bj0 + D[1] * bj1 + D[2] * bj2;
-                const double db1 = D[3] * bj0 + D[4] * bj1 + D[5] * bj2;
-                const double db2 = D[6] * bj0 + D[7] * bj1 + D[8] * bj2;
-                const double z = bi0 * db0 + bi1 * db1 + bi2 * db2 + Gp(G_BDV, wv) * (pxi * cj0 + pyi * cj1);
-                const double geo = pxi * (Gp(G_N, wv) * pxj + Gp(G_N + 2, wv) * pyj)
+                const TT db0 = Dc[0] * bj0 + Dc[1] * bj1 + Dc[2] * bj2;
+                const TT db1 = Dc[3] * bj0 + Dc[4] * bj1 + Dc[5] * bj2;
+                const TT db2 = Dc[6] * bj0 + Dc[7] * bj1 + Dc[8] * bj2;
+                const TT z = bi0 * db0 + bi1 * db1 + bi2 * db2 + Gp(G_BDV, wv) * (pxi * cj0 + pyi * cj1);
+                const TT geo = pxi * (Gp(G_N, wv) * pxj + Gp(G_N + 2, wv) * pyj)
                                  + pyi * (Gp(G_N + 2, wv) * pxj + Gp(G_N + 1, wv) * pyj);
-                double m0 = 0.0, m1 = 0.0, q0 = 0.0, q1 = 0.0;
+                TT m0 = 0, m1 = 0, q0 = 0, q1 = 0;
                 if (HAS_B) {
-                    const double bbj0 = B[0] * bj0 + B[1] * bj1 + B[2] * bj2;
-                    const double bbj1 = B[3] * bj0 + B[4] * bj1 + B[5] * bj2;
-                    const double bbj2 = B[6] * bj0 + B[7] * bj1 + B[8] * bj2;
-                    const double bbi0 = B[0] * bi0 + B[1] * bi1 + B[2] * bi2;
-                    const double bbi1 = B[3] * bi0 + B[4] * bi1 + B[5] * bi2;
-                    const double bbi2 = B[6] * bi0 + B[7] * bi1 + B[8] * bi2;
+                    const TT bbj0 = Bc[0] * bj0 + Bc[1] * bj1 + Bc[2] * bj2;
+                    const TT bbj1 = Bc[3] * bj0 + Bc[4] * bj1 + Bc[5] * bj2;
+                    const TT bbj2 = Bc[6] * bj0 + Bc[7] * bj1 + Bc[8] * bj2;
+                    const TT bbi0 = Bc[0] * bi0 + Bc[1] * bi1 + Bc[2] * bi2;
+                    const TT bbi1 = Bc[3] * bi0 + Bc[4] * bi1 + Bc[5] * bi2;
+                    const TT bbi2 = Bc[6] * bi0 + Bc[7] * bi1 + Bc[8] * bi2;
                     m0 = pxi * bbj0 + pyi * bbj2; m1 = pyi * bbj1 + pxi * bbj2;
                     q0 = pxj * bbi0 + pyj * bbi2; q1 = pyj * bbi1 + pxj * bbi2;
                 }
-                double xv[3], yv[3], T0[3], T1[3];
+                TT xv[3], yv[3], T0[3], T1[3];
 #pragma unroll
                 for (int r = 0; r < 3; ++r) {
-                    const double gi = a1[r] * ci0 + a2[r] * ci1, gj = a1[r] * cj0 + a2[r] * cj1;
+                    const TT gi = a1[r] * ci0 + a2[r] * ci1, gj = a1[r] * cj0 + a2[r] * cj1;
                     xv[r] = W * (a1[r] * m0 + a2[r] * m1 - tui * gj);
                     yv[r] = W * (a1[r] * q0 + a2[r] * q1 - tuj * gi + z * bv[r]);
                     T0[r] = W * (a1[r] * S00 + a2[r] * S10);
                     T1[r] = W * (a1[r] * S01 + a2[r] * S11);
                 }
-                const double wg = W * geo;
+                const TT wg = W * geo;
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
 #pragma unroll
                     for (int r = 0; r < 3; ++r)
                         kb[3 * c + r] = T0[r] * a1[c] + T1[r] * a2[c] + xv[r] * bv[c] + bv[r] * yv[c]
-                                        + (r == c ? wg : 0.0);
+                                        + (r == c ? wg : TT(0));
             };
             while (mask) {
                 const int w0 = __ffs(mask) - 1;
                 mask &= mask - 1;
-                double k0[9];
+                TT k0[9];
                 pair_block(w0, k0);
                 if (mask) {
                     const int w1 = __ffs(mask) - 1;
                     mask &= mask - 1;
-                    double k1[9];
+                    TT k1[9];
                     pair_block(w1, k1);
 #pragma unroll
                     for (int k = 0; k < 9; ++k) acc[k] = (acc[k] + k0[k]) + k1[k];
@@ -265,7 +275,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
                     for (int k = 0; k < 9; ++k) acc[k] += k0[k];
                 }
             }
-            double* dst = scr + 9 * static_cast<int64_t>(slot_dst[tb + sl]);
+            TT* dst = scr + 9 * static_cast<int64_t>(slot_dst[tb + sl]);
 #pragma unroll
             for (int k = 0; k < 9; ++k) dst[k] = acc[k];
         }
@@ -277,7 +287,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
             const int l = loc[wv * UMAX + tid];
             if (l == 0xff) continue;
             const int r = off[g0 + wv] - pb + l;
-            f0 += R(R_F0, r); f1 += R(R_F1, r); f2 += R(R_F2, r);
+            f0 += RF(R_F0, r); f1 += RF(R_F1, r); f2 += RF(R_F2, r);
         }
         double* dst = fscr + 3 * static_cast<int64_t>(ub + tid);
         dst[0] = f0; dst[1] = f1; dst[2] = f2;
@@ -468,8 +478,9 @@ __global__ void k_fold(int nfgp, const FoldDev* __restrict__ folds, const int* _
 // ---------------------------------------------------------------------------
 // Segmented reductions (one thread per K block / per node).
 // ---------------------------------------------------------------------------
+template <typename TT>
 __global__ void k_reduce_K(int64_t nblocks, const int64_t* __restrict__ kc_off,
-                           const double* __restrict__ scr, const int64_t* __restrict__ kf_off,
+                           const TT* __restrict__ scr, const int64_t* __restrict__ kf_off,
                            const int* __restrict__ kf_idx, const int* __restrict__ nd_gp,
                            const double* __restrict__ fgp_rec, const double* __restrict__ Gnd, int folds,
                            int exact, const int64_t* __restrict__ base, const int* __restrict__ stride,
@@ -478,14 +489,17 @@ __global__ void k_reduce_K(int64_t nblocks, const int64_t* __restrict__ kc_off,
     if (blk >= nblocks) return;
     const int64_t bs = base[blk];
     if (bs < 0) return;
+    TT vt[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) vt[k] = TT(0);
+    for (int64_t a = kc_off[blk]; a < kc_off[blk + 1]; ++a) {
+        const TT* s = scr + 9 * a;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) vt[k] += s[k];
+    }
     double v[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) v[k] = 0.0;
-    for (int64_t a = kc_off[blk]; a < kc_off[blk + 1]; ++a) {
-        const double* s = scr + 9 * a;
-#pragma unroll
-        for (int k = 0; k < 9; ++k) v[k] += s[k];
-    }
+    for (int k = 0; k < 9; ++k) v[k] = static_cast<double>(vt[k]);
     if (folds) {
         double f[9];
 #pragma unroll
@@ -552,9 +566,10 @@ __global__ void k_reduce_F(int nn, const int* __restrict__ fc_off, const int* __
     F[3 * i] = f0; F[3 * i + 1] = f1; F[3 * i + 2] = f2;
 }
 
-size_t tile_lds_bytes(int rec_cap) {
-    return sizeof(double) * (static_cast<size_t>(rec_cap) * RS + TILE_GPS * GS + UMAX * 3)
-         + sizeof(uint32_t) * UMAX + TILE_GPS * UMAX;
+size_t tile_lds_bytes(int rec_cap, bool fp32) {
+    const size_t tt = fp32 ? sizeof(float) : sizeof(double);
+    return sizeof(double) * (static_cast<size_t>(rec_cap) * 3 + UMAX * 3)
+         + tt * (static_cast<size_t>(rec_cap) * 6 + TILE_GPS * GS) + sizeof(uint32_t) * UMAX + TILE_GPS * UMAX;
 }
 
 void launch_assembly(Context& c, int flags, const double* xdev, double* Fdev, double* Kdev) {
@@ -571,16 +586,22 @@ void launch_assembly(Context& c, int flags, const double* xdev, double* Fdev, do
     std::memcpy(m.B, c.B, sizeof(m.B));
     std::memcpy(m.D, c.D, sizeof(m.D));
     c.d_fscr.alloc(std::max<int64_t>(3 * P.nfscratch, 1));
-    if (tangent) c.d_scr.alloc(std::max<int64_t>(9 * P.nscratch, 1));
-    const size_t lds = tile_lds_bytes(P.rec_cap);
-    int slot = timing_begin(c, tangent ? "tile_residual_tangent" : "tile_residual");
+    const bool fp32 = (flags & CVM_FP32_TANGENT) != 0;
+    // scratch rows: 9 values of the tangent type (bytes held in a double buffer)
+    if (tangent) c.d_scr.alloc(std::max<int64_t>(fp32 ? (9 * P.nscratch + 1) / 2 : 9 * P.nscratch, 1));
+    const size_t lds = tile_lds_bytes(P.rec_cap, fp32);
+    int slot = timing_begin(c, !tangent ? "tile_residual" : (fp32 ? "tile_residual_tangent_fp32" : "tile_residual_tangent"));
     const bool has_b = m.B[0] != 0.0 || m.B[1] != 0.0 || m.B[2] != 0.0 || m.B[3] != 0.0 || m.B[4] != 0.0 ||
                        m.B[5] != 0.0 || m.B[6] != 0.0 || m.B[7] != 0.0 || m.B[8] != 0.0;
-    hipLaunchKernelGGL(has_b ? k_tile<true> : k_tile<false>, dim3(P.ntiles), dim3(TILE_THREADS), lds, c.stream,
-                       P.d_tile_gp.p, P.d_tile_uoff.p,
-                       P.d_tile_unodes.p, P.d_tile_toff.p, P.d_tile_tpairs.p, P.d_slot_dst.p, fd.off.p, P.d_pair_uloc.p, fd.sf.p,
-                       static_cast<int64_t>(fd.npairs), fd.d_w.p, xdev, m, P.rec_cap, tangent ? 1 : 0, c.d_scr.p,
-                       c.d_fscr.p);
+    auto tile = [&](auto kern, auto* scr) {
+        hipLaunchKernelGGL(kern, dim3(P.ntiles), dim3(TILE_THREADS), lds, c.stream, P.d_tile_gp.p, P.d_tile_uoff.p,
+                           P.d_tile_unodes.p, P.d_tile_toff.p, P.d_tile_tpairs.p, P.d_slot_dst.p, fd.off.p,
+                           P.d_pair_uloc.p, fd.sf.p, static_cast<int64_t>(fd.npairs), fd.d_w.p, xdev, m, P.rec_cap,
+                           tangent ? 1 : 0, scr, c.d_fscr.p);
+    };
+    float* scr32 = reinterpret_cast<float*>(c.d_scr.p);
+    if (fp32) tile(has_b ? k_tile<true, float> : k_tile<false, float>, scr32);
+    else tile(has_b ? k_tile<true, double> : k_tile<false, double>, c.d_scr.p);
     CVM_HIP(hipGetLastError());
     timing_end(c, slot);
     if (folds) {
@@ -615,11 +636,15 @@ void launch_assembly(Context& c, int flags, const double* xdev, double* Fdev, do
     if (tangent) {
         const bool uni = (flags & CVM_ELASTIC_FOLDS) != 0;
         slot = timing_begin(c, "reduce_K");
-        hipLaunchKernelGGL(k_reduce_K, dim3(static_cast<unsigned>((P.nblocks + 255) / 256)), dim3(256), 0, c.stream,
-                           P.nblocks, P.d_kc_off.p, c.d_scr.p, P.d_kf_off.p, P.d_kf_idx.p, P.d_fnd_gp.p,
-                           c.d_fold_gp.p, c.d_fold_nd.p, folds ? 1 : 0, exact ? 1 : 0,
-                           uni ? P.d_blk_base_u.p : P.d_blk_base_f.p,
-                           uni ? P.d_blk_stride_u.p : P.d_blk_stride_f.p, Kdev);
+        auto red = [&](auto kern, auto* scr) {
+            hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>((P.nblocks + 255) / 256)), dim3(256), 0, c.stream,
+                               P.nblocks, P.d_kc_off.p, scr, P.d_kf_off.p, P.d_kf_idx.p, P.d_fnd_gp.p,
+                               c.d_fold_gp.p, c.d_fold_nd.p, folds ? 1 : 0, exact ? 1 : 0,
+                               uni ? P.d_blk_base_u.p : P.d_blk_base_f.p,
+                               uni ? P.d_blk_stride_u.p : P.d_blk_stride_f.p, Kdev);
+        };
+        if (fp32) red(k_reduce_K<float>, scr32);
+        else red(k_reduce_K<double>, c.d_scr.p);
         CVM_HIP(hipGetLastError());
         timing_end(c, slot);
     }

@@ -206,7 +206,7 @@ void launch_node_signs(Context& c);
 void launch_frnn(Context& c, Domain& d, double rho);
 void launch_rkpm(Context& c, Domain& d, double rho, int order, int kernel, int* code, int* nbad, int* first);
 void launch_assembly(Context& c, int flags, const double* xdev, double* Fdev, double* Kdev);
-size_t tile_lds_bytes(int rec_cap);
+size_t tile_lds_bytes(int rec_cap, bool fp32);
 constexpr int kTileGps = 16;   // GPs per assembly tile
 constexpr int kTileUnion = 64; // union nodes per assembly tile
 

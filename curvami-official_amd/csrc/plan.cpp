@@ -124,7 +124,7 @@ void build_plan(Context& c) {
         P.max_union = max_union;
         P.rec_cap = std::max(rec_cap, 1);
         P.tile_max_gps = tg;
-        if (tile_lds_bytes(P.rec_cap) <= c.lds_limit) break;
+        if (tile_lds_bytes(P.rec_cap, false) <= c.lds_limit) break;
         if (tg == 1) throw Error(CVM_E_INVALID, "assembly tile of one Gauss point exceeds the LDS limit");
     }
     // face blocks from the touched pairs

@@ -25,7 +25,7 @@ _i64 = C.POINTER(C.c_int64)
 # status codes (cvm_status)
 OK, E_INVALID, E_HIP, E_SINGULAR_MOMENT, E_PIVOT_COLLAPSE, E_PATTERN_MISS, E_NO_MEMORY, E_STATE, E_BUDGET = range(9)
 FACE, EDGE, POINT, FOLD_LEFT, FOLD_RIGHT = range(5)
-RESIDUAL, TANGENT, ELASTIC_FOLDS, DEVICE_PTRS, EXACT_HINGE = 1, 2, 4, 8, 16
+RESIDUAL, TANGENT, ELASTIC_FOLDS, DEVICE_PTRS, EXACT_HINGE, FP32_TANGENT = 1, 2, 4, 8, 16, 32
 
 # every exported symbol of include/curvami_gpu.h with its signature
 SIGNATURES = {
@@ -203,7 +203,7 @@ class Plate:
         self._check(self._L.cvm_csr_mapping(self._h, gp, out.ctypes.data_as(_i), n.value, C.byref(n)))
         return out[:n.value]
 
-    def assemble(self, x, elastic_folds=True, tangent=True, exact_hinge=False):
+    def assemble(self, x, elastic_folds=True, tangent=True, exact_hinge=False, fp32_tangent=False):
         """Meshless::assembleSystem: returns (F_int, K_values) in the pattern
         returned by pattern(with_folds=elastic_folds)."""
         x = _f64(x)
@@ -211,7 +211,7 @@ class Plate:
         nnz = self.nnz_union if elastic_folds else self.nnz_face
         K = np.empty(max(nnz, 1)) if tangent else None
         flags = RESIDUAL | (TANGENT if tangent else 0) | (ELASTIC_FOLDS if elastic_folds else 0) \
-            | (EXACT_HINGE if exact_hinge else 0)
+            | (EXACT_HINGE if exact_hinge else 0) | (FP32_TANGENT if fp32_tangent else 0)
         self._check(self._L.cvm_assemble(self._h, x.ctypes.data, flags, F.ctypes.data,
                                          K.ctypes.data if K is not None else None))
         return F, (K[:nnz] if tangent else None)

@@ -59,8 +59,12 @@ enum {
     CVM_TANGENT = 2,         /* compute K values */
     CVM_ELASTIC_FOLDS = 4,   /* add elastic hinges (Meshless.h:118-126, i.e. flagRig == false) */
     CVM_DEVICE_PTRS = 8,     /* x, F_int and K_values are device pointers */
-    CVM_EXACT_HINGE = 16     /* add the FD dihedral-angle Hessian term to the hinge tangent
+    CVM_EXACT_HINGE = 16,    /* add the FD dihedral-angle Hessian term to the hinge tangent
                                 (Meshless::fold_exact_tangent, Meshless.h:48-50; Constraints.h:394-416) */
+    CVM_FP32_TANGENT = 32    /* mixed precision: face tangent blocks computed and accumulated in FP32
+                                (inexact-Newton Jacobian); shape functions, kinematics, residual and
+                                hinge stay FP64; K values are returned as doubles. No reference
+                                counterpart (the reference is FP64 throughout). */
 };
 
 /* ---- context ------------------------------------------------------------ */

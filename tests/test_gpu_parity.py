@@ -177,3 +177,21 @@ def test_exact_hinge_tangent(case, request):
     print(f"{case}: Hessian term max {hess_part:.3e}, |dK| {ek:.3e} (max|K| {np.abs(Kr).max():.3e})")
     assert ek <= 1e-12 * np.abs(Kr).max() + 1e-6 * hess_part
     assert np.abs(Fg - Fr).max() <= 1e-10 * np.abs(Fr).max()
+
+
+@pytest.mark.parametrize("case", ["c1", "arc"])
+def test_fp32_tangent(case, request):
+    """Mixed precision: the face tangent in FP32 (inexact-Newton Jacobian), the
+    residual still FP64 (equal to the FP64 path up to FMA-contraction round-off
+    of the two kernel instantiations). The tangent's error against the FP64
+    oracle is stated: <= 1e-5 of max|K|."""
+    sh, orc, gpu = request.getfixturevalue(case)
+    x = W.displacement(sh.nn, 1, 1e-3)
+    _, _, Kr, _ = orc.assemble(x, flag_rig=False)
+    F64, _ = gpu.assemble(x, elastic_folds=True)
+    F32, K32 = gpu.assemble(x, elastic_folds=True, fp32_tangent=True)
+    assert np.abs(F64 - F32).max() <= 1e-13 * np.abs(F64).max()
+    ek = np.abs(K32 - Kr).max() / np.abs(Kr).max()
+    ef = np.linalg.norm(K32 - Kr) / np.linalg.norm(Kr)
+    print(f"{case}: FP32 tangent max rel {ek:.3e}, Frobenius rel {ef:.3e}")
+    assert ek <= 1e-5 and ef <= 1e-5
