@@ -197,8 +197,14 @@ def main():
     }
     dom_name = max(cands, key=lambda k: cands[k][1])
     achieved = cands[dom_name][0]
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        tj = json.load(open(tfile))
+        if dom_name in tj and tj[dom_name].get("workload") == sh.name:
+            traffic = tj[dom_name]["bytes_per_launch"]
     roofline = {"kernel": dom_name, "bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic,
                 "precision": "fp64 (vector = matrix peak on MI355X)",
                 "launch_ms": round(cands[dom_name][1], 4)}
     out = {

@@ -66,13 +66,11 @@ void build_plan(Context& c) {
     std::vector<uint16_t> tpairs;
     std::vector<int> stamp(nn, -1), where(nn, -1);
     for (int tg = std::min(P.tile_max_gps, kTileGps);; tg = std::max(1, tg / 2)) {
-        tile_gp.assign(1, 0); tile_uoff.assign(1, 0); unodes.clear(); tile_toff.assign(1, 0); tpairs.clear();
+        tile_gp.assign(1, 0); tile_uoff.assign(1, 0); unodes.clear();
         std::fill(stamp.begin(), stamp.end(), -1);
+        // (1) tiles: greedy runs of consecutive GPs (sequential, O(pairs))
         int t = 0, g = 0;
         std::vector<int> cur;
-        std::vector<uint32_t> pres;
-        std::vector<std::pair<uint32_t, uint16_t>> tlist;
-        int max_touched = 0, max_union = 0, rec_cap = 0;
         while (g < ng) {
             cur.clear();
             const int g0 = g;
@@ -93,14 +91,23 @@ void build_plan(Context& c) {
             unodes.insert(unodes.end(), cur.begin(), cur.end());
             tile_gp.push_back(g);
             tile_uoff.push_back(static_cast<int>(unodes.size()));
-            // touched pairs of the tile: (ua, ub) coupled by at least one GP; sorted by
-            // the number of coupling GPs (descending), then by the presence mask, so that
-            // the lanes of a wave walk equally long, mostly identical GP sequences
-            const int U = static_cast<int>(cur.size());
-            pres.assign(U, 0u);
-            for (int q = g0; q < g; ++q)
-                for (int k = off[q]; k < off[q + 1]; ++k) pres[uloc[k]] |= 1u << (q - g0);
-            tlist.clear();
+            ++t;
+        }
+        P.ntiles = t;
+        // (2) touched pairs of each tile (parallel over tiles): (ua, ub) coupled by at
+        // least one GP; sorted by the number of coupling GPs (descending), then by the
+        // presence mask, so that the lanes of a wave walk equally long, mostly
+        // identical GP sequences
+        std::vector<std::vector<uint16_t>> tl(P.ntiles);
+        int max_touched = 0, max_union = 0, rec_cap = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(max : max_touched, max_union, rec_cap)
+        for (int ti = 0; ti < P.ntiles; ++ti) {
+            const int q0 = tile_gp[ti], q1 = tile_gp[ti + 1];
+            const int U = tile_uoff[ti + 1] - tile_uoff[ti];
+            std::vector<uint32_t> pres(U, 0u);
+            for (int q = q0; q < q1; ++q)
+                for (int k = off[q]; k < off[q + 1]; ++k) pres[uloc[k]] |= 1u << (q - q0);
+            std::vector<std::pair<uint32_t, uint16_t>> tlist;
             for (int a = 0; a < U; ++a)
                 for (int bb = a; bb < U; ++bb) {
                     const uint32_t m = pres[a] & pres[bb];
@@ -111,15 +118,18 @@ void build_plan(Context& c) {
                 if (cx != cy) return cx > cy;
                 return x.first < y.first;
             });
-            const size_t t0 = tpairs.size();
-            for (const auto& e : tlist) tpairs.push_back(e.second);
-            tile_toff.push_back(static_cast<int>(tpairs.size()));
-            max_touched = std::max(max_touched, static_cast<int>(tpairs.size() - t0));
+            tl[ti].resize(tlist.size());
+            for (size_t k = 0; k < tlist.size(); ++k) tl[ti][k] = tlist[k].second;
+            max_touched = std::max(max_touched, static_cast<int>(tlist.size()));
             max_union = std::max(max_union, U);
-            rec_cap = std::max(rec_cap, off[g] - off[g0]);
-            ++t;
+            rec_cap = std::max(rec_cap, off[q1] - off[q0]);
         }
-        P.ntiles = t;
+        tile_toff.assign(P.ntiles + 1, 0);
+        for (int ti = 0; ti < P.ntiles; ++ti) tile_toff[ti + 1] = tile_toff[ti] + static_cast<int>(tl[ti].size());
+        tpairs.resize(tile_toff[P.ntiles]);
+#pragma omp parallel for schedule(static)
+        for (int ti = 0; ti < P.ntiles; ++ti)
+            std::copy(tl[ti].begin(), tl[ti].end(), tpairs.begin() + tile_toff[ti]);
         P.max_touched = max_touched;
         P.max_union = max_union;
         P.rec_cap = std::max(rec_cap, 1);
@@ -127,13 +137,33 @@ void build_plan(Context& c) {
         if (tile_lds_bytes(P.rec_cap, false) <= c.lds_limit) break;
         if (tg == 1) throw Error(CVM_E_INVALID, "assembly tile of one Gauss point exceeds the LDS limit");
     }
-    // face blocks from the touched pairs
+    // face blocks: for each column node hi, the nodes lo <= hi that share a face GP
+    // with it (parallel over columns through the node -> GP incidence)
     std::vector<std::vector<int>> cols(nn);     // face + fold rows per column
-    for (int t = 0; t < P.ntiles; ++t) {
-        const int* un = unodes.data() + tile_uoff[t];
-        for (int s2 = tile_toff[t]; s2 < tile_toff[t + 1]; ++s2) {
-            const int a = tpairs[s2] >> 8, bb = tpairs[s2] & 0xff;
-            cols[un[bb]].push_back(un[a]);
+    {
+        std::vector<int> ncnt(nn + 1, 0);
+        for (size_t k = 0; k < ids.size(); ++k) ncnt[ids[k] + 1]++;
+        for (int i = 0; i < nn; ++i) ncnt[i + 1] += ncnt[i];
+        std::vector<int> ngp_of(ids.size());
+        {
+            std::vector<int> fillp(ncnt.begin(), ncnt.end() - 1);
+            for (int q = 0; q < ng; ++q)
+                for (int k = off[q]; k < off[q + 1]; ++k) ngp_of[fillp[ids[k]]++] = q;
+        }
+#pragma omp parallel
+        {
+            std::vector<int> mark(nn, -1);
+#pragma omp for schedule(dynamic, 256)
+            for (int hi = 0; hi < nn; ++hi) {
+                auto& v = cols[hi];
+                for (int a = ncnt[hi]; a < ncnt[hi + 1]; ++a) {
+                    const int q = ngp_of[a];
+                    for (int k = off[q]; k < off[q + 1]; ++k) {
+                        const int lo = ids[k];
+                        if (lo <= hi && mark[lo] != hi) { mark[lo] = hi; v.push_back(lo); }
+                    }
+                }
+            }
         }
     }
     P.nscratch = static_cast<int64_t>(tpairs.size());
@@ -141,6 +171,7 @@ void build_plan(Context& c) {
 
     // face pattern (per column: sorted unique rows)
     std::vector<std::vector<int>> face_rows(nn);
+#pragma omp parallel for schedule(dynamic, 256)
     for (int j = 0; j < nn; ++j) {
         auto& v = cols[j];
         std::sort(v.begin(), v.end());
@@ -177,12 +208,13 @@ void build_plan(Context& c) {
 
     // union pattern
     P.col_ptr.assign(nn + 1, 0);
+#pragma omp parallel for schedule(dynamic, 256)
     for (int j = 0; j < nn; ++j) {
         auto& v = cols[j];
         std::sort(v.begin(), v.end());
         v.erase(std::unique(v.begin(), v.end()), v.end());
-        P.col_ptr[j + 1] = P.col_ptr[j] + static_cast<int>(v.size());
     }
+    for (int j = 0; j < nn; ++j) P.col_ptr[j + 1] = P.col_ptr[j] + static_cast<int>(cols[j].size());
     P.nblocks = P.col_ptr[nn];
     P.col_rows.resize(P.nblocks);
     P.blk_face.assign(P.nblocks, 0);
@@ -232,17 +264,20 @@ void build_plan(Context& c) {
     // ---- K contributors: face tiles then folds ---------------------------------
     std::vector<int64_t> kc_cnt(P.nblocks + 1, 0);
     std::vector<int64_t> tblk(tpairs.size());
+    bool miss = false;
+#pragma omp parallel for schedule(dynamic, 64) reduction(|| : miss)
     for (int t = 0; t < P.ntiles; ++t) {
         const int* un = unodes.data() + tile_uoff[t];
         for (int s = tile_toff[t]; s < tile_toff[t + 1]; ++s) {
             const int a = tpairs[s] >> 8, bb = tpairs[s] & 0xff;
             const int64_t blk = block_of(un[a], un[bb]);
-            if (blk < 0) throw Error(CVM_E_PATTERN_MISS,
-                "Assembly: Gauss-point neighbour missing from the sparsity pattern; CSR mapping is inconsistent.");
+            if (blk < 0) miss = true;
             tblk[s] = blk;
-            kc_cnt[blk + 1]++;
         }
     }
+    if (miss) throw Error(CVM_E_PATTERN_MISS,
+        "Assembly: Gauss-point neighbour missing from the sparsity pattern; CSR mapping is inconsistent.");
+    for (size_t s = 0; s < tblk.size(); ++s) kc_cnt[tblk[s] + 1]++;
     for (int64_t b = 0; b < P.nblocks; ++b) kc_cnt[b + 1] += kc_cnt[b];
     // block-major scratch: the tile kernel writes touched slot s straight to row
     // slot_dst[s], the position of its contribution in its block's segment; the
