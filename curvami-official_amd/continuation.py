@@ -1,0 +1,288 @@
+"""Constrained Newton with backtracking and the load-increment path.
+
+The host side of the Newton loop that calls the device assembly, with the
+semantics of ContinuationSolver::LineSearch
+(core/include/curvami/solver/ContinuationSolver.h:185-322):
+
+* constraint rows are filtered once per call to an independent set by a
+  column-pivoted QR of J^T with the tolerance max(rows, cols) * eps * max|R_ii|
+  (InitializeTopology, :105-137) and reused (filterRows, :139-164);
+* the force residual R = F_int - lambda F_ext is projected onto the null space
+  of J_ind (ConstraintProjector, core/include/curvami/solver/ConstraintProjector.h:19-55);
+* convergence at iteration j > 1 when max|P R| <= tol_force, |dx . R| <= tol_energy
+  and max|c| <= tol_geom (:225-233); divergence when max|P R| >= div or NaN (:235-239);
+* the step is the geometric corrector xb = -J^T (J J^T)^-1 c plus the KKT step
+  [K J^T; J 0] [dx; mu] = [-(R + K xb); 0] (:241-263), K read as the upper triangle;
+* backtracking: alpha = 1, 1/2, ... (6 trials) until the projected residual of the
+  trial is <= 1.05 times the current one; without descent the last trial is kept
+  unless fail_on_no_descent (:268-312);
+* on failure the state rolls back to the call's start (:238, :305, :318-321).
+
+The KKT solve replaces the reference's symmetric MUMPS factorization
+(core/include/curvami/solver/MumpsSymmetricSolver.h:46-55), which is not
+available here, by SuperLU (scipy) on the assembled symmetric KKT matrix.
+
+A problem object provides the Problem interface
+(core/include/curvami/solver/Problem.h:16-68): num_dofs, assemble(x) -> (K upper
+as scipy CSC, F_int), internal_force(x), external_force(x), constraints(x) ->
+(c, J as scipy CSR), load_stiffness(x) -> K_load upper or None.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List, Optional
+
+import numpy as np
+import scipy.linalg as sla
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+
+def full_from_upper(K):
+    """upper(K) + strictly-upper(K)^T: the selfadjointView<Upper> of a CSC
+    matrix. The assembled K stores its diagonal node blocks as full 3 x 3, so
+    entries below the diagonal exist and must be ignored, as MUMPS (sym = 2)
+    ignores them (MumpsSymmetricSolver.h:84-126)."""
+    return (sp.triu(K) + sp.triu(K, 1).T).tocsc()
+
+
+class ConstraintProjector:
+    """(J J^T) factorized once, used for the minimum-norm corrector and the
+    null-space projection (ConstraintProjector.h:19-55). The Gram matrix is
+    small (rows x rows); a failed Cholesky falls back to a least-squares solve."""
+
+    def __init__(self, J):
+        self.J = J.tocsr()
+        self.nr = J.shape[0]
+        self._chol = None
+        if self.nr:
+            G = (self.J @ self.J.T).toarray()
+            try:
+                self._chol = sla.cho_factor(G, lower=True, check_finite=True)
+            except (np.linalg.LinAlgError, ValueError):
+                self._chol = None
+            self._G = G
+
+    def _gram_solve(self, b):
+        if self._chol is not None:
+            y = sla.cho_solve(self._chol, b)
+            if np.all(np.isfinite(y)):
+                return y
+        return np.linalg.lstsq(self._G, b, rcond=None)[0]
+
+    def corrector(self, c, ns):
+        if c.size == 0:
+            return np.zeros(ns)
+        return self.J.T @ self._gram_solve(-c)
+
+    def project(self, R):
+        if self.nr == 0:
+            return R
+        return R - self.J.T @ self._gram_solve(self.J @ R)
+
+
+class KKTSolver:
+    """Direct solve of [K J^T; J 0] with K given as its upper triangle."""
+
+    def __init__(self, permc_spec: str = "MMD_AT_PLUS_A"):
+        self.permc_spec = permc_spec
+        self.factorizations = 0
+        self.last_error = 0
+
+    def solve(self, K_upper, J, rhs):
+        ns = K_upper.shape[0]
+        Kf = full_from_upper(K_upper)
+        if J.shape[0]:
+            A = sp.bmat([[Kf, J.T], [J, None]], format="csc")
+        else:
+            A = Kf
+        try:
+            lu = spla.splu(A, permc_spec=self.permc_spec)
+        except RuntimeError:
+            self.last_error = -1
+            return None
+        self.factorizations += 1
+        sol = lu.solve(rhs)
+        if not np.all(np.isfinite(sol)):
+            self.last_error = -2
+            return None
+        return sol
+
+
+@dataclasses.dataclass
+class LineSearchReport:
+    converged: bool
+    iterations: int            # Newton iterations at convergence (-1 otherwise)
+    attempts: int
+    history: List[tuple]       # (j, trial merit, energy change, geometric error)
+
+
+class ContinuationSolver:
+    tol_force = 1e-8
+    tol_energy = 1e-8
+    tol_geom = 1e-12
+    div = 1e7
+    ls_max_iters = 500
+    fail_on_no_descent = False
+
+    def __init__(self, kkt: Optional[KKTSolver] = None, verbose: bool = False):
+        self.kkt = kkt or KKTSolver()
+        self.ind_rows = np.zeros(0, np.int64)
+        self.verbose = verbose
+        self.last_report: Optional[LineSearchReport] = None
+
+    # ---- InitializeTopology / filterRows (ContinuationSolver.h:105-164) -------
+    def initialize_topology(self, J):
+        if J.shape[0] == 0 or J.shape[1] == 0:
+            self.ind_rows = np.zeros(0, np.int64)
+            return
+        Jt = J.toarray().T
+        if not np.all(np.isfinite(Jt)):
+            raise RuntimeError("Constraint Jacobian contains NaN or Inf.")
+        R, perm = sla.qr(Jt, mode="r", pivoting=True)
+        d = np.abs(np.diag(R))
+        tol = max(J.shape) * np.finfo(np.float64).eps * (d.max() if d.size else 0.0)
+        self.ind_rows = np.sort(perm[:d.size][d > tol])
+
+    def filter_rows(self, c, J):
+        return c[self.ind_rows], J.tocsr()[self.ind_rows]
+
+    # ---- LineSearch (ContinuationSolver.h:185-322) -----------------------------
+    def line_search(self, prob, lam: float, d: np.ndarray) -> LineSearchReport:
+        ns = d.size
+        if ns != prob.num_dofs:
+            raise ValueError(f"LineSearch: state vector has {ns} entries, problem has {prob.num_dofs} DOFs")
+        d0 = d.copy()
+        c_all, J_all = prob.constraints(d)
+        self.initialize_topology(J_all)
+        energy_change = 1.0
+        hist = []
+        rep = LineSearchReport(False, -1, 0, hist)
+        self.last_report = rep
+        for j in range(1, self.ls_max_iters + 1):
+            rep.attempts = j
+            c_all, J_all = prob.constraints(d)
+            c_ind, J_ind = self.filter_rows(c_all, J_all)
+            R = prob.internal_force(d) - lam * prob.external_force(d)
+            proj = ConstraintProjector(J_ind)
+            max_force = float(np.abs(proj.project(R)).max()) if ns else 0.0
+            geom_error = float(np.abs(c_ind).max()) if c_ind.size else 0.0
+            if j > 1 and max_force <= self.tol_force and energy_change <= self.tol_energy \
+                    and geom_error <= self.tol_geom:
+                rep.converged = True
+                rep.iterations = j
+                return rep
+            if max_force >= self.div or math.isnan(max_force):
+                d[:] = d0
+                return rep
+            K, _ = prob.assemble(d)
+            Kl = prob.load_stiffness(d)
+            if Kl is not None:
+                K = (K - lam * Kl).tocsc()
+            xb = proj.corrector(c_ind, ns)
+            rhs = np.zeros(ns + c_ind.size)
+            rhs[:ns] = -(R + full_from_upper(K) @ xb)
+            sol = self.kkt.solve(K, J_ind, rhs)
+            if sol is None:
+                d[:] = d0
+                return rep
+            dx = sol[:ns]
+            alpha = 1.0
+            trial = max_force
+            descended = False
+            d_trial = d
+            for _ in range(6):
+                d_trial = d + xb + alpha * dx
+                R_t = prob.internal_force(d_trial) - lam * prob.external_force(d_trial)
+                c_t, J_t = prob.constraints(d_trial)
+                _, J_t_ind = self.filter_rows(c_t, J_t)
+                trial = float(np.abs(ConstraintProjector(J_t_ind).project(R_t)).max())
+                if trial <= 1.05 * max_force:
+                    descended = True
+                    break
+                alpha *= 0.5
+            if not descended and self.fail_on_no_descent:
+                d[:] = d0
+                return rep
+            energy_change = abs(float(np.dot(xb + alpha * dx, R)))
+            d[:] = d_trial
+            hist.append((j, trial, energy_change, geom_error))
+            if self.verbose:
+                print(f"{j:3d}  {trial:12.5e}  {energy_change:12.5e}  {geom_error:12.5e}", flush=True)
+        d[:] = d0
+        return rep
+
+
+@dataclasses.dataclass
+class LoadPathResult:
+    x: np.ndarray
+    converged: List[bool]
+    iterations: List[int]
+    lambdas: List[float]
+
+    @property
+    def all_converged(self) -> bool:
+        return all(self.converged)
+
+
+def load_path(prob, increments: int = 20, lam_total: float = 1.0, x0=None,
+              solver: Optional[ContinuationSolver] = None, stop_on_failure: bool = True) -> LoadPathResult:
+    """Proportional loading lambda_k = k / increments * lam_total, one LineSearch
+    per increment from the previous converged state."""
+    solver = solver or ContinuationSolver()
+    x = np.zeros(prob.num_dofs) if x0 is None else np.array(x0, dtype=np.float64)
+    conv, its, lams = [], [], []
+    for k in range(1, increments + 1):
+        lam = lam_total * k / increments
+        rep = solver.line_search(prob, lam, x)
+        conv.append(rep.converged); its.append(rep.iterations); lams.append(lam)
+        if not rep.converged and stop_on_failure:
+            break
+    return LoadPathResult(x, conv, its, lams)
+
+
+class PlateProblem:
+    """PlateProblem (plate/PlateProblem.h:52-91) over the device plate: residual,
+    tangent, constraint rows and external load all come from the HIP library.
+    fp32_tangent assembles the face tangent in FP32 (inexact Newton);
+    exact_hinge adds the hinge's (theta - theta0) d2theta term (fold_exact_tangent,
+    plate/Meshless.h:48-50)."""
+
+    def __init__(self, plate, flag_rig: bool = False, fp32_tangent: bool = False, exact_hinge: bool = False):
+        self.plate = plate
+        self.flag_rig = flag_rig
+        self.fp32_tangent = fp32_tangent
+        self.exact_hinge = exact_hinge
+        self.num_dofs = 3 * plate.nn
+        outer, inner = plate.pattern(with_folds=not flag_rig)
+        self._outer, self._inner = outer, inner
+        self._fext = None
+        self._rows, self._cols = plate.constraint_layout(rigid=flag_rig)
+        self.assemblies = 0
+        self.residuals = 0
+
+    def assemble(self, x):
+        F, K = self.plate.assemble(x, elastic_folds=not self.flag_rig, fp32_tangent=self.fp32_tangent,
+                                   exact_hinge=self.exact_hinge)
+        self.assemblies += 1
+        n = self.num_dofs
+        return sp.csc_matrix((K, self._inner, self._outer), shape=(n, n)), F
+
+    def internal_force(self, x):
+        F, _ = self.plate.assemble(x, elastic_folds=not self.flag_rig, tangent=False)
+        self.residuals += 1
+        return F
+
+    def external_force(self, x):
+        if self._fext is None:
+            self._fext = self.plate.external_force()
+        return self._fext
+
+    def load_stiffness(self, x):
+        return None
+
+    def constraints(self, x):
+        c, vals = self.plate.constraints(x)
+        J = sp.csr_matrix((vals, (self._rows, self._cols)), shape=(c.size, self.num_dofs))
+        return c, J

@@ -236,6 +236,8 @@ int cvm_add_domain(cvm_context* ctx, int kind, int ng, const double* gx, const d
         d->d_w.upload(d->hw, c.stream);
         d->d_tx.upload(d->htx, c.stream);
         d->d_ty.upload(d->hty, c.stream);
+        d->d_nx.upload(d->hnx, c.stream);
+        d->d_ny.upload(d->hny, c.stream);
         CVM_HIP(hipStreamSynchronize(c.stream));
     } catch (...) {
         delete d;
@@ -274,6 +276,7 @@ int cvm_set_fold_gp_state(cvm_context* ctx, int fold_id, int n, const double* th
     if (theta0) f.d_theta0.upload(f.theta0, c.stream);
     CVM_HIP(hipStreamSynchronize(c.stream));
     if (active_changed) c.plan.ready = false;   // the fold pattern depends on activity
+    c.cons.ready = false;                        // so do the fold rows
     CVM_CATCH(ctx)
 }
 
@@ -283,6 +286,7 @@ int cvm_neighbours(cvm_context* ctx, int domain, double rho, int64_t* npairs) {
     CVM_HIP(hipSetDevice(c.device));
     cvm::Domain& d = cvm::domain(c, domain);
     cvm::launch_frnn(c, d, rho);
+    ++d.gen;
     if (npairs) *npairs = d.npairs;
     if (d.kind == CVM_DOMAIN_FACE || d.kind == CVM_DOMAIN_FOLD_LEFT || d.kind == CVM_DOMAIN_FOLD_RIGHT)
         c.plan.ready = false;
@@ -310,6 +314,7 @@ int cvm_shape_functions(cvm_context* ctx, int domain, int order, int kernel, int
     cvm::Domain& d = cvm::domain(c, domain);
     int code = 0, nb = 0, fb = 0;
     cvm::launch_rkpm(c, d, d.rho, order, kernel, &code, &nb, &fb);
+    ++d.gen;
     if (nbad) *nbad = nb;
     if (first_bad) *first_bad = fb;
     if (code == CVM_E_SINGULAR_MOMENT) {
@@ -428,6 +433,89 @@ int cvm_assemble(cvm_context* ctx, const double* x, int flags, double* F_int, do
         if (F_int) CVM_HIP(hipMemcpyAsync(F_int, Fd, 3 * sizeof(double) * c.nn, hipMemcpyDeviceToHost, c.stream));
         if (tangent && K_values && nnz)
             CVM_HIP(hipMemcpyAsync(K_values, Kd, sizeof(double) * nnz, hipMemcpyDeviceToHost, c.stream));
+        CVM_HIP(hipStreamSynchronize(c.stream));
+    }
+    CVM_CATCH(ctx)
+}
+
+int cvm_set_domain_bc(cvm_context* ctx, int domain, int active, const double* val, const double* force,
+                      double moment) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    cvm::Domain& d = cvm::domain(c, domain);
+    if (active & ~15) throw Error(CVM_E_INVALID, "bad active mask (bits 0..3: u, v, w, rotation)");
+    d.bc.active = active;
+    for (int k = 0; k < 4; ++k) d.bc.val[k] = val ? val[k] : 0.0;
+    for (int k = 0; k < 3; ++k) d.bc.force[k] = force ? force[k] : 0.0;
+    d.bc.moment = moment;
+    c.cons.ready = false;
+    c.cons.fext_ready = false;
+    CVM_CATCH(ctx)
+}
+
+int cvm_constraint_layout(cvm_context* ctx, int rigid, int64_t* nrows, int64_t* ntriplets) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    CVM_HIP(hipSetDevice(c.device));
+    cvm::build_constraint_layout(c, rigid != 0);
+    if (nrows) *nrows = c.cons.nrows;
+    if (ntriplets) *ntriplets = c.cons.ntrip;
+    CVM_CATCH(ctx)
+}
+
+int cvm_get_constraint_pattern(cvm_context* ctx, int* rows, int* cols) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    if (!c.cons.ready) throw Error(CVM_E_STATE, "constraint layout not built (cvm_constraint_layout)");
+    if (rows) std::copy(c.cons.rows.begin(), c.cons.rows.end(), rows);
+    if (cols) std::copy(c.cons.cols.begin(), c.cons.cols.end(), cols);
+    CVM_CATCH(ctx)
+}
+
+int cvm_constraints(cvm_context* ctx, const double* x, int flags, double* c_out, double* j_values) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    CVM_HIP(hipSetDevice(c.device));
+    cvm::ConstraintLayout& L = c.cons;
+    if (!L.ready) throw Error(CVM_E_STATE, "constraint layout not built (cvm_constraint_layout)");
+    if (!x) throw Error(CVM_E_INVALID, "x is null");
+    const bool dev = (flags & CVM_DEVICE_PTRS) != 0;
+    if (dev && (!c_out || !j_values) && L.nrows) throw Error(CVM_E_INVALID, "device output pointer is null");
+    const double* xd = x;
+    double* cd = c_out;
+    double* jd = j_values;
+    cvm::DBuf<double> tc, tj;
+    if (!dev) {
+        c.d_x.upload(x, 3 * static_cast<size_t>(c.nn), c.stream);
+        xd = c.d_x.p;
+        tc.alloc(std::max<int64_t>(L.nrows, 1));
+        tj.alloc(std::max<int64_t>(L.ntrip, 1));
+        cd = tc.p;
+        jd = tj.p;
+    }
+    cvm::launch_constraints(c, xd, cd, jd);
+    if (!dev) {
+        if (c_out && L.nrows) tc.download(c_out, L.nrows, c.stream);
+        if (j_values && L.ntrip) tj.download(j_values, L.ntrip, c.stream);
+        CVM_HIP(hipStreamSynchronize(c.stream));
+    }
+    CVM_CATCH(ctx)
+}
+
+int cvm_external_force(cvm_context* ctx, int flags, double* F_ext) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    CVM_HIP(hipSetDevice(c.device));
+    if (!F_ext) throw Error(CVM_E_INVALID, "F_ext is null");
+    const bool dev = (flags & CVM_DEVICE_PTRS) != 0;
+    double* Fd = F_ext;
+    if (!dev) {
+        c.d_F.alloc(std::max<size_t>(3 * static_cast<size_t>(c.nn), 1));
+        Fd = c.d_F.p;
+    }
+    cvm::launch_external_force(c, Fd);
+    if (!dev) {
+        CVM_HIP(hipMemcpyAsync(F_ext, Fd, 3 * sizeof(double) * c.nn, hipMemcpyDeviceToHost, c.stream));
         CVM_HIP(hipStreamSynchronize(c.stream));
     }
     CVM_CATCH(ctx)

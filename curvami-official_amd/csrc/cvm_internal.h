@@ -72,11 +72,22 @@ struct Crease {
 
 // Shape-function cache of one domain, device resident, pair order = FRNN order
 // (GP-major, neighbours in the search order of GlobalFRNN::execute).
+// BoundaryCondition of a domain (plate/PlateData.h:36-50): active bit 0 u,
+// 1 v, 2 w, 3 normal rotation; dead force per GP; edge moment per unit length.
+struct BoundaryCond {
+    int active = 0;
+    double val[4] = {0, 0, 0, 0};
+    double force[3] = {0, 0, 0};
+    double moment = 0.0;
+};
+
 struct Domain {
     int kind = CVM_DOMAIN_FACE;
     int ng = 0;
     std::vector<double> hx, hy, hw, hnx, hny, htx, hty;   // host copies of the GP data
-    DBuf<double> gx, gy, d_w, d_tx, d_ty;
+    DBuf<double> gx, gy, d_w, d_tx, d_ty, d_nx, d_ny;
+    BoundaryCond bc;
+    uint64_t gen = 0;         // bumped whenever neighbours or shape functions change
     // neighbour lists
     double rho = 0.0;
     int64_t npairs = -1;
@@ -99,6 +110,27 @@ struct Fold {
     std::vector<double> theta0;   // per crease GP (empty: Ang0)
     std::vector<char> active;     // per crease GP (empty: all active)
     DBuf<double> d_theta0;
+};
+
+// Constraint rows of Meshless::evaluateConstraints (plate/Meshless.h:129-195):
+// one job per Dirichlet GP or active crease GP, with its first row and first
+// triplet; the triplet pattern (rows, cols) is fixed by the neighbour lists.
+struct ConstraintLayout {
+    bool ready = false;
+    bool rigid = false;
+    int64_t nrows = 0, ntrip = 0;
+    std::vector<int> rows, cols;
+    std::vector<uint64_t> gens;   // domain generations the layout was built from
+    int njobs = 0;
+    DBuf<int> d_job;              // 6 ints per job: kind, a, b, gp, row0, trip0
+    DBuf<unsigned char> d_dom;    // per domain: device descriptor
+    DBuf<unsigned char> d_fold;   // per fold: device descriptor
+    // external force: contributions grouped by dof, in the reference's order
+    bool fext_ready = false;
+    std::vector<uint64_t> fext_gens;
+    DBuf<int> d_fx_off;           // 3 nn + 1
+    DBuf<int> d_fx_rec;           // 4 ints per contribution: dom, pair, gp, type
+    DBuf<double> d_fx_val;        // 3 doubles per contribution: coefficient, n_out x, n_out y
 };
 
 // Assembly plan: tiles of face GPs, block pattern, contributor lists.
@@ -184,6 +216,7 @@ struct Context {
     std::vector<Domain*> doms;
     std::vector<Fold> folds;
     Plan plan;
+    ConstraintLayout cons;
     // assembly buffers
     DBuf<double> d_x;           // 3 nn displacements
     DBuf<double> d_F;           // 3 nn
@@ -215,5 +248,8 @@ int timing_begin(Context& c, const char* name);
 void timing_end(Context& c, int slot);
 
 void build_plan(Context& c);
+void build_constraint_layout(Context& c, bool rigid);
+void launch_constraints(Context& c, const double* xdev, double* cdev, double* jdev);
+void launch_external_force(Context& c, double* Fdev);
 
 }  // namespace cvm

@@ -47,6 +47,11 @@ SIGNATURES = {
     "cvm_get_pattern": (C.c_int, [C.c_void_p, C.c_int, _i, _i]),
     "cvm_csr_mapping": (C.c_int, [C.c_void_p, C.c_int, _i, C.c_int, _i]),
     "cvm_assemble": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "cvm_set_domain_bc": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _d, _d, C.c_double]),
+    "cvm_constraint_layout": (C.c_int, [C.c_void_p, C.c_int, _i64, _i64]),
+    "cvm_get_constraint_pattern": (C.c_int, [C.c_void_p, _i, _i]),
+    "cvm_constraints": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "cvm_external_force": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "cvm_synchronize": (C.c_int, [C.c_void_p]),
     "cvm_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "cvm_kernel_count": (C.c_int, [C.c_void_p]),
@@ -100,6 +105,7 @@ class Plate:
         self._h = h
         self.nn = 0
         self.domain_sizes = []
+        self.n_crows = self.n_ctrip = 0
         self.npairs = {}
 
     def close(self):
@@ -221,6 +227,34 @@ class Plate:
         self._check(self._L.cvm_assemble(self._h, C.c_void_p(x_ptr), flags | DEVICE_PTRS,
                                          C.c_void_p(F_ptr), C.c_void_p(K_ptr) if K_ptr else None))
 
+    # ---- constraints and loads ----------------------------------------------
+    def set_bc(self, dom, active=0, val=(0.0, 0.0, 0.0, 0.0), force=(0.0, 0.0, 0.0), moment=0.0):
+        """BoundaryCondition of a domain (active: bit 0 u, 1 v, 2 w, 3 rotation)."""
+        v = np.ascontiguousarray(val, np.float64); f = np.ascontiguousarray(force, np.float64)
+        self._check(self._L.cvm_set_domain_bc(self._h, dom, int(active), v.ctypes.data_as(_d),
+                                              f.ctypes.data_as(_d), float(moment)))
+
+    def constraint_layout(self, rigid=False):
+        """Builds the constraint-row layout; returns the triplet (rows, cols)."""
+        nr = C.c_int64(0); nt = C.c_int64(0)
+        self._check(self._L.cvm_constraint_layout(self._h, int(rigid), C.byref(nr), C.byref(nt)))
+        self.n_crows, self.n_ctrip = nr.value, nt.value
+        rows = np.empty(max(nt.value, 1), np.int32); cols = np.empty(max(nt.value, 1), np.int32)
+        self._check(self._L.cvm_get_constraint_pattern(self._h, rows.ctypes.data_as(_i), cols.ctypes.data_as(_i)))
+        return rows[:nt.value], cols[:nt.value]
+
+    def constraints(self, x):
+        """c(x) and the triplet values at x (layout from constraint_layout)."""
+        x = _f64(x)
+        c = np.empty(max(self.n_crows, 1)); v = np.empty(max(self.n_ctrip, 1))
+        self._check(self._L.cvm_constraints(self._h, x.ctypes.data, 0, c.ctypes.data, v.ctypes.data))
+        return c[:self.n_crows], v[:self.n_ctrip]
+
+    def external_force(self):
+        F = np.empty(3 * self.nn)
+        self._check(self._L.cvm_external_force(self._h, 0, F.ctypes.data))
+        return F
+
     def synchronize(self):
         self._check(self._L.cvm_synchronize(self._h))
 
@@ -257,6 +291,11 @@ def plate_from_sheet(sh, device: int = 0, compute: bool = True, order=None, kern
         r = p.add_domain(f.right, FOLD_RIGHT)
         p.add_fold(l, r, f.Kt, f.Ang0)
         p.fold_doms.append((l, r))
+    p.bnd_doms = []
+    for bd in getattr(sh, "boundaries", []):
+        d = p.add_domain(bd.gps, POINT if bd.kind == "point" else EDGE)
+        p.set_bc(d, bd.active, bd.val, bd.force, bd.moment)
+        p.bnd_doms.append(d)
     if compute:
         for d in range(len(p.domain_sizes)):
             p.neighbours(d, sh.rho)

@@ -117,6 +117,7 @@ public:
     // SetupHelper::recomputeShapeFunctions (SetupHelper.h:112-138): FRNN + RKPM
     // on every domain.
     void recomputeShapeFunctions() {
+        cons_ready_ = false;
         for (int d = 0; d < static_cast<int>(sizes_.size()); ++d) {
             GlobalFRNN frnn(rho);
             std::vector<int> off(static_cast<size_t>(sizes_[d]) + 1), ids;
@@ -146,6 +147,39 @@ public:
         ctx_.check(cvm_assemble(ctx_.get(), x.data(), flags, F.data(), nullptr));
         return F;
     }
+    // BoundaryCondition of a domain (PlateData.h:36-50); active bit 0 u, 1 v, 2 w, 3 rotation.
+    void setBoundaryCondition(int domain, int active, const double val[4], const double force[3], double moment) {
+        ctx_.check(cvm_set_domain_bc(ctx_.get(), domain, active, val, force, moment));
+        cons_ready_ = false;
+    }
+    // Meshless::evaluateConstraints (Meshless.h:129-195): c and J as triplets in the
+    // reference's push order (J_all.setFromTriplets sums duplicates).
+    struct Triplet { int row, col; double value; };
+    void evaluateConstraints(const std::vector<double>& x, std::vector<double>& c_all,
+                             std::vector<Triplet>& J_triplets, int& nrows, bool flagRig_current) {
+        if (!cons_ready_ || cons_rigid_ != flagRig_current) {
+            int64_t nr = 0, nt = 0;
+            ctx_.check(cvm_constraint_layout(ctx_.get(), flagRig_current, &nr, &nt));
+            crow_.resize(static_cast<size_t>(nt));
+            ccol_.resize(static_cast<size_t>(nt));
+            ctx_.check(cvm_get_constraint_pattern(ctx_.get(), crow_.data(), ccol_.data()));
+            cnrows_ = static_cast<int>(nr);
+            cons_ready_ = true;
+            cons_rigid_ = flagRig_current;
+        }
+        c_all.resize(static_cast<size_t>(cnrows_));
+        std::vector<double> v(crow_.size());
+        ctx_.check(cvm_constraints(ctx_.get(), x.data(), 0, c_all.data(), v.data()));
+        J_triplets.resize(v.size());
+        for (size_t k = 0; k < v.size(); ++k) J_triplets[k] = {crow_[k], ccol_[k], v[k]};
+        nrows = cnrows_;
+    }
+    // Meshless::computeExternalForce (Meshless.h:60-100)
+    std::vector<double> computeExternalForce() {
+        std::vector<double> F(3 * static_cast<size_t>(nn_));
+        ctx_.check(cvm_external_force(ctx_.get(), 0, F.data()));
+        return F;
+    }
     // NonLinearAssembler::csrMapping (Assembly.h:54)
     std::vector<int> csrMapping(int gp) {
         int n = 0;
@@ -161,6 +195,9 @@ private:
     int nn_ = 0;
     int64_t nnz_face_ = 0, nnz_union_ = 0;
     std::vector<int> sizes_;
+    bool cons_ready_ = false, cons_rigid_ = false;
+    int cnrows_ = 0;
+    std::vector<int> crow_, ccol_;
 };
 
 }  // namespace curvami_gpu

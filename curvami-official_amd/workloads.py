@@ -79,6 +79,18 @@ class Fold:
 
 
 @dataclasses.dataclass
+class Boundary:
+    """An edge or point integration domain with its BoundaryCondition
+    (plate/PlateData.h:36-56). active: bit 0 u, 1 v, 2 w, 3 normal rotation."""
+    gps: GPSet
+    kind: str                 # "point" or "edge"
+    active: int = 0
+    val: tuple = (0.0, 0.0, 0.0, 0.0)
+    force: tuple = (0.0, 0.0, 0.0)
+    moment: float = 0.0
+
+
+@dataclasses.dataclass
 class Sheet:
     name: str
     n_side: int
@@ -94,6 +106,7 @@ class Sheet:
     nu: float = 0.3
     t: float = 0.01
     n_split: int = 0
+    boundaries: List[Boundary] = dataclasses.field(default_factory=list)
 
     @property
     def nn(self) -> int:
@@ -348,3 +361,64 @@ def rigid_fold_displacement(sh: Sheet, angle: float, seed: int, sigma: float = 1
 def displacement(nn: int, seed: int = 1, sigma: float = 1e-3) -> np.ndarray:
     """u ~ N(0, sigma) per DOF (3 DOFs per node, node-major)."""
     return normal(seed, 3 * nn, sigma)
+
+
+# ---------------------------------------------------------------------------
+# Boundary domains and the Newton load case
+# ---------------------------------------------------------------------------
+def point_domain(x: float, y: float) -> GPSet:
+    """0-D domain: one GP at the point, weight 1 (Quadrature.h:84-93)."""
+    z = np.zeros(1)
+    return GPSet(np.array([x]), np.array([y]), np.ones(1), z, z.copy(), z.copy(), z.copy())
+
+
+def edge_domain(ax: float, ay: float, bx: float, by: float, nseg: int) -> GPSet:
+    """Straight boundary edge split into `nseg` equal line elements, one GP per
+    element at its midpoint, weight = element length, tangent along a->b,
+    normal = tangent turned by +90 degrees (Quadrature.h:96-121, line_degree 1)."""
+    s0 = np.arange(nseg) / nseg
+    s1 = (np.arange(nseg) + 1) / nseg
+    p1x, p1y = ax + (bx - ax) * s0, ay + (by - ay) * s0
+    p2x, p2y = ax + (bx - ax) * s1, ay + (by - ay) * s1
+    tx, ty = p2x - p1x, p2y - p1y
+    L = np.sqrt(tx * tx + ty * ty)
+    tx, ty = tx / L, ty / L
+    gx = 0.5 * p1x + 0.5 * p2x
+    gy = 0.5 * p1y + 0.5 * p2y
+    return GPSet(gx, gy, L.copy(), -ty, tx.copy(), tx, ty)
+
+
+def bending_d11(sh: Sheet) -> float:
+    return sh.E * sh.t ** 3 / (12.0 * (1.0 - sh.nu ** 2))
+
+
+def with_edge_moment_load(sh: Sheet, seed: int = 2) -> Sheet:
+    """The Newton load case: the three-corner supports of the reference's
+    regression configurations (point 1 at (0,0): u, v, w; point 2 at (1,0): v, w;
+    point 4 at (0,1): w) and equal normal edge moments on x = 0 and x = 1.
+    The moments are self-equilibrated, so the supports only remove the rigid
+    motions. Magnitude m = s Kt, s = 0.5 + U(0,1) from SplitMix64(seed), Kt the
+    hinge stiffness per unit length of crease 0: a straight crease then opens by
+    about s radians at full load, while the facets bend to a curvature m / D of
+    about 0.1 s (D = E t^3 / 12 (1 - nu^2)). Without a crease, m = s D."""
+    n = sh.n_side
+    s = 0.5 + float(uniform01(seed, 1)[0])
+    m = s * (sh.folds[0].Kt if sh.folds else bending_d11(sh))
+    b = [Boundary(point_domain(0.0, 0.0), "point", active=1 | 2 | 4),
+         Boundary(point_domain(1.0, 0.0), "point", active=2 | 4),
+         Boundary(point_domain(0.0, 1.0), "point", active=4),
+         Boundary(edge_domain(0.0, 0.0, 0.0, 1.0, n - 1), "edge", moment=m),
+         Boundary(edge_domain(1.0, 0.0, 1.0, 1.0, n - 1), "edge", moment=m)]
+    return dataclasses.replace(sh, name=sh.name + "_moment", boundaries=b)
+
+
+def newton_tolerances(f_ext_inf: float) -> dict:
+    """Solver tolerances for the load path, scaled to the load: the reference's
+    defaults (tol_force 1e-8, tol_energy 1e-8) are absolute and its examples set
+    them per configuration (solver.tol_force / solver.tol_energy, main.cpp:263-264);
+    with E = 1, t = 0.01 the loads are ~1e-8, so they are set relative to
+    max|F_ext|."""
+    return {"tol_force": 1e-7 * f_ext_inf, "tol_energy": 1e-9 * f_ext_inf}
+
+
+NEWTON_INCREMENTS = 20

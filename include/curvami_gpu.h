@@ -144,6 +144,29 @@ CVM_API int cvm_csr_mapping(cvm_context* ctx, int gp, int* out, int cap, int* n)
  * follows cvm_get_pattern(with_folds = flags & CVM_ELASTIC_FOLDS). */
 CVM_API int cvm_assemble(cvm_context* ctx, const double* x, int flags, double* F_int, double* K_values);
 
+/* ---- constraints and loads (the rest of PlateProblem, plate/PlateProblem.h:52-91) ---- */
+
+/* BoundaryCondition of a domain (plate/PlateData.h:36-50): active bit 0 u, 1 v,
+ * 2 w, 3 normal rotation (values val[4], may be NULL = 0); dead force per unit
+ * (force[3], may be NULL); bending moment per unit length on edge domains. */
+CVM_API int cvm_set_domain_bc(cvm_context* ctx, int domain, int active, const double* val, const double* force,
+                              double moment);
+/* Row and triplet layout of Meshless::evaluateConstraints (plate/Meshless.h:129-195):
+ * Dirichlet rows of the point domains, then of the edge domains
+ * (Constraints::evaluateDirichlet, plate/Constraints.h:24-52), then per fold and
+ * active crease GP three tie rows u_L - u_R and, when rigid != 0 (flagRig), the
+ * angle row theta - theta0 (Constraints::evaluateFold, plate/Constraints.h:247-308).
+ * Needs the shape functions of those domains; rebuild after they change. */
+CVM_API int cvm_constraint_layout(cvm_context* ctx, int rigid, int64_t* nrows, int64_t* ntriplets);
+/* Triplet rows and columns in the reference's push order (duplicates are summed
+ * when J is formed, as Eigen's setFromTriplets does). */
+CVM_API int cvm_get_constraint_pattern(cvm_context* ctx, int* rows, int* cols);
+/* c(x)[nrows] and the triplet values[ntriplets] at x. flags: CVM_DEVICE_PTRS. */
+CVM_API int cvm_constraints(cvm_context* ctx, const double* x, int flags, double* c, double* j_values);
+/* Reference load F_ext[3nn] of Meshless::computeExternalForce (plate/Meshless.h:60-100):
+ * dead forces of point, edge and fold domains and edge moments. flags: CVM_DEVICE_PTRS. */
+CVM_API int cvm_external_force(cvm_context* ctx, int flags, double* F_ext);
+
 /* Wait for the context's stream. */
 CVM_API int cvm_synchronize(cvm_context* ctx);
 

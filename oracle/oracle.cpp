@@ -876,12 +876,23 @@ struct FoldDef {
     std::vector<char> active;       // per crease GP (empty: all)
 };
 
+// BoundaryCondition (plate/PlateData.h:36-50); kind 1 = point domain, 2 = edge
+// domain (Meshless::points / Meshless::edges), 0 = neither.
+struct BC {
+    int kind = 0;
+    bool au = false, av = false, aw = false, arot = false;
+    double vu = 0, vv = 0, vw = 0, vrot = 0;
+    double force[3] = {0, 0, 0};
+    double moment = 0.0;
+};
+
 struct Domain {
     std::vector<V2> X;
     std::vector<double> w;
     std::vector<V2> normal, tangent;
     NbrList nl;
     Shape sh;
+    BC bc;
 };
 
 static void elastic_fold(const Domain& L, const Domain& R, const FoldDef& f, const double* u,
@@ -954,6 +965,82 @@ static void elastic_fold(const Domain& L, const Domain& R, const FoldDef& f, con
 }
 
 // ---------------------------------------------------------------------------
+// Constraint rows: Constraints::evaluateDirichlet (plate/Constraints.h:24-52)
+// and the pointwise tie / rigid angle rows of Constraints::evaluateFold
+// (plate/Constraints.h:247-308). Triplets are pushed in the reference order.
+// ---------------------------------------------------------------------------
+static void dirichlet_rows(const Domain& d, const double* u, int& row, std::vector<double>& c,
+                           std::vector<Triplet>& J) {
+    const BC& bc = d.bc;
+    for (size_t g = 0; g < d.X.size(); ++g) {
+        const int b = d.nl.off[g], n = d.nl.off[g + 1] - b;
+        double uc[3] = {0, 0, 0}, ux[3] = {0, 0, 0}, uy[3] = {0, 0, 0};
+        for (int j = 0; j < n; ++j) {                                 // :31-36
+            const double* uj = u + 3 * static_cast<size_t>(d.nl.ids[b + j]);
+            for (int k = 0; k < 3; ++k) {
+                uc[k] += uj[k] * d.sh.psi[b + j];
+                ux[k] += uj[k] * d.sh.dpx[b + j];
+                uy[k] += uj[k] * d.sh.dpy[b + j];
+            }
+        }
+        const bool act[3] = {bc.au, bc.av, bc.aw};
+        const double val[3] = {bc.vu, bc.vv, bc.vw};
+        for (int k = 0; k < 3; ++k) {                                 // :38-40
+            if (!act[k]) continue;
+            c.push_back(uc[k] - val[k]);
+            for (int j = 0; j < n; ++j) J.push_back({row, d.nl.ids[b + j] * 3 + k, d.sh.psi[b + j]});
+            ++row;
+        }
+        if (bc.arot) {                                                // :42-48
+            const double nx = d.normal[g].x, ny = d.normal[g].y;
+            c.push_back((ux[2] * nx + uy[2] * ny) - bc.vrot);
+            for (int j = 0; j < n; ++j)
+                J.push_back({row, d.nl.ids[b + j] * 3 + 2, d.sh.dpx[b + j] * nx + d.sh.dpy[b + j] * ny});
+            ++row;
+        }
+    }
+}
+
+static void fold_rows(const Domain& L, const Domain& R, const FoldDef& f, const double* u, bool rig,
+                      int& row, std::vector<double>& c, std::vector<Triplet>& J) {
+    for (size_t i = 0; i < L.X.size(); ++i) {
+        if (!f.active.empty() && !f.active[i]) continue;
+        const int bL = L.nl.off[i], nL = L.nl.off[i + 1] - bL;
+        const int bR = R.nl.off[i], nR = R.nl.off[i + 1] - bR;
+        double uL[3] = {0, 0, 0}, uR[3] = {0, 0, 0};                  // :267-269
+        for (int j = 0; j < nL; ++j)
+            for (int k = 0; k < 3; ++k) uL[k] += u[3 * static_cast<size_t>(L.nl.ids[bL + j]) + k] * L.sh.psi[bL + j];
+        for (int j = 0; j < nR; ++j)
+            for (int k = 0; k < 3; ++k) uR[k] += u[3 * static_cast<size_t>(R.nl.ids[bR + j]) + k] * R.sh.psi[bR + j];
+        for (int d = 0; d < 3; ++d) {                                 // :271-278
+            c.push_back(uL[d] - uR[d]);
+            for (int j = 0; j < nL; ++j) J.push_back({row, L.nl.ids[bL + j] * 3 + d, L.sh.psi[bL + j]});
+            for (int j = 0; j < nR; ++j) J.push_back({row, R.nl.ids[bR + j] * 3 + d, -R.sh.psi[bR + j]});
+            ++row;
+        }
+        if (!rig) continue;
+        const Kin kL = kinematics(L.nl.ids.data() + bL, nL, L.sh.dpx.data() + bL, L.sh.dpy.data() + bL,
+                                  L.sh.dxx.data() + bL, L.sh.dyy.data() + bL, L.sh.dxy.data() + bL, u);
+        const Kin kR = kinematics(R.nl.ids.data() + bR, nR, R.sh.dpx.data() + bR, R.sh.dpy.data() + bR,
+                                  R.sh.dxx.data() + bR, R.sh.dyy.data() + bR, R.sh.dxy.data() + bR, u);
+        double Fv[12] = {kL.a1.x, kL.a1.y, kL.a1.z, kL.a2.x, kL.a2.y, kL.a2.z,
+                         kR.a1.x, kR.a1.y, kR.a1.z, kR.a2.x, kR.a2.y, kR.a2.z};
+        double grad[12];
+        const double theta = fold_angle(Fv, L.tangent[i].x, L.tangent[i].y, grad);   // :280-287
+        c.push_back(theta - (f.theta0.empty() ? f.Ang0 : f.theta0[i]));
+        auto add = [&](int block, const Domain& Dm, int b, int n) {    // :293-301
+            for (int j = 0; j < n; ++j) {
+                const double dX = Dm.sh.dpx[b + j], dY = Dm.sh.dpy[b + j];
+                for (int d = 0; d < 3; ++d)
+                    J.push_back({row, Dm.nl.ids[b + j] * 3 + d, grad[block + d] * dX + grad[block + d + 3] * dY});
+            }
+        };
+        add(0, L, bL, nL); add(6, R, bR, nR);
+        ++row;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // The plate: plate/Meshless.h:11-127 (domains, folds, initAssembly,
 // assembleSystem, assembleSystemInternalOnly)
 // ---------------------------------------------------------------------------
@@ -970,6 +1057,9 @@ struct Plate {
     // last assembled system (CSC of block-upper K)
     std::vector<int> K_outer, K_inner;
     std::vector<double> K_val, F;
+    // last evaluated constraints
+    std::vector<double> c;
+    std::vector<Triplet> J;
     std::string err;
     double t_last = 0.0;
 };
@@ -1195,6 +1285,74 @@ void orc_kinematics(Plate* p, int dom, const double* x, double* out) {
         o[9] = k.E0[0][0]; o[10] = k.E0[1][1]; o[11] = k.E0[0][1];
         o[12] = k.E1[0][0]; o[13] = k.E1[1][1]; o[14] = k.E1[0][1];
     }
+}
+
+void orc_set_domain_bc(Plate* p, int dom, int kind, int active, const double* val, const double* force,
+                       double moment) {
+    BC& bc = p->doms[dom].bc;
+    bc.kind = kind;
+    bc.au = (active & 1) != 0; bc.av = (active & 2) != 0; bc.aw = (active & 4) != 0; bc.arot = (active & 8) != 0;
+    bc.vu = val[0]; bc.vv = val[1]; bc.vw = val[2]; bc.vrot = val[3];
+    for (int k = 0; k < 3; ++k) bc.force[k] = force[k];
+    bc.moment = moment;
+}
+
+// Meshless::evaluateConstraints (plate/Meshless.h:129-195): Dirichlet rows of the
+// point domains, then of the edge domains, then each fold's rows. Returns the
+// triplet count; the row count goes to *nrows.
+long orc_constraints(Plate* p, const double* x, int flag_rig, long* nrows) {
+    p->c.clear(); p->J.clear();
+    int row = 0;
+    for (int kind = 1; kind <= 2; ++kind)
+        for (const Domain& d : p->doms)
+            if (d.bc.kind == kind && (d.bc.au || d.bc.av || d.bc.aw || d.bc.arot))
+                dirichlet_rows(d, x, row, p->c, p->J);
+    for (const auto& f : p->folds)
+        fold_rows(p->doms[f.left], p->doms[f.right], f, x, flag_rig != 0, row, p->c, p->J);
+    *nrows = row;
+    return static_cast<long>(p->J.size());
+}
+
+void orc_get_constraints(Plate* p, double* c, int* rows, int* cols, double* vals) {
+    std::memcpy(c, p->c.data(), p->c.size() * sizeof(double));
+    for (size_t k = 0; k < p->J.size(); ++k) { rows[k] = p->J[k].r; cols[k] = p->J[k].c; vals[k] = p->J[k].v; }
+}
+
+// Meshless::computeExternalForce (plate/Meshless.h:60-100): dead forces of the
+// point, edge and fold domains and the edge moments, accumulated in the
+// reference's order (points, then per edge force then moment, then folds).
+void orc_external_force(Plate* p, double* F) {
+    const size_t nd = 3 * p->nodes.size();
+    for (size_t k = 0; k < nd; ++k) F[k] = 0.0;
+    auto force = [&](const Domain& d) {
+        const double* fv = d.bc.force;
+        if (fv[0] * fv[0] + fv[1] * fv[1] + fv[2] * fv[2] <= 0.0) return;
+        for (size_t g = 0; g < d.X.size(); ++g) {
+            const double W = d.w[g];
+            for (int j = d.nl.off[g]; j < d.nl.off[g + 1]; ++j) {
+                const double pw = d.sh.psi[j] * W;
+                for (int k = 0; k < 3; ++k) F[3 * static_cast<size_t>(d.nl.ids[j]) + k] += pw * fv[k];
+            }
+        }
+    };
+    auto moment = [&](const Domain& d) {
+        if (d.bc.moment == 0.0) return;
+        for (size_t g = 0; g < d.X.size(); ++g) {
+            double side = 0.0;                                        // :86-88
+            for (int j = d.nl.off[g]; j < d.nl.off[g + 1]; ++j) {
+                const V2 X = p->nodes[d.nl.ids[j]];
+                side += (X.x - d.X[g].x) * d.normal[g].x + (X.y - d.X[g].y) * d.normal[g].y;
+            }
+            const double nx = side > 0.0 ? -d.normal[g].x : d.normal[g].x;
+            const double ny = side > 0.0 ? -d.normal[g].y : d.normal[g].y;
+            for (int j = d.nl.off[g]; j < d.nl.off[g + 1]; ++j)
+                F[3 * static_cast<size_t>(d.nl.ids[j]) + 2] +=
+                    d.w[g] * d.bc.moment * (d.sh.dpx[j] * nx + d.sh.dpy[j] * ny);
+        }
+    };
+    for (const Domain& d : p->doms) if (d.bc.kind == 1) force(d);
+    for (const Domain& d : p->doms) if (d.bc.kind == 2) { force(d); moment(d); }
+    for (const auto& f : p->folds) { force(p->doms[f.left]); force(p->doms[f.right]); }
 }
 
 }  // extern "C"

@@ -61,6 +61,10 @@ def lib():
             "orc_last_seconds": (C.c_double, [P]),
             "orc_fold_angle": (C.c_double, [_d, C.c_double, C.c_double, _d]),
             "orc_kinematics": (None, [P, C.c_int, _d, _d]),
+            "orc_set_domain_bc": (None, [P, C.c_int, C.c_int, C.c_int, _d, _d, C.c_double]),
+            "orc_constraints": (C.c_long, [P, _d, C.c_int, C.POINTER(C.c_long)]),
+            "orc_get_constraints": (None, [P, _d, _i, _i, _d]),
+            "orc_external_force": (None, [P, _d]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -203,6 +207,26 @@ class OraclePlate:
     def last_seconds(self) -> float:
         return self._L.orc_last_seconds(self._p)
 
+    def set_bc(self, dom, kind, active=0, val=(0, 0, 0, 0), force=(0, 0, 0), moment=0.0):
+        """BoundaryCondition of a domain; kind 1 = point, 2 = edge, 0 = other."""
+        v = _f64(np.asarray(val, float)); f = _f64(np.asarray(force, float))
+        self._L.orc_set_domain_bc(self._p, dom, kind, int(active), _dp(v), _dp(f), float(moment))
+
+    def constraints(self, x, flag_rig=False):
+        """Meshless::evaluateConstraints: c and the COO triplets (rows, cols, vals)
+        in push order."""
+        x = _f64(x)
+        nr = C.c_long(0)
+        nt = self._L.orc_constraints(self._p, _dp(x), int(flag_rig), C.byref(nr))
+        c = np.empty(nr.value); r = np.empty(nt, np.int32); col = np.empty(nt, np.int32); v = np.empty(nt)
+        self._L.orc_get_constraints(self._p, _dp(c), _ip(r), _ip(col), _dp(v))
+        return c, r, col, v
+
+    def external_force(self):
+        F = np.empty(3 * self.nn)
+        self._L.orc_external_force(self._p, _dp(F))
+        return F
+
     def kinematics(self, dom, x):
         x = _f64(x)
         out = np.empty((self.domains[dom], 15))
@@ -227,6 +251,11 @@ def plate_from_sheet(sh, nthreads: int = 1, compute: bool = True):
         fold_doms.append((l, r))
     p.fold_doms = fold_doms
     p.face_dom = face
+    p.bnd_doms = []
+    for bd in getattr(sh, "boundaries", []):
+        d = p.add_domain(bd.gps)
+        p.set_bc(d, 1 if bd.kind == "point" else 2, bd.active, bd.val, bd.force, bd.moment)
+        p.bnd_doms.append(d)
     if compute:
         for d in range(len(p.domains)):
             p.neighbours(d, sh.rho)

@@ -1,0 +1,96 @@
+"""Constraint rows, external load and the Newton load path on the device,
+against the oracle.
+
+Bars: the triplet pattern bit-identical; constraint values, Jacobian values and
+the external load within the shape-function round-off (they are linear in Psi,
+which the device and the oracle compute to ~1e-13, not bit-identically); the converged
+displacement of the 20-increment load path within 1e-8 (inf-norm, relative)
+of the oracle's, with every increment converged (golden fixture written by
+tests/golden/make_newton_fixture.py). The FP32-tangent (mixed) path reports
+its Newton iteration counts and must converge to the same state.
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+import continuation as CS
+import curvami_gpu as G
+import oracle as O
+import workloads as W
+from _cases import RHO_FACTOR
+
+pytestmark = pytest.mark.gpu
+
+FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "newton_straight21_moment.npz")
+
+
+@pytest.fixture(scope="module")
+def c1m():
+    sh = W.with_edge_moment_load(W.config1(rho_factor=RHO_FACTOR))
+    return sh, O.plate_from_sheet(sh, nthreads=4), G.plate_from_sheet(sh)
+
+
+@pytest.mark.parametrize("rigid", [False, True])
+def test_constraint_rows_match_oracle(c1m, rigid):
+    sh, orc, gpu = c1m
+    x = W.rigid_fold_displacement(sh, 0.5, seed=3, sigma=1e-3)
+    rows, cols = gpu.constraint_layout(rigid=rigid)
+    c_o, r_o, col_o, v_o = orc.constraints(x, flag_rig=rigid)
+    assert np.array_equal(rows, r_o) and np.array_equal(cols, col_o), "triplet pattern differs"
+    c_g, v_g = gpu.constraints(x)
+    assert c_g.shape == c_o.shape and v_g.shape == v_o.shape
+    ec = np.abs(c_g - c_o).max()
+    ev = np.abs(v_g - v_o).max()
+    print(f"rigid={rigid}: {c_o.size} rows, {v_o.size} triplets, max|dc| {ec:.2e} (max|c| {np.abs(c_o).max():.2e}), "
+          f"max|dJ| {ev:.2e} (max|J| {np.abs(v_o).max():.2e})")
+    # c is a Psi-weighted sum of |x| <= ~3e-3 (linear rows) or an angle (rigid rows)
+    assert ec <= 1e-12
+    assert ev <= 1e-10 * max(1.0, np.abs(v_o).max())
+
+
+def test_external_force(c1m):
+    sh, orc, gpu = c1m
+    Fo = orc.external_force()
+    Fg = gpu.external_force()
+    assert np.abs(Fo).max() > 0
+    e = np.abs(Fg - Fo).max() / np.abs(Fo).max()
+    print(f"F_ext: max rel {e:.2e}")
+    assert e <= 1e-11
+
+
+def _run_path(gpu, fp32):
+    ref = np.load(FIXTURE)
+    prob = CS.PlateProblem(gpu, flag_rig=False, fp32_tangent=fp32)
+    solver = CS.ContinuationSolver()
+    solver.tol_force = float(ref["tol_force"])
+    solver.tol_energy = float(ref["tol_energy"])
+    t0 = time.perf_counter()
+    res = CS.load_path(prob, increments=W.NEWTON_INCREMENTS, lam_total=1.0, solver=solver)
+    dt = time.perf_counter() - t0
+    return ref, res, prob, dt
+
+
+def test_newton_load_path_matches_oracle(c1m):
+    sh, _, gpu = c1m
+    ref, res, prob, dt = _run_path(gpu, fp32=False)
+    err = np.abs(res.x - ref["x"]).max() / np.abs(ref["x"]).max()
+    print(f"FP64 path: {len(res.converged)} increments, iterations {res.iterations} "
+          f"(oracle {ref['iterations'].tolist()}), |dx|/|x| {err:.3e}, {dt:.1f} s, "
+          f"{prob.assemblies} tangents, {prob.residuals} residuals")
+    assert res.all_converged and len(res.converged) == W.NEWTON_INCREMENTS == len(ref["iterations"])
+    assert err <= 1e-8
+
+
+def test_newton_mixed_precision_tangent(c1m):
+    """FP32 face tangent (inexact Newton), FP64 residual and constraint rows: the
+    path converges to the FP64 state; the iteration counts are reported."""
+    sh, _, gpu = c1m
+    ref, res, prob, dt = _run_path(gpu, fp32=True)
+    err = np.abs(res.x - ref["x"]).max() / np.abs(ref["x"]).max()
+    d_it = int(np.sum(res.iterations) - np.sum(ref["iterations"]))
+    print(f"mixed path: iterations {res.iterations} (sum {sum(res.iterations)}, oracle FP64 sum "
+          f"{int(np.sum(ref['iterations']))}, delta {d_it:+d}), |dx|/|x| {err:.3e}, {dt:.1f} s")
+    assert res.all_converged and len(res.converged) == W.NEWTON_INCREMENTS
+    assert err <= 1e-8
