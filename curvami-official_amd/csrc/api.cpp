@@ -6,10 +6,6 @@
 
 #include "cvm_internal.h"
 
-struct cvm_context {
-    cvm::Context c;
-};
-
 namespace cvm {
 
 Context::~Context() {
@@ -145,6 +141,8 @@ int cvm_create(int device, cvm_context** out) {
         delete ctx;
         return CVM_E_HIP;
     }
+    ctx->c.ws.stream = ctx->c.stream;
+    ctx->c.ws.timed = true;
     int lds = 0;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
         ctx->c.lds_limit = static_cast<size_t>(lds);
@@ -277,6 +275,7 @@ int cvm_set_fold_gp_state(cvm_context* ctx, int fold_id, int n, const double* th
     CVM_HIP(hipStreamSynchronize(c.stream));
     if (active_changed) c.plan.ready = false;   // the fold pattern depends on activity
     c.cons.ready = false;                        // so do the fold rows
+    if (c.plan.ready) cvm::upload_fold_descriptors(c);   // theta0 pointer
     CVM_CATCH(ctx)
 }
 
@@ -428,7 +427,7 @@ int cvm_assemble(cvm_context* ctx, const double* x, int flags, double* F_int, do
     } else if (!Fd || (tangent && !Kd)) {
         throw Error(CVM_E_INVALID, "device output pointer is null");
     }
-    cvm::launch_assembly(c, flags, xd, Fd, Kd);
+    cvm::launch_assembly(c, c.ws, flags, xd, Fd, Kd);
     if (!dev) {
         if (F_int) CVM_HIP(hipMemcpyAsync(F_int, Fd, 3 * sizeof(double) * c.nn, hipMemcpyDeviceToHost, c.stream));
         if (tangent && K_values && nnz)

@@ -494,81 +494,110 @@ size_t tile_lds_bytes(int rec_cap, bool fp32) {
          + tt * (static_cast<size_t>(rec_cap) * 6 + TILE_GPS * GS) + sizeof(uint32_t) * UMAX + TILE_GPS * UMAX;
 }
 
-void launch_assembly(Context& c, int flags, const double* xdev, double* Fdev, double* Kdev) {
+size_t assembly_workspace_bytes(const Context& c, int flags) {
+    const Plan& P = c.plan;
+    const bool tangent = (flags & CVM_TANGENT) != 0;
+    const bool fp32 = (flags & CVM_FP32_TANGENT) != 0;
+    size_t b = sizeof(double) * std::max<int64_t>(3 * P.nfscratch, 1);
+    if (tangent) b += (fp32 ? sizeof(float) : sizeof(double)) * std::max<int64_t>(9 * P.nscratch, 1);
+    b += sizeof(double) * (static_cast<size_t>(FGP_REC) * P.nfgp + static_cast<size_t>(FND_REC) *
+                           std::max<int64_t>(P.nfnodes, 1));
+    return b;
+}
+
+void reserve_workspace(const Context& c, Workspace& ws, int flags) {
+    const Plan& P = c.plan;
+    ws.fscr.ensure(std::max<int64_t>(3 * P.nfscratch, 1));
+    if (flags & CVM_TANGENT) {
+        const bool fp32 = (flags & CVM_FP32_TANGENT) != 0;
+        ws.scr.ensure(std::max<int64_t>(fp32 ? (9 * P.nscratch + 1) / 2 : 9 * P.nscratch, 1));
+    }
+    ws.fold_gp.ensure(std::max<int64_t>(static_cast<int64_t>(FGP_REC) * P.nfgp, 1));
+    ws.fold_nd.ensure(static_cast<size_t>(FND_REC) * std::max<int64_t>(P.nfnodes, 1));
+}
+
+void upload_fold_descriptors(Context& c) {
+    Plan& P = c.plan;
+    std::vector<FoldDev> fdv(std::max<size_t>(c.folds.size(), 1));
+    for (size_t f = 0; f < c.folds.size(); ++f) {
+        const Fold& fo = c.folds[f];
+        Domain& L = *c.doms[fo.left];
+        Domain& R = *c.doms[fo.right];
+        FoldDev& v = fdv[f];
+        v.lsf = L.sf.p; v.rsf = R.sf.p; v.loff = L.off.p; v.roff = R.off.p; v.lids = L.ids.p; v.rids = R.ids.p;
+        v.lnp = L.npairs; v.rnp = R.npairs; v.w = L.d_w.p; v.tx = L.d_tx.p; v.ty = L.d_ty.p;
+        v.theta0 = fo.theta0.empty() ? nullptr : fo.d_theta0.p;
+        v.Kt = fo.Kt; v.Ang0 = fo.Ang0;
+    }
+    P.d_folddev.upload(reinterpret_cast<const unsigned char*>(fdv.data()), fdv.size() * sizeof(FoldDev), c.stream);
+    CVM_HIP(hipStreamSynchronize(c.stream));
+}
+
+// One residual (+ tangent) evaluation on workspace ws. Reads only the plan and
+// the shape functions, so evaluations on different workspaces may run
+// concurrently on different streams.
+void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, double* Fdev, double* Kdev) {
     Plan& P = c.plan;
     if (!P.ready) throw Error(CVM_E_STATE, "assembly plan not initialised (cvm_init_assembly)");
     Domain* fdp = nullptr;
     for (Domain* d : c.doms) if (d->kind == CVM_DOMAIN_FACE) fdp = d;
     Domain& fd = *fdp;
+    const hipStream_t st = ws.stream;
+    auto tbeg = [&](const char* name) { return ws.timed ? timing_begin(c, name) : -1; };
+    auto tend = [&](int slot) { if (ws.timed) timing_end(c, slot); };
     const bool tangent = (flags & CVM_TANGENT) != 0;
     const bool folds = (flags & CVM_ELASTIC_FOLDS) != 0 && P.nfgp > 0;
     const bool exact = folds && tangent && (flags & CVM_EXACT_HINGE) != 0;
+    const bool fp32 = (flags & CVM_FP32_TANGENT) != 0;
     MatC m;
     std::memcpy(m.A, c.A, sizeof(m.A));
     std::memcpy(m.B, c.B, sizeof(m.B));
     std::memcpy(m.D, c.D, sizeof(m.D));
-    c.d_fscr.alloc(std::max<int64_t>(3 * P.nfscratch, 1));
-    const bool fp32 = (flags & CVM_FP32_TANGENT) != 0;
-    // scratch rows: 9 values of the tangent type (bytes held in a double buffer)
-    if (tangent) c.d_scr.alloc(std::max<int64_t>(fp32 ? (9 * P.nscratch + 1) / 2 : 9 * P.nscratch, 1));
+    reserve_workspace(c, ws, flags);
     const size_t lds = tile_lds_bytes(P.rec_cap, fp32);
-    int slot = timing_begin(c, !tangent ? "tile_residual" : (fp32 ? "tile_residual_tangent_fp32" : "tile_residual_tangent"));
+    int slot = tbeg(!tangent ? "tile_residual" : (fp32 ? "tile_residual_tangent_fp32" : "tile_residual_tangent"));
     const bool has_b = m.B[0] != 0.0 || m.B[1] != 0.0 || m.B[2] != 0.0 || m.B[3] != 0.0 || m.B[4] != 0.0 ||
                        m.B[5] != 0.0 || m.B[6] != 0.0 || m.B[7] != 0.0 || m.B[8] != 0.0;
     auto tile = [&](auto kern, auto* scr) {
-        hipLaunchKernelGGL(kern, dim3(P.ntiles), dim3(TILE_THREADS), lds, c.stream, P.d_tile_gp.p, P.d_tile_uoff.p,
+        hipLaunchKernelGGL(kern, dim3(P.ntiles), dim3(TILE_THREADS), lds, st, P.d_tile_gp.p, P.d_tile_uoff.p,
                            P.d_tile_unodes.p, P.d_tile_toff.p, P.d_tile_tpairs.p, P.d_slot_dst.p, fd.off.p,
                            P.d_pair_uloc.p, fd.sf.p, static_cast<int64_t>(fd.npairs), fd.d_w.p, xdev, m, P.rec_cap,
-                           tangent ? 1 : 0, scr, c.d_fscr.p);
+                           tangent ? 1 : 0, scr, ws.fscr.p);
     };
-    float* scr32 = reinterpret_cast<float*>(c.d_scr.p);
+    float* scr32 = reinterpret_cast<float*>(ws.scr.p);
     if (fp32) tile(has_b ? k_tile<true, float> : k_tile<false, float>, scr32);
-    else tile(has_b ? k_tile<true, double> : k_tile<false, double>, c.d_scr.p);
+    else tile(has_b ? k_tile<true, double> : k_tile<false, double>, ws.scr.p);
     CVM_HIP(hipGetLastError());
-    timing_end(c, slot);
+    tend(slot);
     if (folds) {
-        std::vector<FoldDev> fdv(c.folds.size());
-        for (size_t f = 0; f < c.folds.size(); ++f) {
-            const Fold& fo = c.folds[f];
-            Domain& L = *c.doms[fo.left];
-            Domain& R = *c.doms[fo.right];
-            FoldDev& v = fdv[f];
-            v.lsf = L.sf.p; v.rsf = R.sf.p; v.loff = L.off.p; v.roff = R.off.p; v.lids = L.ids.p; v.rids = R.ids.p;
-            v.lnp = L.npairs; v.rnp = R.npairs; v.w = L.d_w.p; v.tx = L.d_tx.p; v.ty = L.d_ty.p;
-            v.theta0 = fo.theta0.empty() ? nullptr : fo.d_theta0.p;
-            v.Kt = fo.Kt; v.Ang0 = fo.Ang0;
-        }
-        c.d_folddev.upload(reinterpret_cast<const unsigned char*>(fdv.data()), fdv.size() * sizeof(FoldDev), c.stream);
-        c.d_fold_gp.alloc(static_cast<size_t>(FGP_REC) * P.nfgp);
-        c.d_fold_nd.alloc(static_cast<size_t>(FND_REC) * std::max<int64_t>(P.nfnodes, 1));
-        slot = timing_begin(c, "fold_hinge");
-        hipLaunchKernelGGL(k_fold, dim3((P.nfgp + 63) / 64), dim3(64), 0, c.stream, P.nfgp,
-                           reinterpret_cast<const FoldDev*>(c.d_folddev.p), P.d_fgp_fold.p, P.d_fgp_local.p,
+        slot = tbeg("fold_hinge");
+        hipLaunchKernelGGL(k_fold, dim3((P.nfgp + 63) / 64), dim3(64), 0, st, P.nfgp,
+                           reinterpret_cast<const FoldDev*>(P.d_folddev.p), P.d_fgp_fold.p, P.d_fgp_local.p,
                            P.d_fgp_noff.p, P.d_fgp_lpo.p, P.d_fgp_rpo.p, P.d_fgp_lpos.p, P.d_fgp_rpos.p, xdev,
-                           exact ? 1 : 0, c.d_fold_gp.p, c.d_fold_nd.p);
+                           exact ? 1 : 0, ws.fold_gp.p, ws.fold_nd.p);
         CVM_HIP(hipGetLastError());
-        timing_end(c, slot);
+        tend(slot);
     }
-    slot = timing_begin(c, "reduce_F");
-    hipLaunchKernelGGL(k_reduce_F, dim3((c.nn + 255) / 256), dim3(256), 0, c.stream, c.nn, P.d_fc_off.p, P.d_fc_idx.p,
-                       c.d_fscr.p, P.d_ff_off.p, P.d_ff_idx.p, P.d_fnd_gp.p, c.d_fold_gp.p, c.d_fold_nd.p,
+    slot = tbeg("reduce_F");
+    hipLaunchKernelGGL(k_reduce_F, dim3((c.nn + 255) / 256), dim3(256), 0, st, c.nn, P.d_fc_off.p, P.d_fc_idx.p,
+                       ws.fscr.p, P.d_ff_off.p, P.d_ff_idx.p, P.d_fnd_gp.p, ws.fold_gp.p, ws.fold_nd.p,
                        folds ? 1 : 0, Fdev);
     CVM_HIP(hipGetLastError());
-    timing_end(c, slot);
+    tend(slot);
     if (tangent) {
         const bool uni = (flags & CVM_ELASTIC_FOLDS) != 0;
-        slot = timing_begin(c, "reduce_K");
+        slot = tbeg("reduce_K");
         auto red = [&](auto kern, auto* scr) {
-            hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>((P.nblocks + 255) / 256)), dim3(256), 0, c.stream,
+            hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>((P.nblocks + 255) / 256)), dim3(256), 0, st,
                                P.nblocks, P.d_kc_off.p, scr, P.d_kf_off.p, P.d_kf_idx.p, P.d_fnd_gp.p,
-                               c.d_fold_gp.p, c.d_fold_nd.p, folds ? 1 : 0, exact ? 1 : 0,
+                               ws.fold_gp.p, ws.fold_nd.p, folds ? 1 : 0, exact ? 1 : 0,
                                uni ? P.d_blk_base_u.p : P.d_blk_base_f.p,
                                uni ? P.d_blk_stride_u.p : P.d_blk_stride_f.p, Kdev);
         };
         if (fp32) red(k_reduce_K<float>, scr32);
-        else red(k_reduce_K<double>, c.d_scr.p);
+        else red(k_reduce_K<double>, ws.scr.p);
         CVM_HIP(hipGetLastError());
-        timing_end(c, slot);
+        tend(slot);
     }
 }
 

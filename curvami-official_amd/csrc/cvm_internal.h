@@ -54,6 +54,11 @@ struct DBuf {
         }
         n = count;
     }
+    // grow-only: keeps a larger buffer (workspaces shared by sheets of different sizes)
+    void ensure(size_t count) {
+        if (count <= n && (p || count == 0)) return;
+        alloc(count);
+    }
     void upload(const T* h, size_t count, hipStream_t s) {
         alloc(count);
         if (count) CVM_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
@@ -133,6 +138,18 @@ struct ConstraintLayout {
     DBuf<double> d_fx_val;        // 3 doubles per contribution: coefficient, n_out x, n_out y
 };
 
+// What one assembly in flight needs besides the plan: its stream, the tile
+// scratch rows, the per-union-node residual rows and the hinge records. The
+// context owns one (on its own stream); a serving loop owns one per slot.
+struct Workspace {
+    hipStream_t stream = nullptr;
+    bool timed = false;          // kernel timings go to the context's Timing
+    DBuf<double> scr;            // 9 * nscratch values of the tangent type
+    DBuf<double> fscr;           // 3 * nfscratch
+    DBuf<double> fold_gp;        // per crease GP record
+    DBuf<double> fold_nd;        // per (crease GP, node) record
+};
+
 // Assembly plan: tiles of face GPs, block pattern, contributor lists.
 struct Plan {
     bool ready = false;
@@ -176,6 +193,7 @@ struct Plan {
     DBuf<int> d_fgp_lpo, d_fgp_rpo;     // per crease GP: offsets into lpos / rpos
     DBuf<int> d_fnd_gp;                 // node-set entry -> crease GP
     int64_t nfnodes = 0;                // node-set entries over all crease GPs
+    DBuf<unsigned char> d_folddev;      // per fold: device descriptor (shape functions, weights, theta0)
     DBuf<int64_t> d_kf_off;             // per block: fold contributor offsets
     DBuf<int> d_kf_idx;                 // per contribution: node-set entries of lo and hi
     DBuf<int> d_ff_off;                 // per node: fold contributions (nodeset entry indices)
@@ -221,11 +239,7 @@ struct Context {
     DBuf<double> d_x;           // 3 nn displacements
     DBuf<double> d_F;           // 3 nn
     DBuf<double> d_K;           // nnz (union)
-    DBuf<double> d_scr;         // 9 * nscratch
-    DBuf<double> d_fscr;        // 3 * nfscratch
-    DBuf<double> d_fold_gp;     // per crease GP: W*Kt, W*moment
-    DBuf<double> d_fold_nd;     // per (crease GP, node of L u R): aggregated dtheta/du
-    DBuf<unsigned char> d_folddev;
+    Workspace ws;               // the context's own assembly workspace (its stream)
     size_t lds_limit = 65536;
     Timing timing;
     int tile_gps = 16;
@@ -238,7 +252,10 @@ void launch_indicator_gp(Context& c, Domain& d);
 void launch_node_signs(Context& c);
 void launch_frnn(Context& c, Domain& d, double rho);
 void launch_rkpm(Context& c, Domain& d, double rho, int order, int kernel, int* code, int* nbad, int* first);
-void launch_assembly(Context& c, int flags, const double* xdev, double* Fdev, double* Kdev);
+void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, double* Fdev, double* Kdev);
+size_t assembly_workspace_bytes(const Context& c, int flags);
+void reserve_workspace(const Context& c, Workspace& ws, int flags);
+void upload_fold_descriptors(Context& c);
 size_t tile_lds_bytes(int rec_cap, bool fp32);
 constexpr int kTileGps = 16;   // GPs per assembly tile
 constexpr int kTileUnion = 64; // union nodes per assembly tile
@@ -253,3 +270,9 @@ void launch_constraints(Context& c, const double* xdev, double* cdev, double* jd
 void launch_external_force(Context& c, double* Fdev);
 
 }  // namespace cvm
+
+// The opaque handle of the C ABI.
+struct cvm_context {
+    cvm::Context c;
+};
+

@@ -384,6 +384,7 @@ void build_plan(Context& c) {
     P.face_ng = ng;
     P.face_npairs = fd.npairs;
     CVM_HIP(hipStreamSynchronize(s));
+    upload_fold_descriptors(c);
     P.ready = true;
 }
 

@@ -1,0 +1,364 @@
+// Serving loop: residual / tangent evaluation requests on device-resident
+// sheets, evaluated concurrently on a pool of HIP streams.
+//
+//  * Sheets are prepared contexts (nodes, neighbour lists, shape functions and
+//    the assembly plan stay in HBM); a request names a sheet and a displacement.
+//  * Slots: up to max_inflight evaluations are in flight at once, each on its own
+//    stream with its own workspace (scratch rows, x, F, optionally K). Requests
+//    that arrive while others run are admitted as soon as a slot is free and the
+//    workspace bytes of the admitted requests stay within the HBM budget, so
+//    their kernels overlap on the device (continuous batching).
+//  * A request's arithmetic does not depend on what else is in flight: each
+//    evaluation reads only its sheet's plan and writes only its own workspace,
+//    so results are bit-identical alone and in any mix.
+//  * One dispatcher thread admits, launches and retires requests (event polling);
+//    submit and wait may be called from any host thread.
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+#include "cvm_internal.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+struct SheetRef {
+    cvm_context* ctx = nullptr;
+    int nn = 0;
+    int64_t nnz_union = 0, nnz_face = 0;
+};
+
+struct Request {
+    int sheet = 0;
+    int flags = 0;
+    std::vector<double> x;          // copied at submit
+    double* F_host = nullptr;
+    double* K_dev = nullptr;        // caller's device buffer, or null (slot buffer)
+    size_t bytes = 0;               // workspace bytes charged against the budget
+    int slot = -1;
+    int status = CVM_OK;
+    std::string err;
+    bool done = false;
+    Clock::time_point t_submit, t_start, t_done;
+};
+
+struct Slot {
+    cvm::Workspace ws;
+    cvm::DBuf<double> x, F, K;
+    double* hx = nullptr;           // pinned staging
+    double* hF = nullptr;
+    size_t hcap = 0;
+    hipEvent_t ev = nullptr;
+    int64_t req = -1;
+    size_t cap_bytes = 0;           // workspace bytes currently allocated
+};
+
+}  // namespace
+
+struct cvm_server {
+    int device = 0;
+    int64_t budget = 0;
+    std::vector<SheetRef> sheets;
+    std::vector<Slot> slots;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<int64_t> pending;
+    std::unordered_map<int64_t, Request> reqs;
+    int64_t next_id = 1;
+    int64_t in_use = 0;              // bytes charged by in-flight requests
+    int64_t completed = 0;
+    int inflight = 0;
+    bool stop = false;
+    std::string err;
+    std::thread worker;
+
+    size_t request_bytes(const SheetRef& s, int flags, bool own_K) const {
+        size_t b = cvm::assembly_workspace_bytes(s.ctx->c, flags);
+        b += 2 * sizeof(double) * 3 * static_cast<size_t>(s.nn);          // x, F
+        if ((flags & CVM_TANGENT) && own_K)
+            b += sizeof(double) * static_cast<size_t>((flags & CVM_ELASTIC_FOLDS) ? s.nnz_union : s.nnz_face);
+        return b;
+    }
+
+    // allocated workspace bytes over all slots
+    size_t allocated() const {
+        size_t t = 0;
+        for (const Slot& s : slots) t += s.cap_bytes;
+        return t;
+    }
+
+    void launch(int64_t id, Request& r, int si) {
+        Slot& sl = slots[si];
+        const SheetRef& sh = sheets[r.sheet];
+        cvm::Context& c = sh.ctx->c;
+        const size_t n3 = 3 * static_cast<size_t>(sh.nn);
+        const bool own_K = r.K_dev == nullptr;
+        const int64_t nnz = (r.flags & CVM_ELASTIC_FOLDS) ? sh.nnz_union : sh.nnz_face;
+        cvm::reserve_workspace(c, sl.ws, r.flags);
+        sl.x.ensure(n3);
+        sl.F.ensure(n3);
+        if ((r.flags & CVM_TANGENT) && own_K) sl.K.ensure(std::max<int64_t>(nnz, 1));
+        if (sl.hcap < n3) {
+            if (sl.hx) (void)hipHostFree(sl.hx);
+            if (sl.hF) (void)hipHostFree(sl.hF);
+            sl.hx = sl.hF = nullptr;
+            sl.hcap = 0;
+            CVM_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.hx), n3 * sizeof(double), hipHostMallocDefault));
+            CVM_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.hF), n3 * sizeof(double), hipHostMallocDefault));
+            sl.hcap = n3;
+        }
+        sl.cap_bytes = std::max(sl.cap_bytes, r.bytes);
+        std::memcpy(sl.hx, r.x.data(), n3 * sizeof(double));
+        CVM_HIP(hipMemcpyAsync(sl.x.p, sl.hx, n3 * sizeof(double), hipMemcpyHostToDevice, sl.ws.stream));
+        double* Kd = (r.flags & CVM_TANGENT) ? (own_K ? sl.K.p : r.K_dev) : nullptr;
+        cvm::launch_assembly(c, sl.ws, r.flags & ~CVM_DEVICE_PTRS, sl.x.p, sl.F.p, Kd);
+        CVM_HIP(hipMemcpyAsync(sl.hF, sl.F.p, n3 * sizeof(double), hipMemcpyDeviceToHost, sl.ws.stream));
+        CVM_HIP(hipEventRecord(sl.ev, sl.ws.stream));
+        sl.req = id;
+        r.slot = si;
+        r.t_start = Clock::now();
+    }
+
+    void retire(int si) {
+        Slot& sl = slots[si];
+        Request& r = reqs[sl.req];
+        const SheetRef& sh = sheets[r.sheet];
+        if (r.F_host) std::memcpy(r.F_host, sl.hF, 3 * sizeof(double) * sh.nn);
+        r.t_done = Clock::now();
+        r.done = true;
+        r.x.clear();
+        r.x.shrink_to_fit();
+        in_use -= static_cast<int64_t>(r.bytes);
+        --inflight;
+        ++completed;
+        sl.req = -1;
+    }
+
+    void fail(int64_t id, Request& r, const cvm::Error& e) {
+        r.status = e.code;
+        r.err = e.what();
+        r.done = true;
+        r.t_done = Clock::now();
+    }
+
+    void run() {
+        (void)hipSetDevice(device);
+        std::unique_lock<std::mutex> lk(mu);
+        while (true) {
+            // retire finished evaluations
+            for (size_t si = 0; si < slots.size(); ++si) {
+                if (slots[si].req < 0) continue;
+                const hipError_t q = hipEventQuery(slots[si].ev);
+                if (q == hipSuccess) {
+                    retire(static_cast<int>(si));
+                    cv_done.notify_all();
+                } else if (q != hipErrorNotReady) {
+                    Request& r = reqs[slots[si].req];
+                    r.status = CVM_E_HIP;
+                    r.err = std::string("HIP error while evaluating a request: ") + hipGetErrorString(q);
+                    r.done = true;
+                    r.t_done = Clock::now();
+                    in_use -= static_cast<int64_t>(r.bytes);
+                    --inflight;
+                    slots[si].req = -1;
+                    cv_done.notify_all();
+                }
+            }
+            // admit in arrival order
+            while (!pending.empty()) {
+                const int64_t id = pending.front();
+                Request& r = reqs[id];
+                int free_fit = -1, free_any = -1;
+                for (size_t si = 0; si < slots.size(); ++si) {
+                    if (slots[si].req >= 0) continue;
+                    if (slots[si].cap_bytes >= r.bytes &&
+                        (free_fit < 0 || slots[si].cap_bytes < slots[free_fit].cap_bytes))
+                        free_fit = static_cast<int>(si);
+                    if (free_any < 0 || slots[si].cap_bytes > slots[free_any].cap_bytes) free_any = static_cast<int>(si);
+                }
+                int use = free_fit;
+                if (use < 0 && free_any >= 0) {
+                    // growing a slot must keep the allocated workspaces within the budget
+                    const size_t grown = allocated() - slots[free_any].cap_bytes + r.bytes;
+                    if (grown <= static_cast<size_t>(budget) || inflight == 0) use = free_any;
+                }
+                if (use < 0) break;
+                if (inflight > 0 && in_use + static_cast<int64_t>(r.bytes) > budget) break;
+                pending.pop_front();
+                try {
+                    launch(id, r, use);
+                    in_use += static_cast<int64_t>(r.bytes);
+                    ++inflight;
+                } catch (const cvm::Error& e) {
+                    fail(id, r, e);
+                    cv_done.notify_all();
+                } catch (const std::exception& e) {
+                    fail(id, r, cvm::Error(CVM_E_NO_MEMORY, e.what()));
+                    cv_done.notify_all();
+                }
+            }
+            if (stop && inflight == 0 && pending.empty()) break;
+            if (inflight == 0) {
+                cv_work.wait(lk, [&] { return stop || !pending.empty(); });
+            } else {
+                lk.unlock();
+                std::this_thread::sleep_for(std::chrono::microseconds(20));
+                lk.lock();
+            }
+        }
+    }
+};
+
+extern "C" {
+
+int cvm_server_create(int device, int64_t hbm_budget, int max_inflight, cvm_server** out) {
+    if (!out || max_inflight < 1 || max_inflight > 64 || hbm_budget <= 0) return CVM_E_INVALID;
+    *out = nullptr;
+    auto* s = new cvm_server();
+    s->device = device;
+    s->budget = hbm_budget;
+    if (hipSetDevice(device) != hipSuccess) { delete s; return CVM_E_HIP; }
+    s->slots.resize(max_inflight);
+    for (Slot& sl : s->slots) {
+        if (hipStreamCreateWithFlags(&sl.ws.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess) {
+            for (Slot& t : s->slots) {
+                if (t.ws.stream) (void)hipStreamDestroy(t.ws.stream);
+                if (t.ev) (void)hipEventDestroy(t.ev);
+            }
+            delete s;
+            return CVM_E_HIP;
+        }
+    }
+    s->worker = std::thread([s] { s->run(); });
+    *out = s;
+    return CVM_OK;
+}
+
+void cvm_server_destroy(cvm_server* s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    (void)hipSetDevice(s->device);
+    for (Slot& sl : s->slots) {
+        if (sl.ws.stream) (void)hipStreamSynchronize(sl.ws.stream);
+        if (sl.hx) (void)hipHostFree(sl.hx);
+        if (sl.hF) (void)hipHostFree(sl.hF);
+        if (sl.ev) (void)hipEventDestroy(sl.ev);
+        sl.ws.scr.release(); sl.ws.fscr.release(); sl.ws.fold_gp.release(); sl.ws.fold_nd.release();
+        sl.x.release(); sl.F.release(); sl.K.release();
+        if (sl.ws.stream) (void)hipStreamDestroy(sl.ws.stream);
+    }
+    delete s;
+}
+
+const char* cvm_server_last_error(cvm_server* s) { return s ? s->err.c_str() : "null server"; }
+
+int cvm_server_add_sheet(cvm_server* s, cvm_context* ctx, int* sheet_id) {
+    if (!s || !ctx) return CVM_E_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    cvm::Context& c = ctx->c;
+    if (c.device != s->device) { s->err = "sheet context is on another device"; return CVM_E_INVALID; }
+    if (!c.plan.ready) { s->err = "sheet has no assembly plan (cvm_init_assembly)"; return CVM_E_STATE; }
+    SheetRef r;
+    r.ctx = ctx;
+    r.nn = c.nn;
+    r.nnz_union = static_cast<int64_t>(c.plan.uni_inner.size());
+    r.nnz_face = static_cast<int64_t>(c.plan.face_inner.size());
+    s->sheets.push_back(r);
+    if (sheet_id) *sheet_id = static_cast<int>(s->sheets.size()) - 1;
+    return CVM_OK;
+}
+
+int cvm_server_request_bytes(cvm_server* s, int sheet, int flags, int with_K, int64_t* bytes) {
+    if (!s || !bytes) return CVM_E_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (sheet < 0 || sheet >= static_cast<int>(s->sheets.size())) { s->err = "bad sheet id"; return CVM_E_INVALID; }
+    *bytes = static_cast<int64_t>(s->request_bytes(s->sheets[sheet], flags, with_K != 0));
+    return CVM_OK;
+}
+
+int cvm_server_submit(cvm_server* s, int sheet, const double* x, int flags, double* F_host, double* K_dev,
+                      int64_t* request_id) {
+    if (!s || !x || !request_id) return CVM_E_INVALID;
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (sheet < 0 || sheet >= static_cast<int>(s->sheets.size())) { s->err = "bad sheet id"; return CVM_E_INVALID; }
+    if (flags & CVM_DEVICE_PTRS) { s->err = "requests take host x and F (K may be a device buffer)"; return CVM_E_INVALID; }
+    const SheetRef& sh = s->sheets[sheet];
+    Request r;
+    r.sheet = sheet;
+    r.flags = flags | CVM_RESIDUAL;
+    r.F_host = F_host;
+    r.K_dev = K_dev;
+    r.bytes = s->request_bytes(sh, r.flags, K_dev == nullptr);
+    if (static_cast<int64_t>(r.bytes) > s->budget) {
+        s->err = "request workspace (" + std::to_string(r.bytes) + " bytes) exceeds the HBM budget";
+        return CVM_E_BUDGET;
+    }
+    r.x.assign(x, x + 3 * static_cast<size_t>(sh.nn));
+    r.t_submit = Clock::now();
+    const int64_t id = s->next_id++;
+    s->reqs.emplace(id, std::move(r));
+    s->pending.push_back(id);
+    *request_id = id;
+    lk.unlock();
+    s->cv_work.notify_one();
+    return CVM_OK;
+}
+
+// Blocks until request `id` is done; times (may be NULL) = queue ms (submit to
+// launch), device ms (launch to completion seen), total ms. The request is
+// forgotten afterwards.
+int cvm_server_wait(cvm_server* s, int64_t id, double* times) {
+    if (!s) return CVM_E_INVALID;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) { s->err = "unknown request id"; return CVM_E_INVALID; }
+    s->cv_done.wait(lk, [&] { return s->reqs[id].done; });
+    Request& r = s->reqs[id];
+    if (times) {
+        const bool started = r.slot >= 0;
+        times[0] = started ? ms_between(r.t_submit, r.t_start) : ms_between(r.t_submit, r.t_done);
+        times[1] = started ? ms_between(r.t_start, r.t_done) : 0.0;
+        times[2] = ms_between(r.t_submit, r.t_done);
+    }
+    const int st = r.status;
+    if (st != CVM_OK) s->err = r.err;
+    s->reqs.erase(id);
+    return st;
+}
+
+int cvm_server_poll(cvm_server* s, int64_t id, int* done) {
+    if (!s || !done) return CVM_E_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) { s->err = "unknown request id"; return CVM_E_INVALID; }
+    *done = it->second.done ? 1 : 0;
+    return CVM_OK;
+}
+
+// completed requests, requests in flight, pending requests, workspace bytes allocated
+int cvm_server_stats(cvm_server* s, int64_t* out4) {
+    if (!s || !out4) return CVM_E_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    out4[0] = s->completed;
+    out4[1] = s->inflight;
+    out4[2] = static_cast<int64_t>(s->pending.size());
+    out4[3] = static_cast<int64_t>(s->allocated());
+    return CVM_OK;
+}
+
+}  // extern "C"

@@ -53,6 +53,15 @@ SIGNATURES = {
     "cvm_constraints": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "cvm_external_force": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "cvm_synchronize": (C.c_int, [C.c_void_p]),
+    "cvm_server_create": (C.c_int, [C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_void_p)]),
+    "cvm_server_destroy": (None, [C.c_void_p]),
+    "cvm_server_last_error": (C.c_char_p, [C.c_void_p]),
+    "cvm_server_add_sheet": (C.c_int, [C.c_void_p, C.c_void_p, _i]),
+    "cvm_server_request_bytes": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _i64]),
+    "cvm_server_submit": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, _i64]),
+    "cvm_server_poll": (C.c_int, [C.c_void_p, C.c_int64, _i]),
+    "cvm_server_wait": (C.c_int, [C.c_void_p, C.c_int64, _d]),
+    "cvm_server_stats": (C.c_int, [C.c_void_p, _i64]),
     "cvm_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "cvm_kernel_count": (C.c_int, [C.c_void_p]),
     "cvm_kernel_stats": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int, _d, _i64]),
@@ -302,3 +311,113 @@ def plate_from_sheet(sh, device: int = 0, compute: bool = True, order=None, kern
             p.shape_functions(d, sh.order if order is None else order, sh.kernel if kernel is None else kernel)
         p.init_assembly()
     return p
+
+
+class DeviceArray:
+    """A float64 device buffer (hipMalloc through the HIP runtime); plumbing
+    for device-pointer entry points and the serving loop."""
+    _hip = None
+
+    @classmethod
+    def _rt(cls):
+        if cls._hip is None:
+            H = C.CDLL("libamdhip64.so")
+            H.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            H.hipFree.argtypes = [C.c_void_p]
+            H.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            cls._hip = H
+        return cls._hip
+
+    def __init__(self, n: int):
+        self.n = int(n)
+        p = C.c_void_p()
+        if self._rt().hipMalloc(C.byref(p), C.c_size_t(max(self.n, 1) * 8)) != 0:
+            raise CvmError(E_NO_MEMORY, f"hipMalloc of {self.n * 8} bytes failed")
+        self.ptr = p.value
+
+    def get(self) -> np.ndarray:
+        out = np.empty(self.n)
+        if self.n and self._rt().hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), self.n * 8, 2) != 0:
+            raise CvmError(E_HIP, "hipMemcpy D2H failed")
+        return out
+
+    def set(self, a):
+        a = _f64(a)
+        if a.size != self.n:
+            raise ValueError("size mismatch")
+        if self.n and self._rt().hipMemcpy(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), self.n * 8, 1) != 0:
+            raise CvmError(E_HIP, "hipMemcpy H2D failed")
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            self._rt().hipFree(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Server:
+    """The serving loop (cvm_server_*): prepared plates are registered as
+    sheets; submit() queues an evaluation and returns a request id; wait()
+    returns (status, queue_ms, device_ms, total_ms)."""
+
+    def __init__(self, hbm_budget_bytes: int, max_inflight: int = 8, device: int = 0):
+        self._L = load_library()
+        h = C.c_void_p()
+        st = self._L.cvm_server_create(device, int(hbm_budget_bytes), int(max_inflight), C.byref(h))
+        if st != OK:
+            raise CvmError(st, f"cvm_server_create failed with status {st}")
+        self._h = h
+        self._plates = []
+        self._keep = {}
+
+    def _check(self, st):
+        if st != OK:
+            raise CvmError(st, self._L.cvm_server_last_error(self._h).decode())
+
+    def add_sheet(self, plate) -> int:
+        sid = C.c_int(-1)
+        self._check(self._L.cvm_server_add_sheet(self._h, plate._h, C.byref(sid)))
+        self._plates.append(plate)
+        return sid.value
+
+    def request_bytes(self, sheet, flags, with_K=True) -> int:
+        b = C.c_int64(0)
+        self._check(self._L.cvm_server_request_bytes(self._h, sheet, flags, int(with_K), C.byref(b)))
+        return b.value
+
+    def submit(self, sheet, x, F_out, flags=RESIDUAL | TANGENT | ELASTIC_FOLDS, K_dev=None) -> int:
+        x = _f64(x)
+        rid = C.c_int64(0)
+        self._check(self._L.cvm_server_submit(self._h, sheet, x.ctypes.data, flags, F_out.ctypes.data,
+                                              C.c_void_p(K_dev.ptr) if K_dev is not None else None,
+                                              C.byref(rid)))
+        self._keep[rid.value] = (x, F_out, K_dev)
+        return rid.value
+
+    def wait(self, rid):
+        t = (C.c_double * 3)()
+        st = self._L.cvm_server_wait(self._h, C.c_int64(rid), t)
+        self._keep.pop(rid, None)
+        self._check(st)
+        return t[0], t[1], t[2]
+
+    def stats(self):
+        out = (C.c_int64 * 4)()
+        self._check(self._L.cvm_server_stats(self._h, out))
+        return {"completed": out[0], "inflight": out[1], "pending": out[2], "workspace_bytes": out[3]}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.cvm_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

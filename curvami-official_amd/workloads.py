@@ -422,3 +422,35 @@ def newton_tolerances(f_ext_inf: float) -> dict:
 
 
 NEWTON_INCREMENTS = 20
+
+
+# ---------------------------------------------------------------------------
+# Serving requests
+# ---------------------------------------------------------------------------
+SERVING_RHO_FACTOR = 3.75
+
+
+def serving_sheet(n_side: int, kernel: int = 0) -> Sheet:
+    """Sheet of a serving request: n x n nodes on the unit square, straight
+    crease on the node column nearest the centre ((n - 1) // 2), elastic hinge
+    1e-2 E t^3, quadratic RKPM with support 3.75 h."""
+    X, Y, h = grid_nodes(n_side)
+    cr = straight_crease_on_row(n_side, h, (n_side - 1) // 2, vertical=True)
+    E, t = 1.0, 0.01
+    return square_sheet(f"serve_{n_side}", n_side, [cr], [1e-2 * E * t ** 3],
+                        rho_factor=SERVING_RHO_FACTOR, kernel=kernel)
+
+
+def serving_requests(nreq: int, seed: int = 4, n_min: int = 21, n_max: int = 201):
+    """(side, fold angle in radians, noise seed) per request: side ~ U{n_min..n_max},
+    angle ~ U[0, 170 deg]."""
+    u = uniform01(seed, 2 * nreq).reshape(nreq, 2)
+    sides = np.minimum(n_min + (u[:, 0] * (n_max - n_min + 1)).astype(int), n_max)
+    angles = u[:, 1] * math.radians(170.0)
+    return [(int(sides[i]), float(angles[i]), seed * 1000003 + i) for i in range(nreq)]
+
+
+def poisson_arrivals(rate: float, n: int, seed: int) -> np.ndarray:
+    """Arrival times (s) of a Poisson stream: exponential gaps -ln(1 - U) / rate."""
+    u = uniform01(seed, n)
+    return np.cumsum(-np.log1p(-u) / rate)

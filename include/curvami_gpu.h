@@ -170,6 +170,38 @@ CVM_API int cvm_external_force(cvm_context* ctx, int flags, double* F_ext);
 /* Wait for the context's stream. */
 CVM_API int cvm_synchronize(cvm_context* ctx);
 
+/* ---- serving ------------------------------------------------------------- */
+/* No counterpart in the reference, which evaluates one system at a time from
+ * its Newton loops. Callers: request loops that evaluate many residual/tangent
+ * requests (landscape sweeps, arc-length probes) on many sheets at once.
+ *
+ * A server owns max_inflight slots (HIP stream + workspace). Sheets are prepared
+ * contexts (shape functions and assembly plan resident in HBM) registered with
+ * the server; they must not be modified while registered. Requests are admitted
+ * in arrival order as soon as a slot is free and the workspace bytes of the
+ * requests in flight stay within hbm_budget; admitted requests run concurrently.
+ * A request's result does not depend on what else is in flight. */
+typedef struct cvm_server cvm_server;
+CVM_API int cvm_server_create(int device, int64_t hbm_budget, int max_inflight, cvm_server** out);
+CVM_API void cvm_server_destroy(cvm_server* s);
+CVM_API const char* cvm_server_last_error(cvm_server* s);
+CVM_API int cvm_server_add_sheet(cvm_server* s, cvm_context* sheet, int* sheet_id);
+/* Workspace bytes a request of this sheet and flags is charged (with_K: the
+ * server holds K rather than a caller device buffer). */
+CVM_API int cvm_server_request_bytes(cvm_server* s, int sheet, int flags, int with_K, int64_t* bytes);
+/* Asynchronous evaluation at host x[3nn] (copied at submit). F_host[3nn] is
+ * written when the request completes (keep it alive until cvm_server_wait);
+ * K_dev: caller device buffer for the K values, or NULL to leave K in the slot.
+ * flags as cvm_assemble (CVM_DEVICE_PTRS not allowed). CVM_E_BUDGET if one
+ * request alone exceeds the budget. */
+CVM_API int cvm_server_submit(cvm_server* s, int sheet, const double* x, int flags, double* F_host, double* K_dev,
+                              int64_t* request_id);
+CVM_API int cvm_server_poll(cvm_server* s, int64_t request_id, int* done);
+/* Wait for a request; times[3] (may be NULL): queue ms, device ms, total ms. */
+CVM_API int cvm_server_wait(cvm_server* s, int64_t request_id, double* times);
+/* completed, in flight, pending, workspace bytes allocated */
+CVM_API int cvm_server_stats(cvm_server* s, int64_t* out4);
+
 /* ---- measurement -------------------------------------------------------- */
 
 /* Per-kernel HIP-event timing on the context's stream (on/off). */
