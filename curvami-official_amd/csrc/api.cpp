@@ -65,6 +65,29 @@ void timing_end(Context& c, int slot) {
     CVM_HIP(hipEventRecord(c.timing.ev1[slot], c.stream));
 }
 
+void sync_nodes(Context& c) {
+    if (!c.nodes_dirty) return;
+    c.X.upload(c.hX, c.stream);
+    c.Y.upload(c.hY, c.stream);
+    if (c.batched) c.d_nsheet.upload(c.h_nsheet, c.stream);
+    CVM_HIP(hipStreamSynchronize(c.stream));
+    c.nodes_dirty = false;
+}
+
+void sync_domain(Context& c, Domain& d) {
+    if (!d.dirty) return;
+    d.gx.upload(d.hx, c.stream);
+    d.gy.upload(d.hy, c.stream);
+    d.d_w.upload(d.hw, c.stream);
+    d.d_tx.upload(d.htx, c.stream);
+    d.d_ty.upload(d.hty, c.stream);
+    d.d_nx.upload(d.hnx, c.stream);
+    d.d_ny.upload(d.hny, c.stream);
+    if (c.batched) d.d_gsheet.upload(d.h_gsheet, c.stream);
+    CVM_HIP(hipStreamSynchronize(c.stream));
+    d.dirty = false;
+}
+
 static Domain& domain(Context& c, int id) {
     if (id < 0 || id >= static_cast<int>(c.doms.size())) throw Error(CVM_E_INVALID, "bad domain id");
     return *c.doms[id];
@@ -163,6 +186,7 @@ int cvm_set_nodes(cvm_context* ctx, int nn, const double* X, const double* Y) {
     CVM_TRY(ctx)
     Context& c = ctx->c;
     if (nn < 0 || (nn > 0 && (!X || !Y))) throw Error(CVM_E_INVALID, "bad node arrays");
+    if (c.batched) throw Error(CVM_E_STATE, "batched context: nodes come with cvm_add_sheet");
     CVM_HIP(hipSetDevice(c.device));
     c.nn = nn;
     c.hX.assign(X, X + nn);
@@ -198,6 +222,10 @@ int cvm_add_crease(cvm_context* ctx, int nv, const double* vx, const double* vy,
                    int is_closed) {
     CVM_TRY(ctx)
     if (nv < 0 || (nv > 0 && (!vx || !vy))) throw Error(CVM_E_INVALID, "bad crease arrays");
+    if (ctx->c.batched) {
+        if (ctx->c.sheets.empty()) throw Error(CVM_E_STATE, "batched context: add a sheet first");
+        ctx->c.sheets.back().ncreases++;      // creases belong to the last sheet added
+    }
     cvm::Crease cr;
     cr.vx.assign(vx, vx + nv);
     cr.vy.assign(vy, vy + nv);
@@ -218,32 +246,69 @@ int cvm_add_domain(cvm_context* ctx, int kind, int ng, const double* gx, const d
     if (kind < CVM_DOMAIN_FACE || kind > CVM_DOMAIN_FOLD_RIGHT) throw Error(CVM_E_INVALID, "bad domain kind");
     if (ng < 0 || (ng > 0 && (!gx || !gy || !w))) throw Error(CVM_E_INVALID, "bad Gauss-point arrays");
     CVM_HIP(hipSetDevice(c.device));
-    auto* d = new cvm::Domain();
-    d->kind = kind;
-    d->ng = ng;
-    d->hx.assign(gx, gx + ng);
-    d->hy.assign(gy, gy + ng);
-    d->hw.assign(w, w + ng);
-    d->hnx = nx ? std::vector<double>(nx, nx + ng) : std::vector<double>(ng, 0.0);
-    d->hny = ny ? std::vector<double>(ny, ny + ng) : std::vector<double>(ng, 0.0);
-    d->htx = tx ? std::vector<double>(tx, tx + ng) : std::vector<double>(ng, 0.0);
-    d->hty = ty ? std::vector<double>(ty, ty + ng) : std::vector<double>(ng, 0.0);
-    try {
-        d->gx.upload(d->hx, c.stream);
-        d->gy.upload(d->hy, c.stream);
-        d->d_w.upload(d->hw, c.stream);
-        d->d_tx.upload(d->htx, c.stream);
-        d->d_ty.upload(d->hty, c.stream);
-        d->d_nx.upload(d->hnx, c.stream);
-        d->d_ny.upload(d->hny, c.stream);
-        CVM_HIP(hipStreamSynchronize(c.stream));
-    } catch (...) {
-        delete d;
-        throw;
+    auto append = [&](cvm::Domain& d) {
+        d.hx.insert(d.hx.end(), gx, gx + ng);
+        d.hy.insert(d.hy.end(), gy, gy + ng);
+        d.hw.insert(d.hw.end(), w, w + ng);
+        auto add = [&](std::vector<double>& v, const double* a) {
+            if (a) v.insert(v.end(), a, a + ng); else v.insert(v.end(), static_cast<size_t>(ng), 0.0);
+        };
+        add(d.hnx, nx); add(d.hny, ny); add(d.htx, tx); add(d.hty, ty);
+        if (c.batched) d.h_gsheet.insert(d.h_gsheet.end(), static_cast<size_t>(ng), static_cast<int>(c.sheets.size()) - 1);
+        d.ng += ng;
+        d.dirty = true;
+        d.npairs = -1;
+        d.have_sf = false;
+        d.ind_creases = -1;
+    };
+    if (c.batched && c.sheets.empty()) throw Error(CVM_E_STATE, "batched context: add a sheet first");
+    int id = -1;
+    if (c.batched && kind == CVM_DOMAIN_FACE) {
+        // one face domain holds the face GPs of every sheet, in sheet order
+        for (size_t i = 0; i < c.doms.size(); ++i)
+            if (c.doms[i]->kind == CVM_DOMAIN_FACE) id = static_cast<int>(i);
     }
-    c.doms.push_back(d);
+    if (id >= 0) {
+        append(*c.doms[id]);
+    } else {
+        auto* d = new cvm::Domain();
+        d->kind = kind;
+        append(*d);
+        c.doms.push_back(d);
+        id = static_cast<int>(c.doms.size()) - 1;
+    }
+    if (!c.batched) cvm::sync_domain(c, *c.doms[id]);
     if (kind == CVM_DOMAIN_FACE) c.plan.ready = false;
-    if (domain_id) *domain_id = static_cast<int>(c.doms.size()) - 1;
+    if (domain_id) *domain_id = id;
+    CVM_CATCH(ctx)
+}
+
+int cvm_add_sheet(cvm_context* ctx, int nn, const double* X, const double* Y, double rho, int* sheet_id,
+                  int* node0) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    if (nn <= 0 || !X || !Y) throw Error(CVM_E_INVALID, "bad node arrays");
+    if (!(rho > 0.0)) throw Error(CVM_E_INVALID, "rho must be > 0");
+    if (!c.batched && (c.nn > 0 || !c.creases.empty() || !c.doms.empty()))
+        throw Error(CVM_E_STATE, "cvm_add_sheet on a context already set up with cvm_set_nodes");
+    c.batched = true;
+    cvm::SheetSeg sg;
+    sg.node0 = c.nn;
+    sg.nn = nn;
+    sg.crease0 = static_cast<int>(c.creases.size());
+    sg.ncreases = 0;
+    sg.rho = rho;
+    c.sheets.push_back(sg);
+    c.hX.insert(c.hX.end(), X, X + nn);
+    c.hY.insert(c.hY.end(), Y, Y + nn);
+    c.h_nsheet.insert(c.h_nsheet.end(), static_cast<size_t>(nn), static_cast<int>(c.sheets.size()) - 1);
+    c.nn += nn;
+    c.nodes_dirty = true;
+    c.plan.ready = false;
+    c.node_sign_creases = -1;
+    for (cvm::Domain* d : c.doms) { d->npairs = -1; d->have_sf = false; d->ind_creases = -1; }
+    if (sheet_id) *sheet_id = static_cast<int>(c.sheets.size()) - 1;
+    if (node0) *node0 = sg.node0;
     CVM_CATCH(ctx)
 }
 

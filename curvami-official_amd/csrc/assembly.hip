@@ -64,7 +64,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 template <bool HAS_B, typename TT>
 __global__ __launch_bounds__(TILE_THREADS) void k_tile(
     const int* __restrict__ tile_gp, const int* __restrict__ tile_uoff, const int* __restrict__ tile_unodes,
-    const int* __restrict__ tile_toff, const uint16_t* __restrict__ tpairs, const int* __restrict__ slot_dst,
+    const int64_t* __restrict__ tile_toff, const uint16_t* __restrict__ tpairs, const uint32_t* __restrict__ slot_dst,
     const int* __restrict__ off, const uint8_t* __restrict__ uloc,
     const double* __restrict__ sf, int64_t npairs, const double* __restrict__ w,
     const double* __restrict__ xu, MatC mat, int rec_cap, int with_tangent,
@@ -74,7 +74,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
     const int tid = threadIdx.x;
     const int g0 = tile_gp[t], ngp = tile_gp[t + 1] - g0;
     const int ub = tile_uoff[t], U = tile_uoff[t + 1] - ub;
-    const int tb = tile_toff[t], T = tile_toff[t + 1] - tb;
+    const int64_t tb = tile_toff[t];
+    const int T = static_cast<int>(tile_toff[t + 1] - tb);
     const int pb = off[g0];
 
     double* recf = smem;                                        // 3 FP64 planes of rec_cap

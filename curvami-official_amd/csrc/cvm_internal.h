@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <stdexcept>
@@ -86,9 +87,29 @@ struct BoundaryCond {
     double moment = 0.0;
 };
 
+// A sheet of a batched context: nodes [node0, node0 + nn), creases
+// [crease0, crease0 + ncreases), its own support radius. A context set up with
+// cvm_set_nodes has one implicit sheet that covers everything.
+struct SheetSeg {
+    int node0 = 0, nn = 0, crease0 = 0, ncreases = 0;
+    double rho = 0.0;
+};
+
+// Device view of a sheet for one domain's neighbour search and shape functions:
+// the sheet's cell grid (keys koff + y * cols + x), radius and crease range.
+struct SheetDev {
+    double minx = 0.0, miny = 0.0, cell = 0.0, rho = 0.0, rho2 = 0.0;
+    long long koff = 0;
+    int cols = 0, nkeys = 0, nc = 0, crease0 = 0;
+};
+
 struct Domain {
     int kind = CVM_DOMAIN_FACE;
     int ng = 0;
+    std::vector<int> h_gsheet;     // per GP: sheet (batched contexts only; empty = sheet 0)
+    DBuf<int> d_gsheet;
+    bool dirty = false;            // host GP arrays not yet uploaded (batched appends)
+    DBuf<unsigned char> d_sheets;  // SheetDev per sheet, built by the neighbour search
     std::vector<double> hx, hy, hw, hnx, hny, htx, hty;   // host copies of the GP data
     DBuf<double> gx, gy, d_w, d_tx, d_ty, d_nx, d_ny;
     BoundaryCond bc;
@@ -170,7 +191,7 @@ struct Plan {
     DBuf<int> d_tile_gp;        // ntiles + 1
     DBuf<int> d_tile_uoff;      // ntiles + 1
     DBuf<int> d_tile_unodes;    // union node ids
-    DBuf<int> d_tile_toff;      // ntiles + 1 (touched pairs)
+    DBuf<int64_t> d_tile_toff;  // ntiles + 1 (touched pairs)
     DBuf<uint16_t> d_tile_tpairs;   // (ua << 8) | ub
     DBuf<uint8_t> d_pair_uloc;  // per face pair: union-local index
     int64_t nscratch = 0;       // total touched pairs
@@ -178,7 +199,7 @@ struct Plan {
     // K contributors: per block, a contiguous segment of scratch rows (ascending
     // tile order); slot_dst maps a tile's touched slot to its row
     DBuf<int64_t> d_kc_off;
-    DBuf<int> d_slot_dst;
+    DBuf<uint32_t> d_slot_dst;
     // F contributors: per node, fscratch indices
     DBuf<int> d_fc_off;
     DBuf<int> d_fc_idx;
@@ -231,6 +252,18 @@ struct Context {
     DBuf<int8_t> d_node_sign;   // per (crease, node): +1 / -1
     bool creases_dirty = true;
     int node_sign_creases = -1;   // crease count the node signs were computed for (-1: stale)
+    // sheets (batched contexts): nodes and creases per sheet, per-node sheet
+    std::vector<SheetSeg> sheets;
+    bool batched = false;
+    std::vector<int> h_nsheet;
+    DBuf<int> d_nsheet;
+    bool nodes_dirty = false;      // host node arrays not yet uploaded (batched appends)
+    int max_sheet_creases() const {
+        if (!batched) return static_cast<int>(creases.size());
+        int m = 0;
+        for (const SheetSeg& s : sheets) m = std::max(m, s.ncreases);
+        return m;
+    }
     std::vector<Domain*> doms;
     std::vector<Fold> folds;
     Plan plan;
@@ -265,6 +298,8 @@ int timing_begin(Context& c, const char* name);
 void timing_end(Context& c, int slot);
 
 void build_plan(Context& c);
+void sync_nodes(Context& c);
+void sync_domain(Context& c, Domain& d);
 void build_constraint_layout(Context& c, bool rigid);
 void launch_constraints(Context& c, const double* xdev, double* cdev, double* jdev);
 void launch_external_force(Context& c, double* Fdev);

@@ -69,35 +69,45 @@ __device__ __forceinline__ IndOut indicator(double Px, double Py, const double* 
     return o;
 }
 
-// Per-GP indicator data, layout ind[(c * 6 + k) * ng + g] (lane-coalesced).
+// Per-GP indicator data for the creases of the GP's sheet, layout
+// ind[(lc * 6 + k) * ng + g] with lc the crease's slot within its sheet
+// (lane-coalesced). gsheet == null: one sheet holding every crease.
 __global__ void k_indicator_gp(const double* __restrict__ gx, const double* __restrict__ gy, int ng,
-                               const double* __restrict__ cv, const int* __restrict__ cinfo, int nc,
+                               const int* __restrict__ gsheet, const SheetDev* __restrict__ sheets,
+                               const double* __restrict__ cv, const int* __restrict__ cinfo,
                                double* __restrict__ ind) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ng) return;
+    const SheetDev& S = sheets[gsheet ? gsheet[g] : 0];
     const double px = gx[g], py = gy[g];
-    for (int c = 0; c < nc; ++c) {
+    for (int lc = 0; lc < S.nc; ++lc) {
+        const int c = S.crease0 + lc;
         const int off = cinfo[5 * c], nv = cinfo[5 * c + 1];
         const IndOut o = indicator(px, py, cv + 2 * off, cv + 2 * off + nv, nv, cinfo[5 * c + 2],
                                    cinfo[5 * c + 3], cinfo[5 * c + 4]);
-        double* dst = ind + static_cast<size_t>(c) * 6 * ng + g;
+        double* dst = ind + static_cast<size_t>(lc) * 6 * ng + g;
         dst[0] = o.f; dst[ng] = o.gx; dst[2 * ng] = o.gy;
         dst[3 * ng] = o.hxx; dst[4 * ng] = o.hyy; dst[5 * ng] = o.hxy;
     }
 }
 
-// Node side of every crease: +1 when f >= 0 (a node on the crease counts as
-// the + side), -1 otherwise (GlobalRKPM.h:93).
+// Node side of every crease of the node's sheet: +1 when f >= 0 (a node on the
+// crease counts as the + side), -1 otherwise (GlobalRKPM.h:93); layout
+// sign[lc * nn + i]. crange: per sheet (crease0, ncreases); nsheet == null: one sheet.
 __global__ void k_node_sign(const double* __restrict__ X, const double* __restrict__ Y, int nn,
-                            const double* __restrict__ cv, const int* __restrict__ cinfo, int nc,
+                            const int* __restrict__ nsheet, const int* __restrict__ crange,
+                            const double* __restrict__ cv, const int* __restrict__ cinfo,
                             int8_t* __restrict__ sign) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nn) return;
-    for (int c = 0; c < nc; ++c) {
+    const int sh = nsheet ? nsheet[i] : 0;
+    const int c0 = crange[2 * sh], nc = crange[2 * sh + 1];
+    for (int lc = 0; lc < nc; ++lc) {
+        const int c = c0 + lc;
         const int off = cinfo[5 * c], nv = cinfo[5 * c + 1];
         const IndOut o = indicator(X[i], Y[i], cv + 2 * off, cv + 2 * off + nv, nv, cinfo[5 * c + 2],
                                    cinfo[5 * c + 3], cinfo[5 * c + 4]);
-        sign[static_cast<size_t>(c) * nn + i] = (o.f >= 0.0) ? 1 : -1;
+        sign[static_cast<size_t>(lc) * nn + i] = (o.f >= 0.0) ? 1 : -1;
     }
 }
 
@@ -118,21 +128,32 @@ static void upload_creases(Context& c) {
     if (info.empty()) info.assign(5, 0);
     c.d_cv.upload(cv, c.stream);
     c.d_cinfo.upload(info, c.stream);
+    CVM_HIP(hipStreamSynchronize(c.stream));
     c.creases_dirty = false;
 }
 
 void launch_node_signs(Context& c) {
+    sync_nodes(c);
     upload_creases(c);
     const int nc = static_cast<int>(c.creases.size());
     if (c.node_sign_creases == nc) return;        // unchanged nodes and creases
     c.node_sign_creases = nc;
-    c.d_node_sign.alloc(std::max<size_t>(1, static_cast<size_t>(nc) * c.nn));
-    if (nc == 0 || c.nn == 0) return;
+    const int mc = c.max_sheet_creases();
+    c.d_node_sign.alloc(std::max<size_t>(1, static_cast<size_t>(mc) * c.nn));
+    if (mc == 0 || c.nn == 0) return;
+    std::vector<int> crange;
+    if (c.batched)
+        for (const SheetSeg& s : c.sheets) crange.insert(crange.end(), {s.crease0, s.ncreases});
+    else
+        crange = {0, nc};
+    DBuf<int> d_cr;
+    d_cr.upload(crange, c.stream);
     const int slot = timing_begin(c, "node_sign");
     hipLaunchKernelGGL(k_node_sign, dim3((c.nn + 255) / 256), dim3(256), 0, c.stream, c.X.p, c.Y.p, c.nn,
-                       c.d_cv.p, c.d_cinfo.p, nc, c.d_node_sign.p);
+                       c.batched ? c.d_nsheet.p : nullptr, d_cr.p, c.d_cv.p, c.d_cinfo.p, c.d_node_sign.p);
     CVM_HIP(hipGetLastError());
     timing_end(c, slot);
+    CVM_HIP(hipStreamSynchronize(c.stream));
 }
 
 void launch_indicator_gp(Context& c, Domain& d) {
@@ -140,11 +161,13 @@ void launch_indicator_gp(Context& c, Domain& d) {
     const int nc = static_cast<int>(c.creases.size());
     if (d.ind_creases == nc) return;              // unchanged GPs and creases
     d.ind_creases = nc;
-    d.ind.alloc(std::max<size_t>(1, static_cast<size_t>(nc) * 6 * d.ng));
-    if (nc == 0 || d.ng == 0) return;
+    const int mc = c.max_sheet_creases();
+    d.ind.alloc(std::max<size_t>(1, static_cast<size_t>(mc) * 6 * d.ng));
+    if (mc == 0 || d.ng == 0) return;
     const int slot = timing_begin(c, "indicator_gp");
     hipLaunchKernelGGL(k_indicator_gp, dim3((d.ng + 255) / 256), dim3(256), 0, c.stream, d.gx.p, d.gy.p, d.ng,
-                       c.d_cv.p, c.d_cinfo.p, nc, d.ind.p);
+                       c.batched ? d.d_gsheet.p : nullptr, reinterpret_cast<const SheetDev*>(d.d_sheets.p),
+                       c.d_cv.p, c.d_cinfo.p, d.ind.p);
     CVM_HIP(hipGetLastError());
     timing_end(c, slot);
 }
@@ -157,30 +180,28 @@ void launch_indicator_gp(Context& c, Domain& d) {
 // reference does, including the wrap of an out-of-range x into the adjacent
 // row's key; keys outside [0, nkeys) are empty, like a failed map lookup.
 // ---------------------------------------------------------------------------
-struct Grid {
-    double minx, miny, cell;
-    int cols, nkeys;
-};
-
 template <bool FILL>
 __global__ void k_frnn(const double* __restrict__ X, const double* __restrict__ Y,
                        const double* __restrict__ gx, const double* __restrict__ gy, int ng,
-                       const int* __restrict__ cstart, const int* __restrict__ cnodes, Grid gr, double rho2,
-                       double rho, int* __restrict__ count, const int* __restrict__ off,
+                       const int* __restrict__ gsheet, const SheetDev* __restrict__ sheets,
+                       const int* __restrict__ cstart, const int* __restrict__ cnodes,
+                       int* __restrict__ count, const int* __restrict__ off,
                        int* __restrict__ ids, double* __restrict__ s) {
 #pragma clang fp contract(off)
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= ng) return;
+    const SheetDev& gr = sheets[gsheet ? gsheet[q] : 0];
     const double px = gx[q], py = gy[q];
     const int qx = static_cast<int>(floor((px - gr.minx) / gr.cell));
     const int qy = static_cast<int>(floor((py - gr.miny) / gr.cell));
+    const double rho = gr.rho, rho2 = gr.rho2;
     int n = 0;
     int o = FILL ? off[q] : 0;
     for (int dx = -1; dx <= 1; ++dx) {
         for (int dy = -1; dy <= 1; ++dy) {
             const long long key = static_cast<long long>(qy + dy) * gr.cols + (qx + dx);
             if (key < 0 || key >= gr.nkeys) continue;
-            const int b = cstart[key], e = cstart[key + 1];
+            const int b = cstart[gr.koff + key], e = cstart[gr.koff + key + 1];
             for (int k = b; k < e; ++k) {
                 const int ni = cnodes[k];
                 const double ex = X[ni] - px, ey = Y[ni] - py;
@@ -202,52 +223,81 @@ __global__ void k_frnn(const double* __restrict__ X, const double* __restrict__ 
 
 __global__ void k_set_zero_int(int* p) { p[0] = 0; }
 
+// Per sheet: node cell grid with the bounds and column count of FRNN.h:30-42
+// computed over that sheet's nodes; the sheets' key ranges are concatenated.
 void launch_frnn(Context& c, Domain& d, double rho) {
     if (c.nn == 0) throw Error(CVM_E_STATE, "nodes not set");
-    if (!(rho > 0.0)) throw Error(CVM_E_INVALID, "rho must be > 0");
-    d.rho = rho;
+    sync_nodes(c);
+    sync_domain(c, d);
+    std::vector<SheetSeg> segs;
+    if (c.batched) {
+        if (rho != 0.0) throw Error(CVM_E_INVALID, "batched context: pass rho = 0 (each sheet has its own radius)");
+        segs = c.sheets;
+    } else {
+        if (!(rho > 0.0)) throw Error(CVM_E_INVALID, "rho must be > 0");
+        SheetSeg one;
+        one.node0 = 0; one.nn = c.nn; one.crease0 = 0; one.ncreases = static_cast<int>(c.creases.size());
+        one.rho = rho;
+        segs.push_back(one);
+    }
+    d.rho = segs[0].rho;
+    std::vector<SheetDev> table(segs.size());
+    std::vector<int> cstart(1, 0), cnodes(c.nn);
+    long long koff = 0;
+    int nfill = 0;
+    for (size_t si = 0; si < segs.size(); ++si) {
+        const SheetSeg& sg = segs[si];
+        SheetDev& gr = table[si];
+        gr.rho = sg.rho; gr.rho2 = sg.rho * sg.rho; gr.cell = sg.rho;
+        gr.nc = sg.ncreases; gr.crease0 = sg.crease0;
+        if (sg.nn == 0) { gr.nkeys = 0; gr.koff = koff; continue; }
+        double mnx = c.hX[sg.node0], mny = c.hY[sg.node0], mxx = mnx, mxy = mny;
+        for (int i = sg.node0; i < sg.node0 + sg.nn; ++i) {
+            mnx = std::min(mnx, c.hX[i]); mny = std::min(mny, c.hY[i]);
+            mxx = std::max(mxx, c.hX[i]); mxy = std::max(mxy, c.hY[i]);
+        }
+        gr.minx = mnx; gr.miny = mny;
+        gr.cols = static_cast<int>(std::ceil((mxx - mnx) / sg.rho)) + 1;
+        std::vector<long long> key(sg.nn);
+        long long maxkey = 0;
+        for (int i = 0; i < sg.nn; ++i) {
+            const int x = static_cast<int>(std::floor((c.hX[sg.node0 + i] - mnx) / sg.rho));
+            const int y = static_cast<int>(std::floor((c.hY[sg.node0 + i] - mny) / sg.rho));
+            key[i] = static_cast<long long>(y) * gr.cols + x;
+            maxkey = std::max(maxkey, key[i]);
+        }
+        if (koff + maxkey + 1 > (1LL << 30)) throw Error(CVM_E_INVALID, "neighbour grid too large for this rho");
+        gr.nkeys = static_cast<int>(maxkey + 1);
+        gr.koff = koff;
+        std::vector<int> cnt(gr.nkeys + 1, 0);
+        for (int i = 0; i < sg.nn; ++i) cnt[key[i] + 1]++;
+        for (int k = 0; k < gr.nkeys; ++k) cnt[k + 1] += cnt[k];
+        for (int k = 1; k <= gr.nkeys; ++k) cstart.push_back(nfill + cnt[k]);
+        std::vector<int> fillp(cnt.begin(), cnt.end() - 1);
+        for (int i = 0; i < sg.nn; ++i) cnodes[nfill + fillp[key[i]]++] = sg.node0 + i;   // ascending per cell
+        nfill += sg.nn;
+        koff += gr.nkeys;
+    }
+    d.d_sheets.upload(reinterpret_cast<const unsigned char*>(table.data()), table.size() * sizeof(SheetDev), c.stream);
     d.off.alloc(d.ng + 1);
     d.h_off.assign(d.ng + 1, 0);
+    d.ind_creases = -1;
     if (d.ng == 0) {
         d.npairs = 0;
         CVM_HIP(hipMemsetAsync(d.off.p, 0, sizeof(int), c.stream));
+        CVM_HIP(hipStreamSynchronize(c.stream));
         return;
     }
-    // Host binning of the nodes (O(nn)); bounds and cols as FRNN.h:30-42.
-    double mnx = c.hX[0], mny = c.hY[0], mxx = c.hX[0], mxy = c.hY[0];
-    for (int i = 0; i < c.nn; ++i) {
-        mnx = std::min(mnx, c.hX[i]); mny = std::min(mny, c.hY[i]);
-        mxx = std::max(mxx, c.hX[i]); mxy = std::max(mxy, c.hY[i]);
-    }
-    Grid gr;
-    gr.minx = mnx; gr.miny = mny; gr.cell = rho;
-    gr.cols = static_cast<int>(std::ceil((mxx - mnx) / rho)) + 1;
-    std::vector<long long> key(c.nn);
-    long long maxkey = 0;
-    for (int i = 0; i < c.nn; ++i) {
-        const int x = static_cast<int>(std::floor((c.hX[i] - mnx) / rho));
-        const int y = static_cast<int>(std::floor((c.hY[i] - mny) / rho));
-        key[i] = static_cast<long long>(y) * gr.cols + x;
-        maxkey = std::max(maxkey, key[i]);
-    }
-    if (maxkey + 1 > (1LL << 30)) throw Error(CVM_E_INVALID, "neighbour grid too large for this rho");
-    gr.nkeys = static_cast<int>(maxkey + 1);
-    std::vector<int> cstart(gr.nkeys + 1, 0), cnodes(c.nn);
-    for (int i = 0; i < c.nn; ++i) cstart[key[i] + 1]++;
-    for (int k = 0; k < gr.nkeys; ++k) cstart[k + 1] += cstart[k];
-    {
-        std::vector<int> fillp(cstart.begin(), cstart.end() - 1);
-        for (int i = 0; i < c.nn; ++i) cnodes[fillp[key[i]]++] = i;   // ascending index per cell
-    }
+    const int* gsh = c.batched ? d.d_gsheet.p : nullptr;
+    const SheetDev* tab = reinterpret_cast<const SheetDev*>(d.d_sheets.p);
     DBuf<int> d_cstart, d_cnodes, d_count;
     d_cstart.upload(cstart, c.stream);
     d_cnodes.upload(cnodes, c.stream);
     d_count.alloc(d.ng);
-    const double rho2 = rho * rho;
     const dim3 grid((d.ng + 255) / 256), blk(256);
     int slot = timing_begin(c, "frnn_count");
-    hipLaunchKernelGGL((k_frnn<false>), grid, blk, 0, c.stream, c.X.p, c.Y.p, d.gx.p, d.gy.p, d.ng,
-                       d_cstart.p, d_cnodes.p, gr, rho2, rho, d_count.p, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL((k_frnn<false>), grid, blk, 0, c.stream, c.X.p, c.Y.p, d.gx.p, d.gy.p, d.ng, gsh, tab,
+                       d_cstart.p, d_cnodes.p, d_count.p, nullptr, nullptr, nullptr);
     CVM_HIP(hipGetLastError());
     timing_end(c, slot);
     // off = [0, inclusive_scan(count)]
@@ -263,8 +313,8 @@ void launch_frnn(Context& c, Domain& d, double rho) {
     d.ids.alloc(std::max<int64_t>(d.npairs, 1));
     d.s.alloc(std::max<int64_t>(d.npairs, 1));
     slot = timing_begin(c, "frnn_fill");
-    hipLaunchKernelGGL((k_frnn<true>), grid, blk, 0, c.stream, c.X.p, c.Y.p, d.gx.p, d.gy.p, d.ng,
-                       d_cstart.p, d_cnodes.p, gr, rho2, rho, nullptr, d.off.p, d.ids.p, d.s.p);
+    hipLaunchKernelGGL((k_frnn<true>), grid, blk, 0, c.stream, c.X.p, c.Y.p, d.gx.p, d.gy.p, d.ng, gsh, tab,
+                       d_cstart.p, d_cnodes.p, nullptr, d.off.p, d.ids.p, d.s.p);
     CVM_HIP(hipGetLastError());
     timing_end(c, slot);
     d.h_ids.resize(d.npairs);

@@ -55,13 +55,15 @@ void build_plan(Context& c) {
     const int ng = fd.ng;
     const std::vector<int>& off = fd.h_off;
     const std::vector<int>& ids = fd.h_ids;
+    const std::vector<int>& gsh = fd.h_gsheet;
     int nmax = 0;
     for (int g = 0; g < ng; ++g) nmax = std::max(nmax, off[g + 1] - off[g]);
     if (nmax > 64) throw Error(CVM_E_INVALID, "a Gauss point has more than 64 neighbours");
     P.union_max = kTileUnion;
 
     // ---- tiles and touched pairs (tile size shrinks until the LDS fits) -----
-    std::vector<int> tile_gp, tile_uoff, unodes, tile_toff;
+    std::vector<int> tile_gp, tile_uoff, unodes;
+    std::vector<int64_t> tile_toff;
     std::vector<uint8_t> uloc(ids.size());
     std::vector<uint16_t> tpairs;
     std::vector<int> stamp(nn, -1), where(nn, -1);
@@ -75,6 +77,9 @@ void build_plan(Context& c) {
             cur.clear();
             const int g0 = g;
             while (g < ng && g - g0 < tg) {
+                // tiles never straddle two sheets of a batched context, so each
+                // sheet's tiles (and sums) are those it has on its own
+                if (!gsh.empty() && gsh[g] != gsh[g0]) break;
                 int add = 0;
                 for (int k = off[g]; k < off[g + 1]; ++k)
                     if (stamp[ids[k]] != t) { stamp[ids[k]] = t; ++add; cur.push_back(ids[k]); }
@@ -125,7 +130,7 @@ void build_plan(Context& c) {
             rec_cap = std::max(rec_cap, off[q1] - off[q0]);
         }
         tile_toff.assign(P.ntiles + 1, 0);
-        for (int ti = 0; ti < P.ntiles; ++ti) tile_toff[ti + 1] = tile_toff[ti] + static_cast<int>(tl[ti].size());
+        for (int ti = 0; ti < P.ntiles; ++ti) tile_toff[ti + 1] = tile_toff[ti] + static_cast<int64_t>(tl[ti].size());
         tpairs.resize(tile_toff[P.ntiles]);
 #pragma omp parallel for schedule(static)
         for (int ti = 0; ti < P.ntiles; ++ti)
@@ -268,7 +273,7 @@ void build_plan(Context& c) {
 #pragma omp parallel for schedule(dynamic, 64) reduction(|| : miss)
     for (int t = 0; t < P.ntiles; ++t) {
         const int* un = unodes.data() + tile_uoff[t];
-        for (int s = tile_toff[t]; s < tile_toff[t + 1]; ++s) {
+        for (int64_t s = tile_toff[t]; s < tile_toff[t + 1]; ++s) {
             const int a = tpairs[s] >> 8, bb = tpairs[s] & 0xff;
             const int64_t blk = block_of(un[a], un[bb]);
             if (blk < 0) miss = true;
@@ -282,12 +287,12 @@ void build_plan(Context& c) {
     // block-major scratch: the tile kernel writes touched slot s straight to row
     // slot_dst[s], the position of its contribution in its block's segment; the
     // segments are in ascending tile order, i.e. GP order
-    std::vector<int> slot_dst(tpairs.size());
+    std::vector<uint32_t> slot_dst(tpairs.size());
     {
         std::vector<int64_t> fillp(kc_cnt.begin(), kc_cnt.end() - 1);
-        for (size_t s = 0; s < tpairs.size(); ++s) slot_dst[s] = static_cast<int>(fillp[tblk[s]]++);
+        for (size_t s = 0; s < tpairs.size(); ++s) slot_dst[s] = static_cast<uint32_t>(fillp[tblk[s]]++);
     }
-    if (P.nscratch > 0x7fffffffLL) throw Error(CVM_E_INVALID, "assembly scratch exceeds 2^31 rows");
+    if (P.nscratch > 0xffffffffLL) throw Error(CVM_E_INVALID, "assembly scratch exceeds 2^32 rows");
 
     // F contributors per node
     std::vector<int> fc_cnt(nn + 1, 0);

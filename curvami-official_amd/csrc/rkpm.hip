@@ -121,7 +121,8 @@ __global__ __launch_bounds__(RKPM_THREADS) void k_rkpm(
     const double* __restrict__ X, const double* __restrict__ Y, int nn,
     const double* __restrict__ gx, const double* __restrict__ gy, int ng,
     const int* __restrict__ off, const int* __restrict__ ids, const double* __restrict__ sarr,
-    const double* __restrict__ ind, const int8_t* __restrict__ nsign, int nc, double rho,
+    const double* __restrict__ ind, const int8_t* __restrict__ nsign,
+    const int* __restrict__ gsheet, const SheetDev* __restrict__ sheets,
     double* __restrict__ sf, int64_t npairs, int* __restrict__ status /* n1, first1, n2, first2 */) {
     using R = RK<P>;
     constexpr int NP = R::NP, NMON = R::NMON, DM = R::DM, NCJ = R::NCJ;
@@ -135,6 +136,9 @@ __global__ __launch_bounds__(RKPM_THREADS) void k_rkpm(
     if (threadIdx.x == 0) off_s[nblk] = off[g0 + nblk];
 
     if (threadIdx.x < nblk) {
+        const SheetDev& S = sheets[gsheet ? gsheet[g] : 0];
+        const double rho = S.rho;
+        const int nc = S.nc;
         const double px = gx[g], py = gy[g];
         const int b = off[g], e = off[g + 1];
         // ---- moments of monomials ---------------------------------------------
@@ -306,6 +310,9 @@ __global__ __launch_bounds__(RKPM_THREADS) void k_rkpm(
             if (off_s[mid] <= k) lo = mid; else hi = mid;
         }
         const int gg = g0 + lo;
+        const SheetDev& S = sheets[gsheet ? gsheet[gg] : 0];
+        const double rho = S.rho;
+        const int nc = S.nc;
         const double* cj = cj_s + lo * NCJ;
         const int I = ids[k];
         const double xi = (gx[gg] - X[I]) / rho;
@@ -348,11 +355,11 @@ __global__ void k_status_init(int* s) {
 }
 
 template <int P, int KER>
-static void run(Context& c, Domain& d, double rho, int* st) {
-    const int nc = static_cast<int>(c.creases.size());
+static void run(Context& c, Domain& d, int* st) {
     hipLaunchKernelGGL((k_rkpm<P, KER>), dim3((d.ng + RKPM_THREADS - 1) / RKPM_THREADS), dim3(RKPM_THREADS), 0,
                        c.stream, c.X.p, c.Y.p, c.nn,
-                       d.gx.p, d.gy.p, d.ng, d.off.p, d.ids.p, d.s.p, d.ind.p, c.d_node_sign.p, nc, rho,
+                       d.gx.p, d.gy.p, d.ng, d.off.p, d.ids.p, d.s.p, d.ind.p, c.d_node_sign.p,
+                       c.batched ? d.d_gsheet.p : nullptr, reinterpret_cast<const SheetDev*>(d.d_sheets.p),
                        d.sf.p, static_cast<int64_t>(d.npairs), st);
 }
 
@@ -371,13 +378,13 @@ void launch_rkpm(Context& c, Domain& d, double rho, int order, int kernel, int* 
     if (d.ng > 0) {
         const int slot = timing_begin(c, d.kind == CVM_DOMAIN_FACE ? "rkpm_face" : "rkpm_line");
         if (kernel == CVM_KERNEL_QUARTIC) {
-            if (order == 1) run<1, CVM_KERNEL_QUARTIC>(c, d, rho, st.p);
-            else if (order == 2) run<2, CVM_KERNEL_QUARTIC>(c, d, rho, st.p);
-            else run<3, CVM_KERNEL_QUARTIC>(c, d, rho, st.p);
+            if (order == 1) run<1, CVM_KERNEL_QUARTIC>(c, d, st.p);
+            else if (order == 2) run<2, CVM_KERNEL_QUARTIC>(c, d, st.p);
+            else run<3, CVM_KERNEL_QUARTIC>(c, d, st.p);
         } else {
-            if (order == 1) run<1, CVM_KERNEL_CUBIC_BSPLINE>(c, d, rho, st.p);
-            else if (order == 2) run<2, CVM_KERNEL_CUBIC_BSPLINE>(c, d, rho, st.p);
-            else run<3, CVM_KERNEL_CUBIC_BSPLINE>(c, d, rho, st.p);
+            if (order == 1) run<1, CVM_KERNEL_CUBIC_BSPLINE>(c, d, st.p);
+            else if (order == 2) run<2, CVM_KERNEL_CUBIC_BSPLINE>(c, d, st.p);
+            else run<3, CVM_KERNEL_CUBIC_BSPLINE>(c, d, st.p);
         }
         CVM_HIP(hipGetLastError());
         timing_end(c, slot);

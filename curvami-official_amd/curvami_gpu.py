@@ -34,6 +34,7 @@ SIGNATURES = {
     "cvm_last_error": (C.c_char_p, [C.c_void_p]),
     "cvm_set_nodes": (C.c_int, [C.c_void_p, C.c_int, _d, _d]),
     "cvm_set_laminate": (C.c_int, [C.c_void_p, C.c_int, _d]),
+    "cvm_add_sheet": (C.c_int, [C.c_void_p, C.c_int, _d, _d, C.c_double, _i, _i]),
     "cvm_get_abd": (C.c_int, [C.c_void_p, _d, _d, _d]),
     "cvm_add_crease": (C.c_int, [C.c_void_p, C.c_int, _d, _d, C.c_int, C.c_int, C.c_int]),
     "cvm_add_domain": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _d, _d, _d, _d, _d, _d, _d, _i]),
@@ -155,8 +156,20 @@ class Plate:
         a = [_f64(x) for x in (gps.gx, gps.gy, gps.w, gps.nx, gps.ny, gps.tx, gps.ty)]
         did = C.c_int(-1)
         self._check(self._L.cvm_add_domain(self._h, kind, a[0].size, *[_dp(x) for x in a], C.byref(did)))
-        self.domain_sizes.append(a[0].size)
+        if did.value < len(self.domain_sizes):       # batched: face GPs appended to the face domain
+            self.domain_sizes[did.value] += a[0].size
+        else:
+            self.domain_sizes.append(a[0].size)
         return did.value
+
+    def add_sheet(self, X, Y, rho):
+        """Batched context: append a sheet (nodes, support radius); returns
+        (sheet id, id of its first node)."""
+        X, Y = _f64(X), _f64(Y)
+        sid = C.c_int(-1); n0 = C.c_int(-1)
+        self._check(self._L.cvm_add_sheet(self._h, X.size, _dp(X), _dp(Y), float(rho), C.byref(sid), C.byref(n0)))
+        self.nn += X.size
+        return sid.value, n0.value
 
     def add_fold(self, left, right, Kt, Ang0=0.0) -> int:
         fid = C.c_int(-1)
@@ -282,6 +295,37 @@ class Plate:
             self._check(self._L.cvm_kernel_stats(self._h, i, name, 128, C.byref(ms), C.byref(cnt)))
             out[name.value.decode()] = (ms.value, cnt.value)
         return out
+
+
+def batch_from_sheets(sheets, device: int = 0, compute: bool = True):
+    """One batched context holding every workloads.Sheet (same laminate, order
+    and window for all): per-sheet nodes, creases, face GPs (shared face
+    domain), fold domains and boundary domains. Attributes: node0 (first node
+    id per sheet), face_range ((first GP, count) per sheet in the face domain)."""
+    p = Plate(device)
+    p.set_plies(sheets[0].plies)
+    p.node0, p.face_range, p.fold_doms = [], [], []
+    face = None
+    for sh in sheets:
+        _, n0 = p.add_sheet(sh.X, sh.Y, sh.rho)
+        p.node0.append(n0)
+        for cr in sh.creases:
+            p.add_crease(cr.vx, cr.vy, cr.ext_s, cr.ext_e, cr.closed)
+        before = p.domain_sizes[face] if face is not None else 0
+        face = p.add_domain(sh.face, FACE)
+        p.face_range.append((before, sh.face.n))
+        for f in sh.folds:
+            l = p.add_domain(f.left, FOLD_LEFT)
+            r = p.add_domain(f.right, FOLD_RIGHT)
+            p.add_fold(l, r, f.Kt, f.Ang0)
+            p.fold_doms.append((l, r))
+    p.face_dom = face
+    if compute:
+        for d in range(len(p.domain_sizes)):
+            p.neighbours(d, 0.0)
+            p.shape_functions(d, sheets[0].order, sheets[0].kernel)
+        p.init_assembly()
+    return p
 
 
 def plate_from_sheet(sh, device: int = 0, compute: bool = True, order=None, kernel=None):

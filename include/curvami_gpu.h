@@ -86,6 +86,19 @@ CVM_API int cvm_set_laminate(cvm_context* ctx, int nply, const double* plies);
 /* Read back A, B, D (row-major 3x3 each). */
 CVM_API int cvm_get_abd(const cvm_context* ctx, double* A, double* B, double* D);
 
+/* Batched contexts: many independent sheets in one context, so that each stage
+ * (neighbour search, shape functions, assembly) runs as one launch over all of
+ * them. No reference counterpart (the reference holds one sheet per Meshless).
+ * Appends a sheet's nodes (ids node0 .. node0 + nn - 1) with its support radius;
+ * creases and domains added afterwards belong to this sheet until the next
+ * cvm_add_sheet. Face domains of all sheets share one face domain (the first
+ * one's id is returned again). Neighbours and shape functions only ever couple
+ * nodes and GPs of the same sheet, so K is block diagonal over sheets and each
+ * sheet's results equal those of a context holding that sheet alone. Pass rho = 0
+ * to cvm_neighbours in a batched context. Not combinable with cvm_set_nodes. */
+CVM_API int cvm_add_sheet(cvm_context* ctx, int nn, const double* X, const double* Y, double rho, int* sheet_id,
+                          int* node0);
+
 /* Crease polyline (Enrichment::FoldLine, core/include/curvami/Enrichment.h:10-15). */
 CVM_API int cvm_add_crease(cvm_context* ctx, int nv, const double* vx, const double* vy,
                            int extend_start, int extend_end, int is_closed);
