@@ -114,6 +114,130 @@ def cpu_baseline(sh, threads):
     return out, sub.n
 
 
+def _step_time(plate, ndom, order, kernel, xd, Fd, Kd, flags, steps, warmup, graph=False):
+    """Seconds per step: shape functions of every domain (queued back to back,
+    one status read-back), then one evaluation (eager, or the captured graph)."""
+    def step():
+        plate.shape_functions_all(order, kernel)
+        if graph:
+            plate.graph_launch()
+        else:
+            plate.assemble_device(xd.ptr, Fd.ptr, Kd.ptr, flags)
+    if graph:
+        plate.graph_capture(xd.ptr, Fd.ptr, Kd.ptr, flags)
+    for _ in range(warmup):
+        step()
+    plate.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    plate.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def batched_leg(nsheets, warmup, steps):
+    """256 independent sheets (sides U{21..121}, one straight crease each): one
+    batched context, one launch per stage, assembly replayed from a captured
+    graph, tile shape from the autotuner (tuned on the first 16 sheets). The
+    plain FP64 GPU path -- one context per sheet, sheets one after another -- is
+    timed on the first 32 sheets for the ratio."""
+    import autotune as AT
+    sheets = [W.config_batched_sheet(s) for s in range(nsheets)]
+    flags64 = G.RESIDUAL | G.TANGENT | G.ELASTIC_FOLDS
+    flags32 = flags64 | G.FP32_TANGENT
+    res = {}
+    # plain FP64 path, sheet by sheet
+    sample = sheets[:min(32, nsheets)]
+    tot_gp, tot_t = 0, 0.0
+    for sh in sample:
+        p = G.plate_from_sheet(sh)
+        x = G.DeviceArray(3 * sh.nn); x.set(W.displacement(sh.nn, 1, 1e-3))
+        F = G.DeviceArray(3 * sh.nn); K = G.DeviceArray(p.nnz_union)
+        tot_t += _step_time(p, len(p.domain_sizes), sh.order, sh.kernel, x, F, K, flags64, steps, 1)
+        tot_gp += sh.face.n
+        for a in (x, F, K):
+            a.free()
+        p.close()
+    res["plain_fp64_per_sheet_gps"] = tot_gp / tot_t
+    res["plain_sample"] = f"first {len(sample)} sheets, {tot_gp} face GPs, one context per sheet, sequential"
+    # autotune on a subset
+    sub = sheets[:min(16, nsheets)]
+    tp = G.batch_from_sheets(sub)
+    xs = np.concatenate([W.displacement(sh.nn, 1 + k, 1e-3) for k, sh in enumerate(sub)])
+    x = G.DeviceArray(xs.size); x.set(xs)
+    F = G.DeviceArray(xs.size); K = G.DeviceArray(tp.nnz_union)
+    tune = AT.autotune(tp, x.ptr, F.ptr, K.ptr, flags32, reps=3)
+    for a in (x, F, K):
+        a.free()
+    tp.close()
+    tg, um, gr = tune["best"]
+    res["autotune"] = {"best": {"tile_gps": tg, "union_max": um, "grid": gr},
+                       "table": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()}
+                                 for r in tune["table"]], "tuned_on": f"first {len(sub)} sheets"}
+    # the full batch
+    t0 = time.perf_counter()
+    bp = G.batch_from_sheets(sheets, compute=False)
+    bp.set_tile_shape(tg, um, gr)
+    for d in range(len(bp.domain_sizes)):
+        bp.neighbours(d, 0.0)
+    bp.shape_functions_all(sheets[0].order, sheets[0].kernel)
+    nf, nu = bp.init_assembly()
+    res["setup_s"] = round(time.perf_counter() - t0, 2)
+    ng = bp.domain_sizes[bp.face_dom]
+    xs = np.concatenate([W.displacement(sh.nn, 1 + k, 1e-3) for k, sh in enumerate(sheets)])
+    x = G.DeviceArray(xs.size); x.set(xs)
+    F = G.DeviceArray(xs.size); K = G.DeviceArray(nu)
+    nd = len(bp.domain_sizes)
+    for name, fl in (("mixed", flags32), ("fp64", flags64)):
+        dt = _step_time(bp, nd, sheets[0].order, sheets[0].kernel, x, F, K, fl, steps, warmup, graph=True)
+        # assembly alone (the per-Newton-iteration work), graph replays
+        bp.graph_capture(x.ptr, F.ptr, K.ptr, fl)
+        bp.graph_launch(); bp.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            bp.graph_launch()
+        bp.synchronize()
+        da = (time.perf_counter() - t0) / steps
+        res[name] = {"gps": ng / dt, "ms_per_step": dt * 1e3, "assembly_ms": da * 1e3,
+                     "assembly_gps": ng / da}
+    res["mixed_over_plain_fp64"] = res["mixed"]["gps"] / res["plain_fp64_per_sheet_gps"]
+    res["mixed_over_batched_fp64"] = res["mixed"]["gps"] / res["fp64"]["gps"]
+    res["workload"] = (f"{nsheets} sheets, sides U{{21..121}} (mean {np.mean([s.n_side for s in sheets]):.1f}), "
+                       f"{ng} face GPs, {bp.npairs[bp.face_dom]} pairs, nnz_K {nu}; step = shape functions "
+                       f"(all {nd} domains) + residual + CSC tangent")
+    for a in (x, F, K):
+        a.free()
+    bp.close()
+    return res
+
+
+def serving_leg(nreq):
+    """Poisson request streams over the serving loop: sides U{21..201}, rigid
+    fold of a seeded angle plus noise; p50 / p99 latency and throughput per
+    rate, FP64 and mixed tangents; the 201 x 201 request alone."""
+    import serving as S
+    reqs = W.serving_requests(nreq)
+    sides = sorted(set(n for n, _, _ in reqs) | {201})
+    sheets, plates, tprep = S.prepare_sheets(sides)
+    xs = S.displacements(reqs, sheets)
+    srv = G.Server(hbm_budget_bytes=96 << 30, max_inflight=8)
+    sid = {n: srv.add_sheet(plates[n]) for n in sides}
+    flags = G.RESIDUAL | G.TANGENT | G.ELASTIC_FOLDS
+    S.run_rate(srv, sid, reqs[:32], xs[:32], 1e5, 1, flags)                   # warm the slots
+    S.run_rate(srv, sid, reqs[:32], xs[:32], 1e5, 1, flags | G.FP32_TANGENT)
+    x201 = W.rigid_fold_displacement(sheets[201], 1.0, seed=5, sigma=1e-3)
+    out = {"requests_per_rate": nreq, "distinct_sheets": len(sides), "prepare_s": round(tprep, 1),
+           "hbm_budget_gb": 96, "slots": 8}
+    for name, fl in (("fp64", flags), ("mixed", flags | G.FP32_TANGENT)):
+        p50, p99 = S.size_class_latency(srv, sid, plates, 201, x201, fl)
+        rows = [S.run_rate(srv, sid, reqs, xs, r, 100 + r, fl) for r in (50, 100, 200, 500, 1000, 2000)]
+        out[name] = {"alone_201_p50_ms": p50, "alone_201_p99_ms": p99,
+                     "rates": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()}
+                               for r in rows]}
+    srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +246,10 @@ def main():
     ap.add_argument("--n-side", type=int, default=201)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-batched", action="store_true", help="skip the 256-sheet batched leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--batch-sheets", type=int, default=W.BATCH_SHEETS)
+    ap.add_argument("--serving-requests", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design "
@@ -153,8 +281,7 @@ def main():
     flags = G.RESIDUAL | G.TANGENT | G.ELASTIC_FOLDS
 
     def step(extra=0):
-        for d in range(ndom):
-            p.shape_functions(d, sh.order, sh.kernel)
+        p.shape_functions_all(sh.order, sh.kernel)
         p.assemble_device(xd, Fd, Kd, flags | extra)
 
     def timed(extra):
@@ -245,6 +372,10 @@ def main():
     for ptr in (xd, Fd, Kd):
         hip.free(ptr)
     p.close()
+    if not args.no_batched:
+        out["batched"] = batched_leg(args.batch_sheets, args.warmup, max(2, args.steps // 3))
+    if not args.no_serving:
+        out["serving"] = serving_leg(args.serving_requests)
     print(json.dumps(out))
 
 

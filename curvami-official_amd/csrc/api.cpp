@@ -15,6 +15,8 @@ Context::~Context() {
         (void)hipEventDestroy(timing.ev0[i]);
         (void)hipEventDestroy(timing.ev1[i]);
     }
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -371,6 +373,21 @@ int cvm_get_neighbours(cvm_context* ctx, int domain, int* off, int* ids, double*
     CVM_CATCH(ctx)
 }
 
+static void rkpm_error(const cvm::Domain& d, int code, int nb, int fb, const std::string& where) {
+    const int nfirst = d.h_off[fb + 1] - d.h_off[fb];
+    if (code == CVM_E_SINGULAR_MOMENT)
+        throw Error(code, where + "RKPM: singular moment matrix at " + std::to_string(nb) + " of " + std::to_string(d.ng) +
+                              " Gauss points (first is index " + std::to_string(fb) + " with " +
+                              std::to_string(nfirst) +
+                              " neighbours). The support is rank-deficient for the requested basis order - "
+                              "typically too few visible nodes, which happens where a crease cuts most of a "
+                              "support away. Increase the support radius, refine the mesh there, or lower the order.");
+    throw Error(code, where + "RKPM: moment-matrix pivot collapsed during block inversion at " + std::to_string(nb) +
+                          " of " + std::to_string(d.ng) + " Gauss points (first is index " + std::to_string(fb) +
+                          " with " + std::to_string(nfirst) +
+                          " neighbours). Same cause and remedies as a singular moment matrix.");
+}
+
 int cvm_shape_functions(cvm_context* ctx, int domain, int order, int kernel, int* nbad, int* first_bad) {
     CVM_TRY(ctx)
     Context& c = ctx->c;
@@ -381,22 +398,21 @@ int cvm_shape_functions(cvm_context* ctx, int domain, int order, int kernel, int
     ++d.gen;
     if (nbad) *nbad = nb;
     if (first_bad) *first_bad = fb;
-    if (code == CVM_E_SINGULAR_MOMENT) {
-        const int nfirst = d.h_off[fb + 1] - d.h_off[fb];
-        throw Error(code, "RKPM: singular moment matrix at " + std::to_string(nb) + " of " + std::to_string(d.ng) +
-                              " Gauss points (first is index " + std::to_string(fb) + " with " +
-                              std::to_string(nfirst) +
-                              " neighbours). The support is rank-deficient for the requested basis order - "
-                              "typically too few visible nodes, which happens where a crease cuts most of a "
-                              "support away. Increase the support radius, refine the mesh there, or lower the order.");
-    }
-    if (code == CVM_E_PIVOT_COLLAPSE) {
-        const int nfirst = d.h_off[fb + 1] - d.h_off[fb];
-        throw Error(code, "RKPM: moment-matrix pivot collapsed during block inversion at " + std::to_string(nb) +
-                              " of " + std::to_string(d.ng) + " Gauss points (first is index " + std::to_string(fb) +
-                              " with " + std::to_string(nfirst) +
-                              " neighbours). Same cause and remedies as a singular moment matrix.");
-    }
+    if (code != CVM_OK) rkpm_error(d, code, nb, fb, "");
+    CVM_CATCH(ctx)
+}
+
+int cvm_shape_functions_all(cvm_context* ctx, int order, int kernel, int* failed_domain, int* nbad, int* first_bad) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    CVM_HIP(hipSetDevice(c.device));
+    int dom = -1, code = 0, nb = 0, fb = 0;
+    cvm::launch_rkpm_all(c, order, kernel, &dom, &code, &nb, &fb);
+    for (cvm::Domain* d : c.doms) ++d->gen;
+    if (failed_domain) *failed_domain = dom;
+    if (nbad) *nbad = nb;
+    if (first_bad) *first_bad = fb;
+    if (code != CVM_OK) rkpm_error(*c.doms[dom], code, nb, fb, "domain " + std::to_string(dom) + ": ");
     CVM_CATCH(ctx)
 }
 
@@ -582,6 +598,71 @@ int cvm_external_force(cvm_context* ctx, int flags, double* F_ext) {
         CVM_HIP(hipMemcpyAsync(F_ext, Fd, 3 * sizeof(double) * c.nn, hipMemcpyDeviceToHost, c.stream));
         CVM_HIP(hipStreamSynchronize(c.stream));
     }
+    CVM_CATCH(ctx)
+}
+
+int cvm_set_tile_shape(cvm_context* ctx, int tile_gps, int union_max, int grid) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    if (tile_gps < 1 || tile_gps > cvm::kTileGps) throw Error(CVM_E_INVALID, "tile_gps must be in 1..16");
+    if (union_max < 8 || union_max > cvm::kTileUnion) throw Error(CVM_E_INVALID, "union_max must be in 8..64");
+    if (grid < 0) throw Error(CVM_E_INVALID, "grid must be >= 0");
+    if (tile_gps != c.tile_gps || union_max != c.union_max) c.plan.ready = false;
+    c.tile_gps = tile_gps;
+    c.union_max = union_max;
+    c.tile_grid = grid;
+    CVM_CATCH(ctx)
+}
+
+int cvm_get_tile_shape(cvm_context* ctx, int* out4) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    if (!out4) throw Error(CVM_E_INVALID, "null output");
+    out4[0] = c.plan.ready ? c.plan.tile_max_gps : c.tile_gps;
+    out4[1] = c.union_max;
+    out4[2] = c.tile_grid;
+    out4[3] = c.plan.ready ? c.plan.ntiles : 0;
+    CVM_CATCH(ctx)
+}
+
+int cvm_graph_capture(cvm_context* ctx, int flags, const double* x_dev, double* F_dev, double* K_dev) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    CVM_HIP(hipSetDevice(c.device));
+    if (!c.plan.ready) throw Error(CVM_E_STATE, "assembly plan not initialised (cvm_init_assembly)");
+    if (!x_dev || !F_dev || ((flags & CVM_TANGENT) && !K_dev)) throw Error(CVM_E_INVALID, "null device pointer");
+    if (c.graph_exec) { (void)hipGraphExecDestroy(c.graph_exec); c.graph_exec = nullptr; }
+    if (c.graph) { (void)hipGraphDestroy(c.graph); c.graph = nullptr; }
+    // everything the launches need is allocated before the capture starts
+    cvm::reserve_workspace(c, c.ws, flags);
+    CVM_HIP(hipStreamSynchronize(c.stream));
+    const bool timed = c.ws.timed;
+    c.ws.timed = false;
+    CVM_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    try {
+        cvm::launch_assembly(c, c.ws, flags & ~CVM_DEVICE_PTRS, x_dev, F_dev, K_dev);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(c.stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        c.ws.timed = timed;
+        throw;
+    }
+    c.ws.timed = timed;
+    CVM_HIP(hipStreamEndCapture(c.stream, &c.graph));
+    CVM_HIP(hipGraphInstantiate(&c.graph_exec, c.graph, nullptr, nullptr, 0));
+    CVM_CATCH(ctx)
+}
+
+int cvm_graph_launch(cvm_context* ctx) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    if (!c.graph_exec) throw Error(CVM_E_STATE, "no captured graph (cvm_graph_capture)");
+    if (!c.plan.ready) throw Error(CVM_E_STATE, "the assembly plan changed since the capture");
+    CVM_HIP(hipSetDevice(c.device));
+    const int slot = cvm::timing_begin(c, "assembly_graph");
+    CVM_HIP(hipGraphLaunch(c.graph_exec, c.stream));
+    cvm::timing_end(c, slot);
     CVM_CATCH(ctx)
 }
 

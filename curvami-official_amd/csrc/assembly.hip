@@ -61,17 +61,18 @@ __device__ __forceinline__ double wave_sum(double v) {
 // TT = double: the FP64 tangent; TT = float: the mixed-precision tangent (FP32
 // pair blocks and FP32 accumulation), with kinematics, resultants and the
 // residual still in FP64.
-template <bool HAS_B, typename TT>
+template <bool HAS_B, typename TT, bool PERSIST>
 __global__ __launch_bounds__(TILE_THREADS) void k_tile(
     const int* __restrict__ tile_gp, const int* __restrict__ tile_uoff, const int* __restrict__ tile_unodes,
     const int64_t* __restrict__ tile_toff, const uint16_t* __restrict__ tpairs, const uint32_t* __restrict__ slot_dst,
     const int* __restrict__ off, const uint8_t* __restrict__ uloc,
     const double* __restrict__ sf, int64_t npairs, const double* __restrict__ w,
     const double* __restrict__ xu, MatC mat, int rec_cap, int with_tangent,
-    TT* __restrict__ scr, double* __restrict__ fscr) {
+    TT* __restrict__ scr, double* __restrict__ fscr, int ntiles) {
     extern __shared__ double smem[];
-    const int t = blockIdx.x;
     const int tid = threadIdx.x;
+    // one tile per workgroup, or (PERSIST) a grid striding over the tiles
+    auto body = [&](const int t) {
     const int g0 = tile_gp[t], ngp = tile_gp[t + 1] - g0;
     const int ub = tile_uoff[t], U = tile_uoff[t + 1] - ub;
     const int64_t tb = tile_toff[t];
@@ -292,6 +293,15 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
         }
         double* dst = fscr + 3 * static_cast<int64_t>(ub + tid);
         dst[0] = f0; dst[1] = f1; dst[2] = f2;
+    }
+    };
+    if constexpr (PERSIST) {
+        for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+            body(t);
+            __syncthreads();   // the next tile reuses the LDS
+        }
+    } else {
+        body(blockIdx.x);
     }
 }
 
@@ -559,15 +569,22 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
     int slot = tbeg(!tangent ? "tile_residual" : (fp32 ? "tile_residual_tangent_fp32" : "tile_residual_tangent"));
     const bool has_b = m.B[0] != 0.0 || m.B[1] != 0.0 || m.B[2] != 0.0 || m.B[3] != 0.0 || m.B[4] != 0.0 ||
                        m.B[5] != 0.0 || m.B[6] != 0.0 || m.B[7] != 0.0 || m.B[8] != 0.0;
+    const bool persist = c.tile_grid > 0 && c.tile_grid < P.ntiles;
+    const int grid = persist ? c.tile_grid : P.ntiles;
     auto tile = [&](auto kern, auto* scr) {
-        hipLaunchKernelGGL(kern, dim3(P.ntiles), dim3(TILE_THREADS), lds, st, P.d_tile_gp.p, P.d_tile_uoff.p,
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(TILE_THREADS), lds, st, P.d_tile_gp.p, P.d_tile_uoff.p,
                            P.d_tile_unodes.p, P.d_tile_toff.p, P.d_tile_tpairs.p, P.d_slot_dst.p, fd.off.p,
                            P.d_pair_uloc.p, fd.sf.p, static_cast<int64_t>(fd.npairs), fd.d_w.p, xdev, m, P.rec_cap,
-                           tangent ? 1 : 0, scr, ws.fscr.p);
+                           tangent ? 1 : 0, scr, ws.fscr.p, P.ntiles);
     };
     float* scr32 = reinterpret_cast<float*>(ws.scr.p);
-    if (fp32) tile(has_b ? k_tile<true, float> : k_tile<false, float>, scr32);
-    else tile(has_b ? k_tile<true, double> : k_tile<false, double>, ws.scr.p);
+    if (persist) {
+        if (fp32) tile(has_b ? k_tile<true, float, true> : k_tile<false, float, true>, scr32);
+        else tile(has_b ? k_tile<true, double, true> : k_tile<false, double, true>, ws.scr.p);
+    } else {
+        if (fp32) tile(has_b ? k_tile<true, float, false> : k_tile<false, float, false>, scr32);
+        else tile(has_b ? k_tile<true, double, false> : k_tile<false, double, false>, ws.scr.p);
+    }
     CVM_HIP(hipGetLastError());
     tend(slot);
     if (folds) {

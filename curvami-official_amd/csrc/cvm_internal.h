@@ -268,6 +268,7 @@ struct Context {
     std::vector<Fold> folds;
     Plan plan;
     ConstraintLayout cons;
+    DBuf<int> d_status;         // RKPM status words, 4 per domain
     // assembly buffers
     DBuf<double> d_x;           // 3 nn displacements
     DBuf<double> d_F;           // 3 nn
@@ -275,8 +276,12 @@ struct Context {
     Workspace ws;               // the context's own assembly workspace (its stream)
     size_t lds_limit = 65536;
     Timing timing;
-    int tile_gps = 16;
-    int union_max = 64;
+    int tile_gps = 16;          // GPs per assembly tile (at most kTileGps)
+    int union_max = 64;         // union nodes per tile (at most kTileUnion)
+    int tile_grid = 0;          // tile kernel grid: 0 = one workgroup per tile, else persistent
+    // captured assembly (cvm_graph_capture)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
     ~Context();
 };
 
@@ -285,6 +290,7 @@ void launch_indicator_gp(Context& c, Domain& d);
 void launch_node_signs(Context& c);
 void launch_frnn(Context& c, Domain& d, double rho);
 void launch_rkpm(Context& c, Domain& d, double rho, int order, int kernel, int* code, int* nbad, int* first);
+void launch_rkpm_all(Context& c, int order, int kernel, int* dom, int* code, int* nbad, int* first);
 void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, double* Fdev, double* Kdev);
 size_t assembly_workspace_bytes(const Context& c, int flags);
 void reserve_workspace(const Context& c, Workspace& ws, int flags);

@@ -58,8 +58,8 @@ void build_plan(Context& c) {
     const std::vector<int>& gsh = fd.h_gsheet;
     int nmax = 0;
     for (int g = 0; g < ng; ++g) nmax = std::max(nmax, off[g + 1] - off[g]);
-    if (nmax > 64) throw Error(CVM_E_INVALID, "a Gauss point has more than 64 neighbours");
-    P.union_max = kTileUnion;
+    if (nmax > kTileUnion) throw Error(CVM_E_INVALID, "a Gauss point has more than 64 neighbours");
+    P.union_max = std::max(std::min(c.union_max, kTileUnion), nmax);
 
     // ---- tiles and touched pairs (tile size shrinks until the LDS fits) -----
     std::vector<int> tile_gp, tile_uoff, unodes;

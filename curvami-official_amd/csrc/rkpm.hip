@@ -363,39 +363,71 @@ static void run(Context& c, Domain& d, int* st) {
                        d.sf.p, static_cast<int64_t>(d.npairs), st);
 }
 
-void launch_rkpm(Context& c, Domain& d, double rho, int order, int kernel, int* code, int* nbad, int* first) {
-    if (d.npairs < 0) throw Error(CVM_E_STATE, "neighbours not computed for this domain");
+static void check_args(int order, int kernel) {
     if (order < 1 || order > 3) throw Error(CVM_E_INVALID, "order must be 1, 2 or 3");
     if (kernel != CVM_KERNEL_QUARTIC && kernel != CVM_KERNEL_CUBIC_BSPLINE)
         throw Error(CVM_E_INVALID, "unknown window kernel");
-    launch_node_signs(c);
+}
+
+// Enqueues the shape functions of one domain; its status (n1, first1, n2,
+// first2) goes to st[0..3] on the device.
+static void enqueue_rkpm(Context& c, Domain& d, int order, int kernel, int* st) {
+    if (d.npairs < 0) throw Error(CVM_E_STATE, "neighbours not computed for this domain");
     launch_indicator_gp(c, d);
     d.sf.alloc(std::max<int64_t>(6 * d.npairs, 1));
-    DBuf<int> st;
-    st.alloc(4);
-    hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, c.stream, st.p);
-    *code = CVM_OK; *nbad = 0; *first = 0;
+    hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, c.stream, st);
     if (d.ng > 0) {
         const int slot = timing_begin(c, d.kind == CVM_DOMAIN_FACE ? "rkpm_face" : "rkpm_line");
         if (kernel == CVM_KERNEL_QUARTIC) {
-            if (order == 1) run<1, CVM_KERNEL_QUARTIC>(c, d, st.p);
-            else if (order == 2) run<2, CVM_KERNEL_QUARTIC>(c, d, st.p);
-            else run<3, CVM_KERNEL_QUARTIC>(c, d, st.p);
+            if (order == 1) run<1, CVM_KERNEL_QUARTIC>(c, d, st);
+            else if (order == 2) run<2, CVM_KERNEL_QUARTIC>(c, d, st);
+            else run<3, CVM_KERNEL_QUARTIC>(c, d, st);
         } else {
-            if (order == 1) run<1, CVM_KERNEL_CUBIC_BSPLINE>(c, d, st.p);
-            else if (order == 2) run<2, CVM_KERNEL_CUBIC_BSPLINE>(c, d, st.p);
-            else run<3, CVM_KERNEL_CUBIC_BSPLINE>(c, d, st.p);
+            if (order == 1) run<1, CVM_KERNEL_CUBIC_BSPLINE>(c, d, st);
+            else if (order == 2) run<2, CVM_KERNEL_CUBIC_BSPLINE>(c, d, st);
+            else run<3, CVM_KERNEL_CUBIC_BSPLINE>(c, d, st);
         }
         CVM_HIP(hipGetLastError());
         timing_end(c, slot);
     }
-    int hs[4];
-    CVM_HIP(hipMemcpyAsync(hs, st.p, sizeof(hs), hipMemcpyDeviceToHost, c.stream));
-    CVM_HIP(hipStreamSynchronize(c.stream));
+}
+
+static void finish_rkpm(Domain& d, int order, int kernel, const int* hs, int* code, int* nbad, int* first) {
     d.order = order; d.kernel = kernel;
+    *code = CVM_OK; *nbad = 0; *first = 0;
     if (hs[0] > 0) { *code = CVM_E_SINGULAR_MOMENT; *nbad = hs[0]; *first = hs[1]; }
     else if (hs[2] > 0) { *code = CVM_E_PIVOT_COLLAPSE; *nbad = hs[2]; *first = hs[3]; }
     d.have_sf = (*code == CVM_OK);
+}
+
+void launch_rkpm(Context& c, Domain& d, double /*rho*/, int order, int kernel, int* code, int* nbad, int* first) {
+    check_args(order, kernel);
+    launch_node_signs(c);
+    c.d_status.ensure(4);
+    enqueue_rkpm(c, d, order, kernel, c.d_status.p);
+    int hs[4];
+    CVM_HIP(hipMemcpyAsync(hs, c.d_status.p, sizeof(hs), hipMemcpyDeviceToHost, c.stream));
+    CVM_HIP(hipStreamSynchronize(c.stream));
+    finish_rkpm(d, order, kernel, hs, code, nbad, first);
+}
+
+// Every domain, one status read-back at the end. On failure: the first failing
+// domain and its status.
+void launch_rkpm_all(Context& c, int order, int kernel, int* dom, int* code, int* nbad, int* first) {
+    check_args(order, kernel);
+    launch_node_signs(c);
+    const size_t nd = c.doms.size();
+    c.d_status.ensure(std::max<size_t>(4 * nd, 4));
+    for (size_t i = 0; i < nd; ++i) enqueue_rkpm(c, *c.doms[i], order, kernel, c.d_status.p + 4 * i);
+    std::vector<int> hs(4 * std::max<size_t>(nd, 1));
+    if (nd) CVM_HIP(hipMemcpyAsync(hs.data(), c.d_status.p, 4 * nd * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    CVM_HIP(hipStreamSynchronize(c.stream));
+    *dom = -1; *code = CVM_OK; *nbad = 0; *first = 0;
+    for (size_t i = 0; i < nd; ++i) {
+        int cd, nb, fb;
+        finish_rkpm(*c.doms[i], order, kernel, hs.data() + 4 * i, &cd, &nb, &fb);
+        if (cd != CVM_OK && *dom < 0) { *dom = static_cast<int>(i); *code = cd; *nbad = nb; *first = fb; }
+    }
 }
 
 }  // namespace cvm

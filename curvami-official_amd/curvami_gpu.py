@@ -44,6 +44,7 @@ SIGNATURES = {
     "cvm_get_neighbours": (C.c_int, [C.c_void_p, C.c_int, _i, _i, _d]),
     "cvm_shape_functions": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _i, _i]),
     "cvm_get_shape_functions": (C.c_int, [C.c_void_p, C.c_int, _d, _d, _d, _d, _d, _d]),
+    "cvm_shape_functions_all": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _i, _i, _i]),
     "cvm_init_assembly": (C.c_int, [C.c_void_p, _i64, _i64]),
     "cvm_get_pattern": (C.c_int, [C.c_void_p, C.c_int, _i, _i]),
     "cvm_csr_mapping": (C.c_int, [C.c_void_p, C.c_int, _i, C.c_int, _i]),
@@ -54,6 +55,10 @@ SIGNATURES = {
     "cvm_constraints": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "cvm_external_force": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "cvm_synchronize": (C.c_int, [C.c_void_p]),
+    "cvm_set_tile_shape": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "cvm_get_tile_shape": (C.c_int, [C.c_void_p, _i]),
+    "cvm_graph_capture": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "cvm_graph_launch": (C.c_int, [C.c_void_p]),
     "cvm_server_create": (C.c_int, [C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_void_p)]),
     "cvm_server_destroy": (None, [C.c_void_p]),
     "cvm_server_last_error": (C.c_char_p, [C.c_void_p]),
@@ -206,6 +211,16 @@ class Plate:
             raise err
         self._check(st)
 
+    def shape_functions_all(self, order=2, kernel=0):
+        """Every domain, one status read-back; raises with .domain, .nbad, .first."""
+        dm, nb, fb = C.c_int(-1), C.c_int(0), C.c_int(0)
+        st = self._L.cvm_shape_functions_all(self._h, order, kernel, C.byref(dm), C.byref(nb), C.byref(fb))
+        if st in (E_SINGULAR_MOMENT, E_PIVOT_COLLAPSE):
+            err = CvmError(st, self._L.cvm_last_error(self._h).decode())
+            err.domain, err.nbad, err.first = dm.value, nb.value, fb.value
+            raise err
+        self._check(st)
+
     def get_shape_functions(self, dom):
         n = max(self.npairs[dom], 1)
         arrs = [np.empty(n) for _ in range(6)]
@@ -280,6 +295,21 @@ class Plate:
     def synchronize(self):
         self._check(self._L.cvm_synchronize(self._h))
 
+    def set_tile_shape(self, tile_gps=16, union_max=64, grid=0):
+        self._check(self._L.cvm_set_tile_shape(self._h, int(tile_gps), int(union_max), int(grid)))
+
+    def tile_shape(self):
+        out = (C.c_int * 4)()
+        self._check(self._L.cvm_get_tile_shape(self._h, out))
+        return {"tile_gps": out[0], "union_max": out[1], "grid": out[2], "ntiles": out[3]}
+
+    def graph_capture(self, x_ptr, F_ptr, K_ptr, flags):
+        self._check(self._L.cvm_graph_capture(self._h, flags, C.c_void_p(x_ptr), C.c_void_p(F_ptr),
+                                              C.c_void_p(K_ptr) if K_ptr else None))
+
+    def graph_launch(self):
+        self._check(self._L.cvm_graph_launch(self._h))
+
     # ---- measurement ----------------------------------------------------------
     def set_profiling(self, on=True):
         self._check(self._L.cvm_set_profiling(self._h, int(on)))
@@ -323,7 +353,7 @@ def batch_from_sheets(sheets, device: int = 0, compute: bool = True):
     if compute:
         for d in range(len(p.domain_sizes)):
             p.neighbours(d, 0.0)
-            p.shape_functions(d, sheets[0].order, sheets[0].kernel)
+        p.shape_functions_all(sheets[0].order, sheets[0].kernel)
         p.init_assembly()
     return p
 

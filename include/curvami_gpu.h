@@ -133,6 +133,12 @@ CVM_API int cvm_get_neighbours(cvm_context* ctx, int domain, int* off, int* ids,
  * *first_bad set as the reference's messages report them. */
 CVM_API int cvm_shape_functions(cvm_context* ctx, int domain, int order, int kernel,
                                 int* nbad, int* first_bad);
+/* Shape functions of every domain with one status read-back (the per-domain
+ * launches are queued back to back): SetupHelper::recomputeShapeFunctions
+ * (plate/SetupHelper.h:112-138). On failure *failed_domain names the first
+ * failing domain and the status is as above. */
+CVM_API int cvm_shape_functions_all(cvm_context* ctx, int order, int kernel, int* failed_domain, int* nbad,
+                                    int* first_bad);
 /* Copy Psi, dPsi/dX, dPsi/dY, d2Psi/dX2, d2Psi/dY2, d2Psi/dXdY (npairs each,
  * FRNN order; GaussPointData::Psi/dPsi/ddPsi). Any pointer may be NULL. */
 CVM_API int cvm_get_shape_functions(cvm_context* ctx, int domain, double* psi, double* dpx, double* dpy,
@@ -179,6 +185,24 @@ CVM_API int cvm_constraints(cvm_context* ctx, const double* x, int flags, double
 /* Reference load F_ext[3nn] of Meshless::computeExternalForce (plate/Meshless.h:60-100):
  * dead forces of point, edge and fold domains and edge moments. flags: CVM_DEVICE_PTRS. */
 CVM_API int cvm_external_force(cvm_context* ctx, int flags, double* F_ext);
+
+/* Tile shape of the assembly plan (no reference counterpart; chosen by the
+ * autotuner): at most tile_gps (1..16) consecutive face GPs per tile, at most
+ * union_max (8..64) distinct nodes per tile, and the tile kernel grid (0: one
+ * workgroup per tile; > 0: a persistent grid of that many workgroups striding
+ * over the tiles). Changing tile_gps or union_max invalidates the plan
+ * (call cvm_init_assembly again); the grid applies to the next launch. */
+CVM_API int cvm_set_tile_shape(cvm_context* ctx, int tile_gps, int union_max, int grid);
+/* out4: tile_gps in effect, union_max, grid, number of tiles (0 without a plan). */
+CVM_API int cvm_get_tile_shape(cvm_context* ctx, int* out4);
+
+/* Capture one evaluation (cvm_assemble with device pointers x, F_int, K_values)
+ * into a HIP graph held by the context; cvm_graph_launch replays it on the
+ * context's stream (asynchronous; cvm_synchronize waits). The graph reads and
+ * writes the captured addresses, so a Newton loop updates x in place and
+ * replays. No reference counterpart (one launch per Newton iteration). */
+CVM_API int cvm_graph_capture(cvm_context* ctx, int flags, const double* x_dev, double* F_dev, double* K_dev);
+CVM_API int cvm_graph_launch(cvm_context* ctx);
 
 /* Wait for the context's stream. */
 CVM_API int cvm_synchronize(cvm_context* ctx);

@@ -195,3 +195,25 @@ def test_fp32_tangent(case, request):
     ef = np.linalg.norm(K32 - Kr) / np.linalg.norm(Kr)
     print(f"{case}: FP32 tangent max rel {ek:.3e}, Frobenius rel {ef:.3e}")
     assert ek <= 1e-5 and ef <= 1e-5
+
+
+def test_shape_functions_all_matches_per_domain(c1):
+    """cvm_shape_functions_all (one status read-back) gives the per-domain results."""
+    sh, _, gpu = c1
+    ref = [gpu.get_shape_functions(d) for d in range(len(gpu.domain_sizes))]
+    gpu.shape_functions_all(sh.order, sh.kernel)
+    for d in range(len(gpu.domain_sizes)):
+        got = gpu.get_shape_functions(d)
+        for q in range(6):
+            assert np.array_equal(got[q].view(np.int64), ref[d][q].view(np.int64))
+
+
+def test_shape_functions_all_reports_failing_domain():
+    sh = sheet("straight21_rho2.5")
+    gpu = G.plate_from_sheet(sh, compute=False)
+    for d in range(len(gpu.domain_sizes)):
+        gpu.neighbours(d, sh.rho)
+    with pytest.raises(G.CvmError) as ei:
+        gpu.shape_functions_all(sh.order, sh.kernel)
+    assert ei.value.code == G.E_PIVOT_COLLAPSE and ei.value.domain == 0
+    assert "domain 0" in str(ei.value)
