@@ -61,6 +61,17 @@ __device__ __forceinline__ double wave_sum(double v) {
 // TT = double: the FP64 tangent; TT = float: the mixed-precision tangent (FP32
 // pair blocks and FP32 accumulation), with kinematics, resultants and the
 // residual still in FP64.
+// Workgroups are dispatched round-robin over the 8 XCDs (b -> XCD b % 8). Map
+// them so that each XCD runs a contiguous range of tiles in order: neighbouring
+// tiles write neighbouring scratch rows of the same K blocks, and those partial
+// cache lines then merge in one XCD's L2 instead of being written back from
+// several.
+__device__ __forceinline__ int xcd_tile(int b, int nwg) {
+    constexpr int NX = 8;
+    const int q = nwg / NX, r = nwg % NX, x = b % NX;
+    return x * q + min(x, r) + b / NX;
+}
+
 template <bool HAS_B, typename TT, bool PERSIST>
 __global__ __launch_bounds__(TILE_THREADS) void k_tile(
     const int* __restrict__ tile_gp, const int* __restrict__ tile_uoff, const int* __restrict__ tile_unodes,
@@ -301,7 +312,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
             __syncthreads();   // the next tile reuses the LDS
         }
     } else {
-        body(blockIdx.x);
+        body(xcd_tile(blockIdx.x, gridDim.x));
     }
 }
 
