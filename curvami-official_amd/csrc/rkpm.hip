@@ -117,7 +117,7 @@ __device__ __forceinline__ void static_for(F&& f) {
 constexpr int RKPM_THREADS = 64;
 
 template <int P, int KER>
-__global__ __launch_bounds__(RKPM_THREADS) void k_rkpm(
+__global__ __launch_bounds__(RKPM_THREADS, 2) void k_rkpm(
     const double* __restrict__ X, const double* __restrict__ Y, int nn,
     const double* __restrict__ gx, const double* __restrict__ gy, int ng,
     const int* __restrict__ off, const int* __restrict__ ids, const double* __restrict__ sarr,

@@ -53,6 +53,14 @@ int main(int argc, char** argv) {
             return g;
         };
         plate.addFold(gp("L"), gp("R"), meta[2], meta[3]);
+        // boundary domains (point / edge) with their BoundaryCondition
+        const int nb = meta.size() > 5 ? static_cast<int>(meta[5]) : 0;
+        for (int k = 0; k < nb; ++k) {
+            const std::string b = "b" + std::to_string(k);
+            const auto bm = load<double>(d + "/" + b + "meta.bin");   // kind, active, val[4], force[3], moment
+            const int dom = plate.addDomain(static_cast<int>(bm[0]), gp(b.c_str()));
+            plate.setBoundaryCondition(dom, static_cast<int>(bm[1]), &bm[2], &bm[6], bm[9]);
+        }
         plate.recomputeShapeFunctions();
         plate.initAssembly();
         curvami_gpu::CSC K;
@@ -63,6 +71,20 @@ int main(int argc, char** argv) {
         save(d + "/K_values.bin", K.values);
         save(d + "/F.bin", F);
         save(d + "/map0.bin", plate.csrMapping(0));
+        if (nb > 0) {
+            std::vector<double> c;
+            std::vector<curvami_gpu::Meshless::Triplet> t;
+            int nrows = 0;
+            plate.evaluateConstraints(load<double>(d + "/x.bin"), c, t, nrows, meta[4] != 0);
+            std::vector<int> rows, cols;
+            std::vector<double> vals;
+            for (const auto& e : t) { rows.push_back(e.row); cols.push_back(e.col); vals.push_back(e.value); }
+            save(d + "/c.bin", c);
+            save(d + "/J_rows.bin", rows);
+            save(d + "/J_cols.bin", cols);
+            save(d + "/J_vals.bin", vals);
+            save(d + "/F_ext.bin", plate.computeExternalForce());
+        }
         std::printf("mirror_check: nnz=%zu\n", K.values.size());
     } catch (const std::exception& e) {
         std::fprintf(stderr, "mirror_check error: %s\n", e.what());
