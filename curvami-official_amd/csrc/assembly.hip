@@ -30,7 +30,9 @@ namespace cvm {
 constexpr int TILE_THREADS = 512;    // 8 waves; wave w prepares GPs w and w + 8 of the tile
 constexpr int TILE_GPS = 16;         // GPs per tile (presence masks are 16-bit)
 constexpr int UMAX = 64;             // union nodes per tile (one lane per neighbour)
-constexpr int GS = 17;               // GP record planes (tangent type, SoA over the tile's GPs)
+constexpr int GS = 17;               // GP record fields (tangent type)
+constexpr int GSP = 20;              // GP record stride: GP-major, 16-byte aligned for wide LDS reads
+constexpr uint16_t NO_REC = 0xffff;  // loc entry of a union node that a GP does not see
 
 struct MatC {
     double A[9], B[9], D[9];
@@ -93,19 +95,19 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
     double* recf = smem;                                        // 3 FP64 planes of rec_cap
     double* uc = recf + static_cast<size_t>(rec_cap) * 3;       // UMAX * 3
     TT* rect = reinterpret_cast<TT*>(uc + UMAX * 3);            // 6 TT planes of rec_cap
-    TT* gs = rect + static_cast<size_t>(rec_cap) * 6;           // GS planes of TILE_GPS
-    uint32_t* pm = reinterpret_cast<uint32_t*>(gs + TILE_GPS * GS);  // UMAX presence masks
-    uint8_t* loc = reinterpret_cast<uint8_t*>(pm + UMAX);       // TILE_GPS * UMAX
+    TT* gs = rect + static_cast<size_t>(rec_cap) * 6;           // TILE_GPS records of GSP
+    uint32_t* pm = reinterpret_cast<uint32_t*>(gs + TILE_GPS * GSP);  // UMAX presence masks
+    uint16_t* loc = reinterpret_cast<uint16_t*>(pm + UMAX);     // TILE_GPS * UMAX record indices
     auto RF = [&](int f, int r) -> double& { return recf[static_cast<size_t>(f) * rec_cap + r]; };
     auto R = [&](int f, int r) -> TT& { return rect[static_cast<size_t>(f) * rec_cap + r]; };
-    auto Gp = [&](int f, int wv) -> TT& { return gs[f * TILE_GPS + wv]; };
+    auto Gp = [&](int f, int wv) -> TT& { return gs[wv * GSP + f]; };
 
     if (tid < U) {
         const int v = tile_unodes[ub + tid];
         uc[3 * tid] = xu[3 * v]; uc[3 * tid + 1] = xu[3 * v + 1]; uc[3 * tid + 2] = xu[3 * v + 2];
         pm[tid] = 0u;
     }
-    for (int k = tid; k < TILE_GPS * UMAX; k += TILE_THREADS) loc[k] = 0xff;
+    for (int k = tid; k < TILE_GPS * UMAX; k += TILE_THREADS) loc[k] = NO_REC;
     __syncthreads();
 
     const double* A = mat.A;
@@ -184,7 +186,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
                 R(R_BE0, r) = static_cast<TT>(be0); R(R_BE1, r) = static_cast<TT>(be1);
                 R(R_BE2, r) = static_cast<TT>(be2);
                 R(R_TU, r) = static_cast<TT>((px * pX + py * pY) - (hxx * M[0] + 2.0 * hxy * M[2] + hyy * M[1]));
-                loc[wv * UMAX + ul] = static_cast<uint8_t>(lane);
+                loc[wv * UMAX + ul] = static_cast<uint16_t>(r);   // the record, tile-relative
                 atomicOr(&pm[ul], 1u << wv);
             }
             if (lane == 0) {
@@ -215,9 +217,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
             // Two coupling GPs per iteration (ascending GP order is kept: the first
             // GP's contribution is added before the second's).
             auto pair_block = [&](int wv, TT (&kb)[9]) {
-                const int base = off[g0 + wv] - pb;
-                const int ri = base + loc[wv * UMAX + ua];
-                const int rj = base + loc[wv * UMAX + ubb];
+                const int ri = loc[wv * UMAX + ua];
+                const int rj = loc[wv * UMAX + ubb];
                 const TT W = Gp(G_W, wv);
                 const TT a1[3] = {Gp(G_A1, wv), Gp(G_A1 + 1, wv), Gp(G_A1 + 2, wv)};
                 const TT a2[3] = {Gp(G_A2, wv), Gp(G_A2 + 1, wv), Gp(G_A2 + 2, wv)};
@@ -297,9 +298,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
     if (tid < U) {
         double f0 = 0.0, f1 = 0.0, f2 = 0.0;
         for (int wv = 0; wv < ngp; ++wv) {
-            const int l = loc[wv * UMAX + tid];
-            if (l == 0xff) continue;
-            const int r = off[g0 + wv] - pb + l;
+            const int r = loc[wv * UMAX + tid];
+            if (r == NO_REC) continue;
             f0 += RF(R_F0, r); f1 += RF(R_F1, r); f2 += RF(R_F2, r);
         }
         double* dst = fscr + 3 * static_cast<int64_t>(ub + tid);
@@ -513,7 +513,8 @@ __global__ void k_reduce_F(int nn, const int* __restrict__ fc_off, const int* __
 size_t tile_lds_bytes(int rec_cap, bool fp32) {
     const size_t tt = fp32 ? sizeof(float) : sizeof(double);
     return sizeof(double) * (static_cast<size_t>(rec_cap) * 3 + UMAX * 3)
-         + tt * (static_cast<size_t>(rec_cap) * 6 + TILE_GPS * GS) + sizeof(uint32_t) * UMAX + TILE_GPS * UMAX;
+         + tt * (static_cast<size_t>(rec_cap) * 6 + TILE_GPS * GSP) + sizeof(uint32_t) * UMAX
+         + sizeof(uint16_t) * TILE_GPS * UMAX;
 }
 
 size_t assembly_workspace_bytes(const Context& c, int flags) {
