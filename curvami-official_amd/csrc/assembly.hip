@@ -94,12 +94,12 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
 
     double* recf = smem;                                        // 3 FP64 planes of rec_cap
     double* uc = recf + static_cast<size_t>(rec_cap) * 3;       // UMAX * 3
-    TT* rect = reinterpret_cast<TT*>(uc + UMAX * 3);            // 6 TT planes of rec_cap
+    TT* rect = reinterpret_cast<TT*>(uc + UMAX * 3);            // rec_cap records of 6 TT
     TT* gs = rect + static_cast<size_t>(rec_cap) * 6;           // TILE_GPS records of GSP
     uint32_t* pm = reinterpret_cast<uint32_t*>(gs + TILE_GPS * GSP);  // UMAX presence masks
     uint16_t* loc = reinterpret_cast<uint16_t*>(pm + UMAX);     // TILE_GPS * UMAX record indices
     auto RF = [&](int f, int r) -> double& { return recf[static_cast<size_t>(f) * rec_cap + r]; };
-    auto R = [&](int f, int r) -> TT& { return rect[static_cast<size_t>(f) * rec_cap + r]; };
+    auto R = [&](int f, int r) -> TT& { return rect[static_cast<size_t>(r) * 6 + f]; };   // record-major
     auto Gp = [&](int f, int wv) -> TT& { return gs[wv * GSP + f]; };
 
     if (tid < U) {
