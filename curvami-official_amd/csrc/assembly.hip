@@ -453,7 +453,10 @@ __global__ void k_reduce_K(int64_t nblocks, const int64_t* __restrict__ kc_off,
     TT vt[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) vt[k] = TT(0);
-    for (int64_t a = kc_off[blk]; a < kc_off[blk + 1]; ++a) {
+    // rows in tile order; unrolled so that several rows' loads are in flight
+    const int64_t a0 = kc_off[blk], a1 = kc_off[blk + 1];
+#pragma unroll 4
+    for (int64_t a = a0; a < a1; ++a) {
         const TT* s = scr + 9 * a;
 #pragma unroll
         for (int k = 0; k < 9; ++k) vt[k] += s[k];
