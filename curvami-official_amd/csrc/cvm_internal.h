@@ -99,6 +99,7 @@ struct SheetSeg {
 // the sheet's cell grid (keys koff + y * cols + x), radius and crease range.
 struct SheetDev {
     double minx = 0.0, miny = 0.0, cell = 0.0, rho = 0.0, rho2 = 0.0;
+    double irho = 0.0, irho2 = 0.0;     // 1 / rho, 1 / rho^2 (IEEE, host)
     long long koff = 0;
     int cols = 0, nkeys = 0, nc = 0, crease0 = 0;
 };

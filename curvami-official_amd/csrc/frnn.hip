@@ -249,6 +249,7 @@ void launch_frnn(Context& c, Domain& d, double rho) {
         const SheetSeg& sg = segs[si];
         SheetDev& gr = table[si];
         gr.rho = sg.rho; gr.rho2 = sg.rho * sg.rho; gr.cell = sg.rho;
+        gr.irho = 1.0 / sg.rho; gr.irho2 = 1.0 / (sg.rho * sg.rho);
         gr.nc = sg.ncreases; gr.crease0 = sg.crease0;
         if (sg.nn == 0) { gr.nkeys = 0; gr.koff = koff; continue; }
         double mnx = c.hX[sg.node0], mny = c.hY[sg.node0], mxx = mnx, mxy = mny;

@@ -72,17 +72,19 @@ __device__ __forceinline__ Jet pair_jet(double xi, double eta, double si, int I,
                                         int nc, double rho) {
     double w, d, dd;
     window<KER>(si, w, d, dd);
-    const double dx = (si == 0.0) ? 0.0 : xi / si;
-    const double dy = (si == 0.0) ? 0.0 : eta / si;
-    const double si3 = si * si * si;
+    // one reciprocal per pair instead of five divisions
+    const double is = (si == 0.0) ? 0.0 : 1.0 / si;
+    const double is3 = is * is * is;
+    const double dx = xi * is;
+    const double dy = eta * is;
     // xi^2/s^3 and eta^2/s^3 as in GlobalRKPM.h:79-80
-    const double d2x = (si == 0.0) ? dd : (xi * xi) / si3;
-    const double d2y = (si == 0.0) ? dd : (eta * eta) / si3;
+    const double d2x = (si == 0.0) ? dd : (xi * xi) * is3;
+    const double d2y = (si == 0.0) ? dd : (eta * eta) * is3;
     Jet j;
     j.v = w; j.x = d * dx; j.y = d * dy;
     j.xx = dd * dx * dx + d * d2x;
     j.yy = dd * dy * dy + d * d2y;
-    j.xy = (si == 0.0) ? 0.0 : dd * dx * dy + d * (-(xi * eta) / si3);
+    j.xy = (si == 0.0) ? 0.0 : dd * dx * dy + d * (-(xi * eta) * is3);
     for (int c = 0; c < nc; ++c) {
         const double sn = static_cast<double>(nsign[static_cast<size_t>(c) * nn + I]);
         const double* ig = ind + static_cast<size_t>(c) * 6 * ng + g;
@@ -137,7 +139,7 @@ __global__ __launch_bounds__(RKPM_THREADS, 2) void k_rkpm(
 
     if (threadIdx.x < nblk) {
         const SheetDev& S = sheets[gsheet ? gsheet[g] : 0];
-        const double rho = S.rho;
+        const double rho = S.rho, irho = S.irho;
         const int nc = S.nc;
         const double px = gx[g], py = gy[g];
         const int b = off[g], e = off[g + 1];
@@ -149,8 +151,8 @@ __global__ __launch_bounds__(RKPM_THREADS, 2) void k_rkpm(
         });
         for (int k = b; k < e; ++k) {
             const int I = ids[k];
-            const double xi = (px - X[I]) / rho;
-            const double eta = (py - Y[I]) / rho;
+            const double xi = (px - X[I]) * irho;
+            const double eta = (py - Y[I]) * irho;
             const Jet jt = pair_jet<KER>(xi, eta, sarr[k], I, g, ng, nn, ind, nsign, nc, rho);
             double xp[DM + 1], yp[DM + 1];
             xp[0] = 1.0; yp[0] = 1.0;
@@ -311,12 +313,12 @@ __global__ __launch_bounds__(RKPM_THREADS, 2) void k_rkpm(
         }
         const int gg = g0 + lo;
         const SheetDev& S = sheets[gsheet ? gsheet[gg] : 0];
-        const double rho = S.rho;
+        const double rho = S.rho, irho = S.irho, ir2 = S.irho2;
         const int nc = S.nc;
         const double* cj = cj_s + lo * NCJ;
         const int I = ids[k];
-        const double xi = (gx[gg] - X[I]) / rho;
-        const double eta = (gy[gg] - Y[I]) / rho;
+        const double xi = (gx[gg] - X[I]) * irho;
+        const double eta = (gy[gg] - Y[I]) * irho;
         const Jet jt = pair_jet<KER>(xi, eta, sarr[k], I, gg, ng, nn, ind, nsign, nc, rho);
         double xp[P + 1], yp[P + 1];
         xp[0] = 1.0; yp[0] = 1.0;
@@ -340,13 +342,12 @@ __global__ __launch_bounds__(RKPM_THREADS, 2) void k_rkpm(
             Cyy += vyy * p + 2.0 * vy * pyv + v * pyy;
             Cxy += vxy * p + vx * pyv + vy * pxv + v * pxy;
         });
-        const double r2 = rho * rho;
         sf[k] = C * jt.v;
-        sf[npairs + k] = (Cx * jt.v + C * jt.x) / rho;
-        sf[2 * npairs + k] = (Cy * jt.v + C * jt.y) / rho;
-        sf[3 * npairs + k] = (Cxx * jt.v + 2.0 * Cx * jt.x + C * jt.xx) / r2;
-        sf[4 * npairs + k] = (Cyy * jt.v + 2.0 * Cy * jt.y + C * jt.yy) / r2;
-        sf[5 * npairs + k] = (Cxy * jt.v + Cx * jt.y + Cy * jt.x + C * jt.xy) / r2;
+        sf[npairs + k] = (Cx * jt.v + C * jt.x) * irho;
+        sf[2 * npairs + k] = (Cy * jt.v + C * jt.y) * irho;
+        sf[3 * npairs + k] = (Cxx * jt.v + 2.0 * Cx * jt.x + C * jt.xx) * ir2;
+        sf[4 * npairs + k] = (Cyy * jt.v + 2.0 * Cy * jt.y + C * jt.yy) * ir2;
+        sf[5 * npairs + k] = (Cxy * jt.v + Cx * jt.y + Cy * jt.x + C * jt.xy) * ir2;
     }
 }
 
