@@ -346,6 +346,18 @@ int cvm_set_fold_gp_state(cvm_context* ctx, int fold_id, int n, const double* th
     CVM_CATCH(ctx)
 }
 
+int cvm_set_fold_rigid(cvm_context* ctx, int fold_id, int rigid) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    if (fold_id < 0 || fold_id >= static_cast<int>(c.folds.size())) throw Error(CVM_E_INVALID, "bad fold id");
+    if (c.folds[fold_id].rigid != (rigid != 0)) {
+        c.folds[fold_id].rigid = rigid != 0;
+        c.plan.ready = false;      // the hinge blocks of the pattern change
+        c.cons.ready = false;      // and so do the angle rows
+    }
+    CVM_CATCH(ctx)
+}
+
 int cvm_neighbours(cvm_context* ctx, int domain, double rho, int64_t* npairs) {
     CVM_TRY(ctx)
     Context& c = ctx->c;

@@ -44,6 +44,7 @@ struct CFold {
     int left, right;
     const double* theta0;    // may be null
     double Ang0;
+    int rigid;               // angle row for this fold
 };
 
 enum : int { JOB_DIRICHLET = 0, JOB_FOLD = 1 };
@@ -108,7 +109,7 @@ __global__ void k_constraints(int njobs, const int* __restrict__ jobs, const CDo
         for (int k = bL; k < eL; ++k) jvals[t++] = L.sf[k];
         for (int k = bR; k < eR; ++k) jvals[t++] = -R.sf[k];
     }
-    if (!rigid) return;
+    if (!rigid && !f.rigid) return;
     // F0 of both sides (Kinematics.h:75-91) and the fold angle row
     double F[12];
     for (int side = 0; side < 2; ++side) {
@@ -231,7 +232,7 @@ void build_constraint_layout(Context& c, bool rigid) {
                 trip += (eL - bL) + (eR - bR);
                 ++row;
             }
-            if (rigid) {
+            if (rigid || f.rigid) {
                 for (int k = bL; k < eL; ++k)
                     for (int comp = 0; comp < 3; ++comp) { L.rows.push_back(row); L.cols.push_back(3 * Ld.h_ids[k] + comp); }
                 for (int k = bR; k < eR; ++k)
@@ -254,6 +255,7 @@ void build_constraint_layout(Context& c, bool rigid) {
         ft[fi].left = f.left; ft[fi].right = f.right;
         ft[fi].theta0 = f.theta0.empty() ? nullptr : f.d_theta0.p;
         ft[fi].Ang0 = f.Ang0;
+        ft[fi].rigid = f.rigid ? 1 : 0;
     }
     L.d_dom.upload(reinterpret_cast<const unsigned char*>(dt.data()), std::max<size_t>(dt.size(), 1) * sizeof(CDom),
                    c.stream);

@@ -134,6 +134,7 @@ struct Domain {
 struct Fold {
     int left = -1, right = -1;
     double Kt = 0.0, Ang0 = 0.0;
+    bool rigid = false;           // per-fold rigid hinge: constraint rows instead of the elastic hinge
     std::vector<double> theta0;   // per crease GP (empty: Ang0)
     std::vector<char> active;     // per crease GP (empty: all active)
     DBuf<double> d_theta0;

@@ -190,6 +190,7 @@ void build_plan(Context& c) {
     std::vector<FGP> fgps;
     for (size_t f = 0; f < c.folds.size(); ++f) {
         const Fold& fo = c.folds[f];
+        if (fo.rigid) continue;             // rigid hinges are constraint rows, not K terms
         Domain& L = *c.doms[fo.left];
         Domain& R = *c.doms[fo.right];
         if (L.npairs < 0 || R.npairs < 0 || !L.have_sf || !R.have_sf)

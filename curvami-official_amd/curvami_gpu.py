@@ -40,6 +40,7 @@ SIGNATURES = {
     "cvm_add_domain": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _d, _d, _d, _d, _d, _d, _d, _i]),
     "cvm_add_fold": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, _i]),
     "cvm_set_fold_gp_state": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _d, C.c_char_p]),
+    "cvm_set_fold_rigid": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "cvm_neighbours": (C.c_int, [C.c_void_p, C.c_int, C.c_double, _i64]),
     "cvm_get_neighbours": (C.c_int, [C.c_void_p, C.c_int, _i, _i, _d]),
     "cvm_shape_functions": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _i, _i]),
@@ -180,6 +181,9 @@ class Plate:
         fid = C.c_int(-1)
         self._check(self._L.cvm_add_fold(self._h, left, right, Kt, Ang0, C.byref(fid)))
         return fid.value
+
+    def set_fold_rigid(self, fold, rigid=True):
+        self._check(self._L.cvm_set_fold_rigid(self._h, fold, int(rigid)))
 
     def set_fold_gp_state(self, fold, theta0=None, active=None):
         n = self.domain_sizes[0] if theta0 is None and active is None else \

@@ -315,6 +315,25 @@ def config2(kernel: int = 0, rho_factor: float = 2.5, n_side: int = 201,
                         rho_factor=rho_factor, kernel=kernel)
 
 
+def config5(n_side: int = 1001, kernel: int = 0, rho_factor: float = 3.75, nseg_arc: int = 0) -> Sheet:
+    """Large single sheet with a four-vertex-like crease layout: straight creases
+    on the middle node column (x = 0.5, fold 0) and the middle node row (y = 0.5,
+    fold 1), crossing at the centre, and a circular arc of radius 0.3 about the
+    centre from 18 to 162 degrees (fold 2; interior tips, so the crease
+    indicator's end caps are exercised). Fold 0 is meant to be the rigid hinge
+    (constraint rows); the others are elastic, Kt = 1e-2 E t^3."""
+    X, Y, h = grid_nodes(n_side)
+    mid = (n_side - 1) // 2
+    cv = straight_crease_on_row(n_side, h, mid, vertical=True)
+    ch = straight_crease_on_row(n_side, h, mid, vertical=False)
+    nseg = nseg_arc or max(32, (n_side - 1) // 2)
+    arc = arc_crease(0.5, 0.5, 0.3, 0.1 * math.pi, 0.9 * math.pi, nseg, False, False)
+    E, t = 1.0, 0.01
+    Kt = 1e-2 * E * t ** 3
+    return square_sheet(f"fourvertex_{n_side}x{n_side}", n_side, [cv, ch, arc], [Kt, Kt, Kt],
+                        rho_factor=rho_factor, kernel=kernel)
+
+
 BATCH_RHO_FACTOR = 3.75
 BATCH_SHEETS = 256
 

@@ -113,6 +113,11 @@ CVM_API int cvm_add_domain(cvm_context* ctx, int kind, int ng, const double* gx,
  * Meshless::addFold, plate/Meshless.h:30-46). */
 CVM_API int cvm_add_fold(cvm_context* ctx, int left_domain, int right_domain, double Kt, double Ang0,
                          int* fold_id);
+/* Per-fold rigid hinge (no reference counterpart: there flagRig is per call and
+ * covers every fold). A rigid fold adds no hinge terms to F_int and K, even with
+ * CVM_ELASTIC_FOLDS, and its angle rows theta - theta0 are constraint rows
+ * (cvm_constraint_layout) whatever the per-call flag. Invalidates the plan. */
+CVM_API int cvm_set_fold_rigid(cvm_context* ctx, int fold_id, int rigid);
 /* Per crease-GP rest angle and activity (progressive creasing; Fold::gp_theta0,
  * gp_active, PlateData.h:90-93). NULL clears. */
 CVM_API int cvm_set_fold_gp_state(cvm_context* ctx, int fold_id, int n, const double* theta0,

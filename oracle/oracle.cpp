@@ -872,6 +872,7 @@ struct Triplet { int r, c; double v; };
 struct FoldDef {
     int left = -1, right = -1;      // domain indices
     double Kt = 0.0, Ang0 = 0.0;
+    bool rigid = false;             // per-fold rigid hinge (constraint rows, no hinge terms)
     std::vector<double> theta0;     // per crease GP (empty: Ang0)
     std::vector<char> active;       // per crease GP (empty: all)
 };
@@ -1127,6 +1128,8 @@ int orc_add_fold(Plate* p, int left, int right, double Kt, double Ang0) {
     return static_cast<int>(p->folds.size()) - 1;
 }
 
+void orc_set_fold_rigid(Plate* p, int fold, int rigid) { p->folds[fold].rigid = rigid != 0; }
+
 void orc_set_fold_gp_state(Plate* p, int fold, int n, const double* theta0, const char* active) {
     FoldDef& f = p->folds[fold];
     f.theta0.assign(theta0, theta0 + (theta0 ? n : 0));
@@ -1217,7 +1220,7 @@ long orc_assemble(Plate* p, const double* x, int flag_rig, int exact_tangent, in
     } else {
         std::vector<Triplet> trips;
         for (const auto& f : p->folds)
-            elastic_fold(p->doms[f.left], p->doms[f.right], f, x, p->F, &trips, exact_tangent != 0);
+            if (!f.rigid) elastic_fold(p->doms[f.left], p->doms[f.right], f, x, p->F, &trips, exact_tangent != 0);
         // setFromTriplets (duplicates summed in triplet order), then K + K_fold
         // over the union of both patterns.
         std::vector<std::map<int, double>> cols(3 * nn);
@@ -1253,7 +1256,8 @@ void orc_internal_force(Plate* p, const double* x, int flag_rig, int with_tangen
     std::vector<double> Kv, F;
     update_stiffness(fd.nl, fd.sh, fd.w, p->mat, p->pat, x, F, Kv, nthreads, with_tangent != 0);
     if (!flag_rig)
-        for (const auto& f : p->folds) elastic_fold(p->doms[f.left], p->doms[f.right], f, x, F, nullptr, false);
+        for (const auto& f : p->folds)
+            if (!f.rigid) elastic_fold(p->doms[f.left], p->doms[f.right], f, x, F, nullptr, false);
     std::memcpy(F_out, F.data(), F.size() * sizeof(double));
 }
 
@@ -1308,7 +1312,7 @@ long orc_constraints(Plate* p, const double* x, int flag_rig, long* nrows) {
             if (d.bc.kind == kind && (d.bc.au || d.bc.av || d.bc.aw || d.bc.arot))
                 dirichlet_rows(d, x, row, p->c, p->J);
     for (const auto& f : p->folds)
-        fold_rows(p->doms[f.left], p->doms[f.right], f, x, flag_rig != 0, row, p->c, p->J);
+        fold_rows(p->doms[f.left], p->doms[f.right], f, x, flag_rig != 0 || f.rigid, row, p->c, p->J);
     *nrows = row;
     return static_cast<long>(p->J.size());
 }

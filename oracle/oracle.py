@@ -44,6 +44,7 @@ def lib():
             "orc_add_domain": (C.c_int, [P, C.c_int, _d, _d, _d, _d, _d, _d, _d, C.c_int]),
             "orc_add_fold": (C.c_int, [P, C.c_int, C.c_int, C.c_double, C.c_double]),
             "orc_set_fold_gp_state": (None, [P, C.c_int, C.c_int, _d, _c]),
+            "orc_set_fold_rigid": (None, [P, C.c_int, C.c_int]),
             "orc_indicator": (None, [P, C.c_int, C.c_double, C.c_double, _d]),
             "orc_window": (None, [C.c_int, C.c_int, _d, _d, _d, _d]),
             "orc_neighbours": (C.c_long, [P, C.c_int, C.c_double]),
@@ -147,6 +148,9 @@ class OraclePlate:
 
     def add_fold(self, left, right, Kt, Ang0=0.0) -> int:
         return self._L.orc_add_fold(self._p, left, right, Kt, Ang0)
+
+    def set_fold_rigid(self, fold, rigid=True):
+        self._L.orc_set_fold_rigid(self._p, fold, int(rigid))
 
     def indicator(self, c, x, y):
         out = np.empty(8)
