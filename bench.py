@@ -211,6 +211,45 @@ def batched_leg(nsheets, warmup, steps):
     return res
 
 
+def large_leg(warmup, steps):
+    """1001 x 1001 nodes (16M face GPs), two crossing straight creases and an
+    arc; fold 0 a rigid hinge (constraint rows, FP64), folds 1 and 2 elastic.
+    Step = shape functions of every domain + residual + tangent (graph replay);
+    the constraint rows of the rigid fold are timed separately."""
+    sh = W.config5(1001)
+    t0 = time.perf_counter()
+    p = G.plate_from_sheet(sh, compute=False)
+    p.set_fold_rigid(0)
+    for d in range(len(p.domain_sizes)):
+        p.neighbours(d, sh.rho)
+    p.shape_functions_all(sh.order, sh.kernel)
+    nf, nu = p.init_assembly()
+    setup = time.perf_counter() - t0
+    ng = sh.face.n
+    x = G.DeviceArray(3 * sh.nn); x.set(W.displacement(sh.nn, 1, 1e-3))
+    F = G.DeviceArray(3 * sh.nn); K = G.DeviceArray(nu)
+    nd = len(p.domain_sizes)
+    res = {"workload": f"four-vertex sheet 1001x1001 nodes, {ng} face GPs, {p.npairs[p.face_dom]} pairs, "
+                       f"nnz_K {nu}, rho 3.75h, creases x=0.5 (rigid), y=0.5, arc r=0.3 (elastic)",
+           "setup_s": round(setup, 1)}
+    base = G.RESIDUAL | G.TANGENT | G.ELASTIC_FOLDS
+    for name, fl in (("mixed", base | G.FP32_TANGENT), ("fp64", base)):
+        dt = _step_time(p, nd, sh.order, sh.kernel, x, F, K, fl, steps, warmup, graph=True)
+        res[name] = {"gps": ng / dt, "ms_per_step": dt * 1e3}
+    rows, cols = p.constraint_layout(rigid=False)
+    xv = W.displacement(sh.nn, 2, 1e-3)
+    p.constraints(xv)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        p.constraints(xv)
+    res["constraint_rows"] = {"rows": p.n_crows, "triplets": p.n_ctrip,
+                              "ms_host_in_out": (time.perf_counter() - t0) / steps * 1e3}
+    for a in (x, F, K):
+        a.free()
+    p.close()
+    return res
+
+
 def serving_leg(nreq):
     """Poisson request streams over the serving loop: sides U{21..201}, rigid
     fold of a seeded angle plus noise; p50 / p99 latency and throughput per
@@ -248,6 +287,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-sheet batched leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-large", action="store_true", help="skip the 1001 x 1001 leg")
     ap.add_argument("--batch-sheets", type=int, default=W.BATCH_SHEETS)
     ap.add_argument("--serving-requests", type=int, default=200)
     args = ap.parse_args()
@@ -376,6 +416,8 @@ def main():
         out["batched"] = batched_leg(args.batch_sheets, args.warmup, max(2, args.steps // 3))
     if not args.no_serving:
         out["serving"] = serving_leg(args.serving_requests)
+    if not args.no_large:
+        out["large"] = large_leg(args.warmup, max(2, args.steps // 3))
     print(json.dumps(out))
 
 
