@@ -17,7 +17,7 @@ import time
 import curvami_gpu as G
 
 TILE_GPS = (16, 12, 8)
-UNION_MAX = (64, 56)
+UNION_MAX = (0, 56)
 GRIDS = (0, 2048, 8192)
 
 

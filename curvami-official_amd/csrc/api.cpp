@@ -617,7 +617,8 @@ int cvm_set_tile_shape(cvm_context* ctx, int tile_gps, int union_max, int grid) 
     CVM_TRY(ctx)
     Context& c = ctx->c;
     if (tile_gps < 1 || tile_gps > cvm::kTileGps) throw Error(CVM_E_INVALID, "tile_gps must be in 1..16");
-    if (union_max < 8 || union_max > cvm::kTileUnion) throw Error(CVM_E_INVALID, "union_max must be in 8..64");
+    if (union_max != 0 && (union_max < 8 || union_max > cvm::kTileUnion))
+        throw Error(CVM_E_INVALID, "union_max must be 0 (automatic) or in 8..255");
     if (grid < 0) throw Error(CVM_E_INVALID, "grid must be >= 0");
     if (tile_gps != c.tile_gps || union_max != c.union_max) c.plan.ready = false;
     c.tile_gps = tile_gps;
@@ -631,7 +632,7 @@ int cvm_get_tile_shape(cvm_context* ctx, int* out4) {
     Context& c = ctx->c;
     if (!out4) throw Error(CVM_E_INVALID, "null output");
     out4[0] = c.plan.ready ? c.plan.tile_max_gps : c.tile_gps;
-    out4[1] = c.union_max;
+    out4[1] = c.plan.ready ? c.plan.union_max : c.union_max;
     out4[2] = c.tile_grid;
     out4[3] = c.plan.ready ? c.plan.ntiles : 0;
     CVM_CATCH(ctx)

@@ -29,7 +29,7 @@ namespace cvm {
 
 constexpr int TILE_THREADS = 512;    // 8 waves; wave w prepares GPs w and w + 8 of the tile
 constexpr int TILE_GPS = 16;         // GPs per tile (presence masks are 16-bit)
-constexpr int UMAX = 64;             // union nodes per tile (one lane per neighbour)
+constexpr int UMAX = 256;            // union nodes per tile (uint8 union indices, at most 255 used)
 constexpr int GS = 17;               // GP record fields (tangent type)
 constexpr int GSP = 20;              // GP record stride: GP-major, 16-byte aligned for wide LDS reads
 constexpr uint16_t NO_REC = 0xffff;  // loc entry of a union node that a GP does not see
@@ -91,13 +91,13 @@ __device__ __forceinline__ int xcd_tile(int b, int nwg) {
     return x * q + min(x, r) + b / NX;
 }
 
-template <bool HAS_B, typename TT, bool PERSIST>
-__global__ __launch_bounds__(TILE_THREADS) void k_tile(
+template <bool HAS_B, typename TT, bool PERSIST, bool WIDE>
+__global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((sizeof(TT) == 4 && !HAS_B && !WIDE && !PERSIST) ? 6 : 1))) void k_tile(
     const int* __restrict__ tile_gp, const int* __restrict__ tile_uoff, const int* __restrict__ tile_unodes,
     const int64_t* __restrict__ tile_toff, const uint16_t* __restrict__ tpairs, const uint32_t* __restrict__ slot_dst,
     const int* __restrict__ off, const uint8_t* __restrict__ uloc,
     const double* __restrict__ sf, int64_t npairs, const double* __restrict__ w,
-    const double* __restrict__ xu, MatC mat, int rec_cap, int with_tangent,
+    const double* __restrict__ xu, MatC mat, int rec_cap, int umax, int with_tangent,
     TT* __restrict__ scr, double* __restrict__ fscr, int ntiles) {
     extern __shared__ double smem[];
     const int tid = threadIdx.x;
@@ -110,11 +110,11 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
     const int pb = off[g0];
 
     double* recf = smem;                                        // 3 FP64 planes of rec_cap
-    double* uc = recf + static_cast<size_t>(rec_cap) * 3;       // UMAX * 3
-    TT* rect = reinterpret_cast<TT*>(uc + UMAX * 3);            // rec_cap records of 6 TT
+    double* uc = recf + static_cast<size_t>(rec_cap) * 3;       // umax * 3
+    TT* rect = reinterpret_cast<TT*>(uc + umax * 3);            // rec_cap records of 6 TT
     TT* gs = rect + static_cast<size_t>(rec_cap) * 6;           // TILE_GPS records of GSP
-    uint32_t* pm = reinterpret_cast<uint32_t*>(gs + TILE_GPS * GSP);  // UMAX presence masks
-    uint16_t* loc = reinterpret_cast<uint16_t*>(pm + UMAX);     // TILE_GPS * UMAX record indices
+    uint32_t* pm = reinterpret_cast<uint32_t*>(gs + TILE_GPS * GSP);  // umax presence masks
+    uint16_t* loc = reinterpret_cast<uint16_t*>(pm + umax);     // ngp rows of U record indices
     auto RF = [&](int f, int r) -> double& { return recf[static_cast<size_t>(f) * rec_cap + r]; };
     auto R = [&](int f, int r) -> TT& { return rect[static_cast<size_t>(r) * 6 + f]; };   // record-major
     auto Gp = [&](int f, int wv) -> TT& { return gs[wv * GSP + f]; };
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
         uc[3 * tid] = xu[3 * v]; uc[3 * tid + 1] = xu[3 * v + 1]; uc[3 * tid + 2] = xu[3 * v + 2];
         pm[tid] = 0u;
     }
-    for (int k = tid; k < TILE_GPS * UMAX; k += TILE_THREADS) loc[k] = NO_REC;
+    for (int k = tid; k < ngp * U; k += TILE_THREADS) loc[k] = NO_REC;
     __syncthreads();
 
     const double* A = mat.A;
@@ -148,11 +148,30 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
                 u0 = uc[3 * ul]; u1 = uc[3 * ul + 1]; u2 = uc[3 * ul + 2];
             }
             // ---- kinematics (Kinematics.h:75-113) ------------------------------------
-            const double a1x = 1.0 + wave_sum(u0 * px), a1y = wave_sum(u1 * px), a1z = wave_sum(u2 * px);
-            const double a2x = wave_sum(u0 * py), a2y = 1.0 + wave_sum(u1 * py), a2z = wave_sum(u2 * py);
-            const double rxx0 = wave_sum(u0 * hxx), rxx1 = wave_sum(u1 * hxx), rxx2 = wave_sum(u2 * hxx);
-            const double ryy0 = wave_sum(u0 * hyy), ryy1 = wave_sum(u1 * hyy), ryy2 = wave_sum(u2 * hyy);
-            const double rxy0 = wave_sum(u0 * hxy), rxy1 = wave_sum(u1 * hxy), rxy2 = wave_sum(u2 * hxy);
+            // lane k holds neighbour k; neighbours 64.. (large supports) are folded
+            // into the lanes' partial sums before the wave reductions
+            double p_a1x = u0 * px, p_a1y = u1 * px, p_a1z = u2 * px;
+            double p_a2x = u0 * py, p_a2y = u1 * py, p_a2z = u2 * py;
+            double p_xx0 = u0 * hxx, p_xx1 = u1 * hxx, p_xx2 = u2 * hxx;
+            double p_yy0 = u0 * hyy, p_yy1 = u1 * hyy, p_yy2 = u2 * hyy;
+            double p_xy0 = u0 * hxy, p_xy1 = u1 * hxy, p_xy2 = u2 * hxy;
+            if constexpr (WIDE) for (int k = lane + 64; k < n; k += 64) {
+                const int64_t p = b + k;
+                const double qx = sf[npairs + p], qy = sf[2 * npairs + p];
+                const double qxx = sf[3 * npairs + p], qyy = sf[4 * npairs + p], qxy = sf[5 * npairs + p];
+                const int ql = uloc[p];
+                const double v0 = uc[3 * ql], v1 = uc[3 * ql + 1], v2 = uc[3 * ql + 2];
+                p_a1x += v0 * qx; p_a1y += v1 * qx; p_a1z += v2 * qx;
+                p_a2x += v0 * qy; p_a2y += v1 * qy; p_a2z += v2 * qy;
+                p_xx0 += v0 * qxx; p_xx1 += v1 * qxx; p_xx2 += v2 * qxx;
+                p_yy0 += v0 * qyy; p_yy1 += v1 * qyy; p_yy2 += v2 * qyy;
+                p_xy0 += v0 * qxy; p_xy1 += v1 * qxy; p_xy2 += v2 * qxy;
+            }
+            const double a1x = 1.0 + wave_sum(p_a1x), a1y = wave_sum(p_a1y), a1z = wave_sum(p_a1z);
+            const double a2x = wave_sum(p_a2x), a2y = 1.0 + wave_sum(p_a2y), a2z = wave_sum(p_a2z);
+            const double rxx0 = wave_sum(p_xx0), rxx1 = wave_sum(p_xx1), rxx2 = wave_sum(p_xx2);
+            const double ryy0 = wave_sum(p_yy0), ryy1 = wave_sum(p_yy1), ryy2 = wave_sum(p_yy2);
+            const double rxy0 = wave_sum(p_xy0), rxy1 = wave_sum(p_xy1), rxy2 = wave_sum(p_xy2);
             const double C00 = a1x * a1x + a1y * a1y + a1z * a1z;
             const double C11 = a2x * a2x + a2y * a2y + a2z * a2z;
             const double C01 = a1x * a2x + a1y * a2y + a1z * a2z;
@@ -185,14 +204,14 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
             const double ft0 = a1x * vcx + a1y * vcy + a1z * vcz, ft1 = a2x * vcx + a2y * vcy + a2z * vcz;
             const double pX = Ci00 * ft0 + Ci01 * ft1, pY = Ci01 * ft0 + Ci11 * ft1;
             const double bdv = bx * vcx + by * vcy + bz * vcz;
-            if (act) {
-                // per-neighbour quantities (Assembly.h:233-268)
+            // per-neighbour quantities (Assembly.h:233-268)
+            auto record = [&](int k, double px, double py, double hxx, double hyy, double hxy, int ul) {
                 const double gI0 = V0x * px + V1x * py, gI1 = V0y * px + V1y * py, gI2 = V0z * px + V1z * py;
                 const double SXX = rxx0 * gI0 + rxx1 * gI1 + rxx2 * gI2;
                 const double SYY = ryy0 * gI0 + ryy1 * gI1 + ryy2 * gI2;
                 const double SXY = rxy0 * gI0 + rxy1 * gI1 + rxy2 * gI2;
                 const double be0 = -hxx + SXX, be1 = -hyy + SYY, be2 = 2.0 * (-hxy + SXY);
-                const int r = rb + lane;
+                const int r = rb + k;
                 // f_local = (Bm^T N + Bb^T M) W (:249-250)
                 const double ca = px * N[0] + py * N[2], cb = py * N[1] + px * N[2];
                 const double cm = be0 * M[0] + be1 * M[1] + be2 * M[2];
@@ -203,8 +222,14 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
                 R(R_BE0, r) = static_cast<TT>(be0); R(R_BE1, r) = static_cast<TT>(be1);
                 R(R_BE2, r) = static_cast<TT>(be2);
                 R(R_TU, r) = static_cast<TT>((px * pX + py * pY) - (hxx * M[0] + 2.0 * hxy * M[2] + hyy * M[1]));
-                loc[wv * UMAX + ul] = static_cast<uint16_t>(r);   // the record, tile-relative
+                loc[wv * U + ul] = static_cast<uint16_t>(r);   // the record, tile-relative
                 atomicOr(&pm[ul], 1u << wv);
+            };
+            if (act) record(lane, px, py, hxx, hyy, hxy, ul);
+            if constexpr (WIDE) for (int k = lane + 64; k < n; k += 64) {
+                const int64_t p = b + k;
+                record(k, sf[npairs + p], sf[2 * npairs + p], sf[3 * npairs + p], sf[4 * npairs + p],
+                       sf[5 * npairs + p], uloc[p]);
             }
             if (lane == 0) {
                 const double gv[GS] = {W, a1x, a1y, a1z, a2x, a2y, a2z, bx, by, bz, bdv, N[0], N[1], N[2],
@@ -234,8 +259,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
             // Two coupling GPs per iteration (ascending GP order is kept: the first
             // GP's contribution is added before the second's).
             auto pair_block = [&](int wv, TT (&kb)[9]) {
-                const int ri = loc[wv * UMAX + ua];
-                const int rj = loc[wv * UMAX + ubb];
+                const int ri = loc[wv * U + ua];
+                const int rj = loc[wv * U + ubb];
                 const TT W = Gp(G_W, wv);
                 const TT a1[3] = {Gp(G_A1, wv), Gp(G_A1 + 1, wv), Gp(G_A1 + 2, wv)};
                 const TT a2[3] = {Gp(G_A2, wv), Gp(G_A2 + 1, wv), Gp(G_A2 + 2, wv)};
@@ -315,7 +340,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(
     if (tid < U) {
         double f0 = 0.0, f1 = 0.0, f2 = 0.0;
         for (int wv = 0; wv < ngp; ++wv) {
-            const int r = loc[wv * UMAX + tid];
+            const int r = loc[wv * U + tid];
             if (r == NO_REC) continue;
             f0 += RF(R_F0, r); f1 += RF(R_F1, r); f2 += RF(R_F2, r);
         }
@@ -530,11 +555,27 @@ __global__ void k_reduce_F(int nn, const int* __restrict__ fc_off, const int* __
     F[3 * i] = f0; F[3 * i + 1] = f1; F[3 * i + 2] = f2;
 }
 
-size_t tile_lds_bytes(int rec_cap, bool fp32) {
+// k_tile instantiation: body couplings, persistent grid, GPs with more than 64
+// neighbours (chunked lanes)
+template <typename TT, bool PERSIST, bool WIDE>
+static auto tile_kernel(bool has_b) {
+    return has_b ? k_tile<true, TT, PERSIST, WIDE> : k_tile<false, TT, PERSIST, WIDE>;
+}
+
+template <typename TT>
+static auto tile_kernel(bool has_b, bool persist, bool wide) {
+    return persist ? (wide ? tile_kernel<TT, true, true>(has_b) : tile_kernel<TT, true, false>(has_b))
+                   : (wide ? tile_kernel<TT, false, true>(has_b) : tile_kernel<TT, false, false>(has_b));
+}
+
+int tile_umax(int max_union) { return std::min(UMAX, (std::max(max_union, 1) + 7) / 8 * 8); }
+
+size_t tile_lds_bytes(int rec_cap, int max_union, bool fp32) {
     const size_t tt = fp32 ? sizeof(float) : sizeof(double);
-    return sizeof(double) * (static_cast<size_t>(rec_cap) * 3 + UMAX * 3)
-         + tt * (static_cast<size_t>(rec_cap) * 6 + TILE_GPS * GSP) + sizeof(uint32_t) * UMAX
-         + sizeof(uint16_t) * TILE_GPS * UMAX;
+    const size_t um = static_cast<size_t>(tile_umax(max_union));
+    return sizeof(double) * (static_cast<size_t>(rec_cap) * 3 + um * 3)
+         + tt * (static_cast<size_t>(rec_cap) * 6 + TILE_GPS * GSP) + sizeof(uint32_t) * um
+         + sizeof(uint16_t) * TILE_GPS * um;
 }
 
 size_t assembly_workspace_bytes(const Context& c, int flags) {
@@ -597,7 +638,8 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
     std::memcpy(m.B, c.B, sizeof(m.B));
     std::memcpy(m.D, c.D, sizeof(m.D));
     reserve_workspace(c, ws, flags);
-    const size_t lds = tile_lds_bytes(P.rec_cap, fp32);
+    const size_t lds = tile_lds_bytes(P.rec_cap, P.max_union, fp32);
+    const int umax = tile_umax(P.max_union);
     int slot = tbeg(!tangent ? "tile_residual" : (fp32 ? "tile_residual_tangent_fp32" : "tile_residual_tangent"));
     const bool has_b = m.B[0] != 0.0 || m.B[1] != 0.0 || m.B[2] != 0.0 || m.B[3] != 0.0 || m.B[4] != 0.0 ||
                        m.B[5] != 0.0 || m.B[6] != 0.0 || m.B[7] != 0.0 || m.B[8] != 0.0;
@@ -607,16 +649,12 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
         hipLaunchKernelGGL(kern, dim3(grid), dim3(TILE_THREADS), lds, st, P.d_tile_gp.p, P.d_tile_uoff.p,
                            P.d_tile_unodes.p, P.d_tile_toff.p, P.d_tile_tpairs.p, P.d_slot_dst.p, fd.off.p,
                            P.d_pair_uloc.p, fd.sf.p, static_cast<int64_t>(fd.npairs), fd.d_w.p, xdev, m, P.rec_cap,
-                           tangent ? 1 : 0, scr, ws.fscr.p, P.ntiles);
+                           umax, tangent ? 1 : 0, scr, ws.fscr.p, P.ntiles);
     };
+    const bool wide = P.max_nbr > 64;
     float* scr32 = reinterpret_cast<float*>(ws.scr.p);
-    if (persist) {
-        if (fp32) tile(has_b ? k_tile<true, float, true> : k_tile<false, float, true>, scr32);
-        else tile(has_b ? k_tile<true, double, true> : k_tile<false, double, true>, ws.scr.p);
-    } else {
-        if (fp32) tile(has_b ? k_tile<true, float, false> : k_tile<false, float, false>, scr32);
-        else tile(has_b ? k_tile<true, double, false> : k_tile<false, double, false>, ws.scr.p);
-    }
+    if (fp32) tile(tile_kernel<float>(has_b, persist, wide), scr32);
+    else tile(tile_kernel<double>(has_b, persist, wide), ws.scr.p);
     CVM_HIP(hipGetLastError());
     tend(slot);
     if (folds) {

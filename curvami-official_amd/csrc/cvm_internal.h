@@ -190,6 +190,7 @@ struct Plan {
     // tiles
     int ntiles = 0;
     int tile_max_gps = 16, union_max = 64;
+    int max_nbr = 0;            // largest face-GP neighbour count
     DBuf<int> d_tile_gp;        // ntiles + 1
     DBuf<int> d_tile_uoff;      // ntiles + 1
     DBuf<int> d_tile_unodes;    // union node ids
@@ -279,7 +280,7 @@ struct Context {
     size_t lds_limit = 65536;
     Timing timing;
     int tile_gps = 16;          // GPs per assembly tile (at most kTileGps)
-    int union_max = 64;         // union nodes per tile (at most kTileUnion)
+    int union_max = 0;          // union nodes per tile (at most kTileUnion); 0: chosen from the neighbour counts
     int tile_grid = 0;          // tile kernel grid: 0 = one workgroup per tile, else persistent
     // captured assembly (cvm_graph_capture)
     hipGraph_t graph = nullptr;
@@ -297,9 +298,10 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
 size_t assembly_workspace_bytes(const Context& c, int flags);
 void reserve_workspace(const Context& c, Workspace& ws, int flags);
 void upload_fold_descriptors(Context& c);
-size_t tile_lds_bytes(int rec_cap, bool fp32);
-constexpr int kTileGps = 16;   // GPs per assembly tile
-constexpr int kTileUnion = 64; // union nodes per assembly tile
+size_t tile_lds_bytes(int rec_cap, int max_union, bool fp32);
+int tile_umax(int max_union);
+constexpr int kTileGps = 16;    // GPs per assembly tile
+constexpr int kTileUnion = 255; // union nodes per assembly tile (uint8 union indices)
 
 // timing helpers
 int timing_begin(Context& c, const char* name);

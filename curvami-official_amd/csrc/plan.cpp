@@ -58,8 +58,12 @@ void build_plan(Context& c) {
     const std::vector<int>& gsh = fd.h_gsheet;
     int nmax = 0;
     for (int g = 0; g < ng; ++g) nmax = std::max(nmax, off[g + 1] - off[g]);
-    if (nmax > kTileUnion) throw Error(CVM_E_INVALID, "a Gauss point has more than 64 neighbours");
-    P.union_max = std::max(std::min(c.union_max, kTileUnion), nmax);
+    if (nmax > kTileUnion) throw Error(CVM_E_INVALID, "a Gauss point has more than 255 neighbours");
+    P.max_nbr = nmax;
+    // default union cap: 64 up to 47 neighbours (4 x 4 Gauss cells at rho ~ 3.75 h),
+    // proportionally more for larger supports
+    const int auto_union = std::max(64, (nmax * 4 + 2) / 3);
+    P.union_max = std::max(std::min(c.union_max > 0 ? c.union_max : auto_union, kTileUnion), nmax);
 
     // ---- tiles and touched pairs (tile size shrinks until the LDS fits) -----
     std::vector<int> tile_gp, tile_uoff, unodes;
@@ -139,7 +143,7 @@ void build_plan(Context& c) {
         P.max_union = max_union;
         P.rec_cap = std::max(rec_cap, 1);
         P.tile_max_gps = tg;
-        if (tile_lds_bytes(P.rec_cap, false) <= c.lds_limit) break;
+        if (tile_lds_bytes(P.rec_cap, P.max_union, false) <= c.lds_limit) break;
         if (tg == 1) throw Error(CVM_E_INVALID, "assembly tile of one Gauss point exceeds the LDS limit");
     }
     // face blocks: for each column node hi, the nodes lo <= hi that share a face GP

@@ -299,7 +299,7 @@ class Plate:
     def synchronize(self):
         self._check(self._L.cvm_synchronize(self._h))
 
-    def set_tile_shape(self, tile_gps=16, union_max=64, grid=0):
+    def set_tile_shape(self, tile_gps=16, union_max=0, grid=0):
         self._check(self._L.cvm_set_tile_shape(self._h, int(tile_gps), int(union_max), int(grid)))
 
     def tile_shape(self):

@@ -193,7 +193,8 @@ CVM_API int cvm_external_force(cvm_context* ctx, int flags, double* F_ext);
 
 /* Tile shape of the assembly plan (no reference counterpart; chosen by the
  * autotuner): at most tile_gps (1..16) consecutive face GPs per tile, at most
- * union_max (8..64) distinct nodes per tile, and the tile kernel grid (0: one
+ * union_max (8..255; 0 = automatic, 64 for up to 47 neighbours per GP and 4/3 of
+ * the largest neighbour count above) distinct nodes per tile, and the tile kernel grid (0: one
  * workgroup per tile; > 0: a persistent grid of that many workgroups striding
  * over the tiles). Changing tile_gps or union_max invalidates the plan
  * (call cvm_init_assembly again); the grid applies to the next launch. */

@@ -17,6 +17,10 @@ def sheet(name: str):
         return W.config1(rho_factor=RHO_FACTOR)
     if name == "straight21_rho2.5":
         return W.config1(rho_factor=2.5)
+    if name == "straight21_rho6":
+        return W.config1(rho_factor=6.0)
+    if name == "straight21_rho8.5":
+        return W.config1(rho_factor=8.5)
     if name == "straight11":
         return W.config1(rho_factor=RHO_FACTOR, n_side=11)
     if name == "straight21_bspline":

@@ -49,11 +49,11 @@ def test_persistent_grid_bit_identical(arc):
     xv = W.displacement(sh.nn, 3, 1e-3)
     F0, K0 = p.assemble(xv)
     for grid in (1, 7, 256):
-        p.set_tile_shape(16, 64, grid)
+        p.set_tile_shape(16, 0, grid)
         F1, K1 = p.assemble(xv)
         assert np.array_equal(F1.view(np.int64), F0.view(np.int64))
         assert np.array_equal(K1.view(np.int64), K0.view(np.int64))
-    p.set_tile_shape(16, 64, 0)
+    p.set_tile_shape(16, 0, 0)
 
 
 def test_tile_shapes_agree_to_roundoff(arc):
@@ -68,7 +68,7 @@ def test_tile_shapes_agree_to_roundoff(arc):
         F1, K1 = p.assemble(xv)
         assert np.abs(F1 - F0).max() <= 1e-13 * np.abs(F0).max()
         assert np.abs(K1 - K0).max() <= 1e-13 * np.abs(K0).max()
-    p.set_tile_shape(16, 64, 0)
+    p.set_tile_shape(16, 0, 0)
     p.init_assembly()
 
 
@@ -80,5 +80,5 @@ def test_autotune_picks_a_candidate(arc):
     assert len(res["table"]) == 4 and res["ms"] == min(r["ms"] for r in res["table"])
     tg, um, gr = res["best"]
     assert p.tile_shape()["grid"] == gr
-    p.set_tile_shape(16, 64, 0)
+    p.set_tile_shape(16, 0, 0)
     p.init_assembly()

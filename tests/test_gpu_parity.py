@@ -57,7 +57,20 @@ def arc():
     return _build("arc41")
 
 
-@pytest.mark.parametrize("case", ["c1", "arc"])
+@pytest.fixture(scope="module")
+def wide():
+    """Support 6h (the reference's default support radius, 3rd-NN distance x
+    (order + 3)): ~113 neighbours per GP, two 64-lane chunks per GP."""
+    return _build("straight21_rho6")
+
+
+@pytest.fixture(scope="module")
+def wider():
+    """Support 8.5h: up to ~227 neighbours per GP, four 64-lane chunks."""
+    return _build("straight21_rho8.5")
+
+
+@pytest.mark.parametrize("case", ["c1", "arc", "wide", "wider"])
 def test_neighbours_bit_exact(case, request):
     sh, orc, gpu = request.getfixturevalue(case)
     for d in range(len(gpu.domain_sizes)):
@@ -68,7 +81,7 @@ def test_neighbours_bit_exact(case, request):
         assert np.array_equal(o_s.view(np.int64), g_s.view(np.int64)), f"domain {d}: s not bit-identical"
 
 
-@pytest.mark.parametrize("case", ["c1", "arc"])
+@pytest.mark.parametrize("case", ["c1", "arc", "wide"])
 def test_shape_functions(case, request):
     sh, orc, gpu = request.getfixturevalue(case)
     for d in range(len(gpu.domain_sizes)):
@@ -94,7 +107,7 @@ def test_shape_functions(case, request):
             assert bad.size == 0, f"domain {d} {nm}: {bad.size} GPs above tolerance, worst {e.max():.3e}"
 
 
-@pytest.mark.parametrize("case", ["c1", "arc"])
+@pytest.mark.parametrize("case", ["c1", "arc", "wide", "wider"])
 def test_patterns_bit_exact(case, request):
     sh, orc, gpu = request.getfixturevalue(case)
     fo, fi = orc.face_pattern()
@@ -110,7 +123,7 @@ def test_patterns_bit_exact(case, request):
         assert np.array_equal(orc.gp_map(gp, n), gpu.csr_mapping(gp))
 
 
-@pytest.mark.parametrize("case", ["c1", "arc"])
+@pytest.mark.parametrize("case", ["c1", "arc", "wide", "wider"])
 @pytest.mark.parametrize("elastic", [True, False])
 def test_residual_and_tangent(case, elastic, request):
     sh, orc, gpu = request.getfixturevalue(case)
@@ -179,7 +192,7 @@ def test_exact_hinge_tangent(case, request):
     assert np.abs(Fg - Fr).max() <= 1e-10 * np.abs(Fr).max()
 
 
-@pytest.mark.parametrize("case", ["c1", "arc"])
+@pytest.mark.parametrize("case", ["c1", "arc", "wide"])
 def test_fp32_tangent(case, request):
     """Mixed precision: the face tangent in FP32 (inexact-Newton Jacobian), the
     residual still FP64 (equal to the FP64 path up to FMA-contraction round-off
