@@ -73,6 +73,16 @@ SIGNATURES = {
     "cvm_kernel_count": (C.c_int, [C.c_void_p]),
     "cvm_kernel_stats": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int, _d, _i64]),
     "cvm_reset_profiling": (C.c_int, [C.c_void_p]),
+    "cvm_mesh_last_error": (C.c_char_p, []),
+    "cvm_mesh_read": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "cvm_mesh_destroy": (None, [C.c_void_p]),
+    "cvm_mesh_info": (C.c_int, [C.c_void_p, _i64]),
+    "cvm_mesh_nodes": (C.c_int, [C.c_void_p, _d, _d]),
+    "cvm_mesh_elements": (C.c_int, [C.c_void_p, _i, _i, _i64, _i]),
+    "cvm_mesh_quadrature_sizes": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _i64]),
+    "cvm_mesh_quadrature": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _i, _d, _i, _d, _d, _i, _d, _i]),
+    "cvm_mesh_extract_curve": (C.c_int, [C.c_void_p, _i, C.c_int, C.c_int64, _d, _d, _i64, _i, _i64]),
+    "cvm_support_radius": (C.c_int, [_d, _d, C.c_int64, C.c_int, _d]),
 }
 
 _lib = None
@@ -373,11 +383,16 @@ def plate_from_sheet(sh, device: int = 0, compute: bool = True, order=None, kern
         p.add_crease(cr.vx, cr.vy, cr.ext_s, cr.ext_e, cr.closed)
     p.face_dom = p.add_domain(sh.face, FACE)
     p.fold_doms = []
-    for f in sh.folds:
+    for k, f in enumerate(sh.folds):
         l = p.add_domain(f.left, FOLD_LEFT)
         r = p.add_domain(f.right, FOLD_RIGHT)
         p.add_fold(l, r, f.Kt, f.Ang0)
         p.fold_doms.append((l, r))
+        if getattr(f, "rigid", False):
+            p.set_fold_rigid(k, True)
+        for d, fv in ((l, getattr(f, "force_left", None)), (r, getattr(f, "force_right", None))):
+            if fv is not None and any(v != 0.0 for v in fv):
+                p.set_bc(d, 0, force=fv)
     p.bnd_doms = []
     for bd in getattr(sh, "boundaries", []):
         d = p.add_domain(bd.gps, POINT if bd.kind == "point" else EDGE)
@@ -389,6 +404,103 @@ def plate_from_sheet(sh, device: int = 0, compute: bool = True, order=None, kern
             p.shape_functions(d, sh.order if order is None else order, sh.kernel if kernel is None else kernel)
         p.init_assembly()
     return p
+
+
+def _mesh_check(L, st):
+    if st != OK:
+        raise CvmError(st, L.cvm_mesh_last_error().decode(errors="replace"))
+
+
+def _ip(a):
+    return a.ctypes.data_as(_i)
+
+
+class MeshQuadrature:
+    """Gauss points of a mesh (Quadrature::generate, core/include/curvami/Quadrature.h:59-233):
+    active = raw indices of the face-element nodes in ascending order (node i of
+    the sheet is raw node active[i]); face/edge/point arrays hold x, y, w per
+    GP, edge_tn holds tx, ty, nx, ny; *_tag the element's entity tag."""
+
+    def __init__(self, active, face, face_tag, edge, edge_tn, edge_tag, point, point_tag):
+        self.active, self.face, self.face_tag = active, face, face_tag
+        self.edge, self.edge_tn, self.edge_tag = edge, edge_tn, edge_tag
+        self.point, self.point_tag = point, point_tag
+
+
+class Mesh:
+    """A Gmsh 4.1 ASCII mesh read by the library (GmshReader::readMesh,
+    core/include/curvami/GmshReader.h:32-146). Host-only: no GPU is needed."""
+
+    def __init__(self, path: str):
+        self._L = load_library()
+        h = C.c_void_p()
+        _mesh_check(self._L, self._L.cvm_mesh_read(os.fsencode(path), C.byref(h)))
+        self._h = h
+        self.path = path
+        info = np.zeros(3, np.int64)
+        _mesh_check(self._L, self._L.cvm_mesh_info(h, info.ctypes.data_as(_i64)))
+        nn, nel, nent = (int(v) for v in info)
+        self.X = np.empty(nn); self.Y = np.empty(nn)
+        _mesh_check(self._L, self._L.cvm_mesh_nodes(h, _dp(self.X), _dp(self.Y)))
+        self.type = np.empty(nel, np.int32); self.entity = np.empty(nel, np.int32)
+        self.node_off = np.empty(nel + 1, np.int64); self.node_idx = np.empty(max(nent, 1), np.int32)
+        _mesh_check(self._L, self._L.cvm_mesh_elements(h, _ip(self.type), _ip(self.entity),
+                                                       self.node_off.ctypes.data_as(_i64), _ip(self.node_idx)))
+        self.node_idx = self.node_idx[:nent]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.cvm_mesh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def quadrature(self, line_degree: int = 1, face_degree: int = 3) -> MeshQuadrature:
+        sz = np.zeros(4, np.int64)
+        _mesh_check(self._L, self._L.cvm_mesh_quadrature_sizes(self._h, line_degree, face_degree,
+                                                               sz.ctypes.data_as(_i64)))
+        na, nf, ne, npt = (int(v) for v in sz)
+        q = MeshQuadrature(np.empty(max(na, 1), np.int32), np.empty((max(nf, 1), 3)), np.empty(max(nf, 1), np.int32),
+                           np.empty((max(ne, 1), 3)), np.empty((max(ne, 1), 4)), np.empty(max(ne, 1), np.int32),
+                           np.empty((max(npt, 1), 3)), np.empty(max(npt, 1), np.int32))
+        _mesh_check(self._L, self._L.cvm_mesh_quadrature(
+            self._h, line_degree, face_degree, _ip(q.active), _dp(q.face), _ip(q.face_tag), _dp(q.edge),
+            _dp(q.edge_tn), _ip(q.edge_tag), _dp(q.point), _ip(q.point_tag)))
+        q.active = q.active[:na]
+        q.face, q.face_tag = q.face[:nf], q.face_tag[:nf]
+        q.edge, q.edge_tn, q.edge_tag = q.edge[:ne], q.edge_tn[:ne], q.edge_tag[:ne]
+        q.point, q.point_tag = q.point[:npt], q.point_tag[:npt]
+        return q
+
+    def extract_curve(self, tags):
+        """Crease polyline of the line elements with these entity tags
+        (MeshUtils::extractCurve, core/include/curvami/MeshUtils.h:57-134):
+        (vx, vy, closed, extend_start, extend_end, dropped)."""
+        t = np.ascontiguousarray(np.atleast_1d(tags), dtype=np.int32)
+        nv = C.c_int64(); fl = C.c_int(); dr = C.c_int64()
+        _mesh_check(self._L, self._L.cvm_mesh_extract_curve(self._h, _ip(t), t.size, 0, None, None,
+                                                            C.byref(nv), C.byref(fl), C.byref(dr)))
+        vx = np.empty(max(nv.value, 1)); vy = np.empty(max(nv.value, 1))
+        _mesh_check(self._L, self._L.cvm_mesh_extract_curve(self._h, _ip(t), t.size, nv.value, _dp(vx), _dp(vy),
+                                                            C.byref(nv), C.byref(fl), C.byref(dr)))
+        n = nv.value
+        return vx[:n], vy[:n], bool(fl.value & 1), bool(fl.value & 2), bool(fl.value & 4), int(dr.value)
+
+
+def support_radius(X, Y, order: int) -> float:
+    """(order + 3) x median 3rd-nearest-neighbour spacing
+    (MeshUtils::computeSupportRadius, core/include/curvami/MeshUtils.h:144-209)."""
+    L = load_library()
+    X = _f64(X); Y = _f64(Y)
+    if X.shape != Y.shape:
+        raise ValueError("X and Y differ in size")
+    r = C.c_double()
+    _mesh_check(L, L.cvm_support_radius(_dp(X), _dp(Y), X.size, int(order), C.byref(r)))
+    return r.value
 
 
 class DeviceArray:

@@ -71,11 +71,17 @@ class GPSet:
 
 @dataclasses.dataclass
 class Fold:
+    """plate/PlateData.h:73-94: hinge of stiffness Kt and rest angle Ang0 (fold
+    sign applied) between the left / right crease GPs; rigid folds are held by
+    angle rows; force_left / force_right are dead loads on those GPs."""
     left: GPSet
     right: GPSet
     Kt: float
     Ang0: float = 0.0
     crease: int = 0
+    rigid: bool = False
+    force_left: tuple = (0.0, 0.0, 0.0)
+    force_right: tuple = (0.0, 0.0, 0.0)
 
 
 @dataclasses.dataclass

@@ -255,6 +255,46 @@ CVM_API int cvm_kernel_stats(cvm_context* ctx, int i, char* name, int name_cap, 
                              int64_t* launches);
 CVM_API int cvm_reset_profiling(cvm_context* ctx);
 
+/* ---- mesh input (host only; no device work) ------------------------------ */
+
+/* A Gmsh 4.1 ASCII mesh: nodes in file order (z dropped) and the point (15),
+ * line (1), triangle (2) and quadrangle (3) elements with their entity tags. */
+typedef struct cvm_mesh cvm_mesh;
+/* Message of the last failed mesh-API call on this thread. */
+CVM_API const char* cvm_mesh_last_error(void);
+/* Replaces GmshReader::readMesh (core/include/curvami/GmshReader.h:32-146):
+ * the same node indexing (file order) and element node indices; unsupported
+ * element types, undefined node tags and empty meshes are errors as there.
+ * Binary files and non-4.x formats are refused; parametric node blocks are read. */
+CVM_API int cvm_mesh_read(const char* path, cvm_mesh** out);
+CVM_API void cvm_mesh_destroy(cvm_mesh* m);
+/* out3: nodes, elements, element node entries */
+CVM_API int cvm_mesh_info(const cvm_mesh* m, int64_t* out3);
+CVM_API int cvm_mesh_nodes(const cvm_mesh* m, double* X, double* Y);
+/* type[nel], entity_tag[nel], node_off[nel + 1], node_idx[node_off[nel]]; NULL skips */
+CVM_API int cvm_mesh_elements(const cvm_mesh* m, int* type, int* entity_tag, int64_t* node_off, int* node_idx);
+/* Replaces Quadrature::generate (core/include/curvami/Quadrature.h:59-233) in
+ * element order: active nodes (raw indices of face-element nodes, ascending:
+ * the renumbering to 0..n_active-1), face GPs (x, y, w), edge GPs (x, y, w and
+ * tx, ty, nx, ny), point GPs (x, y, w = 1), each with its entity tag.
+ * out4: n_active, n_face, n_edge, n_point. Any output pointer may be NULL. */
+CVM_API int cvm_mesh_quadrature_sizes(const cvm_mesh* m, int line_degree, int face_degree, int64_t* out4);
+CVM_API int cvm_mesh_quadrature(const cvm_mesh* m, int line_degree, int face_degree, int* active,
+                                double* face_xyw, int* face_tag, double* edge_xyw, double* edge_tn,
+                                int* edge_tag, double* point_xyw, int* point_tag);
+/* Replaces MeshUtils::extractCurve (core/include/curvami/MeshUtils.h:57-134):
+ * the chain of the line elements with the given entity tags, from the lowest
+ * end node (or the lowest node of a loop), a closed chain repeating its first
+ * vertex; flags bit 0 closed, bit 1 start on the convex hull of all nodes
+ * (extend start), bit 2 end on it; dropped = nodes not on the chain (branching
+ * or disconnected tag sets). With vx = vy = NULL only *nvert is returned. */
+CVM_API int cvm_mesh_extract_curve(const cvm_mesh* m, const int* tags, int ntags, int64_t cap, double* vx,
+                                   double* vy, int64_t* nvert, int* flags, int64_t* dropped);
+/* Replaces MeshUtils::computeSupportRadius (core/include/curvami/MeshUtils.h:144-209):
+ * (order + 3) x the median (element n/2 in ascending order) over the nodes of
+ * the distance to the 3rd-nearest other node; 0.1 for fewer than 4 nodes. */
+CVM_API int cvm_support_radius(const double* X, const double* Y, int64_t n, int order, double* rho);
+
 #ifdef __cplusplus
 }
 #endif

@@ -249,10 +249,15 @@ def plate_from_sheet(sh, nthreads: int = 1, compute: bool = True):
         p.add_crease(cr.vx, cr.vy, cr.ext_s, cr.ext_e, cr.closed)
     face = p.add_domain(sh.face, is_face=True)
     fold_doms = []
-    for f in sh.folds:
+    for k, f in enumerate(sh.folds):
         l = p.add_domain(f.left); r = p.add_domain(f.right)
         p.add_fold(l, r, f.Kt, f.Ang0)
         fold_doms.append((l, r))
+        if getattr(f, "rigid", False):
+            p.set_fold_rigid(k, True)
+        for d, fv in ((l, getattr(f, "force_left", None)), (r, getattr(f, "force_right", None))):
+            if fv is not None and any(v != 0.0 for v in fv):
+                p.set_bc(d, 0, force=fv)
     p.fold_doms = fold_doms
     p.face_dom = face
     p.bnd_doms = []

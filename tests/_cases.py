@@ -43,3 +43,31 @@ def normwise(a, r, g, ng):
         np.maximum.at(num, g, np.abs(x - y))
         np.maximum.at(den, g, np.abs(y))
     return num / np.where(den > 0, den, 1.0)
+
+
+EPS = np.finfo(np.float64).eps
+
+
+def moment_cond(gx, gy, X, Y, rho, off, ids, psi, order):
+    """cond_2 of each GP's moment matrix M = sum_I w(s_I) p_I p_I^T over its
+    visible neighbours (psi != 0), with the quartic window and the basis of
+    the given order in xi = (x - X_I) / rho: the conditioning that bounds the
+    round-off two correct evaluations of the shape functions can differ by."""
+    g = pair_gp(off)
+    xi = (gx[g] - X[ids]) / rho
+    eta = (gy[g] - Y[ids]) / rho
+    s = np.sqrt(xi * xi + eta * eta)
+    w = np.where(s < 1, 1 - 6 * s ** 2 + 8 * s ** 3 - 3 * s ** 4, 0.0) * (psi != 0)
+    P = [xi ** (d - k) * eta ** k for d in range(order + 1) for k in range(d + 1)]
+    ng = off.size - 1
+    M = np.zeros((ng, len(P), len(P)))
+    for a in range(len(P)):
+        for b in range(a, len(P)):
+            M[:, a, b] = M[:, b, a] = np.bincount(g, w * P[a] * P[b], ng)
+    return np.linalg.cond(M)
+
+
+def sf_tolerance(cond, floor=1e-12):
+    """Per-GP normwise bars for (psi, gradient, Hessian): max(floor, C eps
+    cond(M)) with C = 4 / 16 / 64."""
+    return tuple(np.maximum(floor, C * EPS * cond) for C in (4, 16, 64))

@@ -9,7 +9,7 @@ import pytest
 import curvami_gpu as G
 import oracle as O
 import workloads as W
-from _cases import normwise, pair_gp, sheet
+from _cases import moment_cond, normwise, pair_gp, sf_tolerance, sheet
 
 pytestmark = pytest.mark.gpu
 
@@ -18,26 +18,6 @@ pytestmark = pytest.mark.gpu
 # moves by ~1e-11 between builds with and without FMA contraction, so the bar is
 # max(1e-12, C * eps * cond(M)) with C = 4 / 16 / 64 for Psi / grad / Hessian.
 SF_TOL = 1e-12
-EPS = np.finfo(np.float64).eps
-
-
-def _moment_cond(sh, off, ids, psi_w):
-    """cond_2 of each GP's moment matrix M = sum_I phi_I p_I p_I^T (order 2),
-    rebuilt from the oracle's window weights phi_I = Psi_I / C(x) is not
-    available, so use the visible-neighbour set with unit weights scaled by the
-    quartic window: a conditioning proxy."""
-    g = pair_gp(off)
-    gx = sh.face.gx[g]; gy = sh.face.gy[g]
-    xi = (gx - sh.X[ids]) / sh.rho; eta = (gy - sh.Y[ids]) / sh.rho
-    s = np.sqrt(xi * xi + eta * eta)
-    w = np.where(s < 1, 1 - 6 * s**2 + 8 * s**3 - 3 * s**4, 0.0) * (psi_w != 0)
-    P = np.stack([np.ones_like(xi), xi, eta, xi * xi, xi * eta, eta * eta])
-    ng = off.size - 1
-    M = np.zeros((ng, 6, 6))
-    for a in range(6):
-        for b in range(6):
-            M[:, a, b] = np.bincount(g, w * P[a] * P[b], ng)
-    return np.linalg.cond(M)
 
 
 def _build(name):
@@ -96,11 +76,10 @@ def test_shape_functions(case, request):
         e1 = normwise(got[1:3], ref[1:3], g, ng)
         e2 = normwise(got[3:6], ref[3:6], g, ng)
         if d == 0:
-            cond = _moment_cond(sh, off, ids, ref[0])
+            cond = moment_cond(sh.face.gx, sh.face.gy, sh.X, sh.Y, sh.rho, off, ids, ref[0], sh.order)
         else:
             cond = np.full(ng, 1e4)
-        for e, C, nm in ((e0, 4, "psi"), (e1, 16, "grad"), (e2, 64, "hess")):
-            tol = np.maximum(SF_TOL, C * EPS * cond)
+        for e, tol, nm in zip((e0, e1, e2), sf_tolerance(cond, SF_TOL), ("psi", "grad", "hess")):
             frac = float(np.mean(e <= SF_TOL))
             print(f"domain {d} {nm}: max {e.max():.3e}, median {np.median(e):.3e}, within 1e-12: {frac:.4f}")
             bad = np.nonzero(e > tol)[0]
