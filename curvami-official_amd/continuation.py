@@ -214,6 +214,28 @@ class ContinuationSolver:
         return rep
 
 
+def initial_condition_with_folds(prob_rigid, solver: "ContinuationSolver") -> np.ndarray:
+    """NonLinearSolver::GetInitialConditionWithFolds (plate/NonLinearSolver.h:101-148):
+    from x = 0, the minimum-norm corrector xb of every constraint row (fold
+    angle rows included: prob_rigid is built with flag_rig = True) plus the KKT
+    solution of [K J^T; J 0] [dx; mu] = [-K xb; 0]."""
+    n = prob_rigid.num_dofs
+    x = np.zeros(n)
+    c, J = prob_rigid.constraints(x)
+    if c.size == 0:
+        return x
+    solver.initialize_topology(J)
+    c_ind, J_ind = solver.filter_rows(c, J)
+    K, _ = prob_rigid.assemble(x)
+    xb = ConstraintProjector(J_ind).corrector(c_ind, n)
+    rhs = np.zeros(n + c_ind.size)
+    rhs[:n] = -(full_from_upper(K) @ xb)
+    sol = solver.kkt.solve(K, J_ind, rhs)
+    if sol is None:
+        raise RuntimeError(f"initial folded-condition KKT solve failed with error code {solver.kkt.last_error}")
+    return xb + sol[:n]
+
+
 @dataclasses.dataclass
 class LoadPathResult:
     x: np.ndarray

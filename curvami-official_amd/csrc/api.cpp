@@ -613,6 +613,62 @@ int cvm_external_force(cvm_context* ctx, int flags, double* F_ext) {
     CVM_CATCH(ctx)
 }
 
+int cvm_energies(cvm_context* ctx, const double* x, int flags, double* out3) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    CVM_HIP(hipSetDevice(c.device));
+    if (!x || !out3) throw Error(CVM_E_INVALID, "null argument");
+    if (!c.have_mat) throw Error(CVM_E_STATE, "laminate not set");
+    const bool dev = (flags & CVM_DEVICE_PTRS) != 0;
+    const double* xd = x;
+    if (!dev) {
+        c.d_x.upload(x, 3 * static_cast<size_t>(c.nn), c.stream);
+        xd = c.d_x.p;
+    }
+    double* od = out3;
+    if (!dev) {
+        c.en_out.alloc(3);
+        od = c.en_out.p;
+    }
+    cvm::launch_energies(c, xd, od);
+    if (!dev) CVM_HIP(hipMemcpyAsync(out3, od, 3 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    CVM_HIP(hipStreamSynchronize(c.stream));   // the activity copies read host vectors
+    CVM_CATCH(ctx)
+}
+
+int cvm_fold_angles(cvm_context* ctx, int fold_id, const double* x, int flags, double* theta) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    CVM_HIP(hipSetDevice(c.device));
+    if (fold_id < 0 || fold_id >= static_cast<int>(c.folds.size())) throw Error(CVM_E_INVALID, "bad fold id");
+    if (!x || !theta) throw Error(CVM_E_INVALID, "null argument");
+    const bool dev = (flags & CVM_DEVICE_PTRS) != 0;
+    const int n = c.doms[c.folds[fold_id].left]->ng;
+    const double* xd = x;
+    double* td = theta;
+    if (!dev) {
+        c.d_x.upload(x, 3 * static_cast<size_t>(c.nn), c.stream);
+        xd = c.d_x.p;
+        c.en_scr.ensure(static_cast<size_t>(std::max(n, 1)));
+        td = c.en_scr.p;
+    }
+    cvm::launch_fold_angles(c, fold_id, xd, td);
+    if (!dev && n > 0) CVM_HIP(hipMemcpyAsync(theta, td, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    CVM_HIP(hipStreamSynchronize(c.stream));
+    CVM_CATCH(ctx)
+}
+
+int cvm_clear_creases(cvm_context* ctx) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    if (c.batched) throw Error(CVM_E_STATE, "creases of a batched context cannot be replaced");
+    c.creases.clear();
+    c.creases_dirty = true;
+    c.node_sign_creases = -1;
+    for (cvm::Domain* d : c.doms) { d->have_sf = false; d->ind_creases = -1; }
+    CVM_CATCH(ctx)
+}
+
 int cvm_set_tile_shape(cvm_context* ctx, int tile_gps, int union_max, int grid) {
     CVM_TRY(ctx)
     Context& c = ctx->c;

@@ -23,6 +23,7 @@
 #include <algorithm>
 
 #include "cvm_internal.h"
+#include "device_common.h"
 #include "fold_angle.h"
 
 namespace cvm {
@@ -34,10 +35,6 @@ constexpr int GS = 17;               // GP record fields (tangent type)
 constexpr int GSP = 20;              // GP record stride: GP-major, 16-byte aligned for wide LDS reads
 constexpr uint16_t NO_REC = 0xffff;  // loc entry of a union node that a GP does not see
 
-struct MatC {
-    double A[9], B[9], D[9];
-};
-
 // neighbour record planes: grad Psi, beta (bending B-row factor,
 // Assembly.h:240-247), Tu (:265-267) in the tangent type TT, and the residual
 // contribution f_local (:249) in FP64
@@ -45,29 +42,6 @@ enum { R_PX = 0, R_PY, R_BE0, R_BE1, R_BE2, R_TU };
 enum { R_F0 = 0, R_F1, R_F2 };
 // GP record planes: weight, F0 columns, normal, b.v_curv, membrane resultant N, C0^-1
 enum { G_W = 0, G_A1 = 1, G_A2 = 4, G_B = 7, G_BDV = 10, G_N = 11, G_CI = 14 };
-
-// Sum over the 64 lanes of a wave, returned to every lane, with DPP lane moves
-// (no LDS traffic): pairs, quads, half rows and rows by quad_perm / mirror
-// moves, then rows 0+1 and 2+3 by row_bcast15 and the total by row_bcast31 into
-// lane 63, read back as a scalar.
-template <int CTRL, int ROWS>
-__device__ __forceinline__ double dpp_move(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, false);
-    return __hiloint2double(hi, lo);
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-    v += dpp_move<0xb1, 0xf>(v);     // quad_perm(1,0,3,2)
-    v += dpp_move<0x4e, 0xf>(v);     // quad_perm(2,3,0,1)
-    v += dpp_move<0x141, 0xf>(v);    // row_half_mirror
-    v += dpp_move<0x140, 0xf>(v);    // row_mirror: every lane holds its row's sum
-    v += dpp_move<0x142, 0xa>(v);    // row_bcast15 into rows 1 and 3
-    v += dpp_move<0x143, 0xc>(v);    // row_bcast31 into rows 2 and 3: lane 63 holds the total
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
-    return __hiloint2double(hi, lo);
-}
 
 // Per tile: (1) wave w prepares GP w of the tile (kinematics, resultants and the
 // per-neighbour records, Assembly.h:210-268); (2) every thread owns touched

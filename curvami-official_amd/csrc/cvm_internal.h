@@ -111,6 +111,7 @@ struct Domain {
     DBuf<int> d_gsheet;
     bool dirty = false;            // host GP arrays not yet uploaded (batched appends)
     DBuf<unsigned char> d_sheets;  // SheetDev per sheet, built by the neighbour search
+    std::vector<SheetDev> h_sheets;  // host copy; crease ranges refreshed before each shape-function launch
     std::vector<double> hx, hy, hw, hnx, hny, htx, hty;   // host copies of the GP data
     DBuf<double> gx, gy, d_w, d_tx, d_ty, d_nx, d_ny;
     BoundaryCond bc;
@@ -274,6 +275,9 @@ struct Context {
     DBuf<int> d_status;         // RKPM status words, 4 per domain
     // assembly buffers
     DBuf<double> d_x;           // 3 nn displacements
+    DBuf<double> en_scr;        // per-GP energies (face membrane, face bending, crease hinge)
+    DBuf<char> en_active;       // crease-GP activity copies for the hinge energy
+    DBuf<double> en_out;        // 3 energies
     DBuf<double> d_F;           // 3 nn
     DBuf<double> d_K;           // nnz (union)
     Workspace ws;               // the context's own assembly workspace (its stream)
@@ -313,6 +317,8 @@ void sync_domain(Context& c, Domain& d);
 void build_constraint_layout(Context& c, bool rigid);
 void launch_constraints(Context& c, const double* xdev, double* cdev, double* jdev);
 void launch_external_force(Context& c, double* Fdev);
+void launch_energies(Context& c, const double* xdev, double* out3_dev);
+void launch_fold_angles(Context& c, int fold_id, const double* xdev, double* theta_dev);
 
 }  // namespace cvm
 

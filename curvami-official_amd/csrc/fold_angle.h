@@ -37,23 +37,41 @@ __device__ inline void normal_derivs(const double* a1, const double* a2, const d
         }
 }
 
-__device__ inline double fold_angle_grad(const double* F, double tx, double ty, double* grad) {
+// Unit normals, unit crease tangent (left side) and the angle's cosine / sine.
+struct FoldFrame {
+    double normL, normR, naT;
+    double bL[3], bR[3], t[3], bLxbR[3];
+    double c, s;
+};
+
+__device__ inline double fold_frame(const double* F, double tx, double ty, FoldFrame& f) {
     const double* a1L = F; const double* a2L = F + 3; const double* a1R = F + 6; const double* a2R = F + 9;
     double nL[3], nR[3];
     cross3(a1L, a2L, nL);
     cross3(a1R, a2R, nR);
-    const double normL = sqrt(nL[0] * nL[0] + nL[1] * nL[1] + nL[2] * nL[2]);
-    const double normR = sqrt(nR[0] * nR[0] + nR[1] * nR[1] + nR[2] * nR[2]);
-    const double bL[3] = {nL[0] / normL, nL[1] / normL, nL[2] / normL};
-    const double bR[3] = {nR[0] / normR, nR[1] / normR, nR[2] / normR};
+    f.normL = sqrt(nL[0] * nL[0] + nL[1] * nL[1] + nL[2] * nL[2]);
+    f.normR = sqrt(nR[0] * nR[0] + nR[1] * nR[1] + nR[2] * nR[2]);
+    for (int k = 0; k < 3; ++k) { f.bL[k] = nL[k] / f.normL; f.bR[k] = nR[k] / f.normR; }
     const double aT[3] = {a1L[0] * tx + a2L[0] * ty, a1L[1] * tx + a2L[1] * ty, a1L[2] * tx + a2L[2] * ty};
-    const double naT = sqrt(aT[0] * aT[0] + aT[1] * aT[1] + aT[2] * aT[2]);
-    const double t[3] = {aT[0] / naT, aT[1] / naT, aT[2] / naT};
-    const double c = bL[0] * bR[0] + bL[1] * bR[1] + bL[2] * bR[2];
-    double bLxbR[3];
-    cross3(bL, bR, bLxbR);
-    const double s = bLxbR[0] * t[0] + bLxbR[1] * t[1] + bLxbR[2] * t[2];
-    const double theta = atan2(-s, c);
+    f.naT = sqrt(aT[0] * aT[0] + aT[1] * aT[1] + aT[2] * aT[2]);
+    for (int k = 0; k < 3; ++k) f.t[k] = aT[k] / f.naT;
+    f.c = f.bL[0] * f.bR[0] + f.bL[1] * f.bR[1] + f.bL[2] * f.bR[2];
+    cross3(f.bL, f.bR, f.bLxbR);
+    f.s = f.bLxbR[0] * f.t[0] + f.bLxbR[1] * f.t[1] + f.bLxbR[2] * f.t[2];
+    return atan2(-f.s, f.c);
+}
+
+__device__ inline double fold_angle(const double* F, double tx, double ty) {
+    FoldFrame f;
+    return fold_frame(F, tx, ty, f);
+}
+
+__device__ inline double fold_angle_grad(const double* F, double tx, double ty, double* grad) {
+    const double* a1L = F; const double* a2L = F + 3; const double* a1R = F + 6; const double* a2R = F + 9;
+    FoldFrame f;
+    const double theta = fold_frame(F, tx, ty, f);
+    const double normL = f.normL, normR = f.normR, naT = f.naT, c = f.c, s = f.s;
+    const double* bL = f.bL; const double* bR = f.bR; const double* t = f.t; const double* bLxbR = f.bLxbR;
     double dbL[3][6], dbR[3][6];
     normal_derivs(a1L, a2L, bL, normL, dbL);
     normal_derivs(a1R, a2R, bR, normR, dbR);

@@ -156,8 +156,26 @@ void launch_node_signs(Context& c) {
     CVM_HIP(hipStreamSynchronize(c.stream));
 }
 
+// The crease range (first crease, count) of every sheet in the domain's
+// SheetDev table must be the current one: creases can be added or replaced
+// after the neighbour search, and the indicator and RKPM kernels index the
+// per-crease arrays (sized for the current crease count) by it.
+static void sync_sheet_creases(Context& c, Domain& d) {
+    bool changed = false;
+    for (size_t si = 0; si < d.h_sheets.size(); ++si) {
+        const int nc = c.batched ? c.sheets.at(si).ncreases : static_cast<int>(c.creases.size());
+        const int c0 = c.batched ? c.sheets.at(si).crease0 : 0;
+        SheetDev& t = d.h_sheets[si];
+        if (t.nc != nc || t.crease0 != c0) { t.nc = nc; t.crease0 = c0; changed = true; }
+    }
+    if (changed)
+        d.d_sheets.upload(reinterpret_cast<const unsigned char*>(d.h_sheets.data()),
+                          d.h_sheets.size() * sizeof(SheetDev), c.stream);
+}
+
 void launch_indicator_gp(Context& c, Domain& d) {
     upload_creases(c);
+    sync_sheet_creases(c, d);
     const int nc = static_cast<int>(c.creases.size());
     if (d.ind_creases == nc) return;              // unchanged GPs and creases
     d.ind_creases = nc;
@@ -280,6 +298,7 @@ void launch_frnn(Context& c, Domain& d, double rho) {
         koff += gr.nkeys;
     }
     d.d_sheets.upload(reinterpret_cast<const unsigned char*>(table.data()), table.size() * sizeof(SheetDev), c.stream);
+    d.h_sheets = table;
     d.off.alloc(d.ng + 1);
     d.h_off.assign(d.ng + 1, 0);
     d.ind_creases = -1;

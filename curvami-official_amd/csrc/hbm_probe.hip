@@ -22,24 +22,29 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "hipMalloc failed\n");
         return 1;
     }
-    if (hipMemset(a, 0, bytes) != hipSuccess) return 1;
+    auto ok = [](hipError_t e, const char* what) {
+        if (e != hipSuccess) std::fprintf(stderr, "%s: %s\n", what, hipGetErrorString(e));
+        return e == hipSuccess;
+    };
+    if (!ok(hipMemset(a, 0, bytes), "hipMemset")) return 1;
     hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
+    if (!ok(hipEventCreate(&e0), "hipEventCreate") || !ok(hipEventCreate(&e1), "hipEventCreate")) return 1;
     float best = 1e30f;
     for (int r = 0; r < reps; ++r) {
-        hipEventRecord(e0);
+        if (!ok(hipEventRecord(e0), "hipEventRecord")) return 1;
         hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, 0, a, b, n);
-        hipEventRecord(e1);
-        hipEventSynchronize(e1);
+        if (!ok(hipGetLastError(), "k_copy launch") || !ok(hipEventRecord(e1), "hipEventRecord") ||
+            !ok(hipEventSynchronize(e1), "hipEventSynchronize"))
+            return 1;
         float ms = 0;
-        hipEventElapsedTime(&ms, e0, e1);
+        if (!ok(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime")) return 1;
         if (ms < best) best = ms;
     }
-    if (hipGetLastError() != hipSuccess) return 1;
     std::printf("hbm_probe: %zu MiB copy, best %.3f ms, %.1f GB/s (read + write)\n", mib, best,
                 2.0 * bytes / (best * 1e-3) / 1e9);
-    hipFree(a);
-    hipFree(b);
+    ok(hipEventDestroy(e0), "hipEventDestroy");
+    ok(hipEventDestroy(e1), "hipEventDestroy");
+    ok(hipFree(a), "hipFree");
+    ok(hipFree(b), "hipFree");
     return 0;
 }

@@ -375,6 +375,14 @@ static void check_args(int order, int kernel) {
 static void enqueue_rkpm(Context& c, Domain& d, int order, int kernel, int* st) {
     if (d.npairs < 0) throw Error(CVM_E_STATE, "neighbours not computed for this domain");
     launch_indicator_gp(c, d);
+    // the kernel reads ind[(lc * 6 + k) * ng + g] and sign[lc * nn + I] for
+    // lc < its sheet's crease count: both buffers must hold that many creases
+    const size_t mc = static_cast<size_t>(c.max_sheet_creases());
+    for (const SheetDev& t : d.h_sheets)
+        if (t.nc < 0 || static_cast<size_t>(t.nc) > mc || t.crease0 + t.nc > static_cast<int>(c.creases.size()))
+            throw Error(CVM_E_STATE, "crease table of the domain is out of date");
+    if (mc > 0 && (d.ind.n < mc * 6 * static_cast<size_t>(d.ng) || c.d_node_sign.n < mc * static_cast<size_t>(c.nn)))
+        throw Error(CVM_E_STATE, "crease indicator buffers are smaller than the crease count");
     d.sf.alloc(std::max<int64_t>(6 * d.npairs, 1));
     hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, c.stream, st);
     if (d.ng > 0) {

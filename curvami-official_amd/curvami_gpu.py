@@ -55,6 +55,9 @@ SIGNATURES = {
     "cvm_get_constraint_pattern": (C.c_int, [C.c_void_p, _i, _i]),
     "cvm_constraints": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "cvm_external_force": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "cvm_energies": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "cvm_fold_angles": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]),
+    "cvm_clear_creases": (C.c_int, [C.c_void_p]),
     "cvm_synchronize": (C.c_int, [C.c_void_p]),
     "cvm_set_tile_shape": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "cvm_get_tile_shape": (C.c_int, [C.c_void_p, _i]),
@@ -167,6 +170,9 @@ class Plate:
     def add_crease(self, vx, vy, ext_s=True, ext_e=True, closed=False):
         vx, vy = _f64(vx), _f64(vy)
         self._check(self._L.cvm_add_crease(self._h, vx.size, _dp(vx), _dp(vy), int(ext_s), int(ext_e), int(closed)))
+
+    def clear_creases(self):
+        self._check(self._L.cvm_clear_creases(self._h))
 
     def add_domain(self, gps, kind=FACE) -> int:
         a = [_f64(x) for x in (gps.gx, gps.gy, gps.w, gps.nx, gps.ny, gps.tx, gps.ty)]
@@ -305,6 +311,22 @@ class Plate:
         F = np.empty(3 * self.nn)
         self._check(self._L.cvm_external_force(self._h, 0, F.ctypes.data))
         return F
+
+    def energies(self, x):
+        """(E_mem, E_ben, E_hin) at x (Meshless::computeStrainEnergies,
+        computeTotalHingeEnergy)."""
+        x = _f64(x)
+        out = np.empty(3)
+        self._check(self._L.cvm_energies(self._h, x.ctypes.data, 0, out.ctypes.data))
+        return tuple(float(v) for v in out)
+
+    def fold_angles(self, fold, x):
+        """Dihedral angle at every crease GP of a fold."""
+        x = _f64(x)
+        l, _ = self.fold_doms[fold]
+        th = np.empty(max(self.domain_sizes[l], 1))
+        self._check(self._L.cvm_fold_angles(self._h, int(fold), x.ctypes.data, 0, th.ctypes.data))
+        return th[:self.domain_sizes[l]]
 
     def synchronize(self):
         self._check(self._L.cvm_synchronize(self._h))

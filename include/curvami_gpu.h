@@ -103,6 +103,14 @@ CVM_API int cvm_add_sheet(cvm_context* ctx, int nn, const double* X, const doubl
 CVM_API int cvm_add_crease(cvm_context* ctx, int nv, const double* vx, const double* vy,
                            int extend_start, int extend_end, int is_closed);
 
+/* Removes every crease (single-sheet contexts). Shape functions must be
+ * recomputed (cvm_shape_functions / _all) and the constraint layout rebuilt
+ * before the next use; neighbour lists and the assembly plan stay valid.
+ * The crease re-cut of progressive creasing (SetupHelper::recomputeShapeFunctions,
+ * plate/SetupHelper.h:112-137, called from Progressive::run,
+ * plate/ProgressiveCreasing.h:592-605). */
+CVM_API int cvm_clear_creases(cvm_context* ctx);
+
 /* Integration domain: Gauss points (GaussPointData::X_ref, weight, normal,
  * tangent; MeshfreeData.h:17-35). Normal/tangent may be NULL (zero). */
 CVM_API int cvm_add_domain(cvm_context* ctx, int kind, int ng, const double* gx, const double* gy,
@@ -244,6 +252,18 @@ CVM_API int cvm_server_poll(cvm_server* s, int64_t request_id, int* done);
 CVM_API int cvm_server_wait(cvm_server* s, int64_t request_id, double* times);
 /* completed, in flight, pending, workspace bytes allocated */
 CVM_API int cvm_server_stats(cvm_server* s, int64_t* out4);
+
+/* ---- energies and fold angles --------------------------------------------- */
+
+/* out3 = E_mem, E_ben over the face GPs and E_hin over every fold's active
+ * crease GPs at state x (Meshless::computeStrainEnergies,
+ * computeTotalHingeEnergy, plate/Meshless.h:198-219; Constraints::computeHingeEnergy,
+ * plate/Constraints.h:310-339). flags: CVM_DEVICE_PTRS (x and out3 on the device). */
+CVM_API int cvm_energies(cvm_context* ctx, const double* x, int flags, double* out3);
+/* Dihedral angle theta at every crease GP of a fold, active or not
+ * (Constraints::foldAngleAndGrad, plate/Constraints.h:71-88, as used by
+ * Progressive::activeFoldAngles, plate/ProgressiveCreasing.h:157-179). */
+CVM_API int cvm_fold_angles(cvm_context* ctx, int fold_id, const double* x, int flags, double* theta);
 
 /* ---- measurement -------------------------------------------------------- */
 

@@ -1291,6 +1291,71 @@ void orc_kinematics(Plate* p, int dom, const double* x, double* out) {
     }
 }
 
+// Meshless::computeStrainEnergies (plate/Meshless.h:205-219) over the face
+// domain and computeTotalHingeEnergy (:198-203, Constraints::computeHingeEnergy,
+// plate/Constraints.h:310-339) over every fold's active crease GPs, summed in
+// GP order. out = E_mem, E_ben, E_hin.
+void orc_energies(Plate* p, const double* x, double* out) {
+    const Domain& d = p->doms[p->face];
+    const ABD& m = p->mat;
+    double Em = 0.0, Eb = 0.0;
+    for (size_t g = 0; g + 1 < d.nl.off.size(); ++g) {
+        const int b = d.nl.off[g], n = d.nl.off[g + 1] - b;
+        const Kin k = kinematics(d.nl.ids.data() + b, n, d.sh.dpx.data() + b, d.sh.dpy.data() + b,
+                                 d.sh.dxx.data() + b, d.sh.dyy.data() + b, d.sh.dxy.data() + b, x);
+        const V3 e0 = v3(k.E0[0][0], k.E0[1][1], 2.0 * k.E0[0][1]);
+        const V3 e1 = v3(k.E1[0][0], k.E1[1][1], 2.0 * k.E1[0][1]);
+        const V3 N = add(mv(m.A, e0), mv(m.B, e1));                       // Constitutive.h:67-68
+        const V3 M = add(mv(mT(m.B), e0), mv(m.D, e1));
+        Em += 0.5 * d.w[g] * dot(e0, N);
+        Eb += 0.5 * d.w[g] * dot(e1, M);
+    }
+    double Eh = 0.0;
+    for (const FoldDef& f : p->folds) {
+        const Domain& L = p->doms[f.left];
+        const Domain& R = p->doms[f.right];
+        for (size_t i = 0; i < L.X.size(); ++i) {
+            if (!f.active.empty() && !f.active[i]) continue;
+            double F[12];
+            for (int side = 0; side < 2; ++side) {
+                const Domain& D = side ? R : L;
+                const int b = D.nl.off[i], n = D.nl.off[i + 1] - b;
+                const Kin k = kinematics(D.nl.ids.data() + b, n, D.sh.dpx.data() + b, D.sh.dpy.data() + b,
+                                         D.sh.dxx.data() + b, D.sh.dyy.data() + b, D.sh.dxy.data() + b, x);
+                F[6 * side + 0] = k.a1.x; F[6 * side + 1] = k.a1.y; F[6 * side + 2] = k.a1.z;
+                F[6 * side + 3] = k.a2.x; F[6 * side + 4] = k.a2.y; F[6 * side + 5] = k.a2.z;
+            }
+            const double th = fold_angle(F, L.tangent[i].x, L.tangent[i].y, nullptr);
+            const double dth = th - (f.theta0.empty() ? f.Ang0 : f.theta0[i]);
+            Eh += L.w[i] * (0.5 * f.Kt * (dth * dth));
+        }
+    }
+    out[0] = Em; out[1] = Eb; out[2] = Eh;
+}
+
+// Dihedral angle at every crease GP of a fold (Constraints::foldAngleAndGrad,
+// plate/Constraints.h:71-88), active or not.
+void orc_fold_angles(Plate* p, int fold, const double* x, double* theta) {
+    const FoldDef& f = p->folds[fold];
+    const Domain& L = p->doms[f.left];
+    const Domain& R = p->doms[f.right];
+    for (size_t i = 0; i < L.X.size(); ++i) {
+        double F[12];
+        for (int side = 0; side < 2; ++side) {
+            const Domain& D = side ? R : L;
+            const int b = D.nl.off[i], n = D.nl.off[i + 1] - b;
+            const Kin k = kinematics(D.nl.ids.data() + b, n, D.sh.dpx.data() + b, D.sh.dpy.data() + b,
+                                     D.sh.dxx.data() + b, D.sh.dyy.data() + b, D.sh.dxy.data() + b, x);
+            F[6 * side + 0] = k.a1.x; F[6 * side + 1] = k.a1.y; F[6 * side + 2] = k.a1.z;
+            F[6 * side + 3] = k.a2.x; F[6 * side + 4] = k.a2.y; F[6 * side + 5] = k.a2.z;
+        }
+        theta[i] = fold_angle(F, L.tangent[i].x, L.tangent[i].y, nullptr);
+    }
+}
+
+// Removes every crease; shape functions must be recomputed.
+void orc_clear_creases(Plate* p) { p->creases.clear(); }
+
 void orc_set_domain_bc(Plate* p, int dom, int kind, int active, const double* val, const double* force,
                        double moment) {
     BC& bc = p->doms[dom].bc;

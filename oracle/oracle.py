@@ -66,6 +66,9 @@ def lib():
             "orc_constraints": (C.c_long, [P, _d, C.c_int, C.POINTER(C.c_long)]),
             "orc_get_constraints": (None, [P, _d, _i, _i, _d]),
             "orc_external_force": (None, [P, _d]),
+            "orc_energies": (None, [P, _d, _d]),
+            "orc_fold_angles": (None, [P, C.c_int, _d, _d]),
+            "orc_clear_creases": (None, [P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -149,6 +152,15 @@ class OraclePlate:
     def add_fold(self, left, right, Kt, Ang0=0.0) -> int:
         return self._L.orc_add_fold(self._p, left, right, Kt, Ang0)
 
+    def set_fold_gp_state(self, fold, theta0=None, active=None):
+        """Per crease-GP rest angle and activity (None clears)."""
+        l, _ = self.fold_doms[fold]
+        n = self.domains[l]
+        th = None if theta0 is None else _f64(theta0)
+        ac = None if active is None else np.ascontiguousarray(active, dtype=np.int8)
+        self._L.orc_set_fold_gp_state(self._p, fold, n, _dp(th),
+                                      None if ac is None else ac.ctypes.data_as(_c))
+
     def set_fold_rigid(self, fold, rigid=True):
         self._L.orc_set_fold_rigid(self._p, fold, int(rigid))
 
@@ -230,6 +242,23 @@ class OraclePlate:
         F = np.empty(3 * self.nn)
         self._L.orc_external_force(self._p, _dp(F))
         return F
+
+    def energies(self, x):
+        """(E_mem, E_ben, E_hin) at x."""
+        x = _f64(x)
+        out = np.empty(3)
+        self._L.orc_energies(self._p, _dp(x), _dp(out))
+        return tuple(out)
+
+    def fold_angles(self, fold, x):
+        x = _f64(x)
+        l, _ = self.fold_doms[fold]
+        th = np.empty(self.domains[l])
+        self._L.orc_fold_angles(self._p, fold, _dp(x), _dp(th))
+        return th
+
+    def clear_creases(self):
+        self._L.orc_clear_creases(self._p)
 
     def kinematics(self, dom, x):
         x = _f64(x)

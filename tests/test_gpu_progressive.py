@@ -1,0 +1,160 @@
+"""Progressive creasing on the device: energies, crease angles, per-GP hinge
+state and the crease re-cut against the oracle, and the reference's smoke
+example run end to end (tests/golden/reference_inputs/smoke.json on CB_n20)
+against the answer its README states and the oracle's per-step run
+(tests/golden/progressive_smoke.npz).
+
+Bars: energies to 1e-10 relative and crease angles to 1e-10 absolute (sums of
+shape-function products, which agree to ~1e-13); residual and tangent with a
+partial crease state as the parity tests (F 1e-10, K 1e-12); a re-cut with the
+same creases bit-identical to the first cut; the smoke run's last step within
+half a unit of the README's 4 printed digits, every step within 5 Newton
+iterations, and every step's energies within 1e-7 relative of the oracle's run
+(both converged to |projected residual| <= 1e-10).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import continuation as CS
+import curvami_gpu as G
+import oracle as O
+import progressive as PR
+import workloads as W
+from _cases import RHO_FACTOR
+from _progressive_oracle import SMOKE_KNOWN, SMOKE_MAX_ITERS, smoke_case
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+INPUTS = os.path.join(HERE, "golden", "reference_inputs")
+FIXTURE = os.path.join(HERE, "golden", "progressive_smoke.npz")
+
+
+def within_printed_digits(value, quoted, digits=4):
+    unit = 10.0 ** (math.floor(math.log10(abs(quoted))) - (digits - 1))
+    return abs(value - quoted) <= 0.5 * unit
+
+
+@pytest.fixture(scope="module")
+def c1():
+    sh = W.config1(rho_factor=RHO_FACTOR)
+    return sh, O.plate_from_sheet(sh, nthreads=8), G.plate_from_sheet(sh)
+
+
+def test_energies_and_angles_match_oracle(c1):
+    sh, orc, gpu = c1
+    for x in (W.displacement(sh.nn, 5, 2e-3), W.rigid_fold_displacement(sh, 0.3, seed=2, sigma=1e-3)):
+        eo = np.array(orc.energies(x))
+        eg = np.array(gpu.energies(x))
+        print(f"E_mem/E_ben/E_hin oracle {eo}, device rel {np.abs(eg - eo) / np.abs(eo)}")
+        assert np.all(np.abs(eg - eo) <= 1e-10 * np.abs(eo))
+        to = orc.fold_angles(0, x)
+        tg = gpu.fold_angles(0, x)
+        assert np.abs(tg - to).max() <= 1e-10
+    e1 = gpu.energies(x)
+    assert e1 == gpu.energies(x)    # fixed-order sums
+
+
+def test_partial_crease_state_matches_oracle(c1):
+    """Hinge and tie rows on every other crease GP with per-GP rest angles."""
+    sh, orc, gpu = c1
+    n = sh.folds[0].left.n
+    act = (np.arange(n) % 2 == 0).astype(np.int8)
+    th0 = np.linspace(-0.2, 0.3, n)
+    orc.set_fold_gp_state(0, th0, act)
+    gpu.set_fold_gp_state(0, th0, act)
+    try:
+        gpu.init_assembly()
+        x = W.rigid_fold_displacement(sh, 0.25, seed=4, sigma=1e-3)
+        uo, ui, Kr, Fr = orc.assemble(x, flag_rig=False, exact_tangent=True)
+        go, gi = gpu.pattern(with_folds=True)
+        assert np.array_equal(uo, go) and np.array_equal(ui, gi)
+        Fg, Kg = gpu.assemble(x, elastic_folds=True, exact_hinge=True)
+        ef = np.abs(Fg - Fr).max() / np.abs(Fr).max()
+        ek = np.abs(Kg - Kr).max() / np.abs(Kr).max()
+        print(f"partial state: F rel {ef:.2e}, K rel {ek:.2e}")
+        assert ef <= 1e-10 and ek <= 1e-12
+        rows, cols = gpu.constraint_layout(rigid=False)
+        c_o, r_o, col_o, v_o = orc.constraints(x, flag_rig=False)
+        assert np.array_equal(rows, r_o) and np.array_equal(cols, col_o) and c_o.size == 3 * int(act.sum())
+        eo = np.array(orc.energies(x)); eg = np.array(gpu.energies(x))
+        assert np.all(np.abs(eg - eo) <= 1e-10 * np.abs(eo))
+    finally:
+        orc.set_fold_gp_state(0, None, None)
+        gpu.set_fold_gp_state(0, None, None)
+        gpu.init_assembly()
+
+
+def test_recut_is_bit_identical_and_uncut_matches_oracle(c1):
+    sh, orc, gpu = c1
+    ref = [gpu.get_shape_functions(d) for d in range(len(gpu.domain_sizes))]
+    gpu.clear_creases()
+    for cr in sh.creases:
+        gpu.add_crease(cr.vx, cr.vy, cr.ext_s, cr.ext_e, cr.closed)
+    gpu.shape_functions_all(sh.order, sh.kernel)
+    for d in range(len(gpu.domain_sizes)):
+        got = gpu.get_shape_functions(d)
+        for q in range(6):
+            assert np.array_equal(got[q].view(np.int64), ref[d][q].view(np.int64))
+    # no crease at all: equal to the oracle cut the same way
+    gpu.clear_creases()
+    gpu.shape_functions_all(sh.order, sh.kernel)
+    orc.clear_creases()
+    off, _, _ = orc.neighbours(0, sh.rho)
+    orc.shape_functions(0, sh.rho, sh.order, sh.kernel, 8)
+    po = orc.get_shape_functions(0, off[-1])[0]
+    pg = gpu.get_shape_functions(0)[0]
+    assert np.abs(pg - po).max() <= 1e-12
+    # restore
+    for p in (gpu, orc):
+        for cr in sh.creases:
+            p.add_crease(cr.vx, cr.vy, cr.ext_s, cr.ext_e, cr.closed)
+    gpu.shape_functions_all(sh.order, sh.kernel)
+    orc.shape_functions(0, sh.rho, sh.order, sh.kernel, 8)
+
+
+def test_crease_added_after_neighbours_is_honoured(c1):
+    """Creases added after the neighbour search cut the next shape functions
+    exactly as creases added before it."""
+    import dataclasses
+    sh, _, _ = c1
+    late = G.plate_from_sheet(dataclasses.replace(sh, creases=[], folds=[]))
+    for cr in sh.creases:
+        late.add_crease(cr.vx, cr.vy, cr.ext_s, cr.ext_e, cr.closed)
+    late.shape_functions_all(sh.order, sh.kernel)
+    early = G.plate_from_sheet(dataclasses.replace(sh, folds=[]))
+    a, b = late.get_shape_functions(0), early.get_shape_functions(0)
+    for q in range(6):
+        assert np.array_equal(a[q].view(np.int64), b[q].view(np.int64))
+
+
+def test_smoke_example_known_answer():
+    cfg, ms = smoke_case(INPUTS)
+    sh = ms.sheet
+    print(f"smoke: {sh.nn} nodes, {sh.face.n} face GPs, rho {sh.rho:.6f}, order {sh.order}", flush=True)
+    plate = G.plate_from_sheet(sh)
+    solver = CS.ContinuationSolver()
+    solver.tol_force = float(cfg["solver"]["tol_force"])
+    solver.tol_energy = float(cfg["solver"]["tol_energy"])
+    solver.ls_max_iters = int(cfg["solver"].get("ls_max_iters", 500))
+    x = CS.initial_condition_with_folds(CS.PlateProblem(plate, flag_rig=True), solver)
+    assert np.abs(x).max() == 0.0     # all constraints hold at x = 0
+    be = PR.DeviceBackend(plate, sh, solver, exact_hinge=bool(cfg["solver"].get("fold_exact_tangent", False)),
+                          theta_star=[fi.sign * fi.angle_target for fi in ms.fold_info])
+    lines, static = PR.lines_from_config(ms, cfg["creasing"])
+    recs = PR.run(be, x, lines, static, PR.CreasingConfig.from_json(cfg["creasing"]), log=print)
+    ref = np.load(FIXTURE)
+    assert [(r.a, r.b) for r in recs] == list(zip(ref["a"].tolist(), ref["b"].tolist()))
+    for r, em, eb, eh in zip(recs, ref["E_mem"], ref["E_ben"], ref["E_hin"]):
+        assert r.iters <= SMOKE_MAX_ITERS
+        for got, want in ((r.E_mem, em), (r.E_ben, eb), (r.E_hin, eh)):
+            assert abs(got - want) <= 1e-7 * max(abs(want), 1e-30), (r.step, got, want)
+    last = recs[-1]
+    assert last.step == SMOKE_KNOWN["step"]
+    for k in ("E_mem", "E_ben", "E_hin", "theta_mean_deg"):
+        assert within_printed_digits(getattr(last, k), SMOKE_KNOWN[k]), (k, getattr(last, k))
+    print(f"smoke step {last.step}: E_mem {last.E_mem:.4e} E_ben {last.E_ben:.4e} E_hin {last.E_hin:.4e} "
+          f"theta mean {last.theta_mean_deg:.4f} deg (README: {SMOKE_KNOWN})")
