@@ -27,6 +27,14 @@ sys.path.insert(0, os.path.join(ROOT, "curvami-official_amd"))
 import curvami_gpu as G  # noqa: E402
 import workloads as W  # noqa: E402
 
+_T0 = time.perf_counter()
+
+
+def log(msg):
+    """Progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector / matrix
 RHO_FACTOR = 3.75
@@ -96,6 +104,7 @@ def cpu_baseline(sh, threads):
                      sh.order, sh.kernel, sh.E, sh.nu, sh.t)
     out = {}
     for T in sorted({1, threads}):
+        log(f"cpu baseline: {sub.n} face GPs on {T} thread(s)")
         p = O.plate_from_sheet(sample, compute=False)
         for d in range(len(p.domains)):
             p.neighbours(d, sample.rho)
@@ -142,6 +151,7 @@ def batched_leg(nsheets, warmup, steps):
     plain FP64 GPU path -- one context per sheet, sheets one after another -- is
     timed on the first 32 sheets for the ratio."""
     import autotune as AT
+    log(f"batched: generating {nsheets} sheets")
     sheets = [W.config_batched_sheet(s) for s in range(nsheets)]
     flags64 = G.RESIDUAL | G.TANGENT | G.ELASTIC_FOLDS
     flags32 = flags64 | G.FP32_TANGENT
@@ -149,7 +159,10 @@ def batched_leg(nsheets, warmup, steps):
     # plain FP64 path, sheet by sheet
     sample = sheets[:min(32, nsheets)]
     tot_gp, tot_t = 0, 0.0
-    for sh in sample:
+    log(f"batched: plain FP64 path on {len(sample)} sheets, one context each")
+    for k, sh in enumerate(sample):
+        if k % 8 == 7:
+            log(f"batched: plain path, sheet {k + 1}/{len(sample)}")
         p = G.plate_from_sheet(sh)
         x = G.DeviceArray(3 * sh.nn); x.set(W.displacement(sh.nn, 1, 1e-3))
         F = G.DeviceArray(3 * sh.nn); K = G.DeviceArray(p.nnz_union)
@@ -162,6 +175,7 @@ def batched_leg(nsheets, warmup, steps):
     res["plain_sample"] = f"first {len(sample)} sheets, {tot_gp} face GPs, one context per sheet, sequential"
     # autotune on a subset
     sub = sheets[:min(16, nsheets)]
+    log(f"batched: autotuning the tile shape on {len(sub)} sheets")
     tp = G.batch_from_sheets(sub)
     xs = np.concatenate([W.displacement(sh.nn, 1 + k, 1e-3) for k, sh in enumerate(sub)])
     x = G.DeviceArray(xs.size); x.set(xs)
@@ -175,6 +189,7 @@ def batched_leg(nsheets, warmup, steps):
                        "table": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()}
                                  for r in tune["table"]], "tuned_on": f"first {len(sub)} sheets"}
     # the full batch
+    log(f"batched: set-up of the {nsheets}-sheet context")
     t0 = time.perf_counter()
     bp = G.batch_from_sheets(sheets, compute=False)
     bp.set_tile_shape(tg, um, gr)
@@ -189,6 +204,7 @@ def batched_leg(nsheets, warmup, steps):
     F = G.DeviceArray(xs.size); K = G.DeviceArray(nu)
     nd = len(bp.domain_sizes)
     for name, fl in (("mixed", flags32), ("fp64", flags64)):
+        log(f"batched: timing the {name} step")
         dt = _step_time(bp, nd, sheets[0].order, sheets[0].kernel, x, F, K, fl, steps, warmup, graph=True)
         # assembly alone (the per-Newton-iteration work), graph replays
         bp.graph_capture(x.ptr, F.ptr, K.ptr, fl)
@@ -216,6 +232,7 @@ def large_leg(warmup, steps):
     arc; fold 0 a rigid hinge (constraint rows, FP64), folds 1 and 2 elastic.
     Step = shape functions of every domain + residual + tangent (graph replay);
     the constraint rows of the rigid fold are timed separately."""
+    log("large: generating and setting up the 1001 x 1001 sheet")
     sh = W.config5(1001)
     t0 = time.perf_counter()
     p = G.plate_from_sheet(sh, compute=False)
@@ -234,6 +251,7 @@ def large_leg(warmup, steps):
            "setup_s": round(setup, 1)}
     base = G.RESIDUAL | G.TANGENT | G.ELASTIC_FOLDS
     for name, fl in (("mixed", base | G.FP32_TANGENT), ("fp64", base)):
+        log(f"large: timing the {name} step")
         dt = _step_time(p, nd, sh.order, sh.kernel, x, F, K, fl, steps, warmup, graph=True)
         res[name] = {"gps": ng / dt, "ms_per_step": dt * 1e3}
     rows, cols = p.constraint_layout(rigid=False)
@@ -257,6 +275,7 @@ def serving_leg(nreq):
     import serving as S
     reqs = W.serving_requests(nreq)
     sides = sorted(set(n for n, _, _ in reqs) | {201})
+    log(f"serving: preparing {len(sides)} sheets")
     sheets, plates, tprep = S.prepare_sheets(sides)
     xs = S.displacements(reqs, sheets)
     srv = G.Server(hbm_budget_bytes=96 << 30, max_inflight=8)
@@ -268,6 +287,7 @@ def serving_leg(nreq):
     out = {"requests_per_rate": nreq, "distinct_sheets": len(sides), "prepare_s": round(tprep, 1),
            "hbm_budget_gb": 96, "slots": 8}
     for name, fl in (("fp64", flags), ("mixed", flags | G.FP32_TANGENT)):
+        log(f"serving: {name} tangents, 201 x 201 alone and six request rates")
         p50, p99 = S.size_class_latency(srv, sid, plates, 201, x201, fl)
         rows = [S.run_rate(srv, sid, reqs, xs, r, 100 + r, fl) for r in (50, 100, 200, 500, 1000, 2000)]
         out[name] = {"alone_201_p50_ms": p50, "alone_201_p99_ms": p99,
@@ -296,6 +316,7 @@ def main():
                                    "(see DESIGN.md), not a measured path", "n_gpus": args.gpus}))
         sys.exit(2)
 
+    log(f"main: {args.n_side} x {args.n_side} arc-crease sheet, set-up")
     sh = W.config2(rho_factor=RHO_FACTOR, n_side=args.n_side)
     ng = sh.face.n
     t0 = time.perf_counter()
@@ -342,8 +363,11 @@ def main():
         return dt_, st_
 
     # plain FP64 path (the reported value), then the mixed path (FP32 tangent)
+    log(f"main: {args.warmup} + {args.steps} FP64 steps")
     dt, stats = timed(0)
+    log(f"main: FP64 {dt * 1e3:.3f} ms/step; {args.warmup} + {args.steps} mixed steps")
     dt_mixed, stats_mixed = timed(G.FP32_TANGENT)
+    log(f"main: mixed {dt_mixed * 1e3:.3f} ms/step")
     kern_ms = {k: v[0] / max(v[1], 1) for k, v in stats.items()}
     # per-launch kernel time (face domain launches dominate the rkpm total)
     F = np.empty(3 * sh.nn)
@@ -418,6 +442,7 @@ def main():
         out["serving"] = serving_leg(args.serving_requests)
     if not args.no_large:
         out["large"] = large_leg(args.warmup, max(2, args.steps // 3))
+    log("done")
     print(json.dumps(out))
 
 
