@@ -31,8 +31,8 @@ namespace cvm {
 constexpr int TILE_THREADS = 512;    // 8 waves; wave w prepares GPs w and w + 8 of the tile
 constexpr int TILE_GPS = 16;         // GPs per tile (presence masks are 16-bit)
 constexpr int UMAX = 256;            // union nodes per tile (uint8 union indices, at most 255 used)
-constexpr int GS = 17;               // GP record fields (tangent type)
-constexpr int GSP = 20;              // GP record stride: GP-major, 16-byte aligned for wide LDS reads
+constexpr int GS = 23;               // GP record fields (tangent type)
+constexpr int GSP = 24;              // GP record stride: GP-major, 16-byte aligned for wide LDS reads
 constexpr uint16_t NO_REC = 0xffff;  // loc entry of a union node that a GP does not see
 
 // neighbour record planes: grad Psi, beta (bending B-row factor,
@@ -40,8 +40,9 @@ constexpr uint16_t NO_REC = 0xffff;  // loc entry of a union node that a GP does
 // contribution f_local (:249) in FP64
 enum { R_PX = 0, R_PY, R_BE0, R_BE1, R_BE2, R_TU };
 enum { R_F0 = 0, R_F1, R_F2 };
-// GP record planes: weight, F0 columns, normal, b.v_curv, membrane resultant N, C0^-1
-enum { G_W = 0, G_A1 = 1, G_A2 = 4, G_B = 7, G_BDV = 10, G_N = 11, G_CI = 14 };
+// GP record fields: weight, F0 columns, normal, b.v_curv, W N (membrane resultant times
+// the weight), C0^-1, and the weighted F0 columns W a1, W a2
+enum { G_W = 0, G_A1 = 1, G_A2 = 4, G_B = 7, G_BDV = 10, G_WN = 11, G_CI = 14, G_WA1 = 17, G_WA2 = 20 };
 
 // Per tile: (1) wave w prepares GP w of the tile (kinematics, resultants and the
 // per-neighbour records, Assembly.h:210-268); (2) every thread owns touched
@@ -206,8 +207,9 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
                        sf[5 * npairs + p], uloc[p]);
             }
             if (lane == 0) {
-                const double gv[GS] = {W, a1x, a1y, a1z, a2x, a2y, a2z, bx, by, bz, bdv, N[0], N[1], N[2],
-                                       Ci00, Ci01, Ci11};
+                const double gv[GS] = {W, a1x, a1y, a1z, a2x, a2y, a2z, bx, by, bz, bdv, W * N[0], W * N[1],
+                                       W * N[2], Ci00, Ci01, Ci11, W * a1x, W * a1y, W * a1z, W * a2x,
+                                       W * a2y, W * a2z};
 #pragma unroll
                 for (int f = 0; f < GS; ++f) Gp(f, wv) = static_cast<TT>(gv[f]);
             }
@@ -230,15 +232,21 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
             TT acc[9];
 #pragma unroll
             for (int k = 0; k < 9; ++k) acc[k] = TT(0);
-            // Two coupling GPs per iteration (ascending GP order is kept: the first
-            // GP's contribution is added before the second's).
-            auto pair_block = [&](int wv, TT (&kb)[9]) {
+            // Per coupling GP the block is K_IJ = T0 a1^T + T1 a2^T + x b^T + b y^T + W geo I
+            // with the weight folded into T0, T1, x, y (W a1, W a2 from the GP record);
+            // terms() forms those factors, add() accumulates them into the slot's block.
+            // Two coupling GPs per iteration, the first GP's terms added first (ascending
+            // GP order).
+            struct Terms { TT T0[3], T1[3], xv[3], yv[3], a1[3], a2[3], bv[3], wgeo; };
+            auto terms = [&](int wv, Terms& t) {
                 const int ri = loc[wv * U + ua];
                 const int rj = loc[wv * U + ubb];
-                const TT W = Gp(G_W, wv);
-                const TT a1[3] = {Gp(G_A1, wv), Gp(G_A1 + 1, wv), Gp(G_A1 + 2, wv)};
-                const TT a2[3] = {Gp(G_A2, wv), Gp(G_A2 + 1, wv), Gp(G_A2 + 2, wv)};
-                const TT bv[3] = {Gp(G_B, wv), Gp(G_B + 1, wv), Gp(G_B + 2, wv)};
+                const TT wa1[3] = {Gp(G_WA1, wv), Gp(G_WA1 + 1, wv), Gp(G_WA1 + 2, wv)};
+                const TT wa2[3] = {Gp(G_WA2, wv), Gp(G_WA2 + 1, wv), Gp(G_WA2 + 2, wv)};
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    t.a1[r] = Gp(G_A1 + r, wv); t.a2[r] = Gp(G_A2 + r, wv); t.bv[r] = Gp(G_B + r, wv);
+                }
                 const TT ci00 = Gp(G_CI, wv), ci01 = Gp(G_CI + 1, wv), ci11 = Gp(G_CI + 2, wv);
                 const TT pxi = R(R_PX, ri), pyi = R(R_PY, ri), pxj = R(R_PX, rj), pyj = R(R_PY, rj);
                 const TT bi0 = R(R_BE0, ri), bi1 = R(R_BE1, ri), bi2 = R(R_BE2, ri);
@@ -250,59 +258,65 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
                 const TT ap20 = Ac[6] * pxj + Ac[8] * pyj, ap21 = Ac[7] * pyj + Ac[8] * pxj;
                 const TT S00 = pxi * ap00 + pyi * ap20, S01 = pxi * ap01 + pyi * ap21;
                 const TT S10 = pyi * ap10 + pxi * ap20, S11 = pyi * ap11 + pxi * ap21;
-                // g = F0 C0^-1 grad Psi for both nodes
+                // C0^-1 grad Psi for both nodes (g = F0 C0^-1 grad Psi)
                 const TT ci0 = ci00 * pxi + ci01 * pyi, ci1 = ci01 * pxi + ci11 * pyi;
                 const TT cj0 = ci00 * pxj + ci01 * pyj, cj1 = ci01 * pxj + ci11 * pyj;
                 // z = beta_i . D beta_j + (b.v) grad Psi_i^T C0^-1 grad Psi_j
                 const TT db0 = Dc[0] * bj0 + Dc[1] * bj1 + Dc[2] * bj2;
                 const TT db1 = Dc[3] * bj0 + Dc[4] * bj1 + Dc[5] * bj2;
                 const TT db2 = Dc[6] * bj0 + Dc[7] * bj1 + Dc[8] * bj2;
-                const TT z = bi0 * db0 + bi1 * db1 + bi2 * db2 + Gp(G_BDV, wv) * (pxi * cj0 + pyi * cj1);
-                const TT geo = pxi * (Gp(G_N, wv) * pxj + Gp(G_N + 2, wv) * pyj)
-                                 + pyi * (Gp(G_N + 2, wv) * pxj + Gp(G_N + 1, wv) * pyj);
-                TT m0 = 0, m1 = 0, q0 = 0, q1 = 0;
-                if (HAS_B) {
+                const TT Wz = Gp(G_W, wv) * (bi0 * db0 + bi1 * db1 + bi2 * db2 + Gp(G_BDV, wv) * (pxi * cj0 + pyi * cj1));
+                t.wgeo = pxi * (Gp(G_WN, wv) * pxj + Gp(G_WN + 2, wv) * pyj)
+                         + pyi * (Gp(G_WN + 2, wv) * pxj + Gp(G_WN + 1, wv) * pyj);
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    const TT Wgi = wa1[r] * ci0 + wa2[r] * ci1, Wgj = wa1[r] * cj0 + wa2[r] * cj1;
+                    t.xv[r] = -(tui * Wgj);
+                    t.yv[r] = Wz * t.bv[r] - tuj * Wgi;
+                    t.T0[r] = wa1[r] * S00 + wa2[r] * S10;
+                    t.T1[r] = wa1[r] * S01 + wa2[r] * S11;
+                }
+                if constexpr (HAS_B) {
                     const TT bbj0 = Bc[0] * bj0 + Bc[1] * bj1 + Bc[2] * bj2;
                     const TT bbj1 = Bc[3] * bj0 + Bc[4] * bj1 + Bc[5] * bj2;
                     const TT bbj2 = Bc[6] * bj0 + Bc[7] * bj1 + Bc[8] * bj2;
                     const TT bbi0 = Bc[0] * bi0 + Bc[1] * bi1 + Bc[2] * bi2;
                     const TT bbi1 = Bc[3] * bi0 + Bc[4] * bi1 + Bc[5] * bi2;
                     const TT bbi2 = Bc[6] * bi0 + Bc[7] * bi1 + Bc[8] * bi2;
-                    m0 = pxi * bbj0 + pyi * bbj2; m1 = pyi * bbj1 + pxi * bbj2;
-                    q0 = pxj * bbi0 + pyj * bbi2; q1 = pyj * bbi1 + pxj * bbi2;
-                }
-                TT xv[3], yv[3], T0[3], T1[3];
+                    const TT m0 = pxi * bbj0 + pyi * bbj2, m1 = pyi * bbj1 + pxi * bbj2;
+                    const TT q0 = pxj * bbi0 + pyj * bbi2, q1 = pyj * bbi1 + pxj * bbi2;
 #pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    const TT gi = a1[r] * ci0 + a2[r] * ci1, gj = a1[r] * cj0 + a2[r] * cj1;
-                    xv[r] = W * (a1[r] * m0 + a2[r] * m1 - tui * gj);
-                    yv[r] = W * (a1[r] * q0 + a2[r] * q1 - tuj * gi + z * bv[r]);
-                    T0[r] = W * (a1[r] * S00 + a2[r] * S10);
-                    T1[r] = W * (a1[r] * S01 + a2[r] * S11);
+                    for (int r = 0; r < 3; ++r) {
+                        t.xv[r] += wa1[r] * m0 + wa2[r] * m1;
+                        t.yv[r] += wa1[r] * q0 + wa2[r] * q1;
+                    }
                 }
-                const TT wg = W * geo;
+            };
+            auto add = [&](const Terms& t) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
 #pragma unroll
-                    for (int r = 0; r < 3; ++r)
-                        kb[3 * c + r] = T0[r] * a1[c] + T1[r] * a2[c] + xv[r] * bv[c] + bv[r] * yv[c]
-                                        + (r == c ? wg : TT(0));
+                    for (int r = 0; r < 3; ++r) {
+                        TT v = acc[3 * c + r] + t.T0[r] * t.a1[c] + t.T1[r] * t.a2[c] + t.xv[r] * t.bv[c]
+                               + t.bv[r] * t.yv[c];
+                        if (r == c) v += t.wgeo;
+                        acc[3 * c + r] = v;
+                    }
             };
             while (mask) {
                 const int w0 = __ffs(mask) - 1;
                 mask &= mask - 1;
-                TT k0[9];
-                pair_block(w0, k0);
+                Terms t0;
+                terms(w0, t0);
                 if (mask) {
                     const int w1 = __ffs(mask) - 1;
                     mask &= mask - 1;
-                    TT k1[9];
-                    pair_block(w1, k1);
-#pragma unroll
-                    for (int k = 0; k < 9; ++k) acc[k] = (acc[k] + k0[k]) + k1[k];
+                    Terms t1;
+                    terms(w1, t1);
+                    add(t0);
+                    add(t1);
                 } else {
-#pragma unroll
-                    for (int k = 0; k < 9; ++k) acc[k] += k0[k];
+                    add(t0);
                 }
             }
             TT* dst = scr + 9 * static_cast<int64_t>(slot_dst[tb + sl]);
