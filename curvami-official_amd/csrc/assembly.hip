@@ -34,6 +34,11 @@ constexpr int UMAX = 256;            // union nodes per tile (uint8 union indice
 constexpr int GS = 23;               // GP record fields (tangent type)
 constexpr int GSP = 24;              // GP record stride: GP-major, 16-byte aligned for wide LDS reads
 constexpr uint16_t NO_REC = 0xffff;  // loc entry of a union node that a GP does not see
+// Neighbour record stride (tangent-type elements): FP64 records get one pad
+// element so that records start on 32 different LDS bank positions instead of
+// 16; FP32 records stay at 6 (24 bytes), which keeps them 8-byte aligned for
+// paired loads.
+template <typename TT> constexpr int rec_stride() { return sizeof(TT) == 8 ? 7 : 6; }
 
 // neighbour record planes: grad Psi, beta (bending B-row factor,
 // Assembly.h:240-247), Tu (:265-267) in the tangent type TT, and the residual
@@ -86,12 +91,13 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
 
     double* recf = smem;                                        // 3 FP64 planes of rec_cap
     double* uc = recf + static_cast<size_t>(rec_cap) * 3;       // umax * 3
-    TT* rect = reinterpret_cast<TT*>(uc + umax * 3);            // rec_cap records of 6 TT
-    TT* gs = rect + static_cast<size_t>(rec_cap) * 6;           // TILE_GPS records of GSP
+    TT* rect = reinterpret_cast<TT*>(uc + umax * 3);            // rec_cap records of RS TT
+    constexpr int RS = rec_stride<TT>();
+    TT* gs = rect + static_cast<size_t>(rec_cap) * RS;          // TILE_GPS records of GSP
     uint32_t* pm = reinterpret_cast<uint32_t*>(gs + TILE_GPS * GSP);  // umax presence masks
     uint16_t* loc = reinterpret_cast<uint16_t*>(pm + umax);     // ngp rows of U record indices
     auto RF = [&](int f, int r) -> double& { return recf[static_cast<size_t>(f) * rec_cap + r]; };
-    auto R = [&](int f, int r) -> TT& { return rect[static_cast<size_t>(r) * 6 + f]; };   // record-major
+    auto R = [&](int f, int r) -> TT& { return rect[static_cast<size_t>(r) * RS + f]; };  // record-major
     auto Gp = [&](int f, int wv) -> TT& { return gs[wv * GSP + f]; };
 
     if (tid < U) {
@@ -562,7 +568,8 @@ size_t tile_lds_bytes(int rec_cap, int max_union, bool fp32) {
     const size_t tt = fp32 ? sizeof(float) : sizeof(double);
     const size_t um = static_cast<size_t>(tile_umax(max_union));
     return sizeof(double) * (static_cast<size_t>(rec_cap) * 3 + um * 3)
-         + tt * (static_cast<size_t>(rec_cap) * 6 + TILE_GPS * GSP) + sizeof(uint32_t) * um
+         + tt * (static_cast<size_t>(rec_cap) * (fp32 ? rec_stride<float>() : rec_stride<double>()) +
+                 TILE_GPS * GSP) + sizeof(uint32_t) * um
          + sizeof(uint16_t) * TILE_GPS * um;
 }
 
