@@ -658,6 +658,36 @@ int cvm_fold_angles(cvm_context* ctx, int fold_id, const double* x, int flags, d
     CVM_CATCH(ctx)
 }
 
+int cvm_export_fields(cvm_context* ctx, const double* x, int flags, int nfields, const int* field_codes,
+                      double* out) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    CVM_HIP(hipSetDevice(c.device));
+    if (!x || !out || nfields < 0 || (nfields > 0 && !field_codes)) throw Error(CVM_E_INVALID, "null argument");
+    cvm::Domain* face = nullptr;
+    for (cvm::Domain* d : c.doms) if (d->kind == CVM_DOMAIN_FACE) { face = d; break; }
+    if (!face) throw Error(CVM_E_STATE, "no face domain");
+    const bool dev = (flags & CVM_DEVICE_PTRS) != 0;
+    const double* xd = x;
+    if (!dev) {
+        c.d_x.upload(x, 3 * static_cast<size_t>(c.nn), c.stream);
+        xd = c.d_x.p;
+    }
+    std::vector<int> codes(field_codes, field_codes + nfields);
+    if (codes.empty()) codes.push_back(-1);
+    c.ex_codes.upload(codes, c.stream);
+    const size_t n = static_cast<size_t>(face->ng) * static_cast<size_t>(7 + nfields);
+    double* od = out;
+    if (!dev) {
+        c.ex_out.alloc(std::max<size_t>(n, 1));
+        od = c.ex_out.p;
+    }
+    cvm::launch_export_fields(c, xd, nfields, c.ex_codes.p, od);
+    if (!dev && n) CVM_HIP(hipMemcpyAsync(out, od, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    CVM_HIP(hipStreamSynchronize(c.stream));
+    CVM_CATCH(ctx)
+}
+
 int cvm_clear_creases(cvm_context* ctx) {
     CVM_TRY(ctx)
     Context& c = ctx->c;

@@ -278,6 +278,8 @@ struct Context {
     DBuf<double> en_scr;        // per-GP energies (face membrane, face bending, crease hinge)
     DBuf<char> en_active;       // crease-GP activity copies for the hinge energy
     DBuf<double> en_out;        // 3 energies
+    DBuf<int> ex_codes;         // export field codes
+    DBuf<double> ex_out;        // export columns (ng x (7 + fields))
     DBuf<double> d_F;           // 3 nn
     DBuf<double> d_K;           // nnz (union)
     Workspace ws;               // the context's own assembly workspace (its stream)
@@ -319,6 +321,7 @@ void launch_constraints(Context& c, const double* xdev, double* cdev, double* jd
 void launch_external_force(Context& c, double* Fdev);
 void launch_energies(Context& c, const double* xdev, double* out3_dev);
 void launch_fold_angles(Context& c, int fold_id, const double* xdev, double* theta_dev);
+void launch_export_fields(Context& c, const double* xdev, int nf, const int* fcode_dev, double* out_dev);
 
 }  // namespace cvm
 

@@ -10,7 +10,8 @@
 // left / right deformation gradients and the dihedral angle, plus the GP's
 // hinge energy 1/2 W Kt (theta - theta0)^2 when active. k_sum3 sums the three
 // per-GP arrays in a fixed order (strided per thread, then a fixed LDS tree),
-// so repeated calls give identical bits.
+// so repeated calls give identical bits. k_export_fields evaluates the
+// exporter's per-GP columns the same way (wave per GP).
 #include <algorithm>
 
 #include "cvm_internal.h"
@@ -74,6 +75,61 @@ __global__ __launch_bounds__(EN_THREADS) void k_face_energy(int ng, const int* _
     const double hw = 0.5 * w[g];
     em[g] = hw * (e0[0] * N[0] + e0[1] * N[1] + e0[2] * N[2]);
     eb[g] = hw * (e1[0] * M[0] + e1[1] * M[1] + e1[2] * M[2]);
+}
+
+// Export columns per face GP (Exporter::writeBinaryStep, plate/Exporter.h:34-91):
+// X, Y, u, v, w, rotX, rotY (degrees) and the requested fields (codes 0..5:
+// E_XX, E_YY, E_XY, K_XX, K_YY, K_XY; anything else writes 0). One wave per GP.
+__global__ __launch_bounds__(EN_THREADS) void k_export_fields(int ng, const int* __restrict__ off,
+                                                             const int* __restrict__ ids,
+                                                             const double* __restrict__ sf, int64_t npairs,
+                                                             const double* __restrict__ gx,
+                                                             const double* __restrict__ gy,
+                                                             const double* __restrict__ xu, int nf,
+                                                             const int* __restrict__ fcode,
+                                                             double* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int g = blockIdx.x * (EN_THREADS / 64) + (threadIdx.x >> 6);
+    if (g >= ng) return;   // uniform per wave
+    double p[18];
+    for (int k = 0; k < 18; ++k) p[k] = 0.0;
+    for (int k = off[g] + lane; k < off[g + 1]; k += 64) {
+        const int v = ids[k];
+        const double u0 = xu[3 * v], u1 = xu[3 * v + 1], u2 = xu[3 * v + 2];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const double d = sf[q * npairs + k];
+            p[3 * q] += u0 * d; p[3 * q + 1] += u1 * d; p[3 * q + 2] += u2 * d;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 18; ++k) p[k] = wave_sum(p[k]);
+    if (lane != 0) return;
+    const double a1[3] = {1.0 + p[3], p[4], p[5]}, a2[3] = {p[6], 1.0 + p[7], p[8]};
+    const double* rxx = p + 9; const double* ryy = p + 12; const double* rxy = p + 15;
+    double n[3];
+    cross3(a1, a2, n);
+    const double J = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    const double b[3] = {n[0] / J, n[1] / J, n[2] / J};
+    const double r2d = 180.0 / M_PI;
+    double* o = out + static_cast<int64_t>(g) * (7 + nf);
+    o[0] = gx[g]; o[1] = gy[g];
+    o[2] = p[0]; o[3] = p[1]; o[4] = p[2];
+    o[5] = atan2(-b[1], b[2]) * r2d;
+    o[6] = atan2(b[0], b[2]) * r2d;
+    for (int f = 0; f < nf; ++f) {
+        double v = 0.0;
+        switch (fcode[f]) {
+            case 0: v = 0.5 * ((a1[0] * a1[0] + a1[1] * a1[1] + a1[2] * a1[2]) - 1.0); break;
+            case 1: v = 0.5 * ((a2[0] * a2[0] + a2[1] * a2[1] + a2[2] * a2[2]) - 1.0); break;
+            case 2: v = 0.5 * (a1[0] * a2[0] + a1[1] * a2[1] + a1[2] * a2[2]); break;
+            case 3: v = -(rxx[0] * b[0] + rxx[1] * b[1] + rxx[2] * b[2]); break;
+            case 4: v = -(ryy[0] * b[0] + ryy[1] * b[1] + ryy[2] * b[2]); break;
+            case 5: v = -(rxy[0] * b[0] + rxy[1] * b[1] + rxy[2] * b[2]); break;
+            default: break;
+        }
+        o[7 + f] = v;
+    }
 }
 
 struct CreaseSide {
@@ -187,6 +243,19 @@ void launch_energies(Context& c, const double* xd, double* out_dev) {
         o += n;
     }
     hipLaunchKernelGGL(k_sum3, dim3(3), dim3(SUM_THREADS), 0, st, em, ng, eb, ng, eh, ncr, out_dev);
+    CVM_HIP(hipGetLastError());
+}
+
+void launch_export_fields(Context& c, const double* xd, int nf, const int* fcode_dev, double* out_dev) {
+    Domain* face = nullptr;
+    for (Domain* d : c.doms) if (d->kind == CVM_DOMAIN_FACE) { face = d; break; }
+    if (!face) throw Error(CVM_E_STATE, "no face domain");
+    need_sf(*face, "export fields");
+    if (face->ng == 0) return;
+    const unsigned grid = static_cast<unsigned>((face->ng + EN_THREADS / 64 - 1) / (EN_THREADS / 64));
+    hipLaunchKernelGGL(k_export_fields, dim3(grid), dim3(EN_THREADS), 0, c.stream, face->ng, face->off.p,
+                       face->ids.p, face->sf.p, static_cast<int64_t>(face->npairs), face->gx.p, face->gy.p, xd, nf,
+                       fcode_dev, out_dev);
     CVM_HIP(hipGetLastError());
 }
 

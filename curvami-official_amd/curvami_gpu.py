@@ -58,6 +58,11 @@ SIGNATURES = {
     "cvm_energies": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "cvm_fold_angles": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]),
     "cvm_clear_creases": (C.c_int, [C.c_void_p]),
+    "cvm_export_fields": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, _i, C.c_void_p]),
+    "cvm_restart_last_error": (C.c_char_p, []),
+    "cvm_restart_save": (C.c_int, [C.c_char_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                   _d, _d]),
+    "cvm_restart_load": (C.c_int, [C.c_char_p, C.c_int, _i, _d, _i, _d, _d, C.c_int]),
     "cvm_synchronize": (C.c_int, [C.c_void_p]),
     "cvm_set_tile_shape": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "cvm_get_tile_shape": (C.c_int, [C.c_void_p, _i]),
@@ -320,6 +325,17 @@ class Plate:
         self._check(self._L.cvm_energies(self._h, x.ctypes.data, 0, out.ctypes.data))
         return tuple(float(v) for v in out)
 
+    def export_fields(self, x, codes=()):
+        """(face GPs, 7 + len(codes)) exporter columns: X, Y, u, v, w, rotX, rotY
+        (degrees) and one column per field code (Exporter::writeBinaryStep)."""
+        x = _f64(x)
+        codes = np.ascontiguousarray(codes, dtype=np.int32)
+        ng = self.domain_sizes[self.face_dom]
+        out = np.empty((max(ng, 1), 7 + codes.size))
+        self._check(self._L.cvm_export_fields(self._h, x.ctypes.data, 0, int(codes.size),
+                                              codes.ctypes.data_as(_i) if codes.size else None, out.ctypes.data))
+        return out[:ng]
+
     def fold_angles(self, fold, x):
         """Dihedral angle at every crease GP of a fold."""
         x = _f64(x)
@@ -523,6 +539,34 @@ def support_radius(X, Y, order: int) -> float:
     r = C.c_double()
     _mesh_check(L, L.cvm_support_radius(_dp(X), _dp(Y), X.size, int(order), C.byref(r)))
     return r.value
+
+
+def restart_save(path, k, lam, ds, dl_old, W_ext, x, dU_old):
+    """RestartManager::save: the binary .res record."""
+    L = load_library()
+    x = _f64(x); dU_old = _f64(dU_old)
+    if x.shape != dU_old.shape:
+        raise CvmError(E_INVALID, "Restart save refused: x and dU_old differ in size.")
+    st = L.cvm_restart_save(os.fsencode(path), int(k), float(lam), float(ds), float(dl_old), float(W_ext),
+                            int(x.size), _dp(x), _dp(dU_old))
+    if st != OK:
+        raise CvmError(st, L.cvm_restart_last_error().decode(errors="replace"))
+
+
+def restart_load(path, expected_ns: int = -1):
+    """RestartManager::load: dict(k, lam, ds, dl_old, W_ext, x, dU_old); raises
+    CvmError for a foreign, truncated, mismatched or non-finite record."""
+    L = load_library()
+    k = C.c_int(); ns = C.c_int(); sc = np.zeros(4)
+    st = L.cvm_restart_load(os.fsencode(path), int(expected_ns), C.byref(k), _dp(sc), C.byref(ns), None, None, 0)
+    if st != OK:
+        raise CvmError(st, L.cvm_restart_last_error().decode(errors="replace"))
+    x = np.empty(ns.value); dU = np.empty(ns.value)
+    st = L.cvm_restart_load(os.fsencode(path), int(expected_ns), C.byref(k), _dp(sc), C.byref(ns), _dp(x), _dp(dU),
+                            ns.value)
+    if st != OK:
+        raise CvmError(st, L.cvm_restart_last_error().decode(errors="replace"))
+    return {"k": k.value, "lam": sc[0], "ds": sc[1], "dl_old": sc[2], "W_ext": sc[3], "x": x, "dU_old": dU}
 
 
 class DeviceArray:

@@ -13,9 +13,10 @@ raising theta0 on the newest crease points over `fallback_substeps`
 sub-steps. Energies (cvm_energies) and crease angles (cvm_fold_angles) are
 device results; the per-step bookkeeping here is host logic.
 
-Not reproduced: the coefficient re-fit (refit other than "none"), the
-diagnose_step mode, the negative-mode count of the KKT factorization (reported
-as None) and the HDF5 export.
+Each accepted step can be exported through `on_step` (export.step_record /
+StepWriter). Not reproduced: the coefficient re-fit (refit other than
+"none"), the diagnose_step mode and the negative-mode count of the KKT
+factorization (reported as None).
 """
 from __future__ import annotations
 
@@ -227,11 +228,13 @@ class Backend:
 
 
 def run(backend, x: np.ndarray, lines: List[Line], static_creases: List[Crease], cfg: CreasingConfig,
-        log: Optional[Callable[[str], None]] = None) -> List[StepRecord]:
+        log: Optional[Callable[[str], None]] = None,
+        on_step: Optional[Callable[[StepRecord, np.ndarray], None]] = None) -> List[StepRecord]:
     """Progressive::run. x (the start, e.g. the initial folded condition) is
     updated in place to the last accepted state; returns one record per
     accepted step. On entry the shape functions must be those of every cut in
-    full."""
+    full. on_step(record, x) runs after each accepted step (the reference's
+    on_converged export)."""
     say = log or (lambda s: None)
     if not lines:
         raise ValueError("Progressive creasing: no line to make")
@@ -412,6 +415,8 @@ def run(backend, x: np.ndarray, lines: List[Line], static_creases: List[Crease],
             f"{rec.theta_max_deg:.4f} deg | step change {rec.change_over_L:.3e} L")
         records.append(rec)
         prev_E_tot = rec.E_mem + rec.E_ben + rec.E_hin
+        if on_step is not None:
+            on_step(rec, x)
     return records
 
 

@@ -265,6 +265,26 @@ CVM_API int cvm_energies(cvm_context* ctx, const double* x, int flags, double* o
  * Progressive::activeFoldAngles, plate/ProgressiveCreasing.h:157-179). */
 CVM_API int cvm_fold_angles(cvm_context* ctx, int fold_id, const double* x, int flags, double* theta);
 
+/* Exporter columns per face GP (Exporter::writeBinaryStep, plate/Exporter.h:34-91):
+ * out[g * (7 + nfields) + c] for c = X, Y, u, v, w, rotX (deg), rotY (deg), then
+ * one column per field code (0 E_XX, 1 E_YY, 2 E_XY, 3 K_XX, 4 K_YY, 5 K_XY;
+ * any other code writes zeros, as the reference does for an unknown name).
+ * flags: CVM_DEVICE_PTRS (x and out on the device). */
+CVM_API int cvm_export_fields(cvm_context* ctx, const double* x, int flags, int nfields, const int* field_codes,
+                              double* out);
+
+/* ---- restart record (host only) ------------------------------------------ */
+
+/* RestartManager::save / load (core/include/curvami/RestartManager.h:11-117),
+ * the same byte layout. load: expected_ns <= 0 skips the DOF check; with
+ * x = dU_old = NULL only *ns is returned (after the header checks);
+ * scalars4 = lambda, ds, dl_old, W_ext. Messages: cvm_restart_last_error. */
+CVM_API const char* cvm_restart_last_error(void);
+CVM_API int cvm_restart_save(const char* path, int k, double lambda, double ds, double dl_old, double W_ext, int ns,
+                             const double* x, const double* dU_old);
+CVM_API int cvm_restart_load(const char* path, int expected_ns, int* k, double* scalars4, int* ns, double* x,
+                             double* dU_old, int cap);
+
 /* ---- measurement -------------------------------------------------------- */
 
 /* Per-kernel HIP-event timing on the context's stream (on/off). */

@@ -266,6 +266,33 @@ class OraclePlate:
         self._L.orc_kinematics(self._p, dom, _dp(x), _dp(out))
         return out
 
+    def export_columns(self, gx, gy, x, rho, codes=()):
+        """Exporter::writeBinaryStep's face-GP columns (plate/Exporter.h:47-91):
+        X, Y, u, v, w (Psi-weighted sums of the displacements, in neighbour
+        order), rotX = atan2(-b_y, b_z), rotY = atan2(b_x, b_z) in degrees, and
+        E0 / E1 components for codes 0..5 (zeros otherwise)."""
+        x = _f64(x)
+        dom = self.face_dom
+        off, ids, _ = self.neighbours(dom, rho)
+        psi = self.get_shape_functions(dom, int(off[-1]))[0]
+        kin = self.kinematics(dom, x)
+        ng = off.size - 1
+        out = np.zeros((ng, 7 + len(codes)))
+        out[:, 0], out[:, 1] = gx, gy
+        X = x.reshape(-1, 3)
+        for g in range(ng):
+            u = np.zeros(3)
+            for j in range(off[g], off[g + 1]):
+                u += X[ids[j]] * psi[j]
+            out[g, 2:5] = u
+        b = kin[:, 6:9]
+        out[:, 5] = np.degrees(np.arctan2(-b[:, 1], b[:, 2]))
+        out[:, 6] = np.degrees(np.arctan2(b[:, 0], b[:, 2]))
+        for c, code in enumerate(codes):
+            if 0 <= code <= 5:
+                out[:, 7 + c] = kin[:, 9 + code]
+        return out
+
 
 def plate_from_sheet(sh, nthreads: int = 1, compute: bool = True):
     """Build the oracle plate for a workloads.Sheet: nodes, material, creases,
