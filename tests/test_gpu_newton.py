@@ -94,3 +94,34 @@ def test_newton_mixed_precision_tangent(c1m):
           f"{int(np.sum(ref['iterations']))}, delta {d_it:+d}), |dx|/|x| {err:.3e}, {dt:.1f} s")
     assert res.all_converged and len(res.converged) == W.NEWTON_INCREMENTS
     assert err <= 1e-8
+
+
+def test_ls_miura_line_search_matches_oracle():
+    """The reference's regression configuration ls_miura (four rigid folds
+    pre-set to +-15.26 / 30 deg, two cuts, corner supports; cubic basis,
+    rho = 6h): the initial folded condition and one LineSearch at target_load 0
+    with the rigid-fold rows, on the device, against the oracle's run
+    (tests/golden/lsmiura_linesearch.npz, make_lsmiura_fixture.py). Bars: x0
+    and the converged x within 1e-8 (inf-norm, relative), the same number of
+    Newton iterations."""
+    import json
+    import mesh_setup as MS
+    inputs = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_inputs")
+    ref = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "lsmiura_linesearch.npz"))
+    with open(os.path.join(inputs, "ls_miura.json")) as f:
+        cfg = json.load(f)
+    ms = MS.sheet_from_config(cfg, base_dir=inputs, mesh_file=os.path.join(inputs, "Miura.msh"))
+    plate = G.plate_from_sheet(ms.sheet)
+    prob = CS.PlateProblem(plate, flag_rig=True)
+    solver = CS.ContinuationSolver()
+    t0 = time.perf_counter()
+    x0 = CS.initial_condition_with_folds(prob, solver)
+    e0 = np.abs(x0 - ref["x0"]).max() / np.abs(ref["x0"]).max()
+    x = x0.copy()
+    rep = solver.line_search(prob, float(cfg.get("target_load", 0.0)), x)
+    dt = time.perf_counter() - t0
+    e = np.abs(x - ref["x"]).max() / np.abs(ref["x"]).max()
+    print(f"ls_miura: x0 rel {e0:.2e}; converged {rep.converged} in {rep.iterations} iterations "
+          f"(oracle {int(ref['iterations'])}), x rel {e:.2e}, {dt:.1f} s; energies {plate.energies(x)} "
+          f"(oracle {ref['energies']})")
+    assert e0 <= 1e-8 and rep.converged and rep.iterations == int(ref["iterations"]) and e <= 1e-8
