@@ -97,6 +97,9 @@ struct SheetSeg {
 
 // Device view of a sheet for one domain's neighbour search and shape functions:
 // the sheet's cell grid (keys koff + y * cols + x), radius and crease range.
+// GPs per block of the shape-function kernel (one wave)
+constexpr int RKPM_BLOCK = 64;
+
 struct SheetDev {
     double minx = 0.0, miny = 0.0, cell = 0.0, rho = 0.0, rho2 = 0.0;
     double irho = 0.0, irho2 = 0.0;     // 1 / rho, 1 / rho^2 (IEEE, host)
@@ -123,6 +126,13 @@ struct Domain {
     DBuf<int> ids;            // npairs
     DBuf<double> s;           // npairs
     std::vector<int> h_off, h_ids;     // host mirrors (needed for the pattern plan)
+    // The same lists interleaved per block of RKPM_BLOCK GPs for the moment
+    // pass, where each lane walks its own GP's pairs: the t-th pair of GP lane
+    // l of block b sits at (il_row[b] + t) * RKPM_BLOCK + l, so one wave-wide
+    // load reads consecutive words. Padding slots (t >= count) are never read.
+    DBuf<int> il_row;         // nblocks + 1 (rows of RKPM_BLOCK slots)
+    DBuf<int> il_ids;         // il_row[nblocks] * RKPM_BLOCK
+    DBuf<double> il_s;
     // shape functions: psi, dpx, dpy, dxx, dyy, dxy (SoA, npairs each)
     DBuf<double> sf;          // 6 * npairs
     int order = 0, kernel = 0;

@@ -241,6 +241,21 @@ __global__ void k_frnn(const double* __restrict__ X, const double* __restrict__ 
 
 __global__ void k_set_zero_int(int* p) { p[0] = 0; }
 
+// Interleaved copy of the pair lists for the moment pass (layout on Domain::il_row).
+__global__ __launch_bounds__(RKPM_BLOCK) void k_interleave(const int* __restrict__ off, const int* __restrict__ ids,
+                                                           const double* __restrict__ s, int ng,
+                                                           const int* __restrict__ row, int* __restrict__ il_ids,
+                                                           double* __restrict__ il_s) {
+    const int g = blockIdx.x * RKPM_BLOCK + threadIdx.x;
+    if (g >= ng) return;
+    const int b = off[g], n = off[g + 1] - b;
+    const size_t base = static_cast<size_t>(row[blockIdx.x]) * RKPM_BLOCK + threadIdx.x;
+    for (int t = 0; t < n; ++t) {
+        il_ids[base + static_cast<size_t>(t) * RKPM_BLOCK] = ids[b + t];
+        il_s[base + static_cast<size_t>(t) * RKPM_BLOCK] = s[b + t];
+    }
+}
+
 // Per sheet: node cell grid with the bounds and column count of FRNN.h:30-42
 // computed over that sheet's nodes; the sheets' key ranges are concatenated.
 void launch_frnn(Context& c, Domain& d, double rho) {
@@ -302,6 +317,7 @@ void launch_frnn(Context& c, Domain& d, double rho) {
     d.off.alloc(d.ng + 1);
     d.h_off.assign(d.ng + 1, 0);
     d.ind_creases = -1;
+    d.il_row.release(); d.il_ids.release(); d.il_s.release();
     if (d.ng == 0) {
         d.npairs = 0;
         CVM_HIP(hipMemsetAsync(d.off.p, 0, sizeof(int), c.stream));
@@ -337,6 +353,24 @@ void launch_frnn(Context& c, Domain& d, double rho) {
                        d_cstart.p, d_cnodes.p, nullptr, d.off.p, d.ids.p, d.s.p);
     CVM_HIP(hipGetLastError());
     timing_end(c, slot);
+    // interleaved lists: a block's rows = its largest neighbour count
+    const int nblk = (d.ng + RKPM_BLOCK - 1) / RKPM_BLOCK;
+    std::vector<int> row(nblk + 1, 0);
+    for (int b = 0; b < nblk; ++b) {
+        int mx = 0;
+        for (int g = b * RKPM_BLOCK; g < std::min(d.ng, (b + 1) * RKPM_BLOCK); ++g)
+            mx = std::max(mx, d.h_off[g + 1] - d.h_off[g]);
+        if (static_cast<int64_t>(row[b]) + mx > INT32_MAX / RKPM_BLOCK)
+            throw Error(CVM_E_NO_MEMORY, "interleaved neighbour lists exceed 2^31 slots");
+        row[b + 1] = row[b] + mx;
+    }
+    const size_t slots = std::max<size_t>(static_cast<size_t>(row[nblk]) * RKPM_BLOCK, 1);
+    d.il_row.upload(row, c.stream);
+    d.il_ids.alloc(slots);
+    d.il_s.alloc(slots);
+    hipLaunchKernelGGL(k_interleave, dim3(nblk), dim3(RKPM_BLOCK), 0, c.stream, d.off.p, d.ids.p, d.s.p, d.ng,
+                       d.il_row.p, d.il_ids.p, d.il_s.p);
+    CVM_HIP(hipGetLastError());
     d.h_ids.resize(d.npairs);
     if (d.npairs) CVM_HIP(hipMemcpyAsync(d.h_ids.data(), d.ids.p, d.npairs * sizeof(int), hipMemcpyDeviceToHost, c.stream));
     CVM_HIP(hipStreamSynchronize(c.stream));

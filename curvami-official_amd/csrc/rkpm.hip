@@ -116,13 +116,14 @@ __device__ __forceinline__ void static_for(F&& f) {
     }
 }
 
-constexpr int RKPM_THREADS = 64;
+constexpr int RKPM_THREADS = RKPM_BLOCK;
 
 template <int P, int KER>
 __global__ __launch_bounds__(RKPM_THREADS, 2) void k_rkpm(
     const double* __restrict__ X, const double* __restrict__ Y, int nn,
     const double* __restrict__ gx, const double* __restrict__ gy, int ng,
     const int* __restrict__ off, const int* __restrict__ ids, const double* __restrict__ sarr,
+    const int* __restrict__ il_row, const int* __restrict__ il_ids, const double* __restrict__ il_s,
     const double* __restrict__ ind, const int8_t* __restrict__ nsign,
     const int* __restrict__ gsheet, const SheetDev* __restrict__ sheets,
     double* __restrict__ sf, int64_t npairs, int* __restrict__ status /* n1, first1, n2, first2 */) {
@@ -142,18 +143,31 @@ __global__ __launch_bounds__(RKPM_THREADS, 2) void k_rkpm(
         const double rho = S.rho, irho = S.irho;
         const int nc = S.nc;
         const double px = gx[g], py = gy[g];
-        const int b = off[g], e = off[g + 1];
+        const int cnt = off[g + 1] - off[g];
+        // this lane's pairs in the interleaved lists: stride RKPM_THREADS
+        const int* __restrict__ lid = il_ids + static_cast<size_t>(il_row[blockIdx.x]) * RKPM_THREADS + threadIdx.x;
+        const double* __restrict__ ls = il_s + static_cast<size_t>(il_row[blockIdx.x]) * RKPM_THREADS + threadIdx.x;
         // ---- moments of monomials ---------------------------------------------
         double mu[6][NMON];
         static_for<0, NMON>([&](auto m) {
 #pragma unroll
             for (int j = 0; j < 6; ++j) mu[j][m.value] = 0.0;
         });
-        for (int k = b; k < e; ++k) {
-            const int I = ids[k];
-            const double xi = (px - X[I]) * irho;
-            const double eta = (py - Y[I]) * irho;
-            const Jet jt = pair_jet<KER>(xi, eta, sarr[k], I, g, ng, nn, ind, nsign, nc, rho);
+        // software pipeline: the node and distance of pair t + 1 are loaded one
+        // pass ahead, its coordinates at the start of the pass before it is used
+        int In = cnt > 0 ? lid[0] : 0;
+        double sn = cnt > 0 ? ls[0] : 0.0;
+        for (int t = 0; t < cnt; ++t) {
+            const int I = In;
+            const double si = sn;
+            const double XI = X[I], YI = Y[I];
+            if (t + 1 < cnt) {
+                In = lid[static_cast<size_t>(t + 1) * RKPM_THREADS];
+                sn = ls[static_cast<size_t>(t + 1) * RKPM_THREADS];
+            }
+            const double xi = (px - XI) * irho;
+            const double eta = (py - YI) * irho;
+            const Jet jt = pair_jet<KER>(xi, eta, si, I, g, ng, nn, ind, nsign, nc, rho);
             double xp[DM + 1], yp[DM + 1];
             xp[0] = 1.0; yp[0] = 1.0;
 #pragma unroll
@@ -305,7 +319,9 @@ __global__ __launch_bounds__(RKPM_THREADS, 2) void k_rkpm(
     // ---- Psi, grad Psi, Hessian of Psi (GlobalRKPM.h:363-397), one lane per pair
     // over the block's contiguous pair range (coalesced stores) ---------------------
     const int p0 = off_s[0], p1 = off_s[nblk];
-    for (int k = p0 + threadIdx.x; k < p1; k += RKPM_THREADS) {
+    // Two pairs per lane and pass, both evaluated unconditionally (the second
+    // clamped to the range) so that their load chains overlap.
+    auto eval = [&](int k, double (&o)[6]) {
         int lo = 0, hi = nblk;                 // GP of pair k: off_s[lo] <= k < off_s[lo + 1]
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
@@ -342,12 +358,24 @@ __global__ __launch_bounds__(RKPM_THREADS, 2) void k_rkpm(
             Cyy += vyy * p + 2.0 * vy * pyv + v * pyy;
             Cxy += vxy * p + vx * pyv + vy * pxv + v * pxy;
         });
-        sf[k] = C * jt.v;
-        sf[npairs + k] = (Cx * jt.v + C * jt.x) * irho;
-        sf[2 * npairs + k] = (Cy * jt.v + C * jt.y) * irho;
-        sf[3 * npairs + k] = (Cxx * jt.v + 2.0 * Cx * jt.x + C * jt.xx) * ir2;
-        sf[4 * npairs + k] = (Cyy * jt.v + 2.0 * Cy * jt.y + C * jt.yy) * ir2;
-        sf[5 * npairs + k] = (Cxy * jt.v + Cx * jt.y + Cy * jt.x + C * jt.xy) * ir2;
+        o[0] = C * jt.v;
+        o[1] = (Cx * jt.v + C * jt.x) * irho;
+        o[2] = (Cy * jt.v + C * jt.y) * irho;
+        o[3] = (Cxx * jt.v + 2.0 * Cx * jt.x + C * jt.xx) * ir2;
+        o[4] = (Cyy * jt.v + 2.0 * Cy * jt.y + C * jt.yy) * ir2;
+        o[5] = (Cxy * jt.v + Cx * jt.y + Cy * jt.x + C * jt.xy) * ir2;
+    };
+    for (int k = p0 + threadIdx.x; k < p1; k += 2 * RKPM_THREADS) {
+        const int k2 = k + RKPM_THREADS;
+        double a[6], b[6];
+        eval(k, a);
+        eval(min(k2, p1 - 1), b);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) sf[q * npairs + k] = a[q];
+        if (k2 < p1) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) sf[q * npairs + k2] = b[q];
+        }
     }
 }
 
@@ -359,7 +387,8 @@ template <int P, int KER>
 static void run(Context& c, Domain& d, int* st) {
     hipLaunchKernelGGL((k_rkpm<P, KER>), dim3((d.ng + RKPM_THREADS - 1) / RKPM_THREADS), dim3(RKPM_THREADS), 0,
                        c.stream, c.X.p, c.Y.p, c.nn,
-                       d.gx.p, d.gy.p, d.ng, d.off.p, d.ids.p, d.s.p, d.ind.p, c.d_node_sign.p,
+                       d.gx.p, d.gy.p, d.ng, d.off.p, d.ids.p, d.s.p, d.il_row.p, d.il_ids.p, d.il_s.p,
+                       d.ind.p, c.d_node_sign.p,
                        c.batched ? d.d_gsheet.p : nullptr, reinterpret_cast<const SheetDev*>(d.d_sheets.p),
                        d.sf.p, static_cast<int64_t>(d.npairs), st);
 }
@@ -383,6 +412,8 @@ static void enqueue_rkpm(Context& c, Domain& d, int order, int kernel, int* st) 
             throw Error(CVM_E_STATE, "crease table of the domain is out of date");
     if (mc > 0 && (d.ind.n < mc * 6 * static_cast<size_t>(d.ng) || c.d_node_sign.n < mc * static_cast<size_t>(c.nn)))
         throw Error(CVM_E_STATE, "crease indicator buffers are smaller than the crease count");
+    if (d.ng > 0 && d.il_row.n != static_cast<size_t>((d.ng + RKPM_THREADS - 1) / RKPM_THREADS + 1))
+        throw Error(CVM_E_STATE, "interleaved neighbour lists missing for this domain");
     d.sf.alloc(std::max<int64_t>(6 * d.npairs, 1));
     hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, c.stream, st);
     if (d.ng > 0) {
