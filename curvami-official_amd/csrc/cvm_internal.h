@@ -283,6 +283,7 @@ struct Context {
     Plan plan;
     ConstraintLayout cons;
     DBuf<int> d_status;         // RKPM status words, 4 per domain
+    DBuf<double> rk_cj;         // RKPM per-GP coefficient records (grow-only, reused domain by domain)
     // assembly buffers
     DBuf<double> d_x;           // 3 nn displacements
     DBuf<double> en_scr;        // per-GP energies (face membrane, face bending, crease hinge)

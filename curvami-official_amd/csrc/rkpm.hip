@@ -65,11 +65,9 @@ __device__ __forceinline__ void window(double s, double& w, double& dw, double& 
 
 struct Jet { double v, x, y, xx, yy, xy; };
 
-// Window jet of one pair with crease visibility (GlobalRKPM.h:59-112).
+// Window jet of one pair without creases (GlobalRKPM.h:59-84).
 template <int KER>
-__device__ __forceinline__ Jet pair_jet(double xi, double eta, double si, int I, int g, int ng, int nn,
-                                        const double* __restrict__ ind, const int8_t* __restrict__ nsign,
-                                        int nc, double rho) {
+__device__ __forceinline__ Jet window_jet(double xi, double eta, double si) {
     double w, d, dd;
     window<KER>(si, w, d, dd);
     // one reciprocal per pair instead of five divisions
@@ -85,23 +83,39 @@ __device__ __forceinline__ Jet pair_jet(double xi, double eta, double si, int I,
     j.xx = dd * dx * dx + d * d2x;
     j.yy = dd * dy * dy + d * d2y;
     j.xy = (si == 0.0) ? 0.0 : dd * dx * dy + d * (-(xi * eta) * is3);
+    return j;
+}
+
+// One crease's visibility factor W = (1 + sn f) / 2 applied to the jet
+// (GlobalRKPM.h:86-112): sn = the node's side sign, f and its derivatives the
+// GP's indicator values (in 1/rho units).
+__device__ __forceinline__ void crease_cut(Jet& j, double sn, double f, double fx, double fy, double fxx,
+                                           double fyy, double fxy, double rho) {
+    const double W = 0.5 * (1.0 + sn * f);
+    const double hs = 0.5 * sn;
+    const double dWx = hs * fx * rho, dWy = hs * fy * rho;
+    const double ddWxx = hs * fxx * rho * rho;
+    const double ddWyy = hs * fyy * rho * rho;
+    const double ddWxy = hs * fxy * rho * rho;
+    const Jet o = j;
+    j.v = o.v * W;
+    j.x = o.x * W + o.v * dWx;
+    j.y = o.y * W + o.v * dWy;
+    j.xx = o.xx * W + 2.0 * o.x * dWx + o.v * ddWxx;
+    j.yy = o.yy * W + 2.0 * o.y * dWy + o.v * ddWyy;
+    j.xy = o.xy * W + o.x * dWy + o.y * dWx + o.v * ddWxy;
+}
+
+// Window jet of one pair with crease visibility (GlobalRKPM.h:59-112).
+template <int KER>
+__device__ __forceinline__ Jet pair_jet(double xi, double eta, double si, int I, int g, int ng, int nn,
+                                        const double* __restrict__ ind, const int8_t* __restrict__ nsign,
+                                        int nc, double rho) {
+    Jet j = window_jet<KER>(xi, eta, si);
     for (int c = 0; c < nc; ++c) {
         const double sn = static_cast<double>(nsign[static_cast<size_t>(c) * nn + I]);
         const double* ig = ind + static_cast<size_t>(c) * 6 * ng + g;
-        const double f = ig[0];
-        const double W = 0.5 * (1.0 + sn * f);
-        const double hs = 0.5 * sn;
-        const double dWx = hs * ig[ng] * rho, dWy = hs * ig[2 * ng] * rho;
-        const double ddWxx = hs * ig[3 * ng] * rho * rho;
-        const double ddWyy = hs * ig[4 * ng] * rho * rho;
-        const double ddWxy = hs * ig[5 * ng] * rho * rho;
-        const Jet o = j;
-        j.v = o.v * W;
-        j.x = o.x * W + o.v * dWx;
-        j.y = o.y * W + o.v * dWy;
-        j.xx = o.xx * W + 2.0 * o.x * dWx + o.v * ddWxx;
-        j.yy = o.yy * W + 2.0 * o.y * dWy + o.v * ddWyy;
-        j.xy = o.xy * W + o.x * dWy + o.y * dWx + o.v * ddWxy;
+        crease_cut(j, sn, ig[0], ig[ng], ig[2 * ng], ig[3 * ng], ig[4 * ng], ig[5 * ng], rho);
     }
     return j;
 }
@@ -116,29 +130,25 @@ __device__ __forceinline__ void static_for(F&& f) {
     }
 }
 
-constexpr int RKPM_THREADS = RKPM_BLOCK;
+constexpr int RKPM_THREADS = RKPM_BLOCK;   // coefficient kernel: one lane per GP
+constexpr int RKSF_THREADS = 512;          // shape-function kernel: lanes over a block's pairs
 
-template <int P, int KER>
-__global__ __launch_bounds__(RKPM_THREADS, 2) void k_rkpm(
+// Kernel 1: per GP, the moments, the singularity tests and c = M^-1 e0 with
+// its five derivatives, written as a term-major record cj[g][t][6] =
+// (c_t, cx_t, cy_t, cxx_t, cyy_t, cxy_t) for the shape-function kernel.
+template <int P, int KER, bool SPLIT>
+__global__ __launch_bounds__(RKPM_THREADS, 2) void k_rkpm_coef(
     const double* __restrict__ X, const double* __restrict__ Y, int nn,
     const double* __restrict__ gx, const double* __restrict__ gy, int ng,
-    const int* __restrict__ off, const int* __restrict__ ids, const double* __restrict__ sarr,
+    const int* __restrict__ off,
     const int* __restrict__ il_row, const int* __restrict__ il_ids, const double* __restrict__ il_s,
     const double* __restrict__ ind, const int8_t* __restrict__ nsign,
     const int* __restrict__ gsheet, const SheetDev* __restrict__ sheets,
-    double* __restrict__ sf, int64_t npairs, int* __restrict__ status /* n1, first1, n2, first2 */) {
+    double* __restrict__ cjg, int* __restrict__ status /* n1, first1, n2, first2 */) {
     using R = RK<P>;
     constexpr int NP = R::NP, NMON = R::NMON, DM = R::DM, NCJ = R::NCJ;
-    __shared__ double cj_s[RKPM_THREADS * NCJ];
-    __shared__ int off_s[RKPM_THREADS + 1];
-    const int g0 = blockIdx.x * RKPM_THREADS;
-    const int nblk = min(RKPM_THREADS, ng - g0);
-    const int g = g0 + threadIdx.x;
-    // block's pair range: off_s[0..nblk] (nblk + 1 entries, nblk <= blockDim)
-    if (threadIdx.x < nblk) off_s[threadIdx.x] = off[g0 + threadIdx.x];
-    if (threadIdx.x == 0) off_s[nblk] = off[g0 + nblk];
-
-    if (threadIdx.x < nblk) {
+    const int g = blockIdx.x * RKPM_THREADS + threadIdx.x;
+    if (g < ng) {
         const SheetDev& S = sheets[gsheet ? gsheet[g] : 0];
         const double rho = S.rho, irho = S.irho;
         const int nc = S.nc;
@@ -147,27 +157,36 @@ __global__ __launch_bounds__(RKPM_THREADS, 2) void k_rkpm(
         // this lane's pairs in the interleaved lists: stride RKPM_THREADS
         const int* __restrict__ lid = il_ids + static_cast<size_t>(il_row[blockIdx.x]) * RKPM_THREADS + threadIdx.x;
         const double* __restrict__ ls = il_s + static_cast<size_t>(il_row[blockIdx.x]) * RKPM_THREADS + threadIdx.x;
+        // One pass over the GP's pairs: f(window jet, xi, eta) per pair. Software
+        // pipeline: the node and distance of pair t + 1 are loaded a pass ahead.
+        auto for_pairs = [&](auto&& f) {
+            int In = cnt > 0 ? lid[0] : 0;
+            double sn = cnt > 0 ? ls[0] : 0.0;
+            for (int t = 0; t < cnt; ++t) {
+                const int I = In;
+                const double si = sn;
+                const double XI = X[I], YI = Y[I];
+                if (t + 1 < cnt) {
+                    In = lid[static_cast<size_t>(t + 1) * RKPM_THREADS];
+                    sn = ls[static_cast<size_t>(t + 1) * RKPM_THREADS];
+                }
+                const double xi = (px - XI) * irho;
+                const double eta = (py - YI) * irho;
+                f(pair_jet<KER>(xi, eta, si, I, g, ng, nn, ind, nsign, nc, rho), xi, eta);
+            }
+        };
         // ---- moments of monomials ---------------------------------------------
-        double mu[6][NMON];
+        // mu_j for j = phi, phi_x, phi_y and (not SPLIT) phi_xx, phi_yy, phi_xy.
+        // SPLIT: the second-derivative moments enter only as M_xx c, M_yy c,
+        // M_xy c, accumulated contracted in a second pass once c is known
+        // (3 NP instead of 3 NMON registers).
+        constexpr int NJ = SPLIT ? 3 : 6;
+        double mu[NJ][NMON];
         static_for<0, NMON>([&](auto m) {
 #pragma unroll
-            for (int j = 0; j < 6; ++j) mu[j][m.value] = 0.0;
+            for (int j = 0; j < NJ; ++j) mu[j][m.value] = 0.0;
         });
-        // software pipeline: the node and distance of pair t + 1 are loaded one
-        // pass ahead, its coordinates at the start of the pass before it is used
-        int In = cnt > 0 ? lid[0] : 0;
-        double sn = cnt > 0 ? ls[0] : 0.0;
-        for (int t = 0; t < cnt; ++t) {
-            const int I = In;
-            const double si = sn;
-            const double XI = X[I], YI = Y[I];
-            if (t + 1 < cnt) {
-                In = lid[static_cast<size_t>(t + 1) * RKPM_THREADS];
-                sn = ls[static_cast<size_t>(t + 1) * RKPM_THREADS];
-            }
-            const double xi = (px - XI) * irho;
-            const double eta = (py - YI) * irho;
-            const Jet jt = pair_jet<KER>(xi, eta, si, I, g, ng, nn, ind, nsign, nc, rho);
+        for_pairs([&](const Jet& jt, double xi, double eta) {
             double xp[DM + 1], yp[DM + 1];
             xp[0] = 1.0; yp[0] = 1.0;
 #pragma unroll
@@ -181,14 +200,18 @@ __global__ __launch_bounds__(RKPM_THREADS, 2) void k_rkpm(
                     mu[0][m] += jt.v * mon;
                     mu[1][m] += jt.x * mon;
                     mu[2][m] += jt.y * mon;
-                    mu[3][m] += jt.xx * mon;
-                    mu[4][m] += jt.yy * mon;
-                    mu[5][m] += jt.xy * mon;
+                    if constexpr (!SPLIT) {
+                        mu[NJ - 3][m] += jt.xx * mon;
+                        mu[NJ - 2][m] += jt.yy * mon;
+                        mu[NJ - 1][m] += jt.xy * mon;
+                    }
                 }
-        }
+        });
         // Entries of M and its derivatives from the monomial moments (product rule
         // on x^a y^b), all indices compile-time constants.
-        auto MU = [&](int j, int a, int c) -> double { return (a < 0 || c < 0) ? 0.0 : mu[j][midx(a, c)]; };
+        auto MU = [&](int j, int a, int c) -> double {
+            return (a < 0 || c < 0 || j >= NJ) ? 0.0 : mu[j][midx(a, c)];
+        };
         auto Mv = [&](auto I, auto J) { return MU(0, BE<I.value>::x + BE<J.value>::x, BE<I.value>::y + BE<J.value>::y); };
         auto Mxv = [&](auto I, auto J) {
             constexpr int a = BE<I.value>::x + BE<J.value>::x, c = BE<I.value>::y + BE<J.value>::y;
@@ -209,6 +232,19 @@ __global__ __launch_bounds__(RKPM_THREADS, 2) void k_rkpm(
         auto Mxyv = [&](auto I, auto J) {
             constexpr int a = BE<I.value>::x + BE<J.value>::x, c = BE<I.value>::y + BE<J.value>::y;
             return MU(5, a, c) + a * MU(2, a - 1, c) + c * MU(1, a, c - 1) + a * c * MU(0, a - 1, c - 1);
+        };
+        // the same without the second-derivative moment (SPLIT)
+        auto Mxx0v = [&](auto I, auto J) {
+            constexpr int a = BE<I.value>::x + BE<J.value>::x, c = BE<I.value>::y + BE<J.value>::y;
+            return 2 * a * MU(1, a - 1, c) + a * (a - 1) * MU(0, a - 2, c);
+        };
+        auto Myy0v = [&](auto I, auto J) {
+            constexpr int a = BE<I.value>::x + BE<J.value>::x, c = BE<I.value>::y + BE<J.value>::y;
+            return 2 * c * MU(2, a, c - 1) + c * (c - 1) * MU(0, a, c - 2);
+        };
+        auto Mxy0v = [&](auto I, auto J) {
+            constexpr int a = BE<I.value>::x + BE<J.value>::x, c = BE<I.value>::y + BE<J.value>::y;
+            return a * MU(2, a - 1, c) + c * MU(1, a, c - 1) + a * c * MU(0, a - 1, c - 1);
         };
 
         // ---- singularity tests (GlobalRKPM.h:214-220 and :303-305) --------------
@@ -278,9 +314,15 @@ __global__ __launch_bounds__(RKPM_THREADS, 2) void k_rkpm(
         static_for<0, NP>([&](auto I) {
             double sxx = 0.0, syy = 0.0, sxy = 0.0, sx = 0.0, sy = 0.0;
             static_for<0, NP>([&](auto J) {
-                sxx += Mxxv(I, J) * c[J.value];
-                syy += Myyv(I, J) * c[J.value];
-                sxy += Mxyv(I, J) * c[J.value];
+                if constexpr (SPLIT) {
+                    sxx += Mxx0v(I, J) * c[J.value];
+                    syy += Myy0v(I, J) * c[J.value];
+                    sxy += Mxy0v(I, J) * c[J.value];
+                } else {
+                    sxx += Mxxv(I, J) * c[J.value];
+                    syy += Myyv(I, J) * c[J.value];
+                    sxy += Mxyv(I, J) * c[J.value];
+                }
                 sx += Mxv(I, J) * c[J.value];
                 sy += Myv(I, J) * c[J.value];
             });
@@ -296,46 +338,135 @@ __global__ __launch_bounds__(RKPM_THREADS, 2) void k_rkpm(
                 syy += 2.0 * Myv(I, J) * cy[J.value];
                 sxy += Mxv(I, J) * cy[J.value] + Myv(I, J) * cx[J.value];
             });
-            txx[I.value] = -(txx[I.value] + sxx);
-            tyy[I.value] = -(tyy[I.value] + syy);
-            txy[I.value] = -(txy[I.value] + sxy);
+            if constexpr (SPLIT) {
+                txx[I.value] += sxx; tyy[I.value] += syy; txy[I.value] += sxy;
+            } else {
+                txx[I.value] = -(txx[I.value] + sxx);
+                tyy[I.value] = -(tyy[I.value] + syy);
+                txy[I.value] = -(txy[I.value] + sxy);
+            }
         });
+        double2* cj = reinterpret_cast<double2*>(cjg + static_cast<size_t>(g) * NCJ);   // NCJ even
+        if constexpr (SPLIT) {
+            // c, c_x, c_y are final: out of the registers before the second pass
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+                cj[3 * i] = make_double2(c[i], cx[i]);
+                cjg[static_cast<size_t>(g) * NCJ + 6 * i + 2] = cy[i];
+            }
+            // (M_xx c)_I += sum_k phi_xx,k p_I(k) C(k) with C = sum_J c_J p_J, and
+            // likewise yy, xy: the second-derivative moment terms
+            for_pairs([&](const Jet& jt, double xi, double eta) {
+                double xp[P + 1], yp[P + 1];
+                xp[0] = 1.0; yp[0] = 1.0;
+#pragma unroll
+                for (int q = 1; q <= P; ++q) { xp[q] = xp[q - 1] * xi; yp[q] = yp[q - 1] * eta; }
+                double C = 0.0;
+                static_for<0, NP>([&](auto T) { C += c[T.value] * (xp[BE<T.value>::x] * yp[BE<T.value>::y]); });
+                const double ax = jt.xx * C, ay = jt.yy * C, axy = jt.xy * C;
+                static_for<0, NP>([&](auto T) {
+                    const double pt = xp[BE<T.value>::x] * yp[BE<T.value>::y];
+                    txx[T.value] += ax * pt;
+                    tyy[T.value] += ay * pt;
+                    txy[T.value] += axy * pt;
+                });
+            });
+#pragma unroll
+            for (int i = 0; i < NP; ++i) { txx[i] = -txx[i]; tyy[i] = -tyy[i]; txy[i] = -txy[i]; }
+        }
         solve(txx);
         solve(tyy);
         solve(txy);
-        double* cj = cj_s + threadIdx.x * NCJ;
+        if constexpr (SPLIT) {
 #pragma unroll
-        for (int i = 0; i < NP; ++i) {
-            cj[i] = c[i]; cj[NP + i] = cx[i]; cj[2 * NP + i] = cy[i];
-            cj[3 * NP + i] = txx[i]; cj[4 * NP + i] = tyy[i]; cj[5 * NP + i] = txy[i];
+            for (int i = 0; i < NP; ++i) {
+                cjg[static_cast<size_t>(g) * NCJ + 6 * i + 3] = txx[i];
+                cj[3 * i + 2] = make_double2(tyy[i], txy[i]);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+                cj[3 * i] = make_double2(c[i], cx[i]);
+                cj[3 * i + 1] = make_double2(cy[i], txx[i]);
+                cj[3 * i + 2] = make_double2(tyy[i], txy[i]);
+            }
         }
         if (flag) {
             if (flag & 1) { atomicAdd(&status[0], 1); atomicMin(&status[1], g); }
             if (flag & 2) { atomicAdd(&status[2], 1); atomicMin(&status[3], g); }
         }
     }
+}
+
+// Kernel 2: Psi, grad Psi, Hessian of Psi (GlobalRKPM.h:363-397), one lane per
+// pair over the contiguous pair range of a block of RKPM_BLOCK GPs (coalesced
+// stores); the block's GP records are staged in LDS.
+template <int P, int KER>
+__global__ __launch_bounds__(RKSF_THREADS) void k_rkpm_sf(
+    const double* __restrict__ X, const double* __restrict__ Y, int nn,
+    const double* __restrict__ gx, const double* __restrict__ gy, int ng,
+    const int* __restrict__ off, const int* __restrict__ ids, const double* __restrict__ sarr,
+    const double* __restrict__ ind, const int8_t* __restrict__ nsign,
+    const int* __restrict__ gsheet, const SheetDev* __restrict__ sheets,
+    const double* __restrict__ cjg, double* __restrict__ sf, int64_t npairs) {
+    using R = RK<P>;
+    constexpr int NP = R::NP, NCJ = R::NCJ;
+    constexpr int NCS = 2;                      // creases whose GP indicator values are staged
+    __shared__ int off_s[RKPM_BLOCK + 1];
+    __shared__ int nc_s[RKPM_BLOCK];
+    __shared__ double gp_s[5][RKPM_BLOCK];      // gx, gy, rho, 1/rho, 1/rho^2
+    __shared__ double ind_s[NCS * 6][RKPM_BLOCK];
+    __shared__ double2 cj_s[RKPM_BLOCK * NCJ / 2];
+    const int g0 = blockIdx.x * RKPM_BLOCK;
+    const int nblk = min(RKPM_BLOCK, ng - g0);
+    // ---- per-GP data of the block into LDS ------------------------------------
+    for (int i = threadIdx.x; i <= nblk; i += RKSF_THREADS) off_s[i] = off[g0 + i];
+    for (int i = threadIdx.x; i < nblk; i += RKSF_THREADS) {
+        const int g = g0 + i;
+        const SheetDev& S = sheets[gsheet ? gsheet[g] : 0];
+        gp_s[0][i] = gx[g]; gp_s[1][i] = gy[g];
+        gp_s[2][i] = S.rho; gp_s[3][i] = S.irho; gp_s[4][i] = S.irho2;
+        nc_s[i] = S.nc;
+        for (int c = 0; c < min(S.nc, NCS); ++c)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) ind_s[c * 6 + q][i] = ind[(static_cast<size_t>(c) * 6 + q) * ng + g];
+    }
+    // the block's records are contiguous: nblk * NCJ doubles from g0 * NCJ
+    const double2* __restrict__ src = reinterpret_cast<const double2*>(cjg + static_cast<size_t>(g0) * NCJ);
+    for (int i = threadIdx.x; i < nblk * NCJ / 2; i += RKSF_THREADS) cj_s[i] = src[i];
     __syncthreads();
 
-    // ---- Psi, grad Psi, Hessian of Psi (GlobalRKPM.h:363-397), one lane per pair
-    // over the block's contiguous pair range (coalesced stores) ---------------------
     const int p0 = off_s[0], p1 = off_s[nblk];
-    // Two pairs per lane and pass, both evaluated unconditionally (the second
-    // clamped to the range) so that their load chains overlap.
-    auto eval = [&](int k, double (&o)[6]) {
+    int k = p0 + threadIdx.x;
+    // node and distance of the lane's next pair, loaded a pass ahead
+    int In = k < p1 ? ids[k] : 0;
+    double sN = k < p1 ? sarr[k] : 0.0;
+    for (; k < p1; k += RKSF_THREADS) {
+        const int I = In;
+        const double si = sN;
+        if (k + RKSF_THREADS < p1) { In = ids[k + RKSF_THREADS]; sN = sarr[k + RKSF_THREADS]; }
+        const double XI = X[I], YI = Y[I];
         int lo = 0, hi = nblk;                 // GP of pair k: off_s[lo] <= k < off_s[lo + 1]
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
             if (off_s[mid] <= k) lo = mid; else hi = mid;
         }
-        const int gg = g0 + lo;
-        const SheetDev& S = sheets[gsheet ? gsheet[gg] : 0];
-        const double rho = S.rho, irho = S.irho, ir2 = S.irho2;
-        const int nc = S.nc;
-        const double* cj = cj_s + lo * NCJ;
-        const int I = ids[k];
-        const double xi = (gx[gg] - X[I]) * irho;
-        const double eta = (gy[gg] - Y[I]) * irho;
-        const Jet jt = pair_jet<KER>(xi, eta, sarr[k], I, gg, ng, nn, ind, nsign, nc, rho);
+        const double rho = gp_s[2][lo], irho = gp_s[3][lo], ir2 = gp_s[4][lo];
+        const int nc = nc_s[lo];
+        const double2* cj = cj_s + lo * (NCJ / 2);
+        const double xi = (gp_s[0][lo] - XI) * irho;
+        const double eta = (gp_s[1][lo] - YI) * irho;
+        Jet jt = window_jet<KER>(xi, eta, si);
+        for (int c = 0; c < nc; ++c) {
+            const double sn = static_cast<double>(nsign[static_cast<size_t>(c) * nn + I]);
+            if (c < NCS) {
+                crease_cut(jt, sn, ind_s[c * 6][lo], ind_s[c * 6 + 1][lo], ind_s[c * 6 + 2][lo],
+                           ind_s[c * 6 + 3][lo], ind_s[c * 6 + 4][lo], ind_s[c * 6 + 5][lo], rho);
+            } else {
+                const double* ig = ind + static_cast<size_t>(c) * 6 * ng + g0 + lo;
+                crease_cut(jt, sn, ig[0], ig[ng], ig[2 * ng], ig[3 * ng], ig[4 * ng], ig[5 * ng], rho);
+            }
+        }
         double xp[P + 1], yp[P + 1];
         xp[0] = 1.0; yp[0] = 1.0;
 #pragma unroll
@@ -349,8 +480,8 @@ __global__ __launch_bounds__(RKPM_THREADS, 2) void k_rkpm(
             const double pxx = ax > 1 ? ax * (ax - 1) * xp[ax > 1 ? ax - 2 : 0] * yp[ay] : 0.0;
             const double pyy = ay > 1 ? ay * (ay - 1) * xp[ax] * yp[ay > 1 ? ay - 2 : 0] : 0.0;
             const double pxy = (ax > 0 && ay > 0) ? ax * ay * xp[ax > 0 ? ax - 1 : 0] * yp[ay > 0 ? ay - 1 : 0] : 0.0;
-            const double v = cj[t], vx = cj[NP + t], vy = cj[2 * NP + t];
-            const double vxx = cj[3 * NP + t], vyy = cj[4 * NP + t], vxy = cj[5 * NP + t];
+            const double2 r0 = cj[3 * t], r1 = cj[3 * t + 1], r2 = cj[3 * t + 2];
+            const double v = r0.x, vx = r0.y, vy = r1.x, vxx = r1.y, vyy = r2.x, vxy = r2.y;
             C += v * p;
             Cx += vx * p + v * pxv;
             Cy += vy * p + v * pyv;
@@ -358,24 +489,12 @@ __global__ __launch_bounds__(RKPM_THREADS, 2) void k_rkpm(
             Cyy += vyy * p + 2.0 * vy * pyv + v * pyy;
             Cxy += vxy * p + vx * pyv + vy * pxv + v * pxy;
         });
-        o[0] = C * jt.v;
-        o[1] = (Cx * jt.v + C * jt.x) * irho;
-        o[2] = (Cy * jt.v + C * jt.y) * irho;
-        o[3] = (Cxx * jt.v + 2.0 * Cx * jt.x + C * jt.xx) * ir2;
-        o[4] = (Cyy * jt.v + 2.0 * Cy * jt.y + C * jt.yy) * ir2;
-        o[5] = (Cxy * jt.v + Cx * jt.y + Cy * jt.x + C * jt.xy) * ir2;
-    };
-    for (int k = p0 + threadIdx.x; k < p1; k += 2 * RKPM_THREADS) {
-        const int k2 = k + RKPM_THREADS;
-        double a[6], b[6];
-        eval(k, a);
-        eval(min(k2, p1 - 1), b);
-#pragma unroll
-        for (int q = 0; q < 6; ++q) sf[q * npairs + k] = a[q];
-        if (k2 < p1) {
-#pragma unroll
-            for (int q = 0; q < 6; ++q) sf[q * npairs + k2] = b[q];
-        }
+        sf[k] = C * jt.v;
+        sf[npairs + k] = (Cx * jt.v + C * jt.x) * irho;
+        sf[2 * npairs + k] = (Cy * jt.v + C * jt.y) * irho;
+        sf[3 * npairs + k] = (Cxx * jt.v + 2.0 * Cx * jt.x + C * jt.xx) * ir2;
+        sf[4 * npairs + k] = (Cyy * jt.v + 2.0 * Cy * jt.y + C * jt.yy) * ir2;
+        sf[5 * npairs + k] = (Cxy * jt.v + Cx * jt.y + Cy * jt.x + C * jt.xy) * ir2;
     }
 }
 
@@ -383,14 +502,26 @@ __global__ void k_status_init(int* s) {
     s[0] = 0; s[1] = 0x7fffffff; s[2] = 0; s[3] = 0x7fffffff;
 }
 
+// Second-derivative moments contracted in a second pass from this order on:
+// 6 NMON accumulators (order 3: 168 doubles) do not fit the register file.
+#ifndef RKPM_SPLIT
+#define RKPM_SPLIT(P) ((P) >= 3)
+#endif
+
 template <int P, int KER>
 static void run(Context& c, Domain& d, int* st) {
-    hipLaunchKernelGGL((k_rkpm<P, KER>), dim3((d.ng + RKPM_THREADS - 1) / RKPM_THREADS), dim3(RKPM_THREADS), 0,
-                       c.stream, c.X.p, c.Y.p, c.nn,
-                       d.gx.p, d.gy.p, d.ng, d.off.p, d.ids.p, d.s.p, d.il_row.p, d.il_ids.p, d.il_s.p,
-                       d.ind.p, c.d_node_sign.p,
-                       c.batched ? d.d_gsheet.p : nullptr, reinterpret_cast<const SheetDev*>(d.d_sheets.p),
-                       d.sf.p, static_cast<int64_t>(d.npairs), st);
+    constexpr int NCJ = RK<P>::NCJ;
+    c.rk_cj.ensure(static_cast<size_t>(d.ng) * NCJ);
+    const dim3 grid((d.ng + RKPM_BLOCK - 1) / RKPM_BLOCK);
+    const SheetDev* tab = reinterpret_cast<const SheetDev*>(d.d_sheets.p);
+    const int* gsh = c.batched ? d.d_gsheet.p : nullptr;
+    hipLaunchKernelGGL((k_rkpm_coef<P, KER, RKPM_SPLIT(P)>), grid, dim3(RKPM_THREADS), 0, c.stream, c.X.p, c.Y.p, c.nn,
+                       d.gx.p, d.gy.p, d.ng, d.off.p, d.il_row.p, d.il_ids.p, d.il_s.p,
+                       d.ind.p, c.d_node_sign.p, gsh, tab, c.rk_cj.p, st);
+    CVM_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_rkpm_sf<P, KER>), grid, dim3(RKSF_THREADS), 0, c.stream, c.X.p, c.Y.p, c.nn,
+                       d.gx.p, d.gy.p, d.ng, d.off.p, d.ids.p, d.s.p, d.ind.p, c.d_node_sign.p, gsh, tab,
+                       c.rk_cj.p, d.sf.p, static_cast<int64_t>(d.npairs));
 }
 
 static void check_args(int order, int kernel) {
