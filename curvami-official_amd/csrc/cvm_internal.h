@@ -133,6 +133,7 @@ struct Domain {
     DBuf<int> il_row;         // nblocks + 1 (rows of RKPM_BLOCK slots)
     DBuf<int> il_ids;         // il_row[nblocks] * RKPM_BLOCK
     DBuf<double> il_s;
+    DBuf<uint8_t> pair_lgp;   // npairs: GP of the pair minus the first GP of its RKPM block
     // shape functions: psi, dpx, dpy, dxx, dyy, dxy (SoA, npairs each)
     DBuf<double> sf;          // 6 * npairs
     int order = 0, kernel = 0;

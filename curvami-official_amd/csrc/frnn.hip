@@ -245,7 +245,7 @@ __global__ void k_set_zero_int(int* p) { p[0] = 0; }
 __global__ __launch_bounds__(RKPM_BLOCK) void k_interleave(const int* __restrict__ off, const int* __restrict__ ids,
                                                            const double* __restrict__ s, int ng,
                                                            const int* __restrict__ row, int* __restrict__ il_ids,
-                                                           double* __restrict__ il_s) {
+                                                           double* __restrict__ il_s, uint8_t* __restrict__ lgp) {
     const int g = blockIdx.x * RKPM_BLOCK + threadIdx.x;
     if (g >= ng) return;
     const int b = off[g], n = off[g + 1] - b;
@@ -253,6 +253,7 @@ __global__ __launch_bounds__(RKPM_BLOCK) void k_interleave(const int* __restrict
     for (int t = 0; t < n; ++t) {
         il_ids[base + static_cast<size_t>(t) * RKPM_BLOCK] = ids[b + t];
         il_s[base + static_cast<size_t>(t) * RKPM_BLOCK] = s[b + t];
+        lgp[b + t] = static_cast<uint8_t>(threadIdx.x);
     }
 }
 
@@ -317,7 +318,7 @@ void launch_frnn(Context& c, Domain& d, double rho) {
     d.off.alloc(d.ng + 1);
     d.h_off.assign(d.ng + 1, 0);
     d.ind_creases = -1;
-    d.il_row.release(); d.il_ids.release(); d.il_s.release();
+    d.il_row.release(); d.il_ids.release(); d.il_s.release(); d.pair_lgp.release();
     if (d.ng == 0) {
         d.npairs = 0;
         CVM_HIP(hipMemsetAsync(d.off.p, 0, sizeof(int), c.stream));
@@ -368,8 +369,10 @@ void launch_frnn(Context& c, Domain& d, double rho) {
     d.il_row.upload(row, c.stream);
     d.il_ids.alloc(slots);
     d.il_s.alloc(slots);
+    d.pair_lgp.alloc(std::max<int64_t>(d.npairs, 1));
+    static_assert(RKPM_BLOCK <= 256, "GP-in-block index is a byte");
     hipLaunchKernelGGL(k_interleave, dim3(nblk), dim3(RKPM_BLOCK), 0, c.stream, d.off.p, d.ids.p, d.s.p, d.ng,
-                       d.il_row.p, d.il_ids.p, d.il_s.p);
+                       d.il_row.p, d.il_ids.p, d.il_s.p, d.pair_lgp.p);
     CVM_HIP(hipGetLastError());
     d.h_ids.resize(d.npairs);
     if (d.npairs) CVM_HIP(hipMemcpyAsync(d.h_ids.data(), d.ids.p, d.npairs * sizeof(int), hipMemcpyDeviceToHost, c.stream));

@@ -160,15 +160,23 @@ __global__ __launch_bounds__(RKPM_THREADS, 2) void k_rkpm_coef(
         // One pass over the GP's pairs: f(window jet, xi, eta) per pair. Software
         // pipeline: the node and distance of pair t + 1 are loaded a pass ahead.
         auto for_pairs = [&](auto&& f) {
+            // pair t + 1: node, distance and coordinates; pair t + 2: node, distance
             int In = cnt > 0 ? lid[0] : 0;
             double sn = cnt > 0 ? ls[0] : 0.0;
+            double Xn = cnt > 0 ? X[In] : 0.0, Yn = cnt > 0 ? Y[In] : 0.0;
+            int In2 = cnt > 1 ? lid[RKPM_THREADS] : 0;
+            double sn2 = cnt > 1 ? ls[RKPM_THREADS] : 0.0;
             for (int t = 0; t < cnt; ++t) {
                 const int I = In;
                 const double si = sn;
-                const double XI = X[I], YI = Y[I];
+                const double XI = Xn, YI = Yn;
                 if (t + 1 < cnt) {
-                    In = lid[static_cast<size_t>(t + 1) * RKPM_THREADS];
-                    sn = ls[static_cast<size_t>(t + 1) * RKPM_THREADS];
+                    In = In2; sn = sn2;
+                    Xn = X[In]; Yn = Y[In];
+                    if (t + 2 < cnt) {
+                        In2 = lid[static_cast<size_t>(t + 2) * RKPM_THREADS];
+                        sn2 = ls[static_cast<size_t>(t + 2) * RKPM_THREADS];
+                    }
                 }
                 const double xi = (px - XI) * irho;
                 const double eta = (py - YI) * irho;
@@ -402,17 +410,16 @@ __global__ __launch_bounds__(RKPM_THREADS, 2) void k_rkpm_coef(
 // pair over the contiguous pair range of a block of RKPM_BLOCK GPs (coalesced
 // stores); the block's GP records are staged in LDS.
 template <int P, int KER>
-__global__ __launch_bounds__(RKSF_THREADS) void k_rkpm_sf(
+__global__ __launch_bounds__(RKSF_THREADS) __attribute__((amdgpu_waves_per_eu(6))) void k_rkpm_sf(
     const double* __restrict__ X, const double* __restrict__ Y, int nn,
     const double* __restrict__ gx, const double* __restrict__ gy, int ng,
     const int* __restrict__ off, const int* __restrict__ ids, const double* __restrict__ sarr,
-    const double* __restrict__ ind, const int8_t* __restrict__ nsign,
+    const uint8_t* __restrict__ lgp, const double* __restrict__ ind, const int8_t* __restrict__ nsign,
     const int* __restrict__ gsheet, const SheetDev* __restrict__ sheets,
     const double* __restrict__ cjg, double* __restrict__ sf, int64_t npairs) {
     using R = RK<P>;
     constexpr int NP = R::NP, NCJ = R::NCJ;
     constexpr int NCS = 2;                      // creases whose GP indicator values are staged
-    __shared__ int off_s[RKPM_BLOCK + 1];
     __shared__ int nc_s[RKPM_BLOCK];
     __shared__ double gp_s[5][RKPM_BLOCK];      // gx, gy, rho, 1/rho, 1/rho^2
     __shared__ double ind_s[NCS * 6][RKPM_BLOCK];
@@ -420,7 +427,6 @@ __global__ __launch_bounds__(RKSF_THREADS) void k_rkpm_sf(
     const int g0 = blockIdx.x * RKPM_BLOCK;
     const int nblk = min(RKPM_BLOCK, ng - g0);
     // ---- per-GP data of the block into LDS ------------------------------------
-    for (int i = threadIdx.x; i <= nblk; i += RKSF_THREADS) off_s[i] = off[g0 + i];
     for (int i = threadIdx.x; i < nblk; i += RKSF_THREADS) {
         const int g = g0 + i;
         const SheetDev& S = sheets[gsheet ? gsheet[g] : 0];
@@ -436,21 +442,20 @@ __global__ __launch_bounds__(RKSF_THREADS) void k_rkpm_sf(
     for (int i = threadIdx.x; i < nblk * NCJ / 2; i += RKSF_THREADS) cj_s[i] = src[i];
     __syncthreads();
 
-    const int p0 = off_s[0], p1 = off_s[nblk];
+    const int p0 = off[g0], p1 = off[g0 + nblk];
     int k = p0 + threadIdx.x;
-    // node and distance of the lane's next pair, loaded a pass ahead
+    // node, distance and GP of the lane's next pair, loaded a pass ahead
     int In = k < p1 ? ids[k] : 0;
     double sN = k < p1 ? sarr[k] : 0.0;
+    int Ln = k < p1 ? lgp[k] : 0;
     for (; k < p1; k += RKSF_THREADS) {
         const int I = In;
         const double si = sN;
-        if (k + RKSF_THREADS < p1) { In = ids[k + RKSF_THREADS]; sN = sarr[k + RKSF_THREADS]; }
-        const double XI = X[I], YI = Y[I];
-        int lo = 0, hi = nblk;                 // GP of pair k: off_s[lo] <= k < off_s[lo + 1]
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (off_s[mid] <= k) lo = mid; else hi = mid;
+        const int lo = Ln;                     // GP of pair k within the block
+        if (k + RKSF_THREADS < p1) {
+            In = ids[k + RKSF_THREADS]; sN = sarr[k + RKSF_THREADS]; Ln = lgp[k + RKSF_THREADS];
         }
+        const double XI = X[I], YI = Y[I];
         const double rho = gp_s[2][lo], irho = gp_s[3][lo], ir2 = gp_s[4][lo];
         const int nc = nc_s[lo];
         const double2* cj = cj_s + lo * (NCJ / 2);
@@ -520,7 +525,7 @@ static void run(Context& c, Domain& d, int* st) {
                        d.ind.p, c.d_node_sign.p, gsh, tab, c.rk_cj.p, st);
     CVM_HIP(hipGetLastError());
     hipLaunchKernelGGL((k_rkpm_sf<P, KER>), grid, dim3(RKSF_THREADS), 0, c.stream, c.X.p, c.Y.p, c.nn,
-                       d.gx.p, d.gy.p, d.ng, d.off.p, d.ids.p, d.s.p, d.ind.p, c.d_node_sign.p, gsh, tab,
+                       d.gx.p, d.gy.p, d.ng, d.off.p, d.ids.p, d.s.p, d.pair_lgp.p, d.ind.p, c.d_node_sign.p, gsh, tab,
                        c.rk_cj.p, d.sf.p, static_cast<int64_t>(d.npairs));
 }
 
