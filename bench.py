@@ -433,6 +433,25 @@ def main():
                                          f"assembleSystem; 1 thread: {res[1][0]:.0f} GP/s",
                                "seconds": {str(T): [round(v[1], 3), round(v[2], 3)] for T, v in res.items()},
                                "cpu": platform.processor() or platform.machine()}
+    # the reference's default basis (cubic, order 3) on the face domain of the
+    # same sheet: shape functions only, device time per launch
+    log("main: cubic-basis shape functions (face domain)")
+    try:
+        p.shape_functions(0, 3, sh.kernel)
+        p.synchronize()
+        p.set_profiling(True)
+        p.reset_profiling()
+        ncub = max(3, args.steps // 2)
+        for _ in range(ncub):
+            p.shape_functions(0, 3, sh.kernel)
+        p.synchronize()
+        cub = p.kernel_stats().get("rkpm_face", (0.0, 1))
+        p.set_profiling(False)
+        cub_ms = cub[0] / max(cub[1], 1)
+        out["cubic_face_shape_functions"] = {"order": 3, "face_gps": ng, "pairs": int(p.npairs[0]),
+                                             "ms": round(cub_ms, 4), "gp_per_s": ng / (cub_ms * 1e-3)}
+    except RuntimeError as e:
+        out["cubic_face_shape_functions"] = {"error": str(e)}
     for ptr in (xd, Fd, Kd):
         hip.free(ptr)
     p.close()
