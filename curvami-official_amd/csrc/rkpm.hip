@@ -136,8 +136,12 @@ constexpr int RKSF_THREADS = 512;          // shape-function kernel: lanes over 
 // Kernel 1: per GP, the moments, the singularity tests and c = M^-1 e0 with
 // its five derivatives, written as a term-major record cj[g][t][6] =
 // (c_t, cx_t, cy_t, cxx_t, cyy_t, cxy_t) for the shape-function kernel.
+// Orders 1-2: at least 2 waves per SIMD (256 VGPRs). Order 3: one wave per
+// SIMD, whose 512 registers (256 VGPRs + 256 AGPRs) hold the per-GP solve's
+// working set that would otherwise spill to scratch memory.
 template <int P, int KER, bool SPLIT>
-__global__ __launch_bounds__(RKPM_THREADS, 2) void k_rkpm_coef(
+__global__ __launch_bounds__(RKPM_THREADS) __attribute__((amdgpu_waves_per_eu(P >= 3 ? 1 : 2, P >= 3 ? 1 : 8)))
+void k_rkpm_coef(
     const double* __restrict__ X, const double* __restrict__ Y, int nn,
     const double* __restrict__ gx, const double* __restrict__ gy, int ng,
     const int* __restrict__ off,
