@@ -376,14 +376,21 @@ struct FoldDev {
 constexpr int FGP_REC = 2 + 144;
 constexpr int FND_REC = 7;
 
-__global__ void k_fold(int nfgp, const FoldDev* __restrict__ folds, const int* __restrict__ fgp_fold,
+// One wave (workgroup) per crease GP: both one-sided F0 by wave sums over the
+// GP's neighbours, the angle and its gradient (every lane), the FD Hessian
+// columns on lanes 0-11, then the per-node aggregated gradients, the left
+// side's occurrences before the right side's (a node seen from both sides gets
+// its left term first).
+__global__ __launch_bounds__(64) void k_fold(int nfgp, const FoldDev* __restrict__ folds, const int* __restrict__ fgp_fold,
                        const int* __restrict__ fgp_local, const int* __restrict__ fgp_noff,
                        const int* __restrict__ lpo, const int* __restrict__ rpo,
                        const int* __restrict__ lpos, const int* __restrict__ rpos,
                        const double* __restrict__ xu, int exact, double* __restrict__ fgp_rec,
                        double* __restrict__ Gnd) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nfgp) return;
+    __shared__ double Hs[144];
+    const int q = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (q >= nfgp) return;                    // whole workgroup
     const FoldDev fd = folds[fgp_fold[q]];
     const int i = fgp_local[q];
     double F[12];
@@ -393,15 +400,18 @@ __global__ void k_fold(int nfgp, const FoldDev* __restrict__ folds, const int* _
         const int* of = side ? fd.roff : fd.loff;
         const int* id = side ? fd.rids : fd.lids;
         const int64_t np = side ? fd.rnp : fd.lnp;
-        double a1[3] = {1.0, 0.0, 0.0}, a2[3] = {0.0, 1.0, 0.0};
-        for (int k = of[i]; k < of[i + 1]; ++k) {
+        double p1[3] = {0.0, 0.0, 0.0}, p2[3] = {0.0, 0.0, 0.0};
+        for (int k = of[i] + lane; k < of[i + 1]; k += 64) {
             const int v = id[k];
             const double px = sf[np + k], py = sf[2 * np + k];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) { a1[d] += xu[3 * v + d] * px; a2[d] += xu[3 * v + d] * py; }
+            for (int d = 0; d < 3; ++d) { p1[d] += xu[3 * v + d] * px; p2[d] += xu[3 * v + d] * py; }
         }
 #pragma unroll
-        for (int d = 0; d < 3; ++d) { F[6 * side + d] = a1[d]; F[6 * side + 3 + d] = a2[d]; }
+        for (int d = 0; d < 3; ++d) {
+            F[6 * side + d] = (d == 0 ? 1.0 : 0.0) + wave_sum(p1[d]);
+            F[6 * side + 3 + d] = (d == 1 ? 1.0 : 0.0) + wave_sum(p2[d]);
+        }
     }
     double grad[12];
     const double tx = fd.tx[i], ty = fd.ty[i];
@@ -410,41 +420,43 @@ __global__ void k_fold(int nfgp, const FoldDev* __restrict__ folds, const int* _
     const double W = fd.w[i];
     double* rec = fgp_rec + static_cast<int64_t>(FGP_REC) * q;
     const double Wm = W * (fd.Kt * (theta - th0));
-    rec[0] = W * fd.Kt;
-    rec[1] = Wm;
+    if (lane == 0) { rec[0] = W * fd.Kt; rec[1] = Wm; }
     if (exact && fabs(Wm) > 0.0) {
         // Constraints.h:402-416: central differences of the analytic gradient,
-        // step 1e-6 max(1, |F|_inf), then symmetrised
+        // step 1e-6 max(1, |F|_inf), then symmetrised; lane k < 12 differentiates
+        // along F_k
         double fmax = 0.0;
         for (int k = 0; k < 12; ++k) fmax = fmax > fabs(F[k]) ? fmax : fabs(F[k]);
         const double hs = 1e-6 * (fmax > 1.0 ? fmax : 1.0);
-        double* H = rec + 2;
-        for (int k = 0; k < 12; ++k) {
+        if (lane < 12) {
+            const int k = lane;
             double Fp[12], Fm[12], gp[12], gm[12];
             for (int r = 0; r < 12; ++r) { Fp[r] = F[r]; Fm[r] = F[r]; }
             Fp[k] += hs; Fm[k] -= hs;
             fold_angle_grad(Fp, tx, ty, gp);
             fold_angle_grad(Fm, tx, ty, gm);
-            for (int r = 0; r < 12; ++r) H[12 * r + k] = (gp[r] - gm[r]) / (2.0 * hs);
+            for (int r = 0; r < 12; ++r) Hs[12 * r + k] = (gp[r] - gm[r]) / (2.0 * hs);
         }
-        for (int r = 0; r < 12; ++r)
-            for (int k = r + 1; k < 12; ++k) {
-                const double m = 0.5 * (H[12 * r + k] + H[12 * k + r]);
-                H[12 * r + k] = m; H[12 * k + r] = m;
-            }
+        __syncthreads();
+        for (int e = lane; e < 144; e += 64) {
+            const int r = e / 12, k = e % 12;
+            rec[2 + e] = (r == k) ? Hs[e] : 0.5 * (Hs[12 * r + k] + Hs[12 * k + r]);
+        }
     } else {
-        for (int k = 0; k < 144; ++k) rec[2 + k] = 0.0;
+        for (int e = lane; e < 144; e += 64) rec[2 + e] = 0.0;
     }
     const int nb0 = fgp_noff[q], nb1 = fgp_noff[q + 1];
-    for (int k = nb0; k < nb1; ++k)
+    for (int k = nb0 + lane; k < nb1; k += 64)
         for (int e = 0; e < FND_REC; ++e) Gnd[static_cast<int64_t>(FND_REC) * k + e] = 0.0;
     for (int side = 0; side < 2; ++side) {
+        __syncthreads();   // the previous writes of the GP's node records are visible
         const double* sf = side ? fd.rsf : fd.lsf;
         const int* of = side ? fd.roff : fd.loff;
         const int64_t np = side ? fd.rnp : fd.lnp;
         const int* pos = side ? (rpos + rpo[q]) : (lpos + lpo[q]);
         const int blk = 6 * side;
-        for (int k = of[i]; k < of[i + 1]; ++k) {
+        // one side's neighbours are distinct nodes: no two lanes share a record
+        for (int k = of[i] + lane; k < of[i + 1]; k += 64) {
             const double dX = sf[np + k], dY = sf[2 * np + k];
             double* G = Gnd + static_cast<int64_t>(FND_REC) * (nb0 + pos[k - of[i]]);
 #pragma unroll
@@ -654,7 +666,7 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
     tend(slot);
     if (folds) {
         slot = tbeg("fold_hinge");
-        hipLaunchKernelGGL(k_fold, dim3((P.nfgp + 63) / 64), dim3(64), 0, st, P.nfgp,
+        if (P.nfgp > 0) hipLaunchKernelGGL(k_fold, dim3(P.nfgp), dim3(64), 0, st, P.nfgp,
                            reinterpret_cast<const FoldDev*>(P.d_folddev.p), P.d_fgp_fold.p, P.d_fgp_local.p,
                            P.d_fgp_noff.p, P.d_fgp_lpo.p, P.d_fgp_rpo.p, P.d_fgp_lpos.p, P.d_fgp_rpos.p, xdev,
                            exact ? 1 : 0, ws.fold_gp.p, ws.fold_nd.p);
