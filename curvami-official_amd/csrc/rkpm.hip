@@ -20,6 +20,7 @@
 //  * the crease visibility (:86-112) uses per-GP indicator values and per-node
 //    side signs computed once (the reference recomputes both per pair).
 #include <algorithm>
+#include <cstring>
 
 #include "cvm_internal.h"
 
@@ -133,6 +134,27 @@ __device__ __forceinline__ void static_for(F&& f) {
 constexpr int RKPM_THREADS = RKPM_BLOCK;   // coefficient kernel: one lane per GP
 constexpr int RKSF_THREADS = 512;          // shape-function kernel: lanes over a block's pairs
 
+// One domain of a grouped launch: the domains of a call share one coefficient
+// launch and one shape-function launch; block b belongs to the domain with the
+// largest blk0 <= b (domains without GPs are not listed).
+struct RkDom {
+    const double* gx; const double* gy; const int* off;
+    const int* il_row; const int* il_ids; const double* il_s;
+    const int* ids; const double* s; const uint8_t* lgp;
+    const double* ind; const int* gsheet; const SheetDev* sheets;
+    double* sf; double* cj; int* status;
+    int64_t npairs; int ng; int blk0;
+};
+
+__device__ __forceinline__ const RkDom& rk_domain(const RkDom* __restrict__ doms, int nd, int b) {
+    int lo = 0, hi = nd;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (doms[mid].blk0 <= b) lo = mid; else hi = mid;
+    }
+    return doms[lo];
+}
+
 // Kernel 1: per GP, the moments, the singularity tests and c = M^-1 e0 with
 // its five derivatives, written as a term-major record cj[g][t][6] =
 // (c_t, cx_t, cy_t, cxx_t, cyy_t, cxy_t) for the shape-function kernel.
@@ -141,17 +163,22 @@ constexpr int RKSF_THREADS = 512;          // shape-function kernel: lanes over 
 // working set that would otherwise spill to scratch memory.
 template <int P, int KER, bool SPLIT>
 __global__ __launch_bounds__(RKPM_THREADS) __attribute__((amdgpu_waves_per_eu(P >= 3 ? 1 : 2, P >= 3 ? 1 : 8)))
-void k_rkpm_coef(
-    const double* __restrict__ X, const double* __restrict__ Y, int nn,
-    const double* __restrict__ gx, const double* __restrict__ gy, int ng,
-    const int* __restrict__ off,
-    const int* __restrict__ il_row, const int* __restrict__ il_ids, const double* __restrict__ il_s,
-    const double* __restrict__ ind, const int8_t* __restrict__ nsign,
-    const int* __restrict__ gsheet, const SheetDev* __restrict__ sheets,
-    double* __restrict__ cjg, int* __restrict__ status /* n1, first1, n2, first2 */) {
+void k_rkpm_coef(const double* __restrict__ X, const double* __restrict__ Y, int nn,
+                 const int8_t* __restrict__ nsign, const RkDom* __restrict__ doms, int nd) {
     using R = RK<P>;
     constexpr int NP = R::NP, NMON = R::NMON, DM = R::DM, NCJ = R::NCJ;
-    const int g = blockIdx.x * RKPM_THREADS + threadIdx.x;
+    const RkDom& D = rk_domain(doms, nd, blockIdx.x);
+    const int lb = blockIdx.x - D.blk0;
+    const double* __restrict__ gx = D.gx;
+    const double* __restrict__ gy = D.gy;
+    const int* __restrict__ off = D.off;
+    const double* __restrict__ ind = D.ind;
+    const int* __restrict__ gsheet = D.gsheet;
+    const SheetDev* __restrict__ sheets = D.sheets;
+    double* __restrict__ cjg = D.cj;
+    int* __restrict__ status = D.status;       // n1, first1, n2, first2
+    const int ng = D.ng;
+    const int g = lb * RKPM_THREADS + threadIdx.x;
     if (g < ng) {
         const SheetDev& S = sheets[gsheet ? gsheet[g] : 0];
         const double rho = S.rho, irho = S.irho;
@@ -159,8 +186,8 @@ void k_rkpm_coef(
         const double px = gx[g], py = gy[g];
         const int cnt = off[g + 1] - off[g];
         // this lane's pairs in the interleaved lists: stride RKPM_THREADS
-        const int* __restrict__ lid = il_ids + static_cast<size_t>(il_row[blockIdx.x]) * RKPM_THREADS + threadIdx.x;
-        const double* __restrict__ ls = il_s + static_cast<size_t>(il_row[blockIdx.x]) * RKPM_THREADS + threadIdx.x;
+        const int* __restrict__ lid = D.il_ids + static_cast<size_t>(D.il_row[lb]) * RKPM_THREADS + threadIdx.x;
+        const double* __restrict__ ls = D.il_s + static_cast<size_t>(D.il_row[lb]) * RKPM_THREADS + threadIdx.x;
         // One pass over the GP's pairs: f(window jet, xi, eta) per pair. Software
         // pipeline: the node and distance of pair t + 1 are loaded a pass ahead.
         auto for_pairs = [&](auto&& f) {
@@ -415,20 +442,31 @@ void k_rkpm_coef(
 // stores); the block's GP records are staged in LDS.
 template <int P, int KER>
 __global__ __launch_bounds__(RKSF_THREADS) __attribute__((amdgpu_waves_per_eu(6))) void k_rkpm_sf(
-    const double* __restrict__ X, const double* __restrict__ Y, int nn,
-    const double* __restrict__ gx, const double* __restrict__ gy, int ng,
-    const int* __restrict__ off, const int* __restrict__ ids, const double* __restrict__ sarr,
-    const uint8_t* __restrict__ lgp, const double* __restrict__ ind, const int8_t* __restrict__ nsign,
-    const int* __restrict__ gsheet, const SheetDev* __restrict__ sheets,
-    const double* __restrict__ cjg, double* __restrict__ sf, int64_t npairs) {
+    const double* __restrict__ X, const double* __restrict__ Y, int nn, const int8_t* __restrict__ nsign,
+    const RkDom* __restrict__ doms, int nd) {
     using R = RK<P>;
     constexpr int NP = R::NP, NCJ = R::NCJ;
+    const RkDom& D = rk_domain(doms, nd, blockIdx.x);
+    const int lb = blockIdx.x - D.blk0;
+    const double* __restrict__ gx = D.gx;
+    const double* __restrict__ gy = D.gy;
+    const int* __restrict__ off = D.off;
+    const int* __restrict__ ids = D.ids;
+    const double* __restrict__ sarr = D.s;
+    const uint8_t* __restrict__ lgp = D.lgp;
+    const double* __restrict__ ind = D.ind;
+    const int* __restrict__ gsheet = D.gsheet;
+    const SheetDev* __restrict__ sheets = D.sheets;
+    const double* __restrict__ cjg = D.cj;
+    double* __restrict__ sf = D.sf;
+    const int64_t npairs = D.npairs;
+    const int ng = D.ng;
     constexpr int NCS = 2;                      // creases whose GP indicator values are staged
     __shared__ int nc_s[RKPM_BLOCK];
     __shared__ double gp_s[5][RKPM_BLOCK];      // gx, gy, rho, 1/rho, 1/rho^2
     __shared__ double ind_s[NCS * 6][RKPM_BLOCK];
     __shared__ double2 cj_s[RKPM_BLOCK * NCJ / 2];
-    const int g0 = blockIdx.x * RKPM_BLOCK;
+    const int g0 = lb * RKPM_BLOCK;
     const int nblk = min(RKPM_BLOCK, ng - g0);
     // ---- per-GP data of the block into LDS ------------------------------------
     for (int i = threadIdx.x; i < nblk; i += RKSF_THREADS) {
@@ -507,8 +545,11 @@ __global__ __launch_bounds__(RKSF_THREADS) __attribute__((amdgpu_waves_per_eu(6)
     }
 }
 
-__global__ void k_status_init(int* s) {
-    s[0] = 0; s[1] = 0x7fffffff; s[2] = 0; s[3] = 0x7fffffff;
+// Status words (n1, first1, n2, first2) of n domains.
+__global__ void k_status_init(int* s, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        s[4 * i] = 0; s[4 * i + 1] = 0x7fffffff; s[4 * i + 2] = 0; s[4 * i + 3] = 0x7fffffff;
+    }
 }
 
 // Second-derivative moments contracted in a second pass from this order on:
@@ -518,19 +559,13 @@ __global__ void k_status_init(int* s) {
 #endif
 
 template <int P, int KER>
-static void run(Context& c, Domain& d, int* st) {
-    constexpr int NCJ = RK<P>::NCJ;
-    c.rk_cj.ensure(static_cast<size_t>(d.ng) * NCJ);
-    const dim3 grid((d.ng + RKPM_BLOCK - 1) / RKPM_BLOCK);
-    const SheetDev* tab = reinterpret_cast<const SheetDev*>(d.d_sheets.p);
-    const int* gsh = c.batched ? d.d_gsheet.p : nullptr;
-    hipLaunchKernelGGL((k_rkpm_coef<P, KER, RKPM_SPLIT(P)>), grid, dim3(RKPM_THREADS), 0, c.stream, c.X.p, c.Y.p, c.nn,
-                       d.gx.p, d.gy.p, d.ng, d.off.p, d.il_row.p, d.il_ids.p, d.il_s.p,
-                       d.ind.p, c.d_node_sign.p, gsh, tab, c.rk_cj.p, st);
+static void run(Context& c, const RkDom* tab, int nd, int nblocks) {
+    hipLaunchKernelGGL((k_rkpm_coef<P, KER, RKPM_SPLIT(P)>), dim3(nblocks), dim3(RKPM_THREADS), 0, c.stream, c.X.p,
+                       c.Y.p, c.nn, c.d_node_sign.p, tab, nd);
     CVM_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_rkpm_sf<P, KER>), grid, dim3(RKSF_THREADS), 0, c.stream, c.X.p, c.Y.p, c.nn,
-                       d.gx.p, d.gy.p, d.ng, d.off.p, d.ids.p, d.s.p, d.pair_lgp.p, d.ind.p, c.d_node_sign.p, gsh, tab,
-                       c.rk_cj.p, d.sf.p, static_cast<int64_t>(d.npairs));
+    hipLaunchKernelGGL((k_rkpm_sf<P, KER>), dim3(nblocks), dim3(RKSF_THREADS), 0, c.stream, c.X.p, c.Y.p, c.nn,
+                       c.d_node_sign.p, tab, nd);
+    CVM_HIP(hipGetLastError());
 }
 
 static void check_args(int order, int kernel) {
@@ -539,12 +574,12 @@ static void check_args(int order, int kernel) {
         throw Error(CVM_E_INVALID, "unknown window kernel");
 }
 
-// Enqueues the shape functions of one domain; its status (n1, first1, n2,
-// first2) goes to st[0..3] on the device.
-static void enqueue_rkpm(Context& c, Domain& d, int order, int kernel, int* st) {
+// Host checks and per-domain preparation (indicator values, shape-function
+// buffer) before a domain joins a launch.
+static void prepare_rkpm(Context& c, Domain& d) {
     if (d.npairs < 0) throw Error(CVM_E_STATE, "neighbours not computed for this domain");
     launch_indicator_gp(c, d);
-    // the kernel reads ind[(lc * 6 + k) * ng + g] and sign[lc * nn + I] for
+    // the kernels read ind[(lc * 6 + k) * ng + g] and sign[lc * nn + I] for
     // lc < its sheet's crease count: both buffers must hold that many creases
     const size_t mc = static_cast<size_t>(c.max_sheet_creases());
     for (const SheetDev& t : d.h_sheets)
@@ -554,22 +589,66 @@ static void enqueue_rkpm(Context& c, Domain& d, int order, int kernel, int* st) 
         throw Error(CVM_E_STATE, "crease indicator buffers are smaller than the crease count");
     if (d.ng > 0 && d.il_row.n != static_cast<size_t>((d.ng + RKPM_THREADS - 1) / RKPM_THREADS + 1))
         throw Error(CVM_E_STATE, "interleaved neighbour lists missing for this domain");
+    if (d.ng > 0 && d.pair_lgp.n < static_cast<size_t>(d.npairs))
+        throw Error(CVM_E_STATE, "pair GP indices missing for this domain");
     d.sf.alloc(std::max<int64_t>(6 * d.npairs, 1));
-    hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, c.stream, st);
-    if (d.ng > 0) {
-        const int slot = timing_begin(c, d.kind == CVM_DOMAIN_FACE ? "rkpm_face" : "rkpm_line");
-        if (kernel == CVM_KERNEL_QUARTIC) {
-            if (order == 1) run<1, CVM_KERNEL_QUARTIC>(c, d, st);
-            else if (order == 2) run<2, CVM_KERNEL_QUARTIC>(c, d, st);
-            else run<3, CVM_KERNEL_QUARTIC>(c, d, st);
-        } else {
-            if (order == 1) run<1, CVM_KERNEL_CUBIC_BSPLINE>(c, d, st);
-            else if (order == 2) run<2, CVM_KERNEL_CUBIC_BSPLINE>(c, d, st);
-            else run<3, CVM_KERNEL_CUBIC_BSPLINE>(c, d, st);
-        }
-        CVM_HIP(hipGetLastError());
-        timing_end(c, slot);
+}
+
+// Enqueues the shape functions of a group of domains in one coefficient and one
+// shape-function launch; domain k's status goes to st[4 k .. 4 k + 3]. `cache`
+// selects the context's descriptor table (uploaded only when it changes).
+static void enqueue_group(Context& c, const std::vector<Domain*>& ds, int order, int kernel, int* st, int cache,
+                          const char* stage) {
+    if (ds.empty()) return;
+    for (Domain* d : ds) prepare_rkpm(c, *d);
+    const size_t ncj = 6 * static_cast<size_t>((order + 1) * (order + 2) / 2);   // RK<order>::NCJ
+    std::vector<RkDom> tab;
+    std::vector<size_t> cj_off;
+    int64_t nblocks = 0;
+    size_t cj_total = 0;
+    for (size_t k = 0; k < ds.size(); ++k) {
+        Domain& d = *ds[k];
+        if (d.ng == 0) continue;
+        RkDom r{};
+        r.gx = d.gx.p; r.gy = d.gy.p; r.off = d.off.p;
+        r.il_row = d.il_row.p; r.il_ids = d.il_ids.p; r.il_s = d.il_s.p;
+        r.ids = d.ids.p; r.s = d.s.p; r.lgp = d.pair_lgp.p;
+        r.ind = d.ind.p; r.gsheet = c.batched ? d.d_gsheet.p : nullptr;
+        r.sheets = reinterpret_cast<const SheetDev*>(d.d_sheets.p);
+        r.sf = d.sf.p; r.cj = nullptr; r.status = st + 4 * k;
+        r.npairs = d.npairs; r.ng = d.ng; r.blk0 = static_cast<int>(nblocks);
+        cj_off.push_back(cj_total);            // records of the domain in the shared workspace
+        cj_total += static_cast<size_t>(d.ng) * ncj;
+        nblocks += (d.ng + RKPM_BLOCK - 1) / RKPM_BLOCK;
+        if (nblocks > INT32_MAX) throw Error(CVM_E_INVALID, "too many Gauss points for one shape-function launch");
+        tab.push_back(r);
     }
+    hipLaunchKernelGGL(k_status_init, dim3(1), dim3(256), 0, c.stream, st, static_cast<int>(ds.size()));
+    CVM_HIP(hipGetLastError());
+    if (tab.empty()) return;
+    c.rk_cj.ensure(std::max<size_t>(cj_total, 1));
+    for (size_t k = 0; k < tab.size(); ++k) tab[k].cj = c.rk_cj.p + cj_off[k];
+    auto& host = c.rk_tab_host[cache];
+    auto& dev = c.rk_tab[cache];
+    const size_t bytes = tab.size() * sizeof(RkDom);
+    if (host.size() != bytes || std::memcmp(host.data(), tab.data(), bytes) != 0) {
+        host.assign(reinterpret_cast<const unsigned char*>(tab.data()),
+                    reinterpret_cast<const unsigned char*>(tab.data()) + bytes);
+        dev.upload(host, c.stream);
+    }
+    const RkDom* dtab = reinterpret_cast<const RkDom*>(dev.p);
+    const int nd = static_cast<int>(tab.size()), nb = static_cast<int>(nblocks);
+    const int slot = timing_begin(c, stage);
+    if (kernel == CVM_KERNEL_QUARTIC) {
+        if (order == 1) run<1, CVM_KERNEL_QUARTIC>(c, dtab, nd, nb);
+        else if (order == 2) run<2, CVM_KERNEL_QUARTIC>(c, dtab, nd, nb);
+        else run<3, CVM_KERNEL_QUARTIC>(c, dtab, nd, nb);
+    } else {
+        if (order == 1) run<1, CVM_KERNEL_CUBIC_BSPLINE>(c, dtab, nd, nb);
+        else if (order == 2) run<2, CVM_KERNEL_CUBIC_BSPLINE>(c, dtab, nd, nb);
+        else run<3, CVM_KERNEL_CUBIC_BSPLINE>(c, dtab, nd, nb);
+    }
+    timing_end(c, slot);
 }
 
 static void finish_rkpm(Domain& d, int order, int kernel, const int* hs, int* code, int* nbad, int* first) {
@@ -580,32 +659,48 @@ static void finish_rkpm(Domain& d, int order, int kernel, const int* hs, int* co
     d.have_sf = (*code == CVM_OK);
 }
 
+static const char* stage_of(const Domain& d) { return d.kind == CVM_DOMAIN_FACE ? "rkpm_face" : "rkpm_line"; }
+
 void launch_rkpm(Context& c, Domain& d, double /*rho*/, int order, int kernel, int* code, int* nbad, int* first) {
     check_args(order, kernel);
     launch_node_signs(c);
     c.d_status.ensure(4);
-    enqueue_rkpm(c, d, order, kernel, c.d_status.p);
+    enqueue_group(c, {&d}, order, kernel, c.d_status.p, 2, stage_of(d));
     int hs[4];
     CVM_HIP(hipMemcpyAsync(hs, c.d_status.p, sizeof(hs), hipMemcpyDeviceToHost, c.stream));
     CVM_HIP(hipStreamSynchronize(c.stream));
     finish_rkpm(d, order, kernel, hs, code, nbad, first);
 }
 
-// Every domain, one status read-back at the end. On failure: the first failing
-// domain and its status.
+// Every domain: the face domains in one launch pair, all others in a second;
+// one status read-back at the end. On failure: the first failing domain (in
+// domain order) and its status.
 void launch_rkpm_all(Context& c, int order, int kernel, int* dom, int* code, int* nbad, int* first) {
     check_args(order, kernel);
     launch_node_signs(c);
     const size_t nd = c.doms.size();
     c.d_status.ensure(std::max<size_t>(4 * nd, 4));
-    for (size_t i = 0; i < nd; ++i) enqueue_rkpm(c, *c.doms[i], order, kernel, c.d_status.p + 4 * i);
+    // status slot of domain i: 4 i, so each group gets its domains' slots in order
+    std::vector<Domain*> face, rest;
+    std::vector<size_t> fi, ri;
+    for (size_t i = 0; i < nd; ++i) {
+        if (c.doms[i]->kind == CVM_DOMAIN_FACE) { face.push_back(c.doms[i]); fi.push_back(i); }
+        else { rest.push_back(c.doms[i]); ri.push_back(i); }
+    }
+    // the groups write their status to scratch ranges [0, 4 |face|) and
+    // [4 |face|, 4 nd); mapped back to domain order after the read-back
+    enqueue_group(c, face, order, kernel, c.d_status.p, 0, "rkpm_face");
+    enqueue_group(c, rest, order, kernel, c.d_status.p + 4 * face.size(), 1, "rkpm_line");
     std::vector<int> hs(4 * std::max<size_t>(nd, 1));
     if (nd) CVM_HIP(hipMemcpyAsync(hs.data(), c.d_status.p, 4 * nd * sizeof(int), hipMemcpyDeviceToHost, c.stream));
     CVM_HIP(hipStreamSynchronize(c.stream));
+    std::vector<int> slot_of(nd);
+    for (size_t k = 0; k < fi.size(); ++k) slot_of[fi[k]] = static_cast<int>(k);
+    for (size_t k = 0; k < ri.size(); ++k) slot_of[ri[k]] = static_cast<int>(fi.size() + k);
     *dom = -1; *code = CVM_OK; *nbad = 0; *first = 0;
     for (size_t i = 0; i < nd; ++i) {
         int cd, nb, fb;
-        finish_rkpm(*c.doms[i], order, kernel, hs.data() + 4 * i, &cd, &nb, &fb);
+        finish_rkpm(*c.doms[i], order, kernel, hs.data() + 4 * slot_of[i], &cd, &nb, &fb);
         if (cd != CVM_OK && *dom < 0) { *dom = static_cast<int>(i); *code = cd; *nbad = nb; *first = fb; }
     }
 }
