@@ -75,7 +75,7 @@ template <bool HAS_B, typename TT, bool PERSIST, bool WIDE>
 __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((sizeof(TT) == 4 && !HAS_B && !WIDE && !PERSIST) ? 6 : 1))) void k_tile(
     const int* __restrict__ tile_gp, const int* __restrict__ tile_uoff, const int* __restrict__ tile_unodes,
     const int64_t* __restrict__ tile_toff, const uint16_t* __restrict__ tpairs, const uint32_t* __restrict__ slot_dst,
-    const int* __restrict__ off, const uint8_t* __restrict__ uloc,
+    const int* __restrict__ off, const uint8_t* __restrict__ uloc, const uint8_t* __restrict__ urank,
     const double* __restrict__ sf, int64_t npairs, const double* __restrict__ w,
     const double* __restrict__ xu, MatC mat, int rec_cap, int umax, int with_tangent,
     TT* __restrict__ scr, double* __restrict__ fscr, int ntiles) {
@@ -119,13 +119,13 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
             const int rb = b - pb;
             const bool act = lane < n;
             double px = 0, py = 0, hxx = 0, hyy = 0, hxy = 0;
-            int ul = 0;
+            int ul = 0, rk = 0;
             double u0 = 0, u1 = 0, u2 = 0;
             if (act) {
                 const int64_t p = b + lane;
                 px = sf[npairs + p]; py = sf[2 * npairs + p];
                 hxx = sf[3 * npairs + p]; hyy = sf[4 * npairs + p]; hxy = sf[5 * npairs + p];
-                ul = uloc[p];
+                ul = uloc[p]; rk = urank[p];
                 u0 = uc[3 * ul]; u1 = uc[3 * ul + 1]; u2 = uc[3 * ul + 2];
             }
             // ---- kinematics (Kinematics.h:75-113) ------------------------------------
@@ -186,13 +186,16 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
             const double pX = Ci00 * ft0 + Ci01 * ft1, pY = Ci01 * ft0 + Ci11 * ft1;
             const double bdv = bx * vcx + by * vcy + bz * vcz;
             // per-neighbour quantities (Assembly.h:233-268)
-            auto record = [&](int k, double px, double py, double hxx, double hyy, double hxy, int ul) {
+            // the GP's records in union order (rank rk of the node among the GP's
+            // neighbours): a wave's slots, consecutive union nodes, then read
+            // consecutive records
+            auto record = [&](int rk, double px, double py, double hxx, double hyy, double hxy, int ul) {
                 const double gI0 = V0x * px + V1x * py, gI1 = V0y * px + V1y * py, gI2 = V0z * px + V1z * py;
                 const double SXX = rxx0 * gI0 + rxx1 * gI1 + rxx2 * gI2;
                 const double SYY = ryy0 * gI0 + ryy1 * gI1 + ryy2 * gI2;
                 const double SXY = rxy0 * gI0 + rxy1 * gI1 + rxy2 * gI2;
                 const double be0 = -hxx + SXX, be1 = -hyy + SYY, be2 = 2.0 * (-hxy + SXY);
-                const int r = rb + k;
+                const int r = rb + rk;
                 // f_local = (Bm^T N + Bb^T M) W (:249-250)
                 const double ca = px * N[0] + py * N[2], cb = py * N[1] + px * N[2];
                 const double cm = be0 * M[0] + be1 * M[1] + be2 * M[2];
@@ -206,10 +209,10 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
                 loc[wv * U + ul] = static_cast<uint16_t>(r);   // the record, tile-relative
                 atomicOr(&pm[ul], 1u << wv);
             };
-            if (act) record(lane, px, py, hxx, hyy, hxy, ul);
+            if (act) record(rk, px, py, hxx, hyy, hxy, ul);
             if constexpr (WIDE) for (int k = lane + 64; k < n; k += 64) {
                 const int64_t p = b + k;
-                record(k, sf[npairs + p], sf[2 * npairs + p], sf[3 * npairs + p], sf[4 * npairs + p],
+                record(urank[p], sf[npairs + p], sf[2 * npairs + p], sf[3 * npairs + p], sf[4 * npairs + p],
                        sf[5 * npairs + p], uloc[p]);
             }
             if (lane == 0) {
@@ -655,7 +658,7 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
     auto tile = [&](auto kern, auto* scr) {
         hipLaunchKernelGGL(kern, dim3(grid), dim3(TILE_THREADS), lds, st, P.d_tile_gp.p, P.d_tile_uoff.p,
                            P.d_tile_unodes.p, P.d_tile_toff.p, P.d_tile_tpairs.p, P.d_slot_dst.p, fd.off.p,
-                           P.d_pair_uloc.p, fd.sf.p, static_cast<int64_t>(fd.npairs), fd.d_w.p, xdev, m, P.rec_cap,
+                           P.d_pair_uloc.p, P.d_pair_urank.p, fd.sf.p, static_cast<int64_t>(fd.npairs), fd.d_w.p, xdev, m, P.rec_cap,
                            umax, tangent ? 1 : 0, scr, ws.fscr.p, P.ntiles);
     };
     const bool wide = P.max_nbr > 64;

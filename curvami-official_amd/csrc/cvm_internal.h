@@ -209,6 +209,7 @@ struct Plan {
     DBuf<int64_t> d_tile_toff;  // ntiles + 1 (touched pairs)
     DBuf<uint16_t> d_tile_tpairs;   // (ua << 8) | ub
     DBuf<uint8_t> d_pair_uloc;  // per face pair: union-local index
+    DBuf<uint8_t> d_pair_urank; // per face pair: rank of its node among the GP's neighbours in union order
     int64_t nscratch = 0;       // total touched pairs
     int64_t nfscratch = 0;      // total union entries
     // K contributors: per block, a contiguous segment of scratch rows (ascending

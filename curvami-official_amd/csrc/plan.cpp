@@ -146,6 +146,18 @@ void build_plan(Context& c) {
         if (tile_lds_bytes(P.rec_cap, P.max_union, false) <= c.lds_limit) break;
         if (tg == 1) throw Error(CVM_E_INVALID, "assembly tile of one Gauss point exceeds the LDS limit");
     }
+    // per face pair: the rank of its node among the GP's neighbours in union
+    // order (union indices are distinct within a GP and below 256, and so are
+    // the neighbour positions: one 16-bit key each)
+    std::vector<uint8_t> urank(ids.size());
+#pragma omp parallel for schedule(static, 1024)
+    for (int g = 0; g < ng; ++g) {
+        const int b = off[g], n = off[g + 1] - b;
+        uint16_t key[kTileUnion];
+        for (int k = 0; k < n; ++k) key[k] = static_cast<uint16_t>((uloc[b + k] << 8) | k);
+        std::sort(key, key + n);
+        for (int r = 0; r < n; ++r) urank[b + (key[r] & 0xff)] = static_cast<uint8_t>(r);
+    }
     // face blocks: for each column node hi, the nodes lo <= hi that share a face GP
     // with it (parallel over columns through the node -> GP incidence)
     std::vector<std::vector<int>> cols(nn);     // face + fold rows per column
@@ -367,6 +379,7 @@ void build_plan(Context& c) {
     P.d_tile_toff.upload(tile_toff, s);
     P.d_tile_tpairs.upload(tpairs, s);
     P.d_pair_uloc.upload(uloc, s);
+    P.d_pair_urank.upload(urank, s);
     P.d_kc_off.upload(kc_cnt, s);
     P.d_slot_dst.upload(slot_dst, s);
     P.d_fc_off.upload(fc_cnt, s);
