@@ -32,7 +32,11 @@ constexpr int TILE_THREADS = 512;    // 8 waves; wave w prepares GPs w and w + 8
 constexpr int TILE_GPS = 16;         // GPs per tile (presence masks are 16-bit)
 constexpr int UMAX = 256;            // union nodes per tile (uint8 union indices, at most 255 used)
 constexpr int GS = 23;               // GP record fields (tangent type)
-constexpr int GSP = 24;              // GP record stride: GP-major, 16-byte aligned for wide LDS reads
+// GP record stride, GP-major: the 16 GPs of a tile start on distinct LDS banks
+// (FP64: 23 doubles = 46 dwords apart; FP32: 26 floats, even so that field
+// pairs stay 8-byte aligned for paired loads), so lanes on different GPs read
+// their GP's fields without bank conflicts
+template <typename TT> constexpr int gp_stride() { return sizeof(TT) == 8 ? 23 : 26; }
 constexpr uint16_t NO_REC = 0xffff;  // loc entry of a union node that a GP does not see
 // Neighbour record stride (tangent-type elements): FP64 records get one pad
 // element so that records start on 32 different LDS bank positions instead of
@@ -93,6 +97,7 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
     double* uc = recf + static_cast<size_t>(rec_cap) * 3;       // umax * 3
     TT* rect = reinterpret_cast<TT*>(uc + umax * 3);            // rec_cap records of RS TT
     constexpr int RS = rec_stride<TT>();
+    constexpr int GSP = gp_stride<TT>();
     TT* gs = rect + static_cast<size_t>(rec_cap) * RS;          // TILE_GPS records of GSP
     uint32_t* pm = reinterpret_cast<uint32_t*>(gs + TILE_GPS * GSP);  // umax presence masks
     uint16_t* loc = reinterpret_cast<uint16_t*>(pm + umax);     // ngp rows of U record indices
@@ -584,7 +589,7 @@ size_t tile_lds_bytes(int rec_cap, int max_union, bool fp32) {
     const size_t um = static_cast<size_t>(tile_umax(max_union));
     return sizeof(double) * (static_cast<size_t>(rec_cap) * 3 + um * 3)
          + tt * (static_cast<size_t>(rec_cap) * (fp32 ? rec_stride<float>() : rec_stride<double>()) +
-                 TILE_GPS * GSP) + sizeof(uint32_t) * um
+                 TILE_GPS * (fp32 ? gp_stride<float>() : gp_stride<double>())) + sizeof(uint32_t) * um
          + sizeof(uint16_t) * TILE_GPS * um;
 }
 
