@@ -286,7 +286,7 @@ struct Context {
     ConstraintLayout cons;
     DBuf<int> d_status;         // RKPM status words, 4 per domain
     DBuf<double> rk_cj;         // RKPM per-GP coefficient records (grow-only, reused launch by launch)
-    DBuf<unsigned char> rk_tab[3];            // RKPM domain tables: face group, other domains, single domain
+    DBuf<unsigned char> rk_tab[3];            // RKPM domain tables: all domains, (unused), single domain
     std::vector<unsigned char> rk_tab_host[3];
     // assembly buffers
     DBuf<double> d_x;           // 3 nn displacements

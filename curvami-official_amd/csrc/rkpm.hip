@@ -672,31 +672,28 @@ void launch_rkpm(Context& c, Domain& d, double /*rho*/, int order, int kernel, i
     finish_rkpm(d, order, kernel, hs, code, nbad, first);
 }
 
-// Every domain: the face domains in one launch pair, all others in a second;
-// one status read-back at the end. On failure: the first failing domain (in
-// domain order) and its status.
+// Every domain in one launch pair, the small (non-face) domains first so that
+// their blocks are dispatched first and finish under the face domain's; one
+// status read-back at the end. On failure: the first failing domain (in domain
+// order) and its status.
 void launch_rkpm_all(Context& c, int order, int kernel, int* dom, int* code, int* nbad, int* first) {
     check_args(order, kernel);
     launch_node_signs(c);
     const size_t nd = c.doms.size();
     c.d_status.ensure(std::max<size_t>(4 * nd, 4));
-    // status slot of domain i: 4 i, so each group gets its domains' slots in order
-    std::vector<Domain*> face, rest;
-    std::vector<size_t> fi, ri;
-    for (size_t i = 0; i < nd; ++i) {
-        if (c.doms[i]->kind == CVM_DOMAIN_FACE) { face.push_back(c.doms[i]); fi.push_back(i); }
-        else { rest.push_back(c.doms[i]); ri.push_back(i); }
-    }
-    // the groups write their status to scratch ranges [0, 4 |face|) and
-    // [4 |face|, 4 nd); mapped back to domain order after the read-back
-    enqueue_group(c, face, order, kernel, c.d_status.p, 0, "rkpm_face");
-    enqueue_group(c, rest, order, kernel, c.d_status.p + 4 * face.size(), 1, "rkpm_line");
+    // launch order: non-face domains, then face domains; status slot k of the
+    // launch order, mapped back to domain order after the read-back
+    std::vector<Domain*> group;
+    std::vector<size_t> order_of;
+    for (int pass = 0; pass < 2; ++pass)
+        for (size_t i = 0; i < nd; ++i)
+            if ((c.doms[i]->kind == CVM_DOMAIN_FACE) == (pass == 1)) { group.push_back(c.doms[i]); order_of.push_back(i); }
+    enqueue_group(c, group, order, kernel, c.d_status.p, 0, "shape_functions");
     std::vector<int> hs(4 * std::max<size_t>(nd, 1));
     if (nd) CVM_HIP(hipMemcpyAsync(hs.data(), c.d_status.p, 4 * nd * sizeof(int), hipMemcpyDeviceToHost, c.stream));
     CVM_HIP(hipStreamSynchronize(c.stream));
     std::vector<int> slot_of(nd);
-    for (size_t k = 0; k < fi.size(); ++k) slot_of[fi[k]] = static_cast<int>(k);
-    for (size_t k = 0; k < ri.size(); ++k) slot_of[ri[k]] = static_cast<int>(fi.size() + k);
+    for (size_t k = 0; k < order_of.size(); ++k) slot_of[order_of[k]] = static_cast<int>(k);
     *dom = -1; *code = CVM_OK; *nbad = 0; *first = 0;
     for (size_t i = 0; i < nd; ++i) {
         int cd, nb, fb;
