@@ -286,8 +286,8 @@ struct Context {
     ConstraintLayout cons;
     DBuf<int> d_status;         // RKPM status words, 4 per domain
     DBuf<double> rk_cj;         // RKPM per-GP coefficient records (grow-only, reused launch by launch)
-    DBuf<unsigned char> rk_tab[3];            // RKPM domain tables: all domains, (unused), single domain
-    std::vector<unsigned char> rk_tab_host[3];
+    DBuf<unsigned char> rk_tab[2];            // RKPM domain tables: all domains, single domain
+    std::vector<unsigned char> rk_tab_host[2];
     // assembly buffers
     DBuf<double> d_x;           // 3 nn displacements
     DBuf<double> en_scr;        // per-GP energies (face membrane, face bending, crease hinge)

@@ -665,7 +665,7 @@ void launch_rkpm(Context& c, Domain& d, double /*rho*/, int order, int kernel, i
     check_args(order, kernel);
     launch_node_signs(c);
     c.d_status.ensure(4);
-    enqueue_group(c, {&d}, order, kernel, c.d_status.p, 2, stage_of(d));
+    enqueue_group(c, {&d}, order, kernel, c.d_status.p, 1, stage_of(d));
     int hs[4];
     CVM_HIP(hipMemcpyAsync(hs, c.d_status.p, sizeof(hs), hipMemcpyDeviceToHost, c.stream));
     CVM_HIP(hipStreamSynchronize(c.stream));
