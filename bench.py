@@ -268,6 +268,52 @@ def large_leg(warmup, steps):
     return res
 
 
+def defaults_leg(n_side, warmup, steps):
+    """The same arc-crease sheet at the reference's own defaults: cubic basis
+    (order 3) and support 6 h (6 x the 3rd-nearest-node spacing), about 110
+    neighbours per GP. Step = shape functions of every domain + residual +
+    tangent, as in the main leg, with per-stage device times."""
+    import dataclasses
+    log(f"defaults: {n_side} x {n_side} arc-crease sheet, cubic basis, rho = 6h, set-up")
+    sh = dataclasses.replace(W.config2(rho_factor=6.0, n_side=n_side), order=3)
+    t0 = time.perf_counter()
+    p = G.plate_from_sheet(sh, compute=False)
+    for d in range(len(p.domain_sizes)):
+        p.neighbours(d, sh.rho)
+    p.shape_functions_all(sh.order, sh.kernel)
+    _, nu = p.init_assembly()
+    setup = time.perf_counter() - t0
+    ng = sh.face.n
+    x = G.DeviceArray(3 * sh.nn); x.set(W.displacement(sh.nn, 1, 1e-3))
+    F = G.DeviceArray(3 * sh.nn); K = G.DeviceArray(nu)
+    res = {"workload": f"arc-crease sheet {n_side}x{n_side} nodes, {ng} face GPs, {p.npairs[p.face_dom]} pairs, "
+                       f"nnz_K {nu}, rho 6h, cubic RKPM, quartic window, elastic hinge",
+           "setup_s": round(setup, 1)}
+    base = G.RESIDUAL | G.TANGENT | G.ELASTIC_FOLDS
+    for name, fl in (("fp64", base), ("mixed", base | G.FP32_TANGENT)):
+        log(f"defaults: timing the {name} step")
+        for _ in range(warmup):
+            p.shape_functions_all(sh.order, sh.kernel)
+            p.assemble_device(x.ptr, F.ptr, K.ptr, fl)
+        p.synchronize()
+        p.set_profiling(True)
+        p.reset_profiling()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            p.shape_functions_all(sh.order, sh.kernel)
+            p.assemble_device(x.ptr, F.ptr, K.ptr, fl)
+        p.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        st = p.kernel_stats()
+        p.set_profiling(False)
+        res[name] = {"gps": ng / dt, "ms_per_step": dt * 1e3,
+                     "stages_ms": {k: round(v[0] / max(v[1], 1), 3) for k, v in st.items()}}
+    for a in (x, F, K):
+        a.free()
+    p.close()
+    return res
+
+
 def serving_leg(nreq):
     """Poisson request streams over the serving loop: sides U{21..201}, rigid
     fold of a seeded angle plus noise; p50 / p99 latency and throughput per
@@ -308,6 +354,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-sheet batched leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-large", action="store_true", help="skip the 1001 x 1001 leg")
+    ap.add_argument("--no-defaults", action="store_true",
+                    help="skip the leg at the reference's defaults (cubic basis, rho = 6h)")
     ap.add_argument("--batch-sheets", type=int, default=W.BATCH_SHEETS)
     ap.add_argument("--serving-requests", type=int, default=200)
     args = ap.parse_args()
@@ -456,6 +504,8 @@ def main():
     for ptr in (xd, Fd, Kd):
         hip.free(ptr)
     p.close()
+    if not args.no_defaults:
+        out["reference_defaults"] = defaults_leg(args.n_side, 1, max(3, args.steps // 3))
     if not args.no_batched:
         out["batched"] = batched_leg(args.batch_sheets, args.warmup, max(2, args.steps // 3))
     if not args.no_serving:
