@@ -360,8 +360,8 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design "
-                                   "(see DESIGN.md), not a measured path", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this benchmark runs on one GPU only: sharding a sheet or a batch over "
+                                   "several GPUs is designed, not built or measured", "n_gpus": args.gpus}))
         sys.exit(2)
 
     log(f"main: {args.n_side} x {args.n_side} arc-crease sheet, set-up")
