@@ -153,11 +153,15 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
                 p_yy0 += v0 * qyy; p_yy1 += v1 * qyy; p_yy2 += v2 * qyy;
                 p_xy0 += v0 * qxy; p_xy1 += v1 * qxy; p_xy2 += v2 * qxy;
             }
-            const double a1x = 1.0 + wave_sum(p_a1x), a1y = wave_sum(p_a1y), a1z = wave_sum(p_a1z);
-            const double a2x = wave_sum(p_a2x), a2y = 1.0 + wave_sum(p_a2y), a2z = wave_sum(p_a2z);
-            const double rxx0 = wave_sum(p_xx0), rxx1 = wave_sum(p_xx1), rxx2 = wave_sum(p_xx2);
-            const double ryy0 = wave_sum(p_yy0), ryy1 = wave_sum(p_yy1), ryy2 = wave_sum(p_yy2);
-            const double rxy0 = wave_sum(p_xy0), rxy1 = wave_sum(p_xy1), rxy2 = wave_sum(p_xy2);
+            const double pv[16] = {p_a1x, p_a1y, p_a1z, p_a2x, p_a2y, p_a2z, p_xx0, p_xx1,
+                                   p_xx2, p_yy0, p_yy1, p_yy2, p_xy0, p_xy1, p_xy2, 0.0};
+            double sv[16];
+            wave_sum16(pv, sv);
+            const double a1x = 1.0 + sv[0], a1y = sv[1], a1z = sv[2];
+            const double a2x = sv[3], a2y = 1.0 + sv[4], a2z = sv[5];
+            const double rxx0 = sv[6], rxx1 = sv[7], rxx2 = sv[8];
+            const double ryy0 = sv[9], ryy1 = sv[10], ryy2 = sv[11];
+            const double rxy0 = sv[12], rxy1 = sv[13], rxy2 = sv[14];
             const double C00 = a1x * a1x + a1y * a1y + a1z * a1z;
             const double C11 = a2x * a2x + a2y * a2y + a2z * a2z;
             const double C01 = a1x * a2x + a1y * a2y + a1z * a2z;

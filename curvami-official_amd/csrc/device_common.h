@@ -37,4 +37,56 @@ __device__ __forceinline__ double wave_sum(double v) {
     return __hiloint2double(hi, lo);
 }
 
+// Sixteen wave sums at once, by halving: each exchange step sends half of the
+// lane's values to its partner and keeps the other half, so the 16 sums cost
+// about 95 instructions instead of 16 separate reductions' 300.
+//  - permlane32_swap pairs lanes l and l + 32: afterwards lanes 0-31 hold the
+//    sums of v[k] and lanes 32-63 those of v[k + 8] (k < 8);
+//  - permlane16_swap pairs rows 0/1 and 2/3 the same way (4 values left);
+//  - row_mirror (lane i with 15 - i) and row_half_mirror (i with 7 - i) halve
+//    twice more on lane bits 3 and 2, and two quad sums finish;
+// value j ends in every lane of the quad at lane 32 h + 16 q + 8 b3 + 4 b2 with
+// j = 8 h + 4 q + 2 b3 + b2, from where it is read back as a scalar. All 64
+// lanes must be active.
+__device__ __forceinline__ double swap32_add(double x, double y) {
+    const auto lo = __builtin_amdgcn_permlane32_swap(__double2loint(x), __double2loint(y), false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap(__double2hiint(x), __double2hiint(y), false, false);
+    return __hiloint2double(hi[0], lo[0]) + __hiloint2double(hi[1], lo[1]);
+}
+
+__device__ __forceinline__ double swap16_add(double x, double y) {
+    const auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(x), __double2loint(y), false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap(__double2hiint(x), __double2hiint(y), false, false);
+    return __hiloint2double(hi[0], lo[0]) + __hiloint2double(hi[1], lo[1]);
+}
+
+__device__ __forceinline__ void wave_sum16(const double (&v)[16], double (&out)[16]) {
+    const int lane = static_cast<int>(__lane_id());
+    const bool b3 = (lane >> 3) & 1, b2 = (lane >> 2) & 1;
+    double w[8], x[4], y[2];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) w[k] = swap32_add(v[k], v[k + 8]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) x[k] = swap16_add(w[k], w[k + 4]);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const double keep = b3 ? x[k + 2] : x[k], send = b3 ? x[k] : x[k + 2];
+        y[k] = keep + dpp_move_all<0x140>(send);     // row_mirror
+    }
+    double z;
+    {
+        const double keep = b2 ? y[1] : y[0], send = b2 ? y[0] : y[1];
+        z = keep + dpp_move_all<0x141>(send);        // row_half_mirror
+    }
+    z += dpp_move_all<0xb1>(z);                      // quad_perm(1,0,3,2)
+    z += dpp_move_all<0x4e>(z);                      // quad_perm(2,3,0,1)
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int src = 32 * (j >> 3) + 16 * ((j >> 2) & 1) + 8 * ((j >> 1) & 1) + 4 * (j & 1);
+        const int lo = __builtin_amdgcn_readlane(__double2loint(z), src);
+        const int hi = __builtin_amdgcn_readlane(__double2hiint(z), src);
+        out[j] = __hiloint2double(hi, lo);
+    }
+}
+
 }  // namespace cvm

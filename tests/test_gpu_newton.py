@@ -7,7 +7,8 @@ which the device and the oracle compute to ~1e-13, not bit-identically); the con
 displacement of the 20-increment load path within 1e-8 (inf-norm, relative)
 of the oracle's, with every increment converged (golden fixture written by
 tests/golden/make_newton_fixture.py). The FP32-tangent (mixed) path reports
-its Newton iteration counts and must converge to the same state.
+its Newton iteration counts and must converge to the same state within its
+stopping band (1e-7 relative).
 """
 import os
 import time
@@ -85,7 +86,13 @@ def test_newton_load_path_matches_oracle(c1m):
 
 def test_newton_mixed_precision_tangent(c1m):
     """FP32 face tangent (inexact Newton), FP64 residual and constraint rows: the
-    path converges to the FP64 state; the iteration counts are reported."""
+    path converges, and to the FP64 state within the stopping band. With the
+    exact tangent the iteration converges quadratically and overshoots the
+    stopping rule (max|P R| <= tol_force = 1e-8) by orders of magnitude; with
+    the FP32 tangent it converges linearly and stops just inside it, at a point
+    that moves with the round-off of the residual's sums (3e-9 to 1.6e-8
+    relative across builds that sum in different orders). Bar: 1e-7 relative;
+    the iteration counts are reported."""
     sh, _, gpu = c1m
     ref, res, prob, dt = _run_path(gpu, fp32=True)
     err = np.abs(res.x - ref["x"]).max() / np.abs(ref["x"]).max()
@@ -93,7 +100,7 @@ def test_newton_mixed_precision_tangent(c1m):
     print(f"mixed path: iterations {res.iterations} (sum {sum(res.iterations)}, oracle FP64 sum "
           f"{int(np.sum(ref['iterations']))}, delta {d_it:+d}), |dx|/|x| {err:.3e}, {dt:.1f} s")
     assert res.all_converged and len(res.converged) == W.NEWTON_INCREMENTS
-    assert err <= 1e-8
+    assert err <= 1e-7
 
 
 def test_ls_miura_line_search_matches_oracle():
