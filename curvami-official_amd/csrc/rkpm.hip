@@ -18,7 +18,12 @@
 //    test (:303-305) is applied to them; the 3x3 leading block is inverted by
 //    its adjugate and checked with the reference's residual test (:214-220).
 //  * the crease visibility (:86-112) uses per-GP indicator values and per-node
-//    side signs computed once (the reference recomputes both per pair).
+//    side signs computed once (the reference recomputes both per pair);
+//  * two kernels: k_rkpm_coef (one lane per GP: moments, tests, solves) writes
+//    c and its derivatives per GP to HBM, and k_rkpm_sf (lanes over the pairs of
+//    a block of GPs) evaluates Psi and its derivatives (:363-397) with coalesced
+//    stores at a much higher occupancy than the register-heavy first kernel.
+//    Both are launched once for all domains of a call.
 #include <algorithm>
 #include <cstring>
 
