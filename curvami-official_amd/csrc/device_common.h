@@ -1,5 +1,6 @@
 // Device helpers shared by the assembly and energy kernels: the laminate
-// stiffness blocks as kernel arguments and the DPP wave reduction.
+// stiffness blocks as kernel arguments and the wave reductions (DPP moves and
+// gfx950 half-wave swaps).
 #pragma once
 #include <hip/hip_runtime.h>
 
