@@ -18,6 +18,17 @@ semantics of ContinuationSolver::LineSearch
   unless fail_on_no_descent (:268-312);
 * on failure the state rolls back to the call's start (:238, :305, :318-321).
 
+Arc-length continuation (ContinuationSolver::ArcLength, :329-538) with the
+step control of core/include/curvami/solver/StepController.h: Crisfield's
+constraint |dU|^2 + psi^2 dlambda^2 |F_ext|^2 = ds^2, a tangent predictor whose
+sign follows the previous increment (snap-back reversal), corrector iterations
+with two KKT right-hand sides, ASLA and the curvature controller for ds, the
+no-progress stop at ds_min, the accumulated external work, and the restart
+record (RestartManager, read and written through the library's `.res` codec).
+The inertia veto of a reversal (snapback_inertia_veto, off by default in the
+reference) needs the KKT factorization's negative pivot count, which SuperLU
+does not report, and is not offered.
+
 The KKT solve replaces the reference's symmetric MUMPS factorization
 (core/include/curvami/solver/MumpsSymmetricSolver.h:46-55), which is not
 available here, by SuperLU (scipy) on the assembled symmetric KKT matrix.
@@ -31,6 +42,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import os
 from typing import List, Optional
 
 import numpy as np
@@ -111,6 +123,25 @@ class KKTSolver:
 
 
 @dataclasses.dataclass
+class ArcLengthStep:
+    step: int
+    lam: float
+    iterations: int            # corrector iterations of the accepted step
+    ds: float                  # arc length after the step control
+    W_ext: float               # accumulated external work
+
+
+@dataclasses.dataclass
+class ArcLengthReport:
+    x: np.ndarray
+    lam: float
+    steps: List[ArcLengthStep]
+    stop: str                  # "lambda_tot", "kmax", "ds_min", "kkt", "stalled"
+    reversals: int = 0
+    restarted_from: int = 0    # step loaded from the restart record (0: none)
+
+
+@dataclasses.dataclass
 class LineSearchReport:
     converged: bool
     iterations: int            # Newton iterations at convergence (-1 otherwise)
@@ -125,6 +156,21 @@ class ContinuationSolver:
     div = 1e7
     ls_max_iters = 500
     fail_on_no_descent = False
+    # arc-length continuation and step control (ContinuationSolver.h:31-44,
+    # StepController.h:16-24, :51-80)
+    kmax = 5000
+    max_iters = 100
+    ds = 0.1
+    dsmin = 1e-6
+    dsmax = 10.0
+    Id = 6
+    psi = 1.0
+    restart_file = ""
+    asla_hold_after_reversal = False
+    no_progress_window = 24
+    no_progress_efficiency = 0.15
+    no_progress_abort = True
+    NO_REVERSAL_YET = 1 << 20
 
     def __init__(self, kkt: Optional[KKTSolver] = None, verbose: bool = False):
         self.kkt = kkt or KKTSolver()
@@ -147,6 +193,201 @@ class ContinuationSolver:
 
     def filter_rows(self, c, J):
         return c[self.ind_rows], J.tocsr()[self.ind_rows]
+
+    # ---- step control (StepController.h) ---------------------------------------
+    def _reset_tracking(self):
+        self._since_reversal = self.NO_REVERSAL_YET
+        self._np_ref = None
+        self._np_count = 0
+        self._np_path = 0.0
+
+    def stalled_at_dsmin(self, x, dU) -> bool:
+        """Net movement over a window of accepted steps at ds_min against the path
+        length walked (StepController.h:100-131)."""
+        if not self.no_progress_abort:
+            return False
+        if self.ds > self.dsmin * (1.0 + 1e-12):
+            self._np_ref, self._np_count, self._np_path = None, 0, 0.0
+            return False
+        if self._np_ref is None or self._np_ref.size != x.size:
+            self._np_ref, self._np_count, self._np_path = x.copy(), 0, 0.0
+            return False
+        self._np_path += float(np.linalg.norm(dU))
+        self._np_count += 1
+        if self._np_count < self.no_progress_window:
+            return False
+        moved = float(np.linalg.norm(x - self._np_ref))
+        path = self._np_path
+        self._np_ref, self._np_count, self._np_path = x.copy(), 0, 0.0
+        return path > 0.0 and moved <= self.no_progress_efficiency * path
+
+    def asla(self, ok: bool, k_corr: int, cos_theta: float):
+        """Crisfield's square-root rule with target relaxation on straight paths
+        (StepController.h:167-197)."""
+        opt = float(self.Id)
+        if cos_theta >= 0.99:
+            opt = max(opt, 12.0)
+        elif cos_theta >= 0.90:
+            opt = max(opt, 8.0)
+        if not ok:
+            self.ds *= 0.50 if k_corr < self.max_iters else 0.75
+            return
+        scale = min(math.sqrt(opt / max(k_corr, 1)), 4.0)
+        if 0.90 < scale < 1.10:
+            scale = 1.0
+        self.ds = min(max(self.ds * scale, self.dsmin), self.dsmax)
+
+    def curvature_controller(self, step: int, cos_theta: float):
+        """Tiered ds control on the angle between successive increments
+        (StepController.h:202-227)."""
+        if step <= 1:
+            return
+        if cos_theta >= 0.99:
+            if not (self.asla_hold_after_reversal and self._since_reversal <= 1):
+                self.ds = min(self.ds * 1.5, self.dsmax)
+        elif cos_theta >= 0.90:
+            pass
+        elif cos_theta >= 0.75:
+            self.ds = max(self.ds * 0.8, self.dsmin)
+        else:
+            self.ds = max(self.ds * 0.5, self.dsmin)
+
+    def _apply_load_stiffness(self, prob, K, x, lam):
+        Kl = prob.load_stiffness(x)
+        return K if Kl is None else (K - lam * Kl).tocsc()
+
+    # ---- ArcLength (ContinuationSolver.h:329-538) --------------------------------
+    def arc_length(self, prob, lambda_0: float, x0, lambda_tot: float, on_converged=None) -> ArcLengthReport:
+        """Continuation from (x0, lambda_0) while lambda <= lambda_tot and at most
+        kmax steps; on_converged(step, lambda, x, W_ext) after every accepted
+        step. The solver's ds is adapted in place, as in the reference."""
+        ns = prob.num_dofs
+        x = np.array(x0, dtype=np.float64)
+        if x.size != ns:
+            raise ValueError(f"ArcLength: state vector has {x.size} entries, problem has {ns} DOFs")
+        lam = float(lambda_0)
+        k = 1
+        dU_old = np.zeros(ns)
+        dl_old = 0.0
+        c_all, J_all = prob.constraints(x)
+        self.initialize_topology(J_all)
+        self._reset_tracking()
+        n_const = int(self.ind_rows.size)
+        W_ext = 0.0
+        F_ref = np.array(prob.external_force(x), dtype=np.float64)
+        x_prev = x.copy()
+        rep = ArcLengthReport(x, lam, [], "lambda_tot")
+        if self.restart_file and os.path.exists(self.restart_file):
+            import curvami_gpu
+            r = curvami_gpu.restart_load(self.restart_file, ns)
+            k, lam, self.ds, dl_old, W_ext = r["k"], r["lam"], r["ds"], r["dl_old"], r["W_ext"]
+            x[:] = r["x"]
+            dU_old = r["dU_old"]
+            x_prev = x.copy()      # work increments are measured from the loaded state
+            rep.restarted_from = k
+        psi2 = self.psi * self.psi
+        while k <= self.kmax and lam <= lambda_tot:
+            # ---- predictor: tangent d_T = K^-1 F_ext under the constraints
+            Fext = prob.external_force(x)
+            K, _ = prob.assemble(x)
+            K = self._apply_load_stiffness(prob, K, x, lam)
+            c_all, J_all = prob.constraints(x)
+            c_ind, J_ind = self.filter_rows(c_all, J_all)
+            xb_p = ConstraintProjector(J_ind).corrector(c_ind, ns)
+            rhs = np.zeros(ns + n_const)
+            rhs[:ns] = Fext
+            sol = self.kkt.solve(K, J_ind, rhs)
+            if sol is None:
+                rep.stop = "kkt"
+                break
+            d_T = sol[:ns]
+            ff = float(np.dot(Fext, Fext))
+            if k == 1:
+                sgn = 1.0 if float(np.dot(d_T, Fext)) >= 0.0 else -1.0
+            elif float(np.dot(dU_old, d_T)) >= 0.0:
+                sgn = 1.0
+            else:
+                # snap-back: the tangent turned against the last increment
+                sgn = -1.0
+                self._since_reversal = 0
+                rep.reversals += 1
+            dl = sgn * self.ds / math.sqrt(float(np.dot(d_T, d_T)) + psi2 * ff)
+            dU = dl * d_T + xb_p
+            ok, k_corr = False, self.max_iters
+            # ---- corrector
+            for j in range(1, self.max_iters + 1):
+                lam_c = lam + dl
+                x_c = x + dU
+                c_all, J_all = prob.constraints(x_c)
+                c_ind_c, J_ind_c = self.filter_rows(c_all, J_all)
+                Fext_c = prob.external_force(x_c)
+                K_c, Fint_c = prob.assemble(x_c)
+                K_c = self._apply_load_stiffness(prob, K_c, x_c, lam_c)
+                R = Fint_c - lam_c * Fext_c
+                v_res = float(np.dot(dU, dU)) + psi2 * dl * dl * ff - self.ds * self.ds
+                xb_c = ConstraintProjector(J_ind_c).corrector(c_ind_c, ns)
+                rhs2 = np.zeros((ns + n_const, 2))
+                rhs2[:ns, 0] = Fext_c
+                rhs2[:ns, 1] = -(R + full_from_upper(K_c) @ xb_c)
+                sol2 = self.kkt.solve(K_c, J_ind_c, rhs2)
+                if sol2 is None:
+                    ok, k_corr = False, j
+                    break
+                daP = sol2[:ns, 0]
+                daPsi = sol2[:ns, 1] + xb_c
+                mult = sol2[ns:, 1]
+                dl_c = -(0.5 * v_res + float(np.dot(dU, daPsi))) / (float(np.dot(dU, daP)) + psi2 * dl * ff)
+                dl += dl_c
+                dU = dU + daPsi + dl_c * daP
+                max_force = float(np.abs(R + J_ind_c.T @ mult).max()) if ns else 0.0
+                energy_change = abs(float(np.dot(daPsi, R)))
+                geom_error = float(np.abs(c_ind_c).max()) if c_ind_c.size else 0.0
+                if j > 1 and max_force <= self.tol_force and energy_change <= self.tol_energy \
+                        and geom_error <= self.tol_geom and abs(v_res) <= self.tol_geom:
+                    ok, k_corr = True, j
+                    break
+                if max_force >= self.div or math.isnan(max_force):
+                    ok, k_corr = False, j
+                    break
+                if self.verbose:
+                    print(f"{j:3d}  {max_force:12.5e}  {energy_change:12.5e}  {geom_error:12.5e}  {abs(v_res):12.5e}",
+                          flush=True)
+            # ---- step control
+            cos_theta = 1.0
+            if k > 1 and ok:
+                old_load, new_load = self.psi * dl_old, self.psi * dl
+                old_n = math.sqrt(float(np.dot(dU_old, dU_old)) + old_load * old_load)
+                new_n = math.sqrt(float(np.dot(dU, dU)) + new_load * new_load)
+                if old_n > 0.0 and new_n > 0.0:
+                    cos_theta = (float(np.dot(dU_old, dU)) + old_load * new_load) / (old_n * new_n)
+                    cos_theta = max(-1.0, min(1.0, cos_theta))
+            self.asla(ok, k_corr, cos_theta)
+            if ok:
+                self.curvature_controller(k, cos_theta)
+                if self._since_reversal < self.NO_REVERSAL_YET:
+                    self._since_reversal += 1
+                dU_old, dl_old = dU, dl
+                x += dU
+                lam += dl
+                W_ext += 0.5 * ((lam - dl) + lam) * float(np.dot(F_ref, x - x_prev))
+                x_prev = x.copy()
+                rep.steps.append(ArcLengthStep(k, lam, k_corr, self.ds, W_ext))
+                if on_converged is not None:
+                    on_converged(k, lam, x, W_ext)
+                k += 1
+                if self.restart_file:
+                    import curvami_gpu
+                    curvami_gpu.restart_save(self.restart_file, k, lam, self.ds, dl_old, W_ext, x, dU_old)
+                if self.stalled_at_dsmin(x, dU):
+                    rep.stop = "stalled"
+                    break
+            elif self.ds <= self.dsmin:
+                rep.stop = "ds_min"
+                break
+        else:
+            rep.stop = "kmax" if k > self.kmax else "lambda_tot"
+        rep.x, rep.lam = x, lam
+        return rep
 
     # ---- LineSearch (ContinuationSolver.h:185-322) -----------------------------
     def line_search(self, prob, lam: float, d: np.ndarray) -> LineSearchReport:

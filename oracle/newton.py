@@ -8,9 +8,14 @@ oracle plate (oracle.py) so that the whole load path -- residual, tangent,
 constraint rows, external load -- is the CPU restatement. The KKT system of
 MumpsSymmetricSolver (core/include/curvami/solver/MumpsSymmetricSolver.h:46-55)
 is solved with scipy's sparse LU (spsolve); the projector's Gram system with a
-dense solve.
+dense solve. arc_length restates ContinuationSolver::ArcLength (:329-538) with
+the default step control of core/include/curvami/solver/StepController.h (ASLA
+:167-197, the curvature controller :202-227, the no-progress stop :100-131;
+no inertia veto, as the reference's default).
 """
 from __future__ import annotations
+
+import math
 
 import numpy as np
 import scipy.linalg as sla
@@ -123,3 +128,128 @@ def load_path(prob, increments=20, lam_total=1.0, **tol):
         if not ok:
             break
     return x, conv, its
+
+
+def _kkt_solve(K, Jd, rhs):
+    Ks = _sym(K)
+    if Jd.shape[0]:
+        A = sp.bmat([[Ks, sp.csr_matrix(Jd).T], [sp.csr_matrix(Jd), None]], format="csc")
+    else:
+        A = Ks.tocsc()
+    return spla.splu(A).solve(rhs)
+
+
+def arc_length(prob, lambda_0, x0, lambda_tot, on_converged=None, ds=0.1, dsmin=1e-6, dsmax=10.0, psi=1.0,
+               kmax=5000, max_iters=100, Id=6, tol_force=1e-8, tol_energy=1e-8, tol_geom=1e-12, div=1e7):
+    """Returns (x, lambda, [(step, lambda, corrector iterations)])."""
+    ns = prob.num_dofs
+    x = np.array(x0, dtype=float)
+    lam, k = float(lambda_0), 1
+    dU_old, dl_old = np.zeros(ns), 0.0
+    c, J = prob.constraints(x)
+    ind = _independent_rows(J)
+    nc = ind.size
+    steps = []
+    np_ref, np_count, np_path = None, 0, 0.0
+
+    def tangent(xx, ll):
+        K, F = prob.assemble(xx)
+        Kl = prob.load_stiffness(xx)
+        return (K if Kl is None else K - ll * Kl), F
+
+    while k <= kmax and lam <= lambda_tot:
+        Fe = prob.external_force(x)
+        K, _ = tangent(x, lam)
+        c, J = prob.constraints(x)
+        Jd = J[ind].toarray()
+        corr, _ = _projector(Jd)
+        xbp = corr(c[ind], ns)
+        dT = _kkt_solve(K, Jd, np.concatenate([Fe, np.zeros(nc)]))[:ns]
+        ff = Fe @ Fe
+        if k == 1:
+            sgn = 1.0 if dT @ Fe >= 0 else -1.0
+        else:
+            sgn = 1.0 if dU_old @ dT >= 0 else -1.0
+        dl = sgn * ds / math.sqrt(dT @ dT + psi * psi * ff)
+        dU = dl * dT + xbp
+        ok, kc = False, max_iters
+        for j in range(1, max_iters + 1):
+            lc, xc = lam + dl, x + dU
+            c, J = prob.constraints(xc)
+            ci, Jd = c[ind], J[ind].toarray()
+            Fec = prob.external_force(xc)
+            Kc, Fi = tangent(xc, lc)
+            R = Fi - lc * Fec
+            vres = dU @ dU + psi * psi * dl * dl * ff - ds * ds
+            corr, _ = _projector(Jd)
+            xbc = corr(ci, ns)
+            rhs = np.zeros((ns + nc, 2))
+            rhs[:ns, 0] = Fec
+            rhs[:ns, 1] = -(R + _sym(Kc) @ xbc)
+            sol = _kkt_solve(Kc, Jd, rhs)
+            daP, daPsi, mu = sol[:ns, 0], sol[:ns, 1] + xbc, sol[ns:, 1]
+            dlc = -(0.5 * vres + dU @ daPsi) / (dU @ daP + psi * psi * dl * ff)
+            dl += dlc
+            dU = dU + daPsi + dlc * daP
+            mf = np.abs(R + Jd.T @ mu).max()
+            ge = np.abs(ci).max() if ci.size else 0.0
+            if j > 1 and mf <= tol_force and abs(daPsi @ R) <= tol_energy and ge <= tol_geom and abs(vres) <= tol_geom:
+                ok, kc = True, j
+                break
+            if mf >= div or np.isnan(mf):
+                ok, kc = False, j
+                break
+        cth = 1.0
+        if k > 1 and ok:
+            a, b = psi * dl_old, psi * dl
+            no, nw = math.sqrt(dU_old @ dU_old + a * a), math.sqrt(dU @ dU + b * b)
+            if no > 0 and nw > 0:
+                cth = min(1.0, max(-1.0, (dU_old @ dU + a * b) / (no * nw)))
+        # ASLA
+        opt = float(Id)
+        if cth >= 0.99:
+            opt = max(opt, 12.0)
+        elif cth >= 0.90:
+            opt = max(opt, 8.0)
+        if not ok:
+            ds *= 0.50 if kc < max_iters else 0.75
+        else:
+            sc = min(math.sqrt(opt / max(kc, 1)), 4.0)
+            if 0.90 < sc < 1.10:
+                sc = 1.0
+            ds = min(max(ds * sc, dsmin), dsmax)
+        if ok:
+            if k > 1:       # curvature controller
+                if cth >= 0.99:
+                    ds = min(ds * 1.5, dsmax)
+                elif cth >= 0.90:
+                    pass
+                elif cth >= 0.75:
+                    ds = max(ds * 0.8, dsmin)
+                else:
+                    ds = max(ds * 0.5, dsmin)
+            dU_old, dl_old = dU, dl
+            x = x + dU
+            lam += dl
+            steps.append((k, lam, kc))
+            if on_converged is not None:
+                on_converged(k, lam, x)
+            k += 1
+            # no-progress stop at ds_min
+            if ds <= dsmin * (1.0 + 1e-12):
+                if np_ref is None:
+                    np_ref, np_count, np_path = x.copy(), 0, 0.0
+                else:
+                    np_path += np.linalg.norm(dU)
+                    np_count += 1
+                    if np_count >= 24:
+                        moved = np.linalg.norm(x - np_ref)
+                        stalled = np_path > 0 and moved <= 0.15 * np_path
+                        np_ref, np_count, np_path = x.copy(), 0, 0.0
+                        if stalled:
+                            break
+            else:
+                np_ref, np_count, np_path = None, 0, 0.0
+        elif ds <= dsmin:
+            break
+    return x, lam, steps

@@ -6,7 +6,9 @@ the external load within the shape-function round-off (they are linear in Psi,
 which the device and the oracle compute to ~1e-13, not bit-identically); the converged
 displacement of the 20-increment load path within 1e-8 (inf-norm, relative)
 of the oracle's, with every increment converged (golden fixture written by
-tests/golden/make_newton_fixture.py). The FP32-tangent (mixed) path reports
+tests/golden/make_newton_fixture.py); four arc-length steps with the device
+assembly against the oracle's (the same steps, the state within 1e-8). The
+FP32-tangent (mixed) path reports
 its Newton iteration counts and must converge to the same state within its
 stopping band (1e-7 relative).
 """
@@ -101,6 +103,28 @@ def test_newton_mixed_precision_tangent(c1m):
           f"{int(np.sum(ref['iterations']))}, delta {d_it:+d}), |dx|/|x| {err:.3e}, {dt:.1f} s")
     assert res.all_converged and len(res.converged) == W.NEWTON_INCREMENTS
     assert err <= 1e-7
+
+
+def test_arc_length_matches_oracle(c1m):
+    """ArcLength (ContinuationSolver.h:329-538) on the moment-loaded straight-crease
+    sheet, four steps from x = 0 with psi = 1, ds = 0.05, ds_max = 0.5: the path
+    with the device assembly and the oracle's (oracle/newton.py) take the same
+    steps (lambda within 1e-8 relative, equal corrector iteration counts) to the
+    same state (1e-8, inf-norm, relative)."""
+    import newton as ON
+    sh, orc, gpu = c1m
+    n = 3 * sh.nn
+    x_o, lam_o, st_o = ON.arc_length(ON.OracleProblem(orc, flag_rig=False, nthreads=8), 0.0, np.zeros(n), 1.0,
+                                     ds=0.05, dsmax=0.5, kmax=4)
+    s = CS.ContinuationSolver()
+    s.psi, s.ds, s.dsmax, s.kmax = 1.0, 0.05, 0.5, 4
+    rep = s.arc_length(CS.PlateProblem(gpu, flag_rig=False), 0.0, np.zeros(n), 1.0)
+    print(f"arc length: device {[(round(a.lam, 6), a.iterations) for a in rep.steps]}, "
+          f"oracle {[(round(float(l), 6), it) for _, l, it in st_o]}")
+    assert len(rep.steps) == len(st_o) == 4
+    for a, (_, lam, it) in zip(rep.steps, st_o):
+        assert a.iterations == it and abs(a.lam - lam) <= 1e-8 * abs(lam)
+    assert np.abs(rep.x - x_o).max() <= 1e-8 * np.abs(x_o).max()
 
 
 def test_ls_miura_line_search_matches_oracle():

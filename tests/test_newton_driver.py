@@ -138,3 +138,110 @@ def test_independent_rows_drop_duplicates():
     assert s.ind_rows.size == 3
     assert np.linalg.matrix_rank(J.toarray()[s.ind_rows]) == 3
     assert ON._independent_rows(J).size == 3
+
+
+# ---- ArcLength (ContinuationSolver.h:329-538) -------------------------------------
+def _arclength_cubic(driver):
+    acc = []
+    if driver == "host":
+        s = CS.ContinuationSolver()
+        s.psi, s.ds, s.dsmax = 1.0, 0.2, 0.5
+        s.arc_length(CubicSpring(), 0.0, np.zeros(2), 5.0, lambda k, lam, x, W: acc.append((lam, x.copy())))
+    else:
+        ON.arc_length(CubicSpring(), 0.0, np.zeros(2), 5.0, lambda k, lam, x: acc.append((lam, x.copy())),
+                      ds=0.2, dsmax=0.5, psi=1.0)
+    return acc
+
+
+@pytest.mark.parametrize("driver", ["host", "oracle"])
+def test_arclength_cubic_spring(driver):
+    """tests/consumer/main.cpp:138-160: psi = 1, ds = 0.2, dsmax = 0.5 to lambda = 5;
+    every accepted step on x1 + x1^3 = lambda (1e-8) with x0 = 0 (1e-12), and the
+    last one past lambda = 5."""
+    acc = _arclength_cubic(driver)
+    assert len(acc) > 0 and acc[-1][0] > 5.0
+    assert max(abs(x[1] + x[1] ** 3 - lam) for lam, x in acc) <= 1e-8
+    assert max(abs(x[0]) for _, x in acc) <= 1e-12
+
+
+def test_arclength_drivers_agree():
+    """The host driver and the oracle's restatement take the same steps."""
+    a, b = _arclength_cubic("host"), _arclength_cubic("oracle")
+    assert len(a) == len(b)
+    for (la, xa), (lb, xb) in zip(a, b):
+        assert abs(la - lb) <= 1e-12 * max(1.0, abs(lb)) and np.abs(xa - xb).max() <= 1e-12
+
+
+def test_arclength_follower_torque():
+    """tests/consumer/main.cpp:189-216: psi = 1, ds = dsmax = 0.2 to lambda = 2, on
+    the exact path lambda = k theta / (m sin(beta - theta)) within 1e-7 with and
+    without the load stiffness, fewer tangent assemblies per step with it."""
+    per_step = {}
+    for use in (True, False):
+        prob = FollowerSpring(use)
+        s = CS.ContinuationSolver()
+        s.psi, s.ds, s.dsmax = 1.0, 0.2, 0.2
+        acc = []
+        s.arc_length(prob, 0.0, np.zeros(1), 2.0, lambda k, lam, x, W: acc.append((lam, x[0])))
+        assert len(acc) > 0 and acc[-1][0] > 2.0
+        assert max(prob.path_error(lam, th) for lam, th in acc) <= 1e-7
+        per_step[use] = prob.assemblies / len(acc)
+    assert per_step[True] < per_step[False]
+
+
+class SnapSpring:
+    """One DOF with a non-monotone spring, F_int = x^3 / 3 - x^2 + 0.8 x (stiffness
+    (x - 1)^2 - 0.2), under a dead load lambda: the equilibrium path
+    lambda(x) = F_int(x) rises to a limit point at x = 1 - sqrt(0.2), falls to a
+    second one at 1 + sqrt(0.2) and rises again. Arc-length follows it through both, reversing the load increment after
+    each limit point."""
+    num_dofs = 1
+
+    def internal_force(self, x):
+        return np.array([x[0] ** 3 / 3.0 - x[0] ** 2 + 0.8 * x[0]])
+
+    def assemble(self, x):
+        return sp.csc_matrix(np.array([[x[0] ** 2 - 2.0 * x[0] + 0.8]])), self.internal_force(x)
+
+    def external_force(self, x):
+        return np.array([1.0])
+
+    def load_stiffness(self, x):
+        return None
+
+    def constraints(self, x):
+        return np.zeros(0), sp.csr_matrix((0, 1))
+
+
+def test_arclength_through_limit_points():
+    s = CS.ContinuationSolver()
+    s.psi, s.ds, s.dsmax = 1.0, 0.05, 0.1
+    acc = []
+    rep = s.arc_length(SnapSpring(), 0.0, np.zeros(1), 1.0, lambda k, lam, x, W: acc.append((lam, x[0])))
+    lams = np.array([a for a, _ in acc]); xs = np.array([b for _, b in acc])
+    assert rep.stop == "lambda_tot" and lams[-1] > 1.0
+    assert np.abs(xs ** 3 / 3 - xs ** 2 + 0.8 * xs - lams).max() <= 1e-8
+    assert np.all(np.diff(xs) > 0)                       # the path never turns back in x
+    assert np.any(np.diff(lams) < 0) and rep.reversals >= 1
+    x1, x2 = 1 - math.sqrt(0.2), 1 + math.sqrt(0.2)
+    assert xs.min() < x1 and xs.max() > x2
+
+
+def test_arclength_restart_resumes_the_run(tmp_path):
+    """RestartManager: a run stopped after 5 steps and resumed from its .res record
+    ends in the uninterrupted run's state, with the same work."""
+    full = CS.ContinuationSolver()
+    full.psi, full.ds, full.dsmax = 1.0, 0.2, 0.5
+    ref = full.arc_length(CubicSpring(), 0.0, np.zeros(2), 5.0)
+    path = str(tmp_path / "cubic.res")
+    first = CS.ContinuationSolver()
+    first.psi, first.ds, first.dsmax, first.kmax, first.restart_file = 1.0, 0.2, 0.5, 5, path
+    rep1 = first.arc_length(CubicSpring(), 0.0, np.zeros(2), 5.0)
+    assert rep1.stop == "kmax" and len(rep1.steps) == 5
+    second = CS.ContinuationSolver()
+    second.psi, second.ds, second.dsmax, second.restart_file = 1.0, 0.2, 0.5, path
+    rep2 = second.arc_length(CubicSpring(), 0.0, np.zeros(2), 5.0)
+    assert rep2.restarted_from == 6
+    assert len(rep1.steps) + len(rep2.steps) == len(ref.steps)
+    assert rep2.lam == ref.lam and np.array_equal(rep2.x, ref.x)
+    assert rep2.steps[-1].W_ext == pytest.approx(ref.steps[-1].W_ext, rel=1e-12)
