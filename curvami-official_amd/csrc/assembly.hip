@@ -11,9 +11,10 @@
 // of Assembly.h:290-300 equals
 //   K_IJ = W [ F0 (Psi_i^T A Psi_j) F0^T + x b^T + b y^T + z b b^T + geo I ],
 //   x = F0 Psi_i^T (B beta_j) - Tu_i g_j,  y = F0 Psi_j^T (B beta_i) - Tu_j g_i,
-//   z = beta_i^T D beta_j + (b.v) gradPsi_i^T C0^-1 gradPsi_j,  geo = gradPsi_i^T N gradPsi_j,
-// (A, B, D symmetric), which needs 27 numbers per neighbour and ~130 flop per
-// pair instead of the 3x3 products of the reference.
+//   z = beta_i^T D beta_j + (b.v) g_i . g_j,  geo = gradPsi_i^T N gradPsi_j,
+// (A, B, D symmetric; g_i . g_j = gradPsi_i^T C0^-1 gradPsi_j since C0 = F0^T F0),
+// which needs 27 numbers per neighbour and ~110 flop per pair instead of the 3x3
+// products of the reference.
 //
 // Scatter: a workgroup owns a tile of consecutive face GPs. Its GPs are taken
 // in order; the pairs of one GP hit distinct slots, so they are added to the
@@ -31,36 +32,36 @@ namespace cvm {
 constexpr int TILE_THREADS = 512;    // 8 waves; wave w prepares GPs w and w + 8 of the tile
 constexpr int TILE_GPS = 16;         // GPs per tile (presence masks are 16-bit)
 constexpr int UMAX = 256;            // union nodes per tile (uint8 union indices, at most 255 used)
-constexpr int GS = 23;               // GP record fields (tangent type)
+constexpr int GS = 14;               // GP record fields (tangent type)
 // GP record stride, GP-major: the 16 GPs of a tile start on distinct LDS banks
-// (FP64: 23 doubles = 46 dwords apart; FP32: 26 floats, even so that field
+// (FP64: 15 doubles = 30 dwords apart; FP32: 14 floats, even so that field
 // pairs stay 8-byte aligned for paired loads), so lanes on different GPs read
 // their GP's fields without bank conflicts
-template <typename TT> constexpr int gp_stride() { return sizeof(TT) == 8 ? 23 : 26; }
+template <typename TT> constexpr int gp_stride() { return sizeof(TT) == 8 ? 15 : 14; }
 constexpr uint16_t NO_REC = 0xffff;  // loc entry of a union node that a GP does not see
-// Neighbour record stride (tangent-type elements): FP64 records get one pad
-// element so that records start on 32 different LDS bank positions instead of
-// 16; FP32 records stay at 6 (24 bytes), which keeps them 8-byte aligned for
-// paired loads.
-template <typename TT> constexpr int rec_stride() { return sizeof(TT) == 8 ? 7 : 6; }
+// Neighbour record stride (tangent-type elements): 9 fields; FP64 records are
+// 18 dwords and FP32 records get one pad element (10 floats, 40 bytes) so that
+// they stay 8-byte aligned for paired loads; both start on 32 different LDS
+// bank positions.
+template <typename TT> constexpr int rec_stride() { return sizeof(TT) == 8 ? 9 : 10; }
 
 // neighbour record planes: grad Psi, beta (bending B-row factor,
-// Assembly.h:240-247), Tu (:265-267) in the tangent type TT, and the residual
-// contribution f_local (:249) in FP64
-enum { R_PX = 0, R_PY, R_BE0, R_BE1, R_BE2, R_TU };
+// Assembly.h:240-247), W Tu (:265-267) and g = F0 C0^-1 grad Psi in the tangent
+// type TT, and the residual contribution f_local (:249) in FP64
+enum { R_PX = 0, R_PY, R_BE0, R_BE1, R_BE2, R_TU, R_G0, R_G1, R_G2 };
 enum { R_F0 = 0, R_F1, R_F2 };
 // GP record fields: weight, F0 columns, normal, b.v_curv, W N (membrane resultant times
-// the weight), C0^-1, and the weighted F0 columns W a1, W a2
-enum { G_W = 0, G_A1 = 1, G_A2 = 4, G_B = 7, G_BDV = 10, G_WN = 11, G_CI = 14, G_WA1 = 17, G_WA2 = 20 };
+// the weight)
+enum { G_W = 0, G_A1 = 1, G_A2 = 4, G_B = 7, G_BDV = 10, G_WN = 11 };
 
 // Per tile: (1) wave w prepares GP w of the tile (kinematics, resultants and the
 // per-neighbour records, Assembly.h:210-268); (2) every thread owns touched
 // blocks (lo, hi) of the tile and sums the GPs that couple them, in ascending GP
 // order, from the records of the two nodes: no LDS accumulation, no atomics on
 // values; (3) the tile's residual per union node, again in GP order.
-// The records hold only grad Psi, beta and Tu: g = F0 C0^-1 grad Psi and
-// D beta (and B beta when B != 0) are recomputed per pair from the GP's
-// F0, C0^-1 and the laminate, which keeps the LDS traffic per pair low.
+// The records hold grad Psi, beta, W Tu and g; D beta (and B beta when B != 0)
+// are recomputed per pair from the laminate, which keeps the records (and the
+// LDS traffic per pair) small.
 // TT = double: the FP64 tangent; TT = float: the mixed-precision tangent (FP32
 // pair blocks and FP32 accumulation), with kinematics, resultants and the
 // residual still in FP64.
@@ -214,7 +215,9 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
                 R(R_PX, r) = static_cast<TT>(px); R(R_PY, r) = static_cast<TT>(py);
                 R(R_BE0, r) = static_cast<TT>(be0); R(R_BE1, r) = static_cast<TT>(be1);
                 R(R_BE2, r) = static_cast<TT>(be2);
-                R(R_TU, r) = static_cast<TT>((px * pX + py * pY) - (hxx * M[0] + 2.0 * hxy * M[2] + hyy * M[1]));
+                R(R_TU, r) = static_cast<TT>(W * ((px * pX + py * pY) - (hxx * M[0] + 2.0 * hxy * M[2] + hyy * M[1])));
+                R(R_G0, r) = static_cast<TT>(gI0); R(R_G1, r) = static_cast<TT>(gI1);
+                R(R_G2, r) = static_cast<TT>(gI2);
                 loc[wv * U + ul] = static_cast<uint16_t>(r);   // the record, tile-relative
                 atomicOr(&pm[ul], 1u << wv);
             };
@@ -226,8 +229,7 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
             }
             if (lane == 0) {
                 const double gv[GS] = {W, a1x, a1y, a1z, a2x, a2y, a2z, bx, by, bz, bdv, W * N[0], W * N[1],
-                                       W * N[2], Ci00, Ci01, Ci11, W * a1x, W * a1y, W * a1z, W * a2x,
-                                       W * a2y, W * a2z};
+                                       W * N[2]};
 #pragma unroll
                 for (int f = 0; f < GS; ++f) Gp(f, wv) = static_cast<TT>(gv[f]);
             }
@@ -251,7 +253,7 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
 #pragma unroll
             for (int k = 0; k < 9; ++k) acc[k] = TT(0);
             // Per coupling GP the block is K_IJ = T0 a1^T + T1 a2^T + x b^T + b y^T + W geo I
-            // with the weight folded into T0, T1, x, y (W a1, W a2 from the GP record);
+            // with the weight folded into T0, T1, x, y (W S, W Tu);
             // terms() forms those factors, add() accumulates them into the slot's block.
             // Two coupling GPs per iteration, the first GP's terms added first (ascending
             // GP order).
@@ -259,40 +261,37 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
             auto terms = [&](int wv, Terms& t) {
                 const int ri = loc[wv * U + ua];
                 const int rj = loc[wv * U + ubb];
-                const TT wa1[3] = {Gp(G_WA1, wv), Gp(G_WA1 + 1, wv), Gp(G_WA1 + 2, wv)};
-                const TT wa2[3] = {Gp(G_WA2, wv), Gp(G_WA2 + 1, wv), Gp(G_WA2 + 2, wv)};
+                const TT W = Gp(G_W, wv);
 #pragma unroll
                 for (int r = 0; r < 3; ++r) {
                     t.a1[r] = Gp(G_A1 + r, wv); t.a2[r] = Gp(G_A2 + r, wv); t.bv[r] = Gp(G_B + r, wv);
                 }
-                const TT ci00 = Gp(G_CI, wv), ci01 = Gp(G_CI + 1, wv), ci11 = Gp(G_CI + 2, wv);
                 const TT pxi = R(R_PX, ri), pyi = R(R_PY, ri), pxj = R(R_PX, rj), pyj = R(R_PY, rj);
                 const TT bi0 = R(R_BE0, ri), bi1 = R(R_BE1, ri), bi2 = R(R_BE2, ri);
                 const TT bj0 = R(R_BE0, rj), bj1 = R(R_BE1, rj), bj2 = R(R_BE2, rj);
-                const TT tui = R(R_TU, ri), tuj = R(R_TU, rj);
-                // A Psi_j (3x2) and S = Psi_i^T A Psi_j
+                const TT tui = R(R_TU, ri), tuj = R(R_TU, rj);      // W Tu
+                const TT gi[3] = {R(R_G0, ri), R(R_G1, ri), R(R_G2, ri)};
+                const TT gj[3] = {R(R_G0, rj), R(R_G1, rj), R(R_G2, rj)};
+                // A Psi_j (3x2) and W S = W Psi_i^T A Psi_j
                 const TT ap00 = Ac[0] * pxj + Ac[2] * pyj, ap01 = Ac[1] * pyj + Ac[2] * pxj;
                 const TT ap10 = Ac[3] * pxj + Ac[5] * pyj, ap11 = Ac[4] * pyj + Ac[5] * pxj;
                 const TT ap20 = Ac[6] * pxj + Ac[8] * pyj, ap21 = Ac[7] * pyj + Ac[8] * pxj;
-                const TT S00 = pxi * ap00 + pyi * ap20, S01 = pxi * ap01 + pyi * ap21;
-                const TT S10 = pyi * ap10 + pxi * ap20, S11 = pyi * ap11 + pxi * ap21;
-                // C0^-1 grad Psi for both nodes (g = F0 C0^-1 grad Psi)
-                const TT ci0 = ci00 * pxi + ci01 * pyi, ci1 = ci01 * pxi + ci11 * pyi;
-                const TT cj0 = ci00 * pxj + ci01 * pyj, cj1 = ci01 * pxj + ci11 * pyj;
-                // z = beta_i . D beta_j + (b.v) grad Psi_i^T C0^-1 grad Psi_j
+                const TT S00 = W * (pxi * ap00 + pyi * ap20), S01 = W * (pxi * ap01 + pyi * ap21);
+                const TT S10 = W * (pyi * ap10 + pxi * ap20), S11 = W * (pyi * ap11 + pxi * ap21);
+                // z = beta_i . D beta_j + (b.v) g_i . g_j
                 const TT db0 = Dc[0] * bj0 + Dc[1] * bj1 + Dc[2] * bj2;
                 const TT db1 = Dc[3] * bj0 + Dc[4] * bj1 + Dc[5] * bj2;
                 const TT db2 = Dc[6] * bj0 + Dc[7] * bj1 + Dc[8] * bj2;
-                const TT Wz = Gp(G_W, wv) * (bi0 * db0 + bi1 * db1 + bi2 * db2 + Gp(G_BDV, wv) * (pxi * cj0 + pyi * cj1));
+                const TT Wz = W * (bi0 * db0 + bi1 * db1 + bi2 * db2
+                                   + Gp(G_BDV, wv) * (gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2]));
                 t.wgeo = pxi * (Gp(G_WN, wv) * pxj + Gp(G_WN + 2, wv) * pyj)
                          + pyi * (Gp(G_WN + 2, wv) * pxj + Gp(G_WN + 1, wv) * pyj);
 #pragma unroll
                 for (int r = 0; r < 3; ++r) {
-                    const TT Wgi = wa1[r] * ci0 + wa2[r] * ci1, Wgj = wa1[r] * cj0 + wa2[r] * cj1;
-                    t.xv[r] = -(tui * Wgj);
-                    t.yv[r] = Wz * t.bv[r] - tuj * Wgi;
-                    t.T0[r] = wa1[r] * S00 + wa2[r] * S10;
-                    t.T1[r] = wa1[r] * S01 + wa2[r] * S11;
+                    t.xv[r] = -(tui * gj[r]);
+                    t.yv[r] = Wz * t.bv[r] - tuj * gi[r];
+                    t.T0[r] = t.a1[r] * S00 + t.a2[r] * S10;
+                    t.T1[r] = t.a1[r] * S01 + t.a2[r] * S11;
                 }
                 if constexpr (HAS_B) {
                     const TT bbj0 = Bc[0] * bj0 + Bc[1] * bj1 + Bc[2] * bj2;
@@ -301,12 +300,12 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
                     const TT bbi0 = Bc[0] * bi0 + Bc[1] * bi1 + Bc[2] * bi2;
                     const TT bbi1 = Bc[3] * bi0 + Bc[4] * bi1 + Bc[5] * bi2;
                     const TT bbi2 = Bc[6] * bi0 + Bc[7] * bi1 + Bc[8] * bi2;
-                    const TT m0 = pxi * bbj0 + pyi * bbj2, m1 = pyi * bbj1 + pxi * bbj2;
-                    const TT q0 = pxj * bbi0 + pyj * bbi2, q1 = pyj * bbi1 + pxj * bbi2;
+                    const TT m0 = W * (pxi * bbj0 + pyi * bbj2), m1 = W * (pyi * bbj1 + pxi * bbj2);
+                    const TT q0 = W * (pxj * bbi0 + pyj * bbi2), q1 = W * (pyj * bbi1 + pxj * bbi2);
 #pragma unroll
                     for (int r = 0; r < 3; ++r) {
-                        t.xv[r] += wa1[r] * m0 + wa2[r] * m1;
-                        t.yv[r] += wa1[r] * q0 + wa2[r] * q1;
+                        t.xv[r] += t.a1[r] * m0 + t.a2[r] * m1;
+                        t.yv[r] += t.a1[r] * q0 + t.a2[r] * q1;
                     }
                 }
             };

@@ -471,3 +471,75 @@ def test_hinge_force_is_energy_gradient(c1):
         xm = x.copy(); xm[k] -= h
         fd = (energy(xp) - energy(xm)) / (2 * h)
         assert abs(fd - F[k]) < 1e-6 * np.abs(F).max() + 1e-16
+
+
+# An unsymmetric orthotropic stack (0 / 45 / 90 degrees, unequal thicknesses):
+# A, B and D are all full, so membrane and bending couple (B != 0).
+COUPLED_PLIES = np.array([[1.0, 0.4, 0.25, 0.2, 0.0, 0.004],
+                          [1.0, 0.4, 0.25, 0.2, 45.0, 0.003],
+                          [1.0, 0.4, 0.25, 0.2, 90.0, 0.005]])
+
+
+def _clt(plies):
+    """Classical lamination theory by strain transformation: with engineering
+    strains, the local strains of a ply rotated by theta are T e (T below), and
+    equal strain energy gives Qbar = T^T Q T; A, B, D are the moments 0, 1, 2
+    of Qbar over the thickness, z from -h/2."""
+    h = plies[:, 5].sum()
+    A = np.zeros((3, 3)); B = np.zeros((3, 3)); D = np.zeros((3, 3))
+    z0 = -h / 2
+    for E1, E2, nu12, G12, ang, t in plies:
+        nu21 = nu12 * E2 / E1
+        den = 1 - nu12 * nu21
+        Q = np.array([[E1 / den, nu12 * E2 / den, 0], [nu12 * E2 / den, E2 / den, 0], [0, 0, G12]])
+        m, n = math.cos(math.radians(ang)), math.sin(math.radians(ang))
+        T = np.array([[m * m, n * n, m * n], [n * n, m * m, -m * n], [-2 * m * n, 2 * m * n, m * m - n * n]])
+        Qb = T.T @ Q @ T
+        z1 = z0 + t
+        A += Qb * (z1 - z0); B += Qb * (z1 ** 2 - z0 ** 2) / 2; D += Qb * (z1 ** 3 - z0 ** 3) / 3
+        z0 = z1
+    return A, B, D
+
+
+def test_laminate_abd():
+    """computeABD (Constitutive.h:10-50) against the strain-transformation
+    restatement above for the unsymmetric stack; a symmetric stack has B = 0
+    (to round-off); one isotropic ply gives A = Q t, D = Q t^3 / 12."""
+    p = O.OraclePlate()
+    p.set_nodes(np.zeros(1), np.zeros(1))
+    p.set_plies(COUPLED_PLIES)
+    for got, ref in zip(p.abd(), _clt(COUPLED_PLIES)):
+        assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+    A, B, D = p.abd()
+    assert np.abs(B).max() > 1e-4 * np.abs(A).max() * COUPLED_PLIES[:, 5].sum()
+    assert abs(A[0, 2]) > 1e-3 * A[0, 0] and abs(D[1, 2]) > 1e-3 * D[1, 1]
+    sym = np.vstack([COUPLED_PLIES, COUPLED_PLIES[::-1]])
+    p.set_plies(sym)
+    A, B, _ = p.abd()
+    assert np.abs(B).max() <= 1e-15 * np.abs(A).max()
+    E, nu, t = 2.0, 0.3, 0.01
+    p.set_plies(np.array([[E, E, nu, E / (2 * (1 + nu)), 30.0, t]]))
+    Q = E / (1 - nu * nu) * np.array([[1, nu, 0], [nu, 1, 0], [0, 0, (1 - nu) / 2]])
+    A, B, D = p.abd()
+    assert np.abs(A - Q * t).max() <= 1e-14 * E
+    assert np.abs(B).max() <= 1e-16 * E
+    assert np.abs(D - Q * t ** 3 / 12).max() <= 1e-14 * E * t ** 2
+
+
+def test_coupled_laminate_residual_and_tangent():
+    """With B != 0 the oracle's residual is the gradient of its strain energy
+    (1/2) W (e0 . N + e1 . M), N = A e0 + B e1, M = B^T e0 + D e1, and its
+    tangent is consistent with the residual (< 1e-6)."""
+    sh = W.config1(rho_factor=RHO_FACTOR)
+    p = O.plate_from_sheet(sh)
+    p.set_plies(COUPLED_PLIES)
+    x = W.displacement(sh.nn, 5, 2e-3)
+    F = p.internal_force(x, flag_rig=True)
+    rng = np.random.default_rng(12)
+    for _ in range(3):
+        v = rng.standard_normal(x.size)
+        h = 1e-6
+        e = lambda y: sum(p.energies(y)[:2])
+        fd = (e(x + h * v) - e(x - h * v)) / (2 * h)
+        assert abs(fd - F @ v) <= 1e-7 * np.linalg.norm(F) * np.linalg.norm(v)
+    assert _tangent_check(p, x, 3 * sh.nn, True) < 1e-6
