@@ -22,6 +22,7 @@
 // blocks go to a scratch row. k_reduce_K sums, per K block, its scratch rows in
 // ascending tile order and then its fold contributions in crease-GP order.
 #include <algorithm>
+#include <initializer_list>
 
 #include "cvm_internal.h"
 #include "device_common.h"
@@ -76,8 +77,14 @@ __device__ __forceinline__ int xcd_tile(int b, int nwg) {
     return x * q + min(x, r) + b / NX;
 }
 
-template <bool HAS_B, typename TT, bool PERSIST, bool WIDE>
-__global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((sizeof(TT) == 4 && !HAS_B && !WIDE && !PERSIST) ? 6 : 1))) void k_tile(
+// Laminate classes (template argument LAM): LAM_ORTHO, B = 0 and A, D without
+// the 16/26 couplings (isotropic plies, cross-ply stacks at 0 and 90 degrees
+// whose 16/26 terms vanish); LAM_SYM, B = 0 with full A and D; LAM_COUPLED,
+// B != 0. The orthotropic class skips the products with those zero entries.
+enum { LAM_ORTHO = 0, LAM_SYM = 1, LAM_COUPLED = 2 };
+
+template <int LAM, typename TT, bool PERSIST, bool WIDE>
+__global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((sizeof(TT) == 4 && LAM != LAM_COUPLED && !WIDE && !PERSIST) ? 6 : 1))) void k_tile(
     const int* __restrict__ tile_gp, const int* __restrict__ tile_uoff, const int* __restrict__ tile_unodes,
     const int64_t* __restrict__ tile_toff, const uint16_t* __restrict__ tpairs, const uint32_t* __restrict__ slot_dst,
     const int* __restrict__ off, const uint8_t* __restrict__ uloc, const uint8_t* __restrict__ urank,
@@ -180,7 +187,7 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
             for (int r = 0; r < 3; ++r) {
                 N[r] = A[3 * r] * e0[0] + A[3 * r + 1] * e0[1] + A[3 * r + 2] * e0[2];
                 M[r] = D[3 * r] * e1[0] + D[3 * r + 1] * e1[1] + D[3 * r + 2] * e1[2];
-                if (HAS_B) {
+                if constexpr (LAM == LAM_COUPLED) {
                     N[r] += B[3 * r] * e1[0] + B[3 * r + 1] * e1[1] + B[3 * r + 2] * e1[2];
                     M[r] += B[r] * e0[0] + B[3 + r] * e0[1] + B[6 + r] * e0[2];
                 }
@@ -273,15 +280,19 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
                 const TT gi[3] = {R(R_G0, ri), R(R_G1, ri), R(R_G2, ri)};
                 const TT gj[3] = {R(R_G0, rj), R(R_G1, rj), R(R_G2, rj)};
                 // A Psi_j (3x2) and W S = W Psi_i^T A Psi_j
-                const TT ap00 = Ac[0] * pxj + Ac[2] * pyj, ap01 = Ac[1] * pyj + Ac[2] * pxj;
-                const TT ap10 = Ac[3] * pxj + Ac[5] * pyj, ap11 = Ac[4] * pyj + Ac[5] * pxj;
-                const TT ap20 = Ac[6] * pxj + Ac[8] * pyj, ap21 = Ac[7] * pyj + Ac[8] * pxj;
+                TT ap00 = Ac[0] * pxj, ap01 = Ac[1] * pyj, ap10 = Ac[3] * pxj, ap11 = Ac[4] * pyj;
+                TT ap20 = Ac[8] * pyj, ap21 = Ac[8] * pxj;
+                if constexpr (LAM != LAM_ORTHO) {
+                    ap00 += Ac[2] * pyj; ap01 += Ac[2] * pxj; ap10 += Ac[5] * pyj; ap11 += Ac[5] * pxj;
+                    ap20 += Ac[6] * pxj; ap21 += Ac[7] * pyj;
+                }
                 const TT S00 = W * (pxi * ap00 + pyi * ap20), S01 = W * (pxi * ap01 + pyi * ap21);
                 const TT S10 = W * (pyi * ap10 + pxi * ap20), S11 = W * (pyi * ap11 + pxi * ap21);
                 // z = beta_i . D beta_j + (b.v) g_i . g_j
-                const TT db0 = Dc[0] * bj0 + Dc[1] * bj1 + Dc[2] * bj2;
-                const TT db1 = Dc[3] * bj0 + Dc[4] * bj1 + Dc[5] * bj2;
-                const TT db2 = Dc[6] * bj0 + Dc[7] * bj1 + Dc[8] * bj2;
+                TT db0 = Dc[0] * bj0 + Dc[1] * bj1, db1 = Dc[3] * bj0 + Dc[4] * bj1, db2 = Dc[8] * bj2;
+                if constexpr (LAM != LAM_ORTHO) {
+                    db0 += Dc[2] * bj2; db1 += Dc[5] * bj2; db2 += Dc[6] * bj0 + Dc[7] * bj1;
+                }
                 const TT Wz = W * (bi0 * db0 + bi1 * db1 + bi2 * db2
                                    + Gp(G_BDV, wv) * (gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2]));
                 t.wgeo = pxi * (Gp(G_WN, wv) * pxj + Gp(G_WN + 2, wv) * pyj)
@@ -293,7 +304,7 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
                     t.T0[r] = t.a1[r] * S00 + t.a2[r] * S10;
                     t.T1[r] = t.a1[r] * S01 + t.a2[r] * S11;
                 }
-                if constexpr (HAS_B) {
+                if constexpr (LAM == LAM_COUPLED) {
                     const TT bbj0 = Bc[0] * bj0 + Bc[1] * bj1 + Bc[2] * bj2;
                     const TT bbj1 = Bc[3] * bj0 + Bc[4] * bj1 + Bc[5] * bj2;
                     const TT bbj2 = Bc[6] * bj0 + Bc[7] * bj1 + Bc[8] * bj2;
@@ -575,14 +586,16 @@ __global__ void k_reduce_F(int nn, const int* __restrict__ fc_off, const int* __
 // k_tile instantiation: body couplings, persistent grid, GPs with more than 64
 // neighbours (chunked lanes)
 template <typename TT, bool PERSIST, bool WIDE>
-static auto tile_kernel(bool has_b) {
-    return has_b ? k_tile<true, TT, PERSIST, WIDE> : k_tile<false, TT, PERSIST, WIDE>;
+static auto tile_kernel(int lam) {
+    return lam == LAM_COUPLED ? k_tile<LAM_COUPLED, TT, PERSIST, WIDE>
+         : lam == LAM_SYM     ? k_tile<LAM_SYM, TT, PERSIST, WIDE>
+                              : k_tile<LAM_ORTHO, TT, PERSIST, WIDE>;
 }
 
 template <typename TT>
-static auto tile_kernel(bool has_b, bool persist, bool wide) {
-    return persist ? (wide ? tile_kernel<TT, true, true>(has_b) : tile_kernel<TT, true, false>(has_b))
-                   : (wide ? tile_kernel<TT, false, true>(has_b) : tile_kernel<TT, false, false>(has_b));
+static auto tile_kernel(int lam, bool persist, bool wide) {
+    return persist ? (wide ? tile_kernel<TT, true, true>(lam) : tile_kernel<TT, true, false>(lam))
+                   : (wide ? tile_kernel<TT, false, true>(lam) : tile_kernel<TT, false, false>(lam));
 }
 
 int tile_umax(int max_union) { return std::min(UMAX, (std::max(max_union, 1) + 7) / 8 * 8); }
@@ -659,8 +672,12 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
     const size_t lds = tile_lds_bytes(P.rec_cap, P.max_union, fp32);
     const int umax = tile_umax(P.max_union);
     int slot = tbeg(!tangent ? "tile_residual" : (fp32 ? "tile_residual_tangent_fp32" : "tile_residual_tangent"));
-    const bool has_b = m.B[0] != 0.0 || m.B[1] != 0.0 || m.B[2] != 0.0 || m.B[3] != 0.0 || m.B[4] != 0.0 ||
-                       m.B[5] != 0.0 || m.B[6] != 0.0 || m.B[7] != 0.0 || m.B[8] != 0.0;
+    auto zero = [](const double* a, std::initializer_list<int> ks) {
+        for (int k : ks) if (a[k] != 0.0) return false;
+        return true;
+    };
+    const int lam = !zero(m.B, {0, 1, 2, 3, 4, 5, 6, 7, 8}) ? LAM_COUPLED
+                  : zero(m.A, {2, 5, 6, 7}) && zero(m.D, {2, 5, 6, 7}) ? LAM_ORTHO : LAM_SYM;
     const bool persist = c.tile_grid > 0 && c.tile_grid < P.ntiles;
     const int grid = persist ? c.tile_grid : P.ntiles;
     auto tile = [&](auto kern, auto* scr) {
@@ -671,8 +688,8 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
     };
     const bool wide = P.max_nbr > 64;
     float* scr32 = reinterpret_cast<float*>(ws.scr.p);
-    if (fp32) tile(tile_kernel<float>(has_b, persist, wide), scr32);
-    else tile(tile_kernel<double>(has_b, persist, wide), ws.scr.p);
+    if (fp32) tile(tile_kernel<float>(lam, persist, wide), scr32);
+    else tile(tile_kernel<double>(lam, persist, wide), ws.scr.p);
     CVM_HIP(hipGetLastError());
     tend(slot);
     if (folds) {

@@ -116,36 +116,48 @@ def test_residual_and_tangent(case, elastic, request):
     assert ek <= 1e-12
 
 
-# An unsymmetric orthotropic stack (0 / 45 / 90 degrees, unequal thicknesses):
-# A, B and D full, membrane and bending coupled (B != 0).
-COUPLED_PLIES = np.array([[1.0, 0.4, 0.25, 0.2, 0.0, 0.004],
-                          [1.0, 0.4, 0.25, 0.2, 45.0, 0.003],
-                          [1.0, 0.4, 0.25, 0.2, 90.0, 0.005]])
+# Laminates beyond the isotropic ply of the other cases (which has B = 0 and no
+# 16/26 couplings in A and D):
+# - "coupled": an unsymmetric orthotropic stack (0 / 45 / 90 degrees, unequal
+#   thicknesses), A, B and D full, membrane and bending coupled (B != 0);
+# - "angle": one orthotropic ply at 30 degrees with a power-of-two thickness,
+#   so that B = 0 exactly while A and D have 16/26 terms.
+LAMINATES = {
+    "coupled": np.array([[1.0, 0.4, 0.25, 0.2, 0.0, 0.004],
+                         [1.0, 0.4, 0.25, 0.2, 45.0, 0.003],
+                         [1.0, 0.4, 0.25, 0.2, 90.0, 0.005]]),
+    "angle": np.array([[1.0, 0.4, 0.25, 0.2, 30.0, 0.0078125]]),
+}
 
 
+@pytest.mark.parametrize("lam", ["coupled", "angle"])
 @pytest.mark.parametrize("name", ["straight21", "straight21_rho6"])
-def test_coupled_laminate(name):
-    """B != 0 (the kernels' coupled-laminate instantiations, with up to 64 and
-    with more than 64 neighbours per GP): the device laminate equals the
-    oracle's, the residual and tangent meet the bars above, and the FP32
+def test_laminates(name, lam):
+    """The kernels' laminate instantiations other than the isotropic one, with
+    up to 64 and with more than 64 neighbours per GP: the device laminate equals
+    the oracle's, the residual and tangent meet the bars above, and the FP32
     tangent its 1e-5 bar."""
     sh, orc, gpu = _build(name)
-    orc.set_plies(COUPLED_PLIES)
-    gpu.set_plies(COUPLED_PLIES)
+    plies = LAMINATES[lam]
+    orc.set_plies(plies)
+    gpu.set_plies(plies)
     for a, b in zip(gpu.abd(), orc.abd()):
         assert np.abs(a - b).max() <= 1e-15 * np.abs(b).max()
+    A, B, D = gpu.abd()
+    assert A[0, 2] != 0.0 and D[1, 2] != 0.0
+    assert (np.abs(B).max() > 0.0) == (lam == "coupled")
     x = W.displacement(sh.nn, 2, 1e-3)
     for elastic in (True, False):
         _, _, Kr, Fr = orc.assemble(x, flag_rig=not elastic)
         Fg, Kg = gpu.assemble(x, elastic_folds=elastic)
         ef = np.abs(Fg - Fr).max() / np.abs(Fr).max()
         ek = np.abs(Kg - Kr).max() / np.abs(Kr).max()
-        print(f"{name} coupled elastic={elastic}: F rel {ef:.3e}, K rel {ek:.3e}")
+        print(f"{name} {lam} elastic={elastic}: F rel {ef:.3e}, K rel {ek:.3e}")
         assert ef <= 1e-10
         assert ek <= 1e-12
     _, K32 = gpu.assemble(x, elastic_folds=False, fp32_tangent=True)
     e32 = np.abs(K32 - Kr).max() / np.abs(Kr).max()
-    print(f"{name} coupled FP32 tangent max rel {e32:.3e}")
+    print(f"{name} {lam} FP32 tangent max rel {e32:.3e}")
     assert e32 <= 1e-5
 
 
