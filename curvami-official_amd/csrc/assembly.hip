@@ -492,28 +492,92 @@ __global__ __launch_bounds__(64) void k_fold(int nfgp, const FoldDev* __restrict
 // ---------------------------------------------------------------------------
 // Segmented reductions (one thread per K block / per node).
 // ---------------------------------------------------------------------------
+constexpr int RK_THREADS = 256;   // K blocks per workgroup of k_reduce_K
+constexpr int RK_UNITS = 8;       // 16-byte units per thread per staged chunk
+typedef unsigned int rk_unit __attribute__((ext_vector_type(4)));   // one 16-byte unit
+// rows per chunk: 16 * RK_UNITS * RK_THREADS bytes of LDS hold the chunk's 9 n
+// elements plus the up to U - 1 elements before it in its first 16-byte unit
+template <typename TT> constexpr int rk_chunk() {
+    return (RK_UNITS * RK_THREADS * static_cast<int>(16 / sizeof(TT)) - static_cast<int>(16 / sizeof(TT) - 1)) / 9;
+}
+
+// A workgroup owns RK_THREADS consecutive K blocks, whose scratch rows are one
+// contiguous range. The range is staged through LDS in chunks of 16-byte units
+// (consecutive lanes, consecutive units; the next chunk's loads are issued
+// before the current chunk is summed); each thread adds the rows of its own
+// block from LDS in ascending order, the same order and rounding as a direct
+// walk over the rows. The scratch buffer carries 16 bytes of padding, so the
+// last unit of the last chunk stays inside it.
 template <typename TT>
-__global__ void k_reduce_K(int64_t nblocks, const int64_t* __restrict__ kc_off,
+__global__ __launch_bounds__(RK_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_reduce_K(int64_t nblocks, const int64_t* __restrict__ kc_off,
                            const TT* __restrict__ scr, const int64_t* __restrict__ kf_off,
                            const int* __restrict__ kf_idx, const int* __restrict__ nd_gp,
                            const double* __restrict__ fgp_rec, const double* __restrict__ Gnd, int folds,
                            int exact, const int64_t* __restrict__ base, const int* __restrict__ stride,
                            double* __restrict__ K) {
-    const int64_t blk = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (blk >= nblocks) return;
-    const int64_t bs = base[blk];
-    if (bs < 0) return;
+    constexpr int U = 16 / sizeof(TT);              // elements per unit
+    constexpr int CH = rk_chunk<TT>();
+    __shared__ rk_unit lds_units[RK_UNITS * RK_THREADS];
+    const TT* rows = reinterpret_cast<const TT*>(lds_units);
+    const int tid = threadIdx.x;
+    const int64_t b0 = static_cast<int64_t>(blockIdx.x) * RK_THREADS;
+    const int64_t b1 = b0 + RK_THREADS < nblocks ? b0 + RK_THREADS : nblocks;
+    const int64_t blk = b0 + tid;
+    const bool own = blk < nblocks;
+    const int64_t r0 = kc_off[b0], r1 = kc_off[b1];
+    const int64_t a0 = own ? kc_off[blk] : 0, a1 = own ? kc_off[blk + 1] : 0;
+    // chunk at row c: units [9 c / U, ceil(9 min(r1, c + CH) / U)) of the scratch
+    auto units = [&](int64_t c, int64_t& u0) {
+        u0 = 9 * c / U;
+        const int64_t e1 = 9 * (r1 < c + CH ? r1 : c + CH);
+        return static_cast<int>((e1 + U - 1) / U - u0);
+    };
+    const rk_unit* src = reinterpret_cast<const rk_unit*>(scr);
+    // the chunk's units go to registers first (every lane loads RK_UNITS units,
+    // indices clamped to the chunk's last unit), then to LDS
+    rk_unit buf[RK_UNITS];
+    int64_t fu0 = 0;
+    int fnu = 0;
+    if (r0 < r1) {
+        fnu = units(r0, fu0);
+#pragma unroll
+        for (int j = 0; j < RK_UNITS; ++j) {
+            const int u = tid + j * RK_THREADS;
+            buf[j] = src[fu0 + (u < fnu ? u : fnu - 1)];
+        }
+    }
     TT vt[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) vt[k] = TT(0);
-    // rows in tile order; unrolled so that several rows' loads are in flight
-    const int64_t a0 = kc_off[blk], a1 = kc_off[blk + 1];
-#pragma unroll 4
-    for (int64_t a = a0; a < a1; ++a) {
-        const TT* s = scr + 9 * a;
+    for (int64_t c = r0; c < r1; c += CH) {
+        const int64_t u0 = fu0;
+        const int nu = fnu;
 #pragma unroll
-        for (int k = 0; k < 9; ++k) vt[k] += s[k];
+        for (int j = 0; j < RK_UNITS; ++j) {
+            const int u = tid + j * RK_THREADS;
+            if (u < nu) lds_units[u] = buf[j];
+        }
+        __syncthreads();
+        if (c + CH < r1) {
+            fnu = units(c + CH, fu0);
+#pragma unroll
+            for (int j = 0; j < RK_UNITS; ++j) {
+                const int u = tid + j * RK_THREADS;
+                buf[j] = src[fu0 + (u < fnu ? u : fnu - 1)];
+            }
+        }
+        const int shift = static_cast<int>(9 * c - u0 * U);
+        const int64_t lo = a0 > c ? a0 : c, hi = a1 < c + CH ? a1 : c + CH;
+        for (int64_t a = lo; a < hi; ++a) {
+            const TT* s = rows + shift + 9 * (a - c);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) vt[k] += s[k];
+        }
+        __syncthreads();
     }
+    if (!own) return;
+    const int64_t bs = base[blk];
+    if (bs < 0) return;
     double v[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) v[k] = static_cast<double>(vt[k]);
@@ -614,7 +678,7 @@ size_t assembly_workspace_bytes(const Context& c, int flags) {
     const bool tangent = (flags & CVM_TANGENT) != 0;
     const bool fp32 = (flags & CVM_FP32_TANGENT) != 0;
     size_t b = sizeof(double) * std::max<int64_t>(3 * P.nfscratch, 1);
-    if (tangent) b += (fp32 ? sizeof(float) : sizeof(double)) * std::max<int64_t>(9 * P.nscratch, 1);
+    if (tangent) b += (fp32 ? sizeof(float) : sizeof(double)) * 9 * P.nscratch + 16;
     b += sizeof(double) * (static_cast<size_t>(FGP_REC) * P.nfgp + static_cast<size_t>(FND_REC) *
                            std::max<int64_t>(P.nfnodes, 1));
     return b;
@@ -625,7 +689,8 @@ void reserve_workspace(const Context& c, Workspace& ws, int flags) {
     ws.fscr.ensure(std::max<int64_t>(3 * P.nfscratch, 1));
     if (flags & CVM_TANGENT) {
         const bool fp32 = (flags & CVM_FP32_TANGENT) != 0;
-        ws.scr.ensure(std::max<int64_t>(fp32 ? (9 * P.nscratch + 1) / 2 : 9 * P.nscratch, 1));
+        // + 16 bytes: k_reduce_K reads the scratch in whole 16-byte units
+        ws.scr.ensure((fp32 ? (9 * P.nscratch + 1) / 2 : 9 * P.nscratch) + 2);
     }
     ws.fold_gp.ensure(std::max<int64_t>(static_cast<int64_t>(FGP_REC) * P.nfgp, 1));
     ws.fold_nd.ensure(static_cast<size_t>(FND_REC) * std::max<int64_t>(P.nfnodes, 1));
@@ -711,7 +776,8 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
         const bool uni = (flags & CVM_ELASTIC_FOLDS) != 0;
         slot = tbeg("reduce_K");
         auto red = [&](auto kern, auto* scr) {
-            hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>((P.nblocks + 255) / 256)), dim3(256), 0, st,
+            hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>((P.nblocks + RK_THREADS - 1) / RK_THREADS)),
+                               dim3(RK_THREADS), 0, st,
                                P.nblocks, P.d_kc_off.p, scr, P.d_kf_off.p, P.d_kf_idx.p, P.d_fnd_gp.p,
                                ws.fold_gp.p, ws.fold_nd.p, folds ? 1 : 0, exact ? 1 : 0,
                                uni ? P.d_blk_base_u.p : P.d_blk_base_f.p,
