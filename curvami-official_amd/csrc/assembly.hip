@@ -33,27 +33,32 @@ namespace cvm {
 constexpr int TILE_THREADS = 512;    // 8 waves; wave w prepares GPs w and w + 8 of the tile
 constexpr int TILE_GPS = 16;         // GPs per tile (presence masks are 16-bit)
 constexpr int UMAX = 256;            // union nodes per tile (uint8 union indices, at most 255 used)
-constexpr int GS = 14;               // GP record fields (tangent type)
+constexpr int GS = 17;               // GP record fields (tangent type)
 // GP record stride, GP-major: the 16 GPs of a tile start on distinct LDS banks
-// (FP64: 15 doubles = 30 dwords apart; FP32: 14 floats, even so that field
+// (FP64: 17 doubles = 34 dwords apart; FP32: 18 floats, even so that field
 // pairs stay 8-byte aligned for paired loads), so lanes on different GPs read
 // their GP's fields without bank conflicts
-template <typename TT> constexpr int gp_stride() { return sizeof(TT) == 8 ? 15 : 14; }
+template <typename TT> constexpr int gp_stride() { return sizeof(TT) == 8 ? 17 : 18; }
 constexpr uint16_t NO_REC = 0xffff;  // loc entry of a union node that a GP does not see
-// Neighbour record stride (tangent-type elements): 9 fields; FP64 records are
-// 18 dwords and FP32 records get one pad element (10 floats, 40 bytes) so that
-// they stay 8-byte aligned for paired loads; both start on 32 different LDS
-// bank positions.
-template <typename TT> constexpr int rec_stride() { return sizeof(TT) == 8 ? 9 : 10; }
+// Neighbour records (tangent-type elements). With up to 64 neighbours per GP
+// (GREC) a record also carries g = F0 C0^-1 grad Psi: 9 fields, FP64 records of
+// 18 dwords, FP32 records padded to 10 floats (40 bytes) so that they stay
+// 8-byte aligned for paired loads. Wider supports keep the compact record (6
+// fields; FP64 padded to 7, FP32 at 6) and rebuild g per pair from the GP's
+// F0 and C0^-1, so that a tile of 16 GPs with ~110 neighbours each still fits
+// the LDS. All four layouts start records on 32 different LDS bank positions.
+template <typename TT, bool GREC> constexpr int rec_stride() {
+    return GREC ? (sizeof(TT) == 8 ? 9 : 10) : (sizeof(TT) == 8 ? 7 : 6);
+}
 
 // neighbour record planes: grad Psi, beta (bending B-row factor,
-// Assembly.h:240-247), W Tu (:265-267) and g = F0 C0^-1 grad Psi in the tangent
-// type TT, and the residual contribution f_local (:249) in FP64
+// Assembly.h:240-247), W Tu (:265-267) and (GREC) g = F0 C0^-1 grad Psi in the
+// tangent type TT, and the residual contribution f_local (:249) in FP64
 enum { R_PX = 0, R_PY, R_BE0, R_BE1, R_BE2, R_TU, R_G0, R_G1, R_G2 };
 enum { R_F0 = 0, R_F1, R_F2 };
 // GP record fields: weight, F0 columns, normal, b.v_curv, W N (membrane resultant times
-// the weight)
-enum { G_W = 0, G_A1 = 1, G_A2 = 4, G_B = 7, G_BDV = 10, G_WN = 11 };
+// the weight), C0^-1
+enum { G_W = 0, G_A1 = 1, G_A2 = 4, G_B = 7, G_BDV = 10, G_WN = 11, G_CI = 14 };
 
 // Per tile: (1) wave w prepares GP w of the tile (kinematics, resultants and the
 // per-neighbour records, Assembly.h:210-268); (2) every thread owns touched
@@ -104,7 +109,8 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
     double* recf = smem;                                        // 3 FP64 planes of rec_cap
     double* uc = recf + static_cast<size_t>(rec_cap) * 3;       // umax * 3
     TT* rect = reinterpret_cast<TT*>(uc + umax * 3);            // rec_cap records of RS TT
-    constexpr int RS = rec_stride<TT>();
+    constexpr bool GREC = !WIDE;
+    constexpr int RS = rec_stride<TT, GREC>();
     constexpr int GSP = gp_stride<TT>();
     TT* gs = rect + static_cast<size_t>(rec_cap) * RS;          // TILE_GPS records of GSP
     uint32_t* pm = reinterpret_cast<uint32_t*>(gs + TILE_GPS * GSP);  // umax presence masks
@@ -223,8 +229,10 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
                 R(R_BE0, r) = static_cast<TT>(be0); R(R_BE1, r) = static_cast<TT>(be1);
                 R(R_BE2, r) = static_cast<TT>(be2);
                 R(R_TU, r) = static_cast<TT>(W * ((px * pX + py * pY) - (hxx * M[0] + 2.0 * hxy * M[2] + hyy * M[1])));
-                R(R_G0, r) = static_cast<TT>(gI0); R(R_G1, r) = static_cast<TT>(gI1);
-                R(R_G2, r) = static_cast<TT>(gI2);
+                if constexpr (GREC) {
+                    R(R_G0, r) = static_cast<TT>(gI0); R(R_G1, r) = static_cast<TT>(gI1);
+                    R(R_G2, r) = static_cast<TT>(gI2);
+                }
                 loc[wv * U + ul] = static_cast<uint16_t>(r);   // the record, tile-relative
                 atomicOr(&pm[ul], 1u << wv);
             };
@@ -236,7 +244,7 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
             }
             if (lane == 0) {
                 const double gv[GS] = {W, a1x, a1y, a1z, a2x, a2y, a2z, bx, by, bz, bdv, W * N[0], W * N[1],
-                                       W * N[2]};
+                                       W * N[2], Ci00, Ci01, Ci11};
 #pragma unroll
                 for (int f = 0; f < GS; ++f) Gp(f, wv) = static_cast<TT>(gv[f]);
             }
@@ -277,8 +285,21 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
                 const TT bi0 = R(R_BE0, ri), bi1 = R(R_BE1, ri), bi2 = R(R_BE2, ri);
                 const TT bj0 = R(R_BE0, rj), bj1 = R(R_BE1, rj), bj2 = R(R_BE2, rj);
                 const TT tui = R(R_TU, ri), tuj = R(R_TU, rj);      // W Tu
-                const TT gi[3] = {R(R_G0, ri), R(R_G1, ri), R(R_G2, ri)};
-                const TT gj[3] = {R(R_G0, rj), R(R_G1, rj), R(R_G2, rj)};
+                TT gi[3], gj[3];
+                if constexpr (GREC) {
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) { gi[r] = R(R_G0 + r, ri); gj[r] = R(R_G0 + r, rj); }
+                } else {
+                    // g = F0 C0^-1 grad Psi from the GP's frame
+                    const TT ci00 = Gp(G_CI, wv), ci01 = Gp(G_CI + 1, wv), ci11 = Gp(G_CI + 2, wv);
+                    const TT ci0 = ci00 * pxi + ci01 * pyi, ci1 = ci01 * pxi + ci11 * pyi;
+                    const TT cj0 = ci00 * pxj + ci01 * pyj, cj1 = ci01 * pxj + ci11 * pyj;
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) {
+                        gi[r] = t.a1[r] * ci0 + t.a2[r] * ci1;
+                        gj[r] = t.a1[r] * cj0 + t.a2[r] * cj1;
+                    }
+                }
                 // A Psi_j (3x2) and W S = W Psi_i^T A Psi_j
                 TT ap00 = Ac[0] * pxj, ap01 = Ac[1] * pyj, ap10 = Ac[3] * pxj, ap11 = Ac[4] * pyj;
                 TT ap20 = Ac[8] * pyj, ap21 = Ac[8] * pxj;
@@ -501,6 +522,61 @@ template <typename TT> constexpr int rk_chunk() {
     return (RK_UNITS * RK_THREADS * static_cast<int>(16 / sizeof(TT)) - static_cast<int>(16 / sizeof(TT) - 1)) / 9;
 }
 
+// The block's fold terms added to its face sum vt, and the 3x3 block written
+// into the CSC of K.
+template <typename TT>
+__device__ __forceinline__ void finish_K_block(int64_t blk, int64_t bs, const TT (&vt)[9],
+                                               const int64_t* __restrict__ kf_off, const int* __restrict__ kf_idx,
+                                               const int* __restrict__ nd_gp, const double* __restrict__ fgp_rec,
+                                               const double* __restrict__ Gnd, int folds, int exact,
+                                               const int* __restrict__ stride, double* __restrict__ K) {
+    double v[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) v[k] = static_cast<double>(vt[k]);
+    if (folds) {
+        double f[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) f[k] = 0.0;
+        for (int64_t a = kf_off[blk]; a < kf_off[blk + 1]; ++a) {
+            const int lo = kf_idx[2 * a], hi = kf_idx[2 * a + 1];
+            const double* rec = fgp_rec + static_cast<int64_t>(FGP_REC) * nd_gp[lo];
+            const double wk = rec[0];
+            const double* Gl = Gnd + static_cast<int64_t>(FND_REC) * lo;
+            const double* Gh = Gnd + static_cast<int64_t>(FND_REC) * hi;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int r = 0; r < 3; ++r) f[3 * c + r] += Gl[r] * Gh[c] * wk;
+            if (exact) {
+                // W moment * B(lo, r)^T H B(hi, c); B(node, d) has dX_L, dY_L, dX_R, dY_R
+                // at positions d, d + 3, 6 + d, 9 + d of the 12-vector
+                const double wm = rec[1];
+                const double* H = rec + 2;
+                const double bl[4] = {Gl[3], Gl[4], Gl[5], Gl[6]};
+                const double bh[4] = {Gh[3], Gh[4], Gh[5], Gh[6]};
+                for (int c = 0; c < 3; ++c) {
+                    const int qc[4] = {c, c + 3, 6 + c, 9 + c};
+                    for (int r = 0; r < 3; ++r) {
+                        const int pr[4] = {r, r + 3, 6 + r, 9 + r};
+                        double acc = 0.0;
+                        for (int u = 0; u < 4; ++u)
+                            for (int v = 0; v < 4; ++v) acc += bl[u] * H[12 * pr[u] + qc[v]] * bh[v];
+                        f[3 * c + r] += wm * acc;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) v[k] += f[k];
+    }
+    const int st = stride[blk];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) K[bs + static_cast<int64_t>(c) * st + r] = v[3 * c + r];
+}
+
+
 // A workgroup owns RK_THREADS consecutive K blocks, whose scratch rows are one
 // contiguous range. The range is staged through LDS in chunks of 16-byte units
 // (consecutive lanes, consecutive units; the next chunk's loads are issued
@@ -578,50 +654,35 @@ __global__ __launch_bounds__(RK_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
     if (!own) return;
     const int64_t bs = base[blk];
     if (bs < 0) return;
-    double v[9];
+    finish_K_block(blk, bs, vt, kf_off, kf_idx, nd_gp, fgp_rec, Gnd, folds, exact, stride, K);
+}
+
+// Direct walk: one lane per block adds its contiguous scratch rows in
+// ascending order (the staged kernel's order and rounding). Used when blocks
+// have many rows (large supports), where the staged kernel would leave most
+// lanes of a chunk idle while a few walk long row ranges.
+template <typename TT>
+__global__ __launch_bounds__(RK_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_reduce_K_rows(int64_t nblocks, const int64_t* __restrict__ kc_off,
+                           const TT* __restrict__ scr, const int64_t* __restrict__ kf_off,
+                           const int* __restrict__ kf_idx, const int* __restrict__ nd_gp,
+                           const double* __restrict__ fgp_rec, const double* __restrict__ Gnd, int folds,
+                           int exact, const int64_t* __restrict__ base, const int* __restrict__ stride,
+                           double* __restrict__ K) {
+    const int64_t blk = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (blk >= nblocks) return;
+    const int64_t bs = base[blk];
+    if (bs < 0) return;
+    TT vt[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) v[k] = static_cast<double>(vt[k]);
-    if (folds) {
-        double f[9];
+    for (int k = 0; k < 9; ++k) vt[k] = TT(0);
+    const int64_t a0 = kc_off[blk], a1 = kc_off[blk + 1];
+#pragma unroll 4
+    for (int64_t a = a0; a < a1; ++a) {
+        const TT* s = scr + 9 * a;
 #pragma unroll
-        for (int k = 0; k < 9; ++k) f[k] = 0.0;
-        for (int64_t a = kf_off[blk]; a < kf_off[blk + 1]; ++a) {
-            const int lo = kf_idx[2 * a], hi = kf_idx[2 * a + 1];
-            const double* rec = fgp_rec + static_cast<int64_t>(FGP_REC) * nd_gp[lo];
-            const double wk = rec[0];
-            const double* Gl = Gnd + static_cast<int64_t>(FND_REC) * lo;
-            const double* Gh = Gnd + static_cast<int64_t>(FND_REC) * hi;
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int r = 0; r < 3; ++r) f[3 * c + r] += Gl[r] * Gh[c] * wk;
-            if (exact) {
-                // W moment * B(lo, r)^T H B(hi, c); B(node, d) has dX_L, dY_L, dX_R, dY_R
-                // at positions d, d + 3, 6 + d, 9 + d of the 12-vector
-                const double wm = rec[1];
-                const double* H = rec + 2;
-                const double bl[4] = {Gl[3], Gl[4], Gl[5], Gl[6]};
-                const double bh[4] = {Gh[3], Gh[4], Gh[5], Gh[6]};
-                for (int c = 0; c < 3; ++c) {
-                    const int qc[4] = {c, c + 3, 6 + c, 9 + c};
-                    for (int r = 0; r < 3; ++r) {
-                        const int pr[4] = {r, r + 3, 6 + r, 9 + r};
-                        double acc = 0.0;
-                        for (int u = 0; u < 4; ++u)
-                            for (int v = 0; v < 4; ++v) acc += bl[u] * H[12 * pr[u] + qc[v]] * bh[v];
-                        f[3 * c + r] += wm * acc;
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < 9; ++k) v[k] += f[k];
+        for (int k = 0; k < 9; ++k) vt[k] += s[k];
     }
-    const int st = stride[blk];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int r = 0; r < 3; ++r) K[bs + static_cast<int64_t>(c) * st + r] = v[3 * c + r];
+    finish_K_block(blk, bs, vt, kf_off, kf_idx, nd_gp, fgp_rec, Gnd, folds, exact, stride, K);
 }
 
 __global__ void k_reduce_F(int nn, const int* __restrict__ fc_off, const int* __restrict__ fc_idx,
@@ -664,11 +725,12 @@ static auto tile_kernel(int lam, bool persist, bool wide) {
 
 int tile_umax(int max_union) { return std::min(UMAX, (std::max(max_union, 1) + 7) / 8 * 8); }
 
-size_t tile_lds_bytes(int rec_cap, int max_union, bool fp32) {
+size_t tile_lds_bytes(int rec_cap, int max_union, bool fp32, bool wide) {
     const size_t tt = fp32 ? sizeof(float) : sizeof(double);
     const size_t um = static_cast<size_t>(tile_umax(max_union));
     return sizeof(double) * (static_cast<size_t>(rec_cap) * 3 + um * 3)
-         + tt * (static_cast<size_t>(rec_cap) * (fp32 ? rec_stride<float>() : rec_stride<double>()) +
+         + tt * (static_cast<size_t>(rec_cap) * (fp32 ? (wide ? rec_stride<float, false>() : rec_stride<float, true>())
+                                                      : (wide ? rec_stride<double, false>() : rec_stride<double, true>())) +
                  TILE_GPS * (fp32 ? gp_stride<float>() : gp_stride<double>())) + sizeof(uint32_t) * um
          + sizeof(uint16_t) * TILE_GPS * um;
 }
@@ -734,7 +796,8 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
     std::memcpy(m.B, c.B, sizeof(m.B));
     std::memcpy(m.D, c.D, sizeof(m.D));
     reserve_workspace(c, ws, flags);
-    const size_t lds = tile_lds_bytes(P.rec_cap, P.max_union, fp32);
+    const bool wide = P.max_nbr > 64;
+    const size_t lds = tile_lds_bytes(P.rec_cap, P.max_union, fp32, wide);
     const int umax = tile_umax(P.max_union);
     int slot = tbeg(!tangent ? "tile_residual" : (fp32 ? "tile_residual_tangent_fp32" : "tile_residual_tangent"));
     auto zero = [](const double* a, std::initializer_list<int> ks) {
@@ -751,7 +814,6 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
                            P.d_pair_uloc.p, P.d_pair_urank.p, fd.sf.p, static_cast<int64_t>(fd.npairs), fd.d_w.p, xdev, m, P.rec_cap,
                            umax, tangent ? 1 : 0, scr, ws.fscr.p, P.ntiles);
     };
-    const bool wide = P.max_nbr > 64;
     float* scr32 = reinterpret_cast<float*>(ws.scr.p);
     if (fp32) tile(tile_kernel<float>(lam, persist, wide), scr32);
     else tile(tile_kernel<double>(lam, persist, wide), ws.scr.p);
@@ -783,8 +845,11 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
                                uni ? P.d_blk_base_u.p : P.d_blk_base_f.p,
                                uni ? P.d_blk_stride_u.p : P.d_blk_stride_f.p, Kdev);
         };
-        if (fp32) red(k_reduce_K<float>, scr32);
-        else red(k_reduce_K<double>, ws.scr.p);
+        // staged through LDS up to 24 scratch rows per block on average (supports
+        // of ~4 h), the direct walk beyond
+        const bool staged = P.nscratch <= 24 * P.nblocks;
+        if (fp32) red(staged ? k_reduce_K<float> : k_reduce_K_rows<float>, scr32);
+        else red(staged ? k_reduce_K<double> : k_reduce_K_rows<double>, ws.scr.p);
         CVM_HIP(hipGetLastError());
         tend(slot);
     }

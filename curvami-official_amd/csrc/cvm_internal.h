@@ -319,7 +319,7 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
 size_t assembly_workspace_bytes(const Context& c, int flags);
 void reserve_workspace(const Context& c, Workspace& ws, int flags);
 void upload_fold_descriptors(Context& c);
-size_t tile_lds_bytes(int rec_cap, int max_union, bool fp32);
+size_t tile_lds_bytes(int rec_cap, int max_union, bool fp32, bool wide);
 int tile_umax(int max_union);
 constexpr int kTileGps = 16;    // GPs per assembly tile
 constexpr int kTileUnion = 255; // union nodes per assembly tile (uint8 union indices)

@@ -143,7 +143,7 @@ void build_plan(Context& c) {
         P.max_union = max_union;
         P.rec_cap = std::max(rec_cap, 1);
         P.tile_max_gps = tg;
-        if (tile_lds_bytes(P.rec_cap, P.max_union, false) <= c.lds_limit) break;
+        if (tile_lds_bytes(P.rec_cap, P.max_union, false, P.max_nbr > 64) <= c.lds_limit) break;
         if (tg == 1) throw Error(CVM_E_INVALID, "assembly tile of one Gauss point exceeds the LDS limit");
     }
     // per face pair: the rank of its node among the GP's neighbours in union
