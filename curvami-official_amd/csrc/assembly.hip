@@ -91,7 +91,8 @@ enum { LAM_ORTHO = 0, LAM_SYM = 1, LAM_COUPLED = 2 };
 template <int LAM, typename TT, bool PERSIST, bool WIDE>
 __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((sizeof(TT) == 4 && LAM != LAM_COUPLED && !WIDE && !PERSIST) ? 6 : 1))) void k_tile(
     const int* __restrict__ tile_gp, const int* __restrict__ tile_uoff, const int* __restrict__ tile_unodes,
-    const int64_t* __restrict__ tile_toff, const uint16_t* __restrict__ tpairs, const uint32_t* __restrict__ slot_dst,
+    const int64_t* __restrict__ tile_toff, const uint16_t* __restrict__ tpairs, const int* __restrict__ tile_nd,
+    const uint32_t* __restrict__ slot_dst,
     const int* __restrict__ off, const uint8_t* __restrict__ uloc, const uint8_t* __restrict__ urank,
     const double* __restrict__ sf, int64_t npairs, const double* __restrict__ w,
     const double* __restrict__ xu, MatC mat, int rec_cap, int umax, int with_tangent,
@@ -260,20 +261,30 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
         }
         // ---- pair blocks (Assembly.h:282-324) for the canonical (lo, hi) orientation:
         // K_IJ with I the neighbour entry of lo and J that of hi (K_JI = K_IJ^T).
-        for (int sl = tid; sl < T; sl += TILE_THREADS) {
+        // Work items: the tile's nd paired slots (two blocks with the same lower
+        // node and the same coupling GPs, summed by one thread from one read of the
+        // GP and lower-node records per GP), then its single slots. FP32 and the
+        // coupled laminate take every slot singly: the second block would not fit
+        // their register budgets (six waves per SIMD; 128 VGPRs) without spilling or
+        // losing a resident workgroup.
+        const int nd = (sizeof(TT) == 8 && LAM != LAM_COUPLED) ? tile_nd[t] : 0;
+        for (int it = tid; it < T - nd; it += TILE_THREADS) {
+            const bool paired = it < nd;
+            const int sl = paired ? 2 * it : nd + it;
             const uint16_t pr = tpairs[tb + sl];
-            const int ua = pr >> 8, ubb = pr & 0xff;
-            uint32_t mask = pm[ua] & pm[ubb];
-            TT acc[9];
+            const int ua = pr >> 8, ub1 = pr & 0xff;
+            const int ub2 = paired ? (tpairs[tb + sl + 1] & 0xff) : ub1;
+            uint32_t mask = pm[ua] & pm[ub1];
+            TT acc[9], acc2[9];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) acc[k] = TT(0);
+            for (int k = 0; k < 9; ++k) { acc[k] = TT(0); acc2[k] = TT(0); }
             // Per coupling GP the block is K_IJ = T0 a1^T + T1 a2^T + x b^T + b y^T + W geo I
             // with the weight folded into T0, T1, x, y (W S, W Tu);
             // terms() forms those factors, add() accumulates them into the slot's block.
             // Two coupling GPs per iteration, the first GP's terms added first (ascending
             // GP order).
             struct Terms { TT T0[3], T1[3], xv[3], yv[3], a1[3], a2[3], bv[3], wgeo; };
-            auto terms = [&](int wv, Terms& t) {
+            auto terms = [&](int wv, int ubb, Terms& t) {
                 const int ri = loc[wv * U + ua];
                 const int rj = loc[wv * U + ubb];
                 const TT W = Gp(G_W, wv);
@@ -341,36 +352,54 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
                     }
                 }
             };
-            auto add = [&](const Terms& t) {
+            auto add = [&](TT (&a)[9], const Terms& t) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
 #pragma unroll
                     for (int r = 0; r < 3; ++r) {
-                        TT v = acc[3 * c + r] + t.T0[r] * t.a1[c] + t.T1[r] * t.a2[c] + t.xv[r] * t.bv[c]
+                        TT v = a[3 * c + r] + t.T0[r] * t.a1[c] + t.T1[r] * t.a2[c] + t.xv[r] * t.bv[c]
                                + t.bv[r] * t.yv[c];
                         if (r == c) v += t.wgeo;
-                        acc[3 * c + r] = v;
+                        a[3 * c + r] = v;
                     }
             };
-            while (mask) {
-                const int w0 = __ffs(mask) - 1;
-                mask &= mask - 1;
-                Terms t0;
-                terms(w0, t0);
-                if (mask) {
-                    const int w1 = __ffs(mask) - 1;
+            if (paired) {
+                // one coupling GP per iteration, both blocks
+                while (mask) {
+                    const int w0 = __ffs(mask) - 1;
                     mask &= mask - 1;
-                    Terms t1;
-                    terms(w1, t1);
-                    add(t0);
-                    add(t1);
-                } else {
-                    add(t0);
+                    Terms t0, t1;
+                    terms(w0, ub1, t0);
+                    terms(w0, ub2, t1);
+                    add(acc, t0);
+                    add(acc2, t1);
+                }
+            } else {
+                while (mask) {
+                    const int w0 = __ffs(mask) - 1;
+                    mask &= mask - 1;
+                    Terms t0;
+                    terms(w0, ub1, t0);
+                    if (mask) {
+                        const int w1 = __ffs(mask) - 1;
+                        mask &= mask - 1;
+                        Terms t1;
+                        terms(w1, ub1, t1);
+                        add(acc, t0);
+                        add(acc, t1);
+                    } else {
+                        add(acc, t0);
+                    }
                 }
             }
             TT* dst = scr + 9 * static_cast<int64_t>(slot_dst[tb + sl]);
 #pragma unroll
             for (int k = 0; k < 9; ++k) dst[k] = acc[k];
+            if (paired) {
+                TT* dst2 = scr + 9 * static_cast<int64_t>(slot_dst[tb + sl + 1]);
+#pragma unroll
+                for (int k = 0; k < 9; ++k) dst2[k] = acc2[k];
+            }
         }
     }
     // ---- residual per union node, GPs in order ---------------------------------------
@@ -810,7 +839,7 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
     const int grid = persist ? c.tile_grid : P.ntiles;
     auto tile = [&](auto kern, auto* scr) {
         hipLaunchKernelGGL(kern, dim3(grid), dim3(TILE_THREADS), lds, st, P.d_tile_gp.p, P.d_tile_uoff.p,
-                           P.d_tile_unodes.p, P.d_tile_toff.p, P.d_tile_tpairs.p, P.d_slot_dst.p, fd.off.p,
+                           P.d_tile_unodes.p, P.d_tile_toff.p, P.d_tile_tpairs.p, P.d_tile_nd.p, P.d_slot_dst.p, fd.off.p,
                            P.d_pair_uloc.p, P.d_pair_urank.p, fd.sf.p, static_cast<int64_t>(fd.npairs), fd.d_w.p, xdev, m, P.rec_cap,
                            umax, tangent ? 1 : 0, scr, ws.fscr.p, P.ntiles);
     };

@@ -207,7 +207,8 @@ struct Plan {
     DBuf<int> d_tile_uoff;      // ntiles + 1
     DBuf<int> d_tile_unodes;    // union node ids
     DBuf<int64_t> d_tile_toff;  // ntiles + 1 (touched pairs)
-    DBuf<uint16_t> d_tile_tpairs;   // (ua << 8) | ub
+    DBuf<uint16_t> d_tile_tpairs;   // (ua << 8) | ub; per tile its paired slots first
+    DBuf<int> d_tile_nd;        // ntiles: paired slots of the tile (pairs of slots)
     DBuf<uint8_t> d_pair_uloc;  // per face pair: union-local index
     DBuf<uint8_t> d_pair_urank; // per face pair: rank of its node among the GP's neighbours in union order
     int64_t nscratch = 0;       // total touched pairs

@@ -68,6 +68,7 @@ void build_plan(Context& c) {
     // ---- tiles and touched pairs (tile size shrinks until the LDS fits) -----
     std::vector<int> tile_gp, tile_uoff, unodes;
     std::vector<int64_t> tile_toff;
+    std::vector<int> tile_nd;       // per tile: paired slots (pairs of slots)
     std::vector<uint8_t> uloc(ids.size());
     std::vector<uint16_t> tpairs;
     std::vector<int> stamp(nn, -1), where(nn, -1);
@@ -106,8 +107,12 @@ void build_plan(Context& c) {
         // (2) touched pairs of each tile (parallel over tiles): (ua, ub) coupled by at
         // least one GP; sorted by the number of coupling GPs (descending), then by the
         // presence mask, so that the lanes of a wave walk equally long, mostly
-        // identical GP sequences
+        // identical GP sequences. Consecutive pairs with the same mask and the same
+        // lower node are paired (one thread sums both blocks and reads their shared
+        // records once per GP): the tile's list holds its paired slots first, two
+        // by two, then the single ones.
         std::vector<std::vector<uint16_t>> tl(P.ntiles);
+        tile_nd.assign(P.ntiles, 0);
         int max_touched = 0, max_union = 0, rec_cap = 0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(max : max_touched, max_union, rec_cap)
         for (int ti = 0; ti < P.ntiles; ++ti) {
@@ -127,8 +132,22 @@ void build_plan(Context& c) {
                 if (cx != cy) return cx > cy;
                 return x.first < y.first;
             });
-            tl[ti].resize(tlist.size());
-            for (size_t k = 0; k < tlist.size(); ++k) tl[ti][k] = tlist[k].second;
+            std::vector<uint16_t> sgl;
+            tl[ti].clear();
+            tl[ti].reserve(tlist.size());
+            for (size_t k = 0; k < tlist.size();) {
+                if (k + 1 < tlist.size() && tlist[k].first == tlist[k + 1].first &&
+                    (tlist[k].second >> 8) == (tlist[k + 1].second >> 8)) {
+                    tl[ti].push_back(tlist[k].second);
+                    tl[ti].push_back(tlist[k + 1].second);
+                    k += 2;
+                } else {
+                    sgl.push_back(tlist[k].second);
+                    ++k;
+                }
+            }
+            tile_nd[ti] = static_cast<int>(tl[ti].size() / 2);
+            tl[ti].insert(tl[ti].end(), sgl.begin(), sgl.end());
             max_touched = std::max(max_touched, static_cast<int>(tlist.size()));
             max_union = std::max(max_union, U);
             rec_cap = std::max(rec_cap, off[q1] - off[q0]);
@@ -378,6 +397,7 @@ void build_plan(Context& c) {
     P.d_tile_unodes.upload(unodes, s);
     P.d_tile_toff.upload(tile_toff, s);
     P.d_tile_tpairs.upload(tpairs, s);
+    P.d_tile_nd.upload(tile_nd, s);
     P.d_pair_uloc.upload(uloc, s);
     P.d_pair_urank.upload(urank, s);
     P.d_kc_off.upload(kc_cnt, s);
