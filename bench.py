@@ -1,11 +1,13 @@
 """Benchmark of the per-Gauss-point hot path on one MI355X.
 
-Workload (one step): RKPM shape functions (order 2, quartic window, crease
-visibility) for every integration domain of the 201 x 201 arc-crease sheet,
-then one residual + block-upper CSC tangent assembly with the elastic hinge,
-all on device-resident data. Neighbour search and the assembly plan are the
-geometry setup (once per sheet, as initSparsityPattern in the reference) and
-are reported separately.
+Workload (one step), on device-resident data: the neighbour search of every
+integration domain of the 201 x 201 arc-crease sheet (done again on the
+device, as SetupHelper::recomputeShapeFunctions searches before every RKPM
+solve), the crease indicators and node sides, the RKPM shape functions (order
+2, quartic window, crease visibility), then one residual + block-upper CSC
+tangent assembly with the elastic hinge. The assembly plan is the geometry
+setup (once per sheet, as initSparsityPattern in the reference) and is
+reported separately.
 
 Prints one JSON line. `value` = face Gauss points per second of the step.
 """
@@ -88,11 +90,24 @@ class Hip:
         self.L.hipDeviceSynchronize()
 
 
-def cpu_baseline(sh, threads):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_baseline(sh, threads, reps=5):
     """The FP64 oracle (restated reference CPU path, OpenMP structure of the
     reference: per-thread F/K copies, guided(8), index-ordered merge) on a
     bounded sample of the same sheet: the face GPs of cell rows 95..104
-    (10 of 200 rows, 32,000 GPs, crossed by the arc) plus every fold GP."""
+    (10 of 200 rows, 32,000 GPs, crossed by the arc) plus every fold GP. Timed
+    scope as the device step: neighbour search + shape functions of every
+    domain, then one assembleSystem; median of `reps` runs after one warm-up."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     O.build()
@@ -103,30 +118,42 @@ def cpu_baseline(sh, threads):
     sample = W.Sheet(sh.name + "_rows95-104", sh.n_side, sh.X, sh.Y, sub, sh.creases, sh.folds, sh.rho,
                      sh.order, sh.kernel, sh.E, sh.nu, sh.t)
     out = {}
+    x = W.displacement(sample.nn, 1, 1e-3)
     for T in sorted({1, threads}):
-        log(f"cpu baseline: {sub.n} face GPs on {T} thread(s)")
+        log(f"cpu baseline: {sub.n} face GPs on {T} thread(s), 1 + {reps} runs")
         p = O.plate_from_sheet(sample, compute=False)
-        for d in range(len(p.domains)):
-            p.neighbours(d, sample.rho)
-        t0 = time.perf_counter()
-        for d in range(len(p.domains)):
-            st, _, _ = p.shape_functions(d, sample.rho, sample.order, sample.kernel, nthreads=T)
-            assert st == 0
-        t_sf = time.perf_counter() - t0
-        assert p.init_assembly() == 0
-        x = W.displacement(sample.nn, 1, 1e-3)
-        p.assemble(x, flag_rig=False, nthreads=T)          # warm-up
-        t0 = time.perf_counter()
-        p.assemble(x, flag_rig=False, nthreads=T)
-        t_asm = time.perf_counter() - t0
-        out[T] = (sub.n / (t_sf + t_asm), t_sf, t_asm)
+        runs = []
+        for r in range(reps + 1):
+            t0 = time.perf_counter()
+            for d in range(len(p.domains)):
+                p.neighbours(d, sample.rho)
+            t_nb = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            for d in range(len(p.domains)):
+                st, _, _ = p.shape_functions(d, sample.rho, sample.order, sample.kernel, nthreads=T)
+                assert st == 0
+            t_sf = time.perf_counter() - t0
+            if r == 0:
+                assert p.init_assembly() == 0
+            t0 = time.perf_counter()
+            p.assemble(x, flag_rig=False, nthreads=T)
+            t_asm = time.perf_counter() - t0
+            if r > 0:
+                runs.append((t_nb + t_sf + t_asm, t_nb, t_sf, t_asm))
+        runs.sort()
+        med = runs[len(runs) // 2]
+        out[T] = (sub.n / med[0],) + med[1:]
+        p = None
     return out, sub.n
 
 
 def _step_time(plate, ndom, order, kernel, xd, Fd, Kd, flags, steps, warmup, graph=False):
-    """Seconds per step: shape functions of every domain (queued back to back,
-    one status read-back), then one evaluation (eager, or the captured graph)."""
+    """Seconds per step: the neighbour search of every domain again on the
+    device, shape functions of every domain (queued back to back, one status
+    read-back), then one evaluation (eager, or the captured graph)."""
     def step():
+        if plate.refresh_neighbours():
+            raise RuntimeError("neighbour lists changed on unchanged geometry")
         plate.shape_functions_all(order, kernel)
         if graph:
             plate.graph_launch()
@@ -254,6 +281,10 @@ def large_leg(warmup, steps):
         log(f"large: timing the {name} step")
         dt = _step_time(p, nd, sh.order, sh.kernel, x, F, K, fl, steps, warmup, graph=True)
         res[name] = {"gps": ng / dt, "ms_per_step": dt * 1e3}
+        Fh = F.get()
+        if not np.all(np.isfinite(Fh)):
+            raise RuntimeError(f"large leg: non-finite residual ({name})")
+        res[name]["max_abs_F"] = float(np.abs(Fh).max())
     rows, cols = p.constraint_layout(rigid=False)
     xv = W.displacement(sh.nn, 2, 1e-3)
     p.constraints(xv)
@@ -268,11 +299,36 @@ def large_leg(warmup, steps):
     return res
 
 
+def bspline_leg(n_side, warmup, steps):
+    """The main step on the same sheet with the configurations' window, the
+    cubic B-spline (the reference has the quartic spline only): neighbour search
+    + shape functions + residual + CSC tangent, FP64 and mixed."""
+    log(f"bspline: {n_side} x {n_side} arc-crease sheet, cubic B-spline window, set-up")
+    sh = W.config2(rho_factor=RHO_FACTOR, n_side=n_side, kernel=1)
+    p = G.plate_from_sheet(sh)
+    ng = sh.face.n
+    x = G.DeviceArray(3 * sh.nn); x.set(W.displacement(sh.nn, 1, 1e-3))
+    F = G.DeviceArray(3 * sh.nn); K = G.DeviceArray(p.nnz_union)
+    res = {"workload": f"arc-crease sheet {n_side}x{n_side} nodes, {ng} face GPs, rho 3.75h, quadratic RKPM, "
+                       f"cubic B-spline window, elastic hinge; step as the main one"}
+    base = G.RESIDUAL | G.TANGENT | G.ELASTIC_FOLDS
+    for name, fl in (("fp64", base), ("mixed", base | G.FP32_TANGENT)):
+        dt = _step_time(p, len(p.domain_sizes), sh.order, sh.kernel, x, F, K, fl, steps, warmup)
+        res[name] = {"gps": ng / dt, "ms_per_step": dt * 1e3}
+    Fh = F.get()
+    assert np.all(np.isfinite(Fh))
+    for a in (x, F, K):
+        a.free()
+    p.close()
+    return res
+
+
 def defaults_leg(n_side, warmup, steps):
     """The same arc-crease sheet at the reference's own defaults: cubic basis
     (order 3) and support 6 h (6 x the 3rd-nearest-node spacing), about 110
-    neighbours per GP. Step = shape functions of every domain + residual +
-    tangent, as in the main leg, with per-stage device times."""
+    neighbours per GP. Step = neighbour search + shape functions of every
+    domain + residual + tangent, as in the main leg, with per-stage device
+    times."""
     import dataclasses
     log(f"defaults: {n_side} x {n_side} arc-crease sheet, cubic basis, rho = 6h, set-up")
     sh = dataclasses.replace(W.config2(rho_factor=6.0, n_side=n_side), order=3)
@@ -293,6 +349,7 @@ def defaults_leg(n_side, warmup, steps):
     for name, fl in (("fp64", base), ("mixed", base | G.FP32_TANGENT)):
         log(f"defaults: timing the {name} step")
         for _ in range(warmup):
+            p.refresh_neighbours()
             p.shape_functions_all(sh.order, sh.kernel)
             p.assemble_device(x.ptr, F.ptr, K.ptr, fl)
         p.synchronize()
@@ -300,6 +357,8 @@ def defaults_leg(n_side, warmup, steps):
         p.reset_profiling()
         t0 = time.perf_counter()
         for _ in range(steps):
+            if p.refresh_neighbours():
+                raise RuntimeError("neighbour lists changed on unchanged geometry")
             p.shape_functions_all(sh.order, sh.kernel)
             p.assemble_device(x.ptr, F.ptr, K.ptr, fl)
         p.synchronize()
@@ -354,6 +413,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-sheet batched leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-large", action="store_true", help="skip the 1001 x 1001 leg")
+    ap.add_argument("--no-bspline", action="store_true", help="skip the cubic B-spline window leg")
     ap.add_argument("--no-defaults", action="store_true",
                     help="skip the leg at the reference's defaults (cubic basis, rho = 6h)")
     ap.add_argument("--batch-sheets", type=int, default=W.BATCH_SHEETS)
@@ -389,20 +449,22 @@ def main():
     hip.h2d(xd, x)
     flags = G.RESIDUAL | G.TANGENT | G.ELASTIC_FOLDS
 
-    def step(extra=0):
+    def step(extra=0, search=True):
+        if search and p.refresh_neighbours():
+            raise RuntimeError("neighbour lists changed on unchanged geometry")
         p.shape_functions_all(sh.order, sh.kernel)
         p.assemble_device(xd, Fd, Kd, flags | extra)
 
-    def timed(extra):
+    def timed(extra, search=True):
         for _ in range(args.warmup):
-            step(extra)
+            step(extra, search)
         p.synchronize()
         p.set_profiling(True)
         p.reset_profiling()
         hip.sync()
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            step(extra)
+            step(extra, search)
         p.synchronize()
         hip.sync()
         dt_ = (time.perf_counter() - t0) / args.steps
@@ -410,12 +472,15 @@ def main():
         p.set_profiling(False)
         return dt_, st_
 
-    # plain FP64 path (the reported value), then the mixed path (FP32 tangent)
+    # plain FP64 path (the reported value), then the mixed path (FP32 tangent);
+    # the same two steps without the neighbour search for comparison
     log(f"main: {args.warmup} + {args.steps} FP64 steps")
     dt, stats = timed(0)
     log(f"main: FP64 {dt * 1e3:.3f} ms/step; {args.warmup} + {args.steps} mixed steps")
     dt_mixed, stats_mixed = timed(G.FP32_TANGENT)
-    log(f"main: mixed {dt_mixed * 1e3:.3f} ms/step")
+    log(f"main: mixed {dt_mixed * 1e3:.3f} ms/step; the same steps without the neighbour search")
+    dt_ns, _ = timed(0, search=False)
+    dt_ns_mixed, _ = timed(G.FP32_TANGENT, search=False)
     kern_ms = {k: v[0] / max(v[1], 1) for k, v in stats.items()}
     F = np.empty(3 * sh.nn)
     hip.d2h(F, Fd)
@@ -443,9 +508,10 @@ def main():
         tj = json.load(open(tfile))
         if dom_name in tj and tj[dom_name].get("workload") == sh.name:
             traffic = tj[dom_name]["bytes_per_launch"]
-    roofline = {"kernel": dom_name, "bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+    roofline = {"kernel": dom_name, "bound": "valu", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic,
-                "precision": "fp64 (vector = matrix peak on MI355X)",
+                "precision": "fp64 VALU (no MFMA; the FP64 vector and matrix peaks are equal on MI355X); "
+                             "limited by VALU issue and LDS latency",
                 "launch_ms": round(cands[dom_name][1], 4)}
     out = {
         "metric": "Gauss pts/s (shape fns+R+K CSR)",
@@ -461,27 +527,39 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"arc-crease sheet {args.n_side}x{args.n_side} nodes, {ng} face GPs (4x4/cell), "
-                               f"rho=3.75h, quadratic RKPM, quartic window, elastic hinge; step = shape functions "
-                               f"(all {ndom} domains) + residual + CSC tangent", "nodes": sh.nn, "face_gps": ng,
+                               f"rho=3.75h, quadratic RKPM, quartic window, elastic hinge; step = neighbour search "
+                               f"+ crease indicators + shape functions (all {ndom} domains) + residual + CSC "
+                               f"tangent", "nodes": sh.nn, "face_gps": ng,
                    "pairs": int(p.npairs[0]), "nnz_K": int(nnz_union), "mean_neighbours": float(nper.mean())},
         "stages_ms": {k: round(v, 4) for k, v in kern_ms.items()},
         "mixed": {"value": ng / dt_mixed, "ms_per_step": dt_mixed * 1e3, "speedup_vs_fp64": dt / dt_mixed,
                   "dtype": "f64 shape functions, kinematics, residual, hinge; f32 face tangent",
                   "stages_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in stats_mixed.items()}},
-        "setup_s": {"neighbour_search": round(t_frnn, 3), "assembly_plan": round(t_plan, 3)},
+        "without_neighbour_search": {"scope": "shape functions + residual + CSC tangent (neighbour lists and "
+                                              "indicators kept from the previous step)",
+                                     "fp64": {"value": ng / dt_ns, "ms_per_step": dt_ns * 1e3},
+                                     "mixed": {"value": ng / dt_ns_mixed, "ms_per_step": dt_ns_mixed * 1e3}},
+        "setup_s": {"first_neighbour_search_host": round(t_frnn, 3), "assembly_plan": round(t_plan, 3)},
         "roofline": roofline,
         "asm_hbm_bytes_algorithmic": B_asm,
     }
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        # the CPU share of this process: a GPU box grants 16 host cores per GPU,
+        # whatever the machine's core count
+        share = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        threads = args.cpu_threads or min(16, share)
         res, nsub = cpu_baseline(sh, threads)
         best_T = max(res)
         out["cpu_baseline"] = {"value": res[best_T][0], "unit": "GP/s", "cores": best_T, "kind": "port",
                                "sample": f"oracle (restated reference path, OpenMP) on {nsub} face GPs of cell rows "
-                                         f"95-104 of the same sheet + all fold GPs: shape functions + one "
-                                         f"assembleSystem; 1 thread: {res[1][0]:.0f} GP/s",
-                               "seconds": {str(T): [round(v[1], 3), round(v[2], 3)] for T, v in res.items()},
-                               "cpu": platform.processor() or platform.machine()}
+                                         f"95-104 of the same sheet + all fold GPs: neighbour search + shape "
+                                         f"functions + one assembleSystem, median of 5 runs after one warm-up; "
+                                         f"1 thread: {res[1][0]:.0f} GP/s",
+                               "seconds_median": {str(T): {"neighbours": round(v[1], 4), "shape_functions":
+                                                           round(v[2], 4), "assembly": round(v[3], 4)}
+                                                  for T, v in res.items()},
+                               "cpu": cpu_model(), "nproc": os.cpu_count(), "cpus_allowed": share,
+                               "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
     # the reference's default basis (cubic, order 3) on the face domain of the
     # same sheet: shape functions only, device time per launch
     log("main: cubic-basis shape functions (face domain)")
@@ -504,6 +582,8 @@ def main():
     for ptr in (xd, Fd, Kd):
         hip.free(ptr)
     p.close()
+    if not args.no_bspline:
+        out["cubic_bspline_window"] = bspline_leg(args.n_side, args.warmup, args.steps)
     if not args.no_defaults:
         out["reference_defaults"] = defaults_leg(args.n_side, 1, max(3, args.steps // 3))
     if not args.no_batched:

@@ -371,6 +371,26 @@ int cvm_neighbours(cvm_context* ctx, int domain, double rho, int64_t* npairs) {
     CVM_CATCH(ctx)
 }
 
+int cvm_refresh_neighbours(cvm_context* ctx, int* changed) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    CVM_HIP(hipSetDevice(c.device));
+    bool any = false, pattern = false;
+    const std::vector<char> ch = cvm::refresh_frnn_all(c);
+    for (size_t k = 0; k < c.doms.size(); ++k) {
+        cvm::Domain* d = c.doms[k];
+        if (!ch[k]) continue;
+        any = true;
+        ++d->gen;
+        if (d->kind == CVM_DOMAIN_FACE || d->kind == CVM_DOMAIN_FOLD_LEFT || d->kind == CVM_DOMAIN_FOLD_RIGHT)
+            pattern = true;
+    }
+    if (pattern) c.plan.ready = false;
+    c.node_sign_creases = -1;          // node sides are evaluated again with the next shape functions
+    if (changed) *changed = any ? 1 : 0;
+    CVM_CATCH(ctx)
+}
+
 int cvm_get_neighbours(cvm_context* ctx, int domain, int* off, int* ids, double* s) {
     CVM_TRY(ctx)
     Context& c = ctx->c;
@@ -732,14 +752,20 @@ int cvm_graph_capture(cvm_context* ctx, int flags, const double* x_dev, double* 
     if (!x_dev || !F_dev || ((flags & CVM_TANGENT) && !K_dev)) throw Error(CVM_E_INVALID, "null device pointer");
     if (c.graph_exec) { (void)hipGraphExecDestroy(c.graph_exec); c.graph_exec = nullptr; }
     if (c.graph) { (void)hipGraphDestroy(c.graph); c.graph = nullptr; }
-    // everything the launches need is allocated before the capture starts
-    cvm::reserve_workspace(c, c.ws, flags);
+    // everything the launches need is allocated before the capture starts; an
+    // FP32-tangent capture also reserves the FP64 scratch, so that an eager FP64
+    // evaluation between replays (a mixed-precision Newton loop's finishing
+    // step) does not reallocate the captured scratch
+    const int eflags = flags & ~CVM_DEVICE_PTRS;
+    cvm::reserve_workspace(c, c.ws, eflags & ~CVM_FP32_TANGENT);
+    cvm::reserve_workspace(c, c.ws, eflags);
     CVM_HIP(hipStreamSynchronize(c.stream));
     const bool timed = c.ws.timed;
     c.ws.timed = false;
+    cvm::LaunchSig sig;
     CVM_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
     try {
-        cvm::launch_assembly(c, c.ws, flags & ~CVM_DEVICE_PTRS, x_dev, F_dev, K_dev);
+        cvm::launch_assembly(c, c.ws, eflags, x_dev, F_dev, K_dev, &sig);
     } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(c.stream, &g);
@@ -750,6 +776,9 @@ int cvm_graph_capture(cvm_context* ctx, int flags, const double* x_dev, double* 
     c.ws.timed = timed;
     CVM_HIP(hipStreamEndCapture(c.stream, &c.graph));
     CVM_HIP(hipGraphInstantiate(&c.graph_exec, c.graph, nullptr, nullptr, 0));
+    c.graph_sig = sig.h;
+    c.graph_flags = eflags;
+    c.graph_x = x_dev; c.graph_F = F_dev; c.graph_K = K_dev;
     CVM_CATCH(ctx)
 }
 
@@ -758,6 +787,13 @@ int cvm_graph_launch(cvm_context* ctx) {
     Context& c = ctx->c;
     if (!c.graph_exec) throw Error(CVM_E_STATE, "no captured graph (cvm_graph_capture)");
     if (!c.plan.ready) throw Error(CVM_E_STATE, "the assembly plan changed since the capture");
+    // the graph is valid while the eager path would issue the captured launches
+    cvm::LaunchSig sig;
+    sig.dry = true;
+    cvm::launch_assembly(c, c.ws, c.graph_flags, c.graph_x, c.graph_F, c.graph_K, &sig);
+    if (sig.grow || sig.h != c.graph_sig)
+        throw Error(CVM_E_STATE, "the assembly changed since the capture (plan, laminate, shape functions, "
+                                 "workspace or tile grid): capture again (cvm_graph_capture)");
     CVM_HIP(hipSetDevice(c.device));
     const int slot = cvm::timing_begin(c, "assembly_graph");
     CVM_HIP(hipGraphLaunch(c.graph_exec, c.stream));

@@ -775,16 +775,50 @@ size_t assembly_workspace_bytes(const Context& c, int flags) {
     return b;
 }
 
-void reserve_workspace(const Context& c, Workspace& ws, int flags) {
-    const Plan& P = c.plan;
-    ws.fscr.ensure(std::max<int64_t>(3 * P.nfscratch, 1));
+// Element counts of the four workspace buffers: fscr, scr (0 without the
+// tangent), fold_gp, fold_nd.
+static void workspace_counts(const Plan& P, int flags, size_t (&n)[4]) {
+    n[0] = static_cast<size_t>(std::max<int64_t>(3 * P.nfscratch, 1));
+    n[1] = 0;
     if (flags & CVM_TANGENT) {
         const bool fp32 = (flags & CVM_FP32_TANGENT) != 0;
         // + 16 bytes: k_reduce_K reads the scratch in whole 16-byte units
-        ws.scr.ensure((fp32 ? (9 * P.nscratch + 1) / 2 : 9 * P.nscratch) + 2);
+        n[1] = static_cast<size_t>((fp32 ? (9 * P.nscratch + 1) / 2 : 9 * P.nscratch) + 2);
     }
-    ws.fold_gp.ensure(std::max<int64_t>(static_cast<int64_t>(FGP_REC) * P.nfgp, 1));
-    ws.fold_nd.ensure(static_cast<size_t>(FND_REC) * std::max<int64_t>(P.nfnodes, 1));
+    n[2] = static_cast<size_t>(std::max<int64_t>(static_cast<int64_t>(FGP_REC) * P.nfgp, 1));
+    n[3] = static_cast<size_t>(FND_REC) * static_cast<size_t>(std::max<int64_t>(P.nfnodes, 1));
+}
+
+void reserve_workspace(const Context& c, Workspace& ws, int flags) {
+    size_t n[4];
+    workspace_counts(c.plan, flags, n);
+    ws.fscr.ensure(n[0]);
+    if (n[1]) ws.scr.ensure(n[1]);
+    ws.fold_gp.ensure(n[2]);
+    ws.fold_nd.ensure(n[3]);
+}
+
+bool workspace_would_grow(const Context& c, const Workspace& ws, int flags) {
+    size_t n[4];
+    workspace_counts(c.plan, flags, n);
+    auto grows = [](const auto& b, size_t k) { return k > b.n || (k && !b.p); };
+    return grows(ws.fscr, n[0]) || (n[1] && grows(ws.scr, n[1])) || grows(ws.fold_gp, n[2]) ||
+           grows(ws.fold_nd, n[3]);
+}
+
+// A kernel launch, or (sig->dry) only its contribution to the launch signature.
+template <typename K, typename... A>
+static void launch_k(LaunchSig* sig, K kern, dim3 grid, dim3 block, size_t lds, hipStream_t st, A... args) {
+    if (sig) {
+        const void* kp = reinterpret_cast<const void*>(kern);
+        sig->mix(&kp, sizeof(kp));
+        const unsigned dims[6] = {grid.x, grid.y, grid.z, block.x, block.y, block.z};
+        sig->mix(dims, sizeof(dims));
+        sig->mix(&lds, sizeof(lds));
+        (sig->mix(&args, sizeof(args)), ...);
+        if (sig->dry) return;
+    }
+    hipLaunchKernelGGL(kern, grid, block, lds, st, args...);
 }
 
 void upload_fold_descriptors(Context& c) {
@@ -807,15 +841,17 @@ void upload_fold_descriptors(Context& c) {
 // One residual (+ tangent) evaluation on workspace ws. Reads only the plan and
 // the shape functions, so evaluations on different workspaces may run
 // concurrently on different streams.
-void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, double* Fdev, double* Kdev) {
+void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, double* Fdev, double* Kdev,
+                     LaunchSig* sig) {
     Plan& P = c.plan;
+    const bool dry = sig && sig->dry;
     if (!P.ready) throw Error(CVM_E_STATE, "assembly plan not initialised (cvm_init_assembly)");
     Domain* fdp = nullptr;
     for (Domain* d : c.doms) if (d->kind == CVM_DOMAIN_FACE) fdp = d;
     Domain& fd = *fdp;
     const hipStream_t st = ws.stream;
-    auto tbeg = [&](const char* name) { return ws.timed ? timing_begin(c, name) : -1; };
-    auto tend = [&](int slot) { if (ws.timed) timing_end(c, slot); };
+    auto tbeg = [&](const char* name) { return (ws.timed && !dry) ? timing_begin(c, name) : -1; };
+    auto tend = [&](int slot) { if (ws.timed && !dry) timing_end(c, slot); };
     const bool tangent = (flags & CVM_TANGENT) != 0;
     const bool folds = (flags & CVM_ELASTIC_FOLDS) != 0 && P.nfgp > 0;
     const bool exact = folds && tangent && (flags & CVM_EXACT_HINGE) != 0;
@@ -824,7 +860,11 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
     std::memcpy(m.A, c.A, sizeof(m.A));
     std::memcpy(m.B, c.B, sizeof(m.B));
     std::memcpy(m.D, c.D, sizeof(m.D));
-    reserve_workspace(c, ws, flags);
+    if (dry) {
+        if (workspace_would_grow(c, ws, flags)) sig->grow = true;
+    } else {
+        reserve_workspace(c, ws, flags);
+    }
     const bool wide = P.max_nbr > 64;
     const size_t lds = tile_lds_bytes(P.rec_cap, P.max_union, fp32, wide);
     const int umax = tile_umax(P.max_union);
@@ -838,7 +878,7 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
     const bool persist = c.tile_grid > 0 && c.tile_grid < P.ntiles;
     const int grid = persist ? c.tile_grid : P.ntiles;
     auto tile = [&](auto kern, auto* scr) {
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(TILE_THREADS), lds, st, P.d_tile_gp.p, P.d_tile_uoff.p,
+        launch_k(sig, kern, dim3(grid), dim3(TILE_THREADS), lds, st, P.d_tile_gp.p, P.d_tile_uoff.p,
                            P.d_tile_unodes.p, P.d_tile_toff.p, P.d_tile_tpairs.p, P.d_tile_nd.p, P.d_slot_dst.p, fd.off.p,
                            P.d_pair_uloc.p, P.d_pair_urank.p, fd.sf.p, static_cast<int64_t>(fd.npairs), fd.d_w.p, xdev, m, P.rec_cap,
                            umax, tangent ? 1 : 0, scr, ws.fscr.p, P.ntiles);
@@ -846,28 +886,28 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
     float* scr32 = reinterpret_cast<float*>(ws.scr.p);
     if (fp32) tile(tile_kernel<float>(lam, persist, wide), scr32);
     else tile(tile_kernel<double>(lam, persist, wide), ws.scr.p);
-    CVM_HIP(hipGetLastError());
+    if (!dry) CVM_HIP(hipGetLastError());
     tend(slot);
     if (folds) {
         slot = tbeg("fold_hinge");
-        if (P.nfgp > 0) hipLaunchKernelGGL(k_fold, dim3(P.nfgp), dim3(64), 0, st, P.nfgp,
+        if (P.nfgp > 0) launch_k(sig, k_fold, dim3(P.nfgp), dim3(64), 0, st, P.nfgp,
                            reinterpret_cast<const FoldDev*>(P.d_folddev.p), P.d_fgp_fold.p, P.d_fgp_local.p,
                            P.d_fgp_noff.p, P.d_fgp_lpo.p, P.d_fgp_rpo.p, P.d_fgp_lpos.p, P.d_fgp_rpos.p, xdev,
                            exact ? 1 : 0, ws.fold_gp.p, ws.fold_nd.p);
-        CVM_HIP(hipGetLastError());
+        if (!dry) CVM_HIP(hipGetLastError());
         tend(slot);
     }
     slot = tbeg("reduce_F");
-    hipLaunchKernelGGL(k_reduce_F, dim3((c.nn + 255) / 256), dim3(256), 0, st, c.nn, P.d_fc_off.p, P.d_fc_idx.p,
+    launch_k(sig, k_reduce_F, dim3((c.nn + 255) / 256), dim3(256), 0, st, c.nn, P.d_fc_off.p, P.d_fc_idx.p,
                        ws.fscr.p, P.d_ff_off.p, P.d_ff_idx.p, P.d_fnd_gp.p, ws.fold_gp.p, ws.fold_nd.p,
                        folds ? 1 : 0, Fdev);
-    CVM_HIP(hipGetLastError());
+    if (!dry) CVM_HIP(hipGetLastError());
     tend(slot);
     if (tangent) {
         const bool uni = (flags & CVM_ELASTIC_FOLDS) != 0;
         slot = tbeg("reduce_K");
         auto red = [&](auto kern, auto* scr) {
-            hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>((P.nblocks + RK_THREADS - 1) / RK_THREADS)),
+            launch_k(sig, kern, dim3(static_cast<unsigned>((P.nblocks + RK_THREADS - 1) / RK_THREADS)),
                                dim3(RK_THREADS), 0, st,
                                P.nblocks, P.d_kc_off.p, scr, P.d_kf_off.p, P.d_kf_idx.p, P.d_fnd_gp.p,
                                ws.fold_gp.p, ws.fold_nd.p, folds ? 1 : 0, exact ? 1 : 0,
@@ -879,7 +919,7 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
         const bool staged = P.nscratch <= 24 * P.nblocks;
         if (fp32) red(staged ? k_reduce_K<float> : k_reduce_K_rows<float>, scr32);
         else red(staged ? k_reduce_K<double> : k_reduce_K_rows<double>, ws.scr.p);
-        CVM_HIP(hipGetLastError());
+        if (!dry) CVM_HIP(hipGetLastError());
         tend(slot);
     }
 }

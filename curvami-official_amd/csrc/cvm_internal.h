@@ -126,6 +126,13 @@ struct Domain {
     DBuf<int> ids;            // npairs
     DBuf<double> s;           // npairs
     std::vector<int> h_off, h_ids;     // host mirrors (needed for the pattern plan)
+    // the node cell grid of the last search (FRNN.h:30-51), kept for
+    // cvm_refresh_neighbours; valid for grid_nn nodes and grid_ng GPs
+    DBuf<int> cell_start, cell_nodes;
+    DBuf<int> cnt, off_new;   // per GP counts and offsets of a refresh
+    DBuf<unsigned char> scan_tmp;
+    DBuf<int> rf_flag;        // refresh change flags (offsets, lists)
+    int grid_nn = -1, grid_ng = -1;
     // The same lists interleaved per block of RKPM_BLOCK GPs for the moment
     // pass, where each lane walks its own GP's pairs: the t-th pair of GP lane
     // l of block b sits at (il_row[b] + t) * RKPM_BLOCK + l, so one wave-wide
@@ -171,6 +178,21 @@ struct ConstraintLayout {
     DBuf<int> d_fx_off;           // 3 nn + 1
     DBuf<int> d_fx_rec;           // 4 ints per contribution: dom, pair, gp, type
     DBuf<double> d_fx_val;        // 3 doubles per contribution: coefficient, n_out x, n_out y
+};
+
+// Signature of the launches one assembly issues: each kernel, its grid, block
+// and LDS size and every argument by value. A captured graph replays exactly
+// the launches of its capture, so it stays valid only while an eager call with
+// the same flags and pointers would issue the same signature (same plan
+// buffers and tile counts, laminate, workspace buffers and tile grid).
+struct LaunchSig {
+    bool dry = false;    // record the signature only: no launch, no allocation
+    bool grow = false;   // (dry) a workspace buffer would have to be reallocated
+    uint64_t h = 1469598103934665603ull;
+    void mix(const void* p, size_t n) {
+        const unsigned char* b = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < n; ++i) { h ^= b[i]; h *= 1099511628211ull; }
+    }
 };
 
 // What one assembly in flight needs besides the plan: its stream, the tile
@@ -307,6 +329,11 @@ struct Context {
     // captured assembly (cvm_graph_capture)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    uint64_t graph_sig = 0;     // launch signature at capture
+    int graph_flags = 0;
+    const double* graph_x = nullptr;
+    double* graph_F = nullptr;
+    double* graph_K = nullptr;
     ~Context();
 };
 
@@ -314,11 +341,18 @@ struct Context {
 void launch_indicator_gp(Context& c, Domain& d);
 void launch_node_signs(Context& c);
 void launch_frnn(Context& c, Domain& d, double rho);
+// Repeats the search of every searched domain on the device (same nodes, GPs
+// and radius): the lists are rewritten in place; per domain, whether its lists
+// changed (lists, mirrors and interleaved copies then replaced).
+std::vector<char> refresh_frnn_all(Context& c);
 void launch_rkpm(Context& c, Domain& d, double rho, int order, int kernel, int* code, int* nbad, int* first);
 void launch_rkpm_all(Context& c, int order, int kernel, int* dom, int* code, int* nbad, int* first);
-void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, double* Fdev, double* Kdev);
+void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, double* Fdev, double* Kdev,
+                     LaunchSig* sig = nullptr);
 size_t assembly_workspace_bytes(const Context& c, int flags);
 void reserve_workspace(const Context& c, Workspace& ws, int flags);
+// true if reserve_workspace(c, ws, flags) would reallocate a buffer
+bool workspace_would_grow(const Context& c, const Workspace& ws, int flags);
 void upload_fold_descriptors(Context& c);
 size_t tile_lds_bytes(int rec_cap, int max_union, bool fp32, bool wide);
 int tile_umax(int max_union);

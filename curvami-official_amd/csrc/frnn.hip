@@ -198,13 +198,16 @@ void launch_indicator_gp(Context& c, Domain& d) {
 // reference does, including the wrap of an out-of-range x into the adjacent
 // row's key; keys outside [0, nkeys) are empty, like a failed map lookup.
 // ---------------------------------------------------------------------------
-template <bool FILL>
+// FILL: write the lists at off (else count per GP); CMP (with FILL): the lists
+// already hold a previous search's result, and any entry that changes sets
+// *changed.
+template <bool FILL, bool CMP = false>
 __global__ void k_frnn(const double* __restrict__ X, const double* __restrict__ Y,
                        const double* __restrict__ gx, const double* __restrict__ gy, int ng,
                        const int* __restrict__ gsheet, const SheetDev* __restrict__ sheets,
                        const int* __restrict__ cstart, const int* __restrict__ cnodes,
                        int* __restrict__ count, const int* __restrict__ off,
-                       int* __restrict__ ids, double* __restrict__ s) {
+                       int* __restrict__ ids, double* __restrict__ s, int* __restrict__ changed = nullptr) {
 #pragma clang fp contract(off)
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= ng) return;
@@ -226,8 +229,11 @@ __global__ void k_frnn(const double* __restrict__ X, const double* __restrict__ 
                 const double d2 = ex * ex + ey * ey;
                 if (d2 <= rho2) {
                     if (FILL) {
+                        const double sv = sqrt(d2) / rho;
+                        if (CMP && (ids[o] != ni || __double_as_longlong(s[o]) != __double_as_longlong(sv)))
+                            atomicOr(changed, 1);
                         ids[o] = ni;
-                        s[o] = sqrt(d2) / rho;
+                        s[o] = sv;
                         ++o;
                     } else {
                         ++n;
@@ -240,6 +246,15 @@ __global__ void k_frnn(const double* __restrict__ X, const double* __restrict__ 
 }
 
 __global__ void k_set_zero_int(int* p) { p[0] = 0; }
+
+// flag[0] |= 1 where two offset arrays differ; clears flag[1] for the fill pass
+__global__ void k_cmp_off(const int* __restrict__ a, const int* __restrict__ b, int n, int* __restrict__ flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) flag[1] = 0;
+    if (i < n && a[i] != b[i]) atomicOr(flag, 1);
+}
+
+__global__ void k_zero_int2(int* p) { p[0] = 0; p[1] = 0; }
 
 // Interleaved copy of the pair lists for the moment pass (layout on Domain::il_row).
 __global__ __launch_bounds__(RKPM_BLOCK) void k_interleave(const int* __restrict__ off, const int* __restrict__ ids,
@@ -327,31 +342,40 @@ void launch_frnn(Context& c, Domain& d, double rho) {
     }
     const int* gsh = c.batched ? d.d_gsheet.p : nullptr;
     const SheetDev* tab = reinterpret_cast<const SheetDev*>(d.d_sheets.p);
-    DBuf<int> d_cstart, d_cnodes, d_count;
+    DBuf<int>& d_cstart = d.cell_start;
+    DBuf<int>& d_cnodes = d.cell_nodes;
+    DBuf<int>& d_count = d.cnt;
     d_cstart.upload(cstart, c.stream);
     d_cnodes.upload(cnodes, c.stream);
     d_count.alloc(d.ng);
+    d.grid_nn = c.nn;
+    d.grid_ng = d.ng;
     const dim3 grid((d.ng + 255) / 256), blk(256);
     int slot = timing_begin(c, "frnn_count");
     hipLaunchKernelGGL((k_frnn<false>), grid, blk, 0, c.stream, c.X.p, c.Y.p, d.gx.p, d.gy.p, d.ng, gsh, tab,
-                       d_cstart.p, d_cnodes.p, d_count.p, nullptr, nullptr, nullptr);
+                       d_cstart.p, d_cnodes.p, d_count.p, nullptr, nullptr, nullptr, nullptr);
     CVM_HIP(hipGetLastError());
     timing_end(c, slot);
     // off = [0, inclusive_scan(count)]
     hipLaunchKernelGGL(k_set_zero_int, dim3(1), dim3(1), 0, c.stream, d.off.p);
     size_t tmp_bytes = 0;
     CVM_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, d_count.p, d.off.p + 1, d.ng, c.stream));
-    DBuf<unsigned char> tmp;
-    tmp.alloc(std::max<size_t>(tmp_bytes, 1));
+    DBuf<unsigned char>& tmp = d.scan_tmp;
+    tmp.ensure(std::max<size_t>(tmp_bytes, 1));
     CVM_HIP(hipcub::DeviceScan::InclusiveSum(tmp.p, tmp_bytes, d_count.p, d.off.p + 1, d.ng, c.stream));
     CVM_HIP(hipMemcpyAsync(d.h_off.data(), d.off.p, (d.ng + 1) * sizeof(int), hipMemcpyDeviceToHost, c.stream));
     CVM_HIP(hipStreamSynchronize(c.stream));
+    // off is int32: a domain with 2^31 or more pairs wraps the scan, which
+    // shows as a decrease (every count is below 2^31)
+    for (int g = 0; g < d.ng; ++g)
+        if (d.h_off[g + 1] < d.h_off[g])
+            throw Error(CVM_E_INVALID, "a domain has 2^31 or more neighbour pairs (int32 offsets)");
     d.npairs = d.h_off[d.ng];
     d.ids.alloc(std::max<int64_t>(d.npairs, 1));
     d.s.alloc(std::max<int64_t>(d.npairs, 1));
     slot = timing_begin(c, "frnn_fill");
     hipLaunchKernelGGL((k_frnn<true>), grid, blk, 0, c.stream, c.X.p, c.Y.p, d.gx.p, d.gy.p, d.ng, gsh, tab,
-                       d_cstart.p, d_cnodes.p, nullptr, d.off.p, d.ids.p, d.s.p);
+                       d_cstart.p, d_cnodes.p, nullptr, d.off.p, d.ids.p, d.s.p, nullptr);
     CVM_HIP(hipGetLastError());
     timing_end(c, slot);
     // interleaved lists: a block's rows = its largest neighbour count
@@ -378,6 +402,90 @@ void launch_frnn(Context& c, Domain& d, double rho) {
     if (d.npairs) CVM_HIP(hipMemcpyAsync(d.h_ids.data(), d.ids.p, d.npairs * sizeof(int), hipMemcpyDeviceToHost, c.stream));
     CVM_HIP(hipStreamSynchronize(c.stream));
     d.have_sf = false;
+}
+
+// The search again on the device with the grid of the last one, for every
+// searched domain of the context: counts and their scan, compared with the
+// current offsets; with equal offsets the lists are rewritten in place (any
+// changed entry flagged) and the interleaved copies redone. Two flag
+// read-backs for all domains together; no host loop over GPs or pairs. A
+// domain whose lists change takes the full path (host grid, mirrors,
+// interleave rows). Returns, per domain, whether its lists changed.
+std::vector<char> refresh_frnn_all(Context& c) {
+    const size_t nd = c.doms.size();
+    std::vector<char> changed(nd, 0), live(nd, 0);
+    std::vector<int> fl(2 * nd, 0);
+    for (size_t k = 0; k < nd; ++k) {
+        Domain& d = *c.doms[k];
+        if (d.npairs < 0) continue;
+        if (c.nodes_dirty || d.dirty || d.grid_nn != c.nn || d.grid_ng != d.ng || !d.cell_start.p) {
+            launch_frnn(c, d, c.batched ? 0.0 : d.rho);
+            changed[k] = 1;
+            continue;
+        }
+        d.ind_creases = -1;             // the indicator is evaluated again, as in every RKPM solve
+        if (d.ng == 0) continue;
+        live[k] = 1;
+        const int* gsh = c.batched ? d.d_gsheet.p : nullptr;
+        const SheetDev* tab = reinterpret_cast<const SheetDev*>(d.d_sheets.p);
+        const dim3 grid((d.ng + 255) / 256), blk(256);
+        d.off_new.ensure(d.ng + 1);
+        d.rf_flag.ensure(2);
+        const int slot = timing_begin(c, "frnn_count");
+        hipLaunchKernelGGL((k_frnn<false>), grid, blk, 0, c.stream, c.X.p, c.Y.p, d.gx.p, d.gy.p, d.ng, gsh, tab,
+                           d.cell_start.p, d.cell_nodes.p, d.cnt.p, nullptr, nullptr, nullptr, nullptr);
+        CVM_HIP(hipGetLastError());
+        timing_end(c, slot);
+        hipLaunchKernelGGL(k_zero_int2, dim3(1), dim3(1), 0, c.stream, d.rf_flag.p);
+        hipLaunchKernelGGL(k_set_zero_int, dim3(1), dim3(1), 0, c.stream, d.off_new.p);
+        size_t tmp_bytes = 0;
+        CVM_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, d.cnt.p, d.off_new.p + 1, d.ng, c.stream));
+        d.scan_tmp.ensure(std::max<size_t>(tmp_bytes, 1));
+        CVM_HIP(hipcub::DeviceScan::InclusiveSum(d.scan_tmp.p, tmp_bytes, d.cnt.p, d.off_new.p + 1, d.ng,
+                                                 c.stream));
+        hipLaunchKernelGGL(k_cmp_off, dim3((d.ng + 256) / 256), dim3(256), 0, c.stream, d.off_new.p, d.off.p,
+                           d.ng + 1, d.rf_flag.p);
+        CVM_HIP(hipGetLastError());
+        CVM_HIP(hipMemcpyAsync(&fl[2 * k], d.rf_flag.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    }
+    CVM_HIP(hipStreamSynchronize(c.stream));
+    for (size_t k = 0; k < nd; ++k) {
+        if (!live[k]) continue;
+        Domain& d = *c.doms[k];
+        if (fl[2 * k]) {                // other counts: the full search
+            live[k] = 0;
+            launch_frnn(c, d, c.batched ? 0.0 : d.rho);
+            changed[k] = 1;
+            continue;
+        }
+        const int* gsh = c.batched ? d.d_gsheet.p : nullptr;
+        const SheetDev* tab = reinterpret_cast<const SheetDev*>(d.d_sheets.p);
+        const dim3 grid((d.ng + 255) / 256), blk(256);
+        int slot = timing_begin(c, "frnn_fill");
+        hipLaunchKernelGGL((k_frnn<true, true>), grid, blk, 0, c.stream, c.X.p, c.Y.p, d.gx.p, d.gy.p, d.ng, gsh,
+                           tab, d.cell_start.p, d.cell_nodes.p, nullptr, d.off.p, d.ids.p, d.s.p, d.rf_flag.p + 1);
+        CVM_HIP(hipGetLastError());
+        timing_end(c, slot);
+        const int nblk = (d.ng + RKPM_BLOCK - 1) / RKPM_BLOCK;
+        slot = timing_begin(c, "interleave");
+        hipLaunchKernelGGL(k_interleave, dim3(nblk), dim3(RKPM_BLOCK), 0, c.stream, d.off.p, d.ids.p, d.s.p, d.ng,
+                           d.il_row.p, d.il_ids.p, d.il_s.p, d.pair_lgp.p);
+        CVM_HIP(hipGetLastError());
+        timing_end(c, slot);
+        CVM_HIP(hipMemcpyAsync(&fl[2 * k + 1], d.rf_flag.p + 1, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    }
+    CVM_HIP(hipStreamSynchronize(c.stream));
+    for (size_t k = 0; k < nd; ++k) {
+        if (!live[k] || !fl[2 * k + 1]) continue;
+        // same counts, other nodes: the host mirror of the ids follows
+        Domain& d = *c.doms[k];
+        if (d.npairs)
+            CVM_HIP(hipMemcpyAsync(d.h_ids.data(), d.ids.p, d.npairs * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        d.have_sf = false;
+        changed[k] = 1;
+    }
+    CVM_HIP(hipStreamSynchronize(c.stream));
+    return changed;
 }
 
 }  // namespace cvm

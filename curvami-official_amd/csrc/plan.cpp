@@ -208,6 +208,9 @@ void build_plan(Context& c) {
     }
     P.nscratch = static_cast<int64_t>(tpairs.size());
     P.nfscratch = static_cast<int64_t>(unodes.size());
+    // scratch rows are addressed by uint32 slot_dst, union entries by int fc_idx
+    if (P.nscratch > 0xffffffffLL) throw Error(CVM_E_INVALID, "assembly scratch exceeds 2^32 rows");
+    if (P.nfscratch > 0x7fffffffLL) throw Error(CVM_E_INVALID, "assembly tiles hold more than 2^31 union entries");
 
     // face pattern (per column: sorted unique rows)
     std::vector<std::vector<int>> face_rows(nn);
@@ -328,7 +331,6 @@ void build_plan(Context& c) {
         std::vector<int64_t> fillp(kc_cnt.begin(), kc_cnt.end() - 1);
         for (size_t s = 0; s < tpairs.size(); ++s) slot_dst[s] = static_cast<uint32_t>(fillp[tblk[s]]++);
     }
-    if (P.nscratch > 0xffffffffLL) throw Error(CVM_E_INVALID, "assembly scratch exceeds 2^32 rows");
 
     // F contributors per node
     std::vector<int> fc_cnt(nn + 1, 0);

@@ -43,6 +43,7 @@ SIGNATURES = {
     "cvm_set_fold_rigid": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "cvm_neighbours": (C.c_int, [C.c_void_p, C.c_int, C.c_double, _i64]),
     "cvm_get_neighbours": (C.c_int, [C.c_void_p, C.c_int, _i, _i, _d]),
+    "cvm_refresh_neighbours": (C.c_int, [C.c_void_p, _i]),
     "cvm_shape_functions": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _i, _i]),
     "cvm_get_shape_functions": (C.c_int, [C.c_void_p, C.c_int, _d, _d, _d, _d, _d, _d]),
     "cvm_shape_functions_all": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _i, _i, _i]),
@@ -219,6 +220,20 @@ class Plate:
         self._check(self._L.cvm_neighbours(self._h, dom, rho, C.byref(n)))
         self.npairs[dom] = n.value
         return n.value
+
+    def refresh_neighbours(self):
+        """The neighbour search of every searched domain again on the device;
+        returns True if any list changed (then the shape functions and, for
+        face and fold domains, the assembly plan must be rebuilt)."""
+        ch = C.c_int(0)
+        self._check(self._L.cvm_refresh_neighbours(self._h, C.byref(ch)))
+        if ch.value:
+            for d in range(len(self.domain_sizes)):
+                if d in self.npairs:
+                    off = np.empty(self.domain_sizes[d] + 1, np.int32)
+                    self._check(self._L.cvm_get_neighbours(self._h, d, off.ctypes.data_as(_i), None, None))
+                    self.npairs[d] = int(off[-1])
+        return bool(ch.value)
 
     def get_neighbours(self, dom):
         ng = self.domain_sizes[dom]
@@ -594,6 +609,17 @@ class DeviceArray:
     def get(self) -> np.ndarray:
         out = np.empty(self.n)
         if self.n and self._rt().hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), self.n * 8, 2) != 0:
+            raise CvmError(E_HIP, "hipMemcpy D2H failed")
+        return out
+
+    def get_range(self, lo: int, hi: int) -> np.ndarray:
+        """Elements [lo, hi) copied to the host."""
+        lo, hi = int(lo), int(hi)
+        if not 0 <= lo <= hi <= self.n:
+            raise ValueError("range outside the buffer")
+        out = np.empty(hi - lo)
+        if hi > lo and self._rt().hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr + 8 * lo),
+                                            (hi - lo) * 8, 2) != 0:
             raise CvmError(E_HIP, "hipMemcpy D2H failed")
         return out
 

@@ -137,6 +137,17 @@ CVM_API int cvm_set_fold_gp_state(cvm_context* ctx, int fold_id, int n, const do
  * Replaces GlobalFRNN::execute (core/include/curvami/FRNN.h:23-78): same
  * neighbour sets and order, s = sqrt(dist2) / rho. */
 CVM_API int cvm_neighbours(cvm_context* ctx, int domain, double rho, int64_t* npairs);
+/* The neighbour search of every searched domain again, on the device, with
+ * the radius and node grid of its last cvm_neighbours call: the FRNN of
+ * SetupHelper::recomputeShapeFunctions (plate/SetupHelper.h:112-138), which
+ * searches again before every RKPM solve. The lists are rewritten in place;
+ * *changed = 1 if any list differs from the previous search (nodes or GPs
+ * moved), in which case the host mirrors follow, the shape functions must be
+ * recomputed and a changed face or fold domain invalidates the assembly plan.
+ * The crease indicators and node sides are evaluated again by the next
+ * shape-function call, as GlobalRKPM::solve evaluates them on every call
+ * (GlobalRKPM.h:86-112). */
+CVM_API int cvm_refresh_neighbours(cvm_context* ctx, int* changed);
 /* Copy the lists out: off[ng+1], ids[npairs], s[npairs] (any may be NULL). */
 CVM_API int cvm_get_neighbours(cvm_context* ctx, int domain, int* off, int* ids, double* s);
 

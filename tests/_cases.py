@@ -27,6 +27,10 @@ def sheet(name: str):
         return W.config1(rho_factor=RHO_FACTOR, kernel=1)
     if name == "arc41":
         return W.config2(rho_factor=RHO_FACTOR, n_side=41, nseg=64)
+    if name == "arc41_bspline":
+        return W.config2(rho_factor=RHO_FACTOR, n_side=41, nseg=64, kernel=1)
+    if name == "arc201_bspline":
+        return W.config2(rho_factor=RHO_FACTOR, kernel=1)
     if name == "arc201":
         return W.config2(rho_factor=RHO_FACTOR)
     raise KeyError(name)
@@ -48,16 +52,26 @@ def normwise(a, r, g, ng):
 EPS = np.finfo(np.float64).eps
 
 
-def moment_cond(gx, gy, X, Y, rho, off, ids, psi, order):
+def window(s, kernel=0):
+    """The window on s = |x - X_I| / rho: 0 the quartic spline
+    (GlobalRKPM.h:15-17), 1 the cubic B-spline."""
+    if kernel == 1:
+        inner = 2.0 / 3.0 - 4 * s ** 2 + 4 * s ** 3
+        outer = 4.0 / 3.0 - 4 * s + 4 * s ** 2 - 4.0 / 3.0 * s ** 3
+        return np.where(s < 1, np.where(s <= 0.5, inner, outer), 0.0)
+    return np.where(s < 1, 1 - 6 * s ** 2 + 8 * s ** 3 - 3 * s ** 4, 0.0)
+
+
+def moment_cond(gx, gy, X, Y, rho, off, ids, psi, order, kernel=0):
     """cond_2 of each GP's moment matrix M = sum_I w(s_I) p_I p_I^T over its
-    visible neighbours (psi != 0), with the quartic window and the basis of
+    visible neighbours (psi != 0), with the given window and the basis of
     the given order in xi = (x - X_I) / rho: the conditioning that bounds the
     round-off two correct evaluations of the shape functions can differ by."""
     g = pair_gp(off)
     xi = (gx[g] - X[ids]) / rho
     eta = (gy[g] - Y[ids]) / rho
     s = np.sqrt(xi * xi + eta * eta)
-    w = np.where(s < 1, 1 - 6 * s ** 2 + 8 * s ** 3 - 3 * s ** 4, 0.0) * (psi != 0)
+    w = window(s, kernel) * (psi != 0)
     P = [xi ** (d - k) * eta ** k for d in range(order + 1) for k in range(d + 1)]
     ng = off.size - 1
     M = np.zeros((ng, len(P), len(P)))
@@ -71,3 +85,57 @@ def sf_tolerance(cond, floor=1e-12):
     """Per-GP normwise bars for (psi, gradient, Hessian): max(floor, C eps
     cond(M)) with C = 4 / 16 / 64."""
     return tuple(np.maximum(floor, C * EPS * cond) for C in (4, 16, 64))
+
+
+def upper_csc(outer, inner, K, n):
+    """The stored block-upper K as a scipy CSC matrix (n x n)."""
+    import scipy.sparse as sp
+    return sp.csc_matrix((K, inner, outer), shape=(n, n))
+
+
+def box_sample_check(sh, x, F, K, outer, inner, box, nthreads=16, label=""):
+    # sh: a workloads.Sheet; x the displacement; F, K, outer, inner the device
+    # results and pattern (with the elastic folds) of the full sheet
+    """Compare a device residual and tangent of a full sheet with the oracle on
+    a bounded sample: the oracle assembles only the face GPs inside box =
+    (x0, x1, y0, y1) (plus every crease GP), in their order on the sheet. A
+    node whose support disk (radius rho) lies inside the box receives every
+    face contribution it has on the full sheet, so its F_int entries and the K
+    entries between two such nodes must agree with the device's full-sheet
+    values. Returns (F rel error, K rel error, number of nodes compared, the
+    largest moment-matrix condition number over the box's GPs)."""
+    import oracle as O
+    x0, x1, y0, y1 = box
+    fx, fy = sh.face.gx, sh.face.gy
+    m = (fx > x0) & (fx < x1) & (fy > y0) & (fy < y1)
+    sub = W.GPSet(*(None if a is None else np.asarray(a)[m]
+                    for a in (sh.face.gx, sh.face.gy, sh.face.w, sh.face.nx, sh.face.ny, sh.face.tx, sh.face.ty)))
+    import dataclasses
+    sample = dataclasses.replace(sh, face=sub, boundaries=[])
+    orc = O.plate_from_sheet(sample, nthreads=nthreads)
+    o_outer, o_inner, o_K, o_F = orc.assemble(x, flag_rig=False, nthreads=1)
+    o_off, o_ids, _ = orc.neighbours(0, sh.rho)
+    psi = orc.get_shape_functions(0, int(o_off[-1]))[0]
+    cond = moment_cond(sub.gx, sub.gy, sh.X, sh.Y, sh.rho, o_off, o_ids, psi, sh.order, sh.kernel)
+    eps = 1e-9
+    inner_nodes = np.nonzero((sh.X - sh.rho > x0 + eps) & (sh.X + sh.rho < x1 - eps) &
+                             (sh.Y - sh.rho > y0 + eps) & (sh.Y + sh.rho < y1 - eps))[0]
+    assert inner_nodes.size > 0, "box too small: no node has its support inside"
+    dofs = (3 * inner_nodes[:, None] + np.arange(3)[None, :]).ravel()
+    n = 3 * sh.nn
+    # columns first (cheap on CSC), then rows
+    Kg = upper_csc(outer, inner, K, n)[:, dofs].tocsr()[dofs].toarray()
+    Ko = upper_csc(o_outer, o_inner, o_K, n)[:, dofs].tocsr()[dofs].toarray()
+    ef = np.abs(F[dofs] - o_F[dofs]).max() / np.abs(o_F[dofs]).max()
+    ek = np.abs(Kg - Ko).max() / np.abs(Ko).max()
+    print(f"{label} oracle box {box}: {m.sum()} GPs, {inner_nodes.size} nodes compared, "
+          f"F rel {ef:.3e}, K rel {ek:.3e}, max cond(M) {cond.max():.3e}", flush=True)
+    return ef, ek, inner_nodes.size, float(cond.max())
+
+
+def k_bar(cond_max, fp32=False):
+    """Bar for K against the oracle: 1e-12 of max|K|, or 64 eps cond(M) where
+    the moment matrices are ill-conditioned (GPs next to a crease, whose second
+    shape derivatives carry that much round-off into the bending terms); 1e-5
+    for the FP32 tangent."""
+    return 1e-5 if fp32 else max(1e-12, 64 * EPS * cond_max)

@@ -51,3 +51,20 @@ def test_assembly_per_sheet_bit_identical(batch, fp32):
         assert np.array_equal(seg - seg[0], o1), f"sheet {k}: column pointers"
         assert np.array_equal(inner[seg[0]:seg[-1]] - 3 * n0, i1), f"sheet {k}: row indices"
         assert np.array_equal(Kb[seg[0]:seg[-1]].view(np.int64), K1.view(np.int64)), f"sheet {k}: K values"
+
+
+def test_refresh_neighbours_batched(batch):
+    """The device re-search of a batched context reports its lists unchanged and
+    leaves neighbours, shape functions and the assembly bit-identical."""
+    sheets, _, bp = batch
+    xs = np.concatenate([W.displacement(sh.nn, 40 + k, 1e-3) for k, sh in enumerate(sheets)])
+    off0, ids0, s0 = bp.get_neighbours(bp.face_dom)
+    F0, K0 = bp.assemble(xs)
+    assert bp.refresh_neighbours() is False
+    bp.shape_functions_all(sheets[0].order, sheets[0].kernel)
+    off1, ids1, s1 = bp.get_neighbours(bp.face_dom)
+    assert np.array_equal(off0, off1) and np.array_equal(ids0, ids1)
+    assert np.array_equal(s0.view(np.int64), s1.view(np.int64))
+    F1, K1 = bp.assemble(xs)
+    assert np.array_equal(F1.view(np.int64), F0.view(np.int64))
+    assert np.array_equal(K1.view(np.int64), K0.view(np.int64))

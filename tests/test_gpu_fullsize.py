@@ -1,6 +1,6 @@
 """The bench workload at full size on the device: the 201 x 201 arc-crease sheet
-(640,000 face GPs, 28.2M pairs, rho = 3.75 h, quadratic basis, quartic window,
-elastic hinge), checked
+(640,000 face GPs, 28.2M pairs, rho = 3.75 h, quadratic basis, elastic hinge),
+with the quartic window and with the cubic B-spline window, checked
 
 - against the oracle on a bounded sample of the same sheet: the face GPs of
   cell rows 95-104 (32,000 GPs, crossed by the arc) have the neighbour lists
@@ -32,9 +32,11 @@ Q = 16                     # GPs per cell (4 x 4)
 ROWS = (95, 105)           # cell rows of the oracle sample
 
 
-@pytest.fixture(scope="module")
-def full():
-    sh = sheet("arc201")
+@pytest.fixture(scope="module", params=["arc201", "arc201_bspline"])
+def full(request):
+    """The sheet with the quartic window (the reference's) and with the cubic
+    B-spline window (the configurations' window)."""
+    sh = sheet(request.param)
     print(f"{sh.name}: {sh.nn} nodes, {sh.face.n} face GPs", flush=True)
     return sh, G.plate_from_sheet(sh)
 
@@ -44,7 +46,7 @@ def face(full):
     sh, gpu = full
     off, ids, s = gpu.get_neighbours(0)
     sf = gpu.get_shape_functions(0)
-    cond = moment_cond(sh.face.gx, sh.face.gy, sh.X, sh.Y, sh.rho, off, ids, sf[0], sh.order)
+    cond = moment_cond(sh.face.gx, sh.face.gy, sh.X, sh.Y, sh.rho, off, ids, sf[0], sh.order, sh.kernel)
     print(f"pairs {off[-1]}, cond(M) median {np.median(cond):.3e}, max {cond.max():.3e}", flush=True)
     return off, ids, s, sf, cond
 
@@ -69,7 +71,7 @@ def test_sample_matches_oracle(full, face):
     got = [q[p0:p1] for q in sf]
     g = pair_gp(o_off)
     ng = o_off.size - 1
-    cond = moment_cond(sub.gx, sub.gy, sh.X, sh.Y, sh.rho, o_off, o_ids, ref[0], sh.order)
+    cond = moment_cond(sub.gx, sub.gy, sh.X, sh.Y, sh.rho, o_off, o_ids, ref[0], sh.order, sh.kernel)
     errs = (normwise([got[0]], [ref[0]], g, ng), normwise(got[1:3], ref[1:3], g, ng),
             normwise(got[3:6], ref[3:6], g, ng))
     for e, tol, nm in zip(errs, sf_tolerance(cond), ("psi", "grad", "hess")):

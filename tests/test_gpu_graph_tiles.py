@@ -82,3 +82,56 @@ def test_autotune_picks_a_candidate(arc):
     assert p.tile_shape()["grid"] == gr
     p.set_tile_shape(16, 0, 0)
     p.init_assembly()
+
+
+def _refused(p):
+    with pytest.raises(G.CvmError) as e:
+        p.graph_launch()
+    assert e.value.code == G.E_STATE, str(e.value)
+
+
+def test_graph_refused_after_changes(arc):
+    """A graph replays the launches of its capture. An eager FP64 evaluation
+    between replays of an FP32-tangent capture keeps it valid (the capture
+    reserved the FP64 scratch, so nothing it uses is reallocated); a changed
+    tile grid, laminate or rebuilt plan makes the replay refuse with
+    CVM_E_STATE until the graph is captured again."""
+    sh, p = arc
+    x, F, K = _dev(p, sh)
+    xv = W.displacement(sh.nn, 7, 1e-3)
+    x.set(xv)
+    p.graph_capture(x.ptr, F.ptr, K.ptr, FLAGS | G.FP32_TANGENT)
+    p.assemble(xv)                                   # eager FP64 between replays
+    p.graph_launch()
+    p.synchronize()
+    Fe, Ke = p.assemble(xv, fp32_tangent=True)
+    assert np.array_equal(F.get().view(np.int64), Fe.view(np.int64))
+    assert np.array_equal(K.get().view(np.int64), Ke.view(np.int64))
+    # tile grid
+    p.set_tile_shape(16, 0, 64)
+    _refused(p)
+    p.set_tile_shape(16, 0, 0)
+    p.graph_launch()                                 # the captured launches again
+    p.synchronize()
+    # laminate
+    plies = sh.plies.copy()
+    other = plies.copy()
+    other[:, 0] *= 2.0
+    other[:, 1] *= 2.0
+    p.set_plies(other)
+    _refused(p)
+    p.set_plies(plies)
+    p.graph_launch()
+    p.synchronize()
+    # plan rebuilt with another tile shape
+    p.set_tile_shape(8, 64, 0)
+    p.init_assembly()
+    _refused(p)
+    p.set_tile_shape(16, 0, 0)
+    p.init_assembly()
+    p.graph_capture(x.ptr, F.ptr, K.ptr, FLAGS)
+    p.graph_launch()
+    p.synchronize()
+    Fe, Ke = p.assemble(xv)
+    assert np.array_equal(F.get().view(np.int64), Fe.view(np.int64))
+    assert np.array_equal(K.get().view(np.int64), Ke.view(np.int64))

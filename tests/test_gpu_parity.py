@@ -18,6 +18,10 @@ pytestmark = pytest.mark.gpu
 # moves by ~1e-11 between builds with and without FMA contraction, so the bar is
 # max(1e-12, C * eps * cond(M)) with C = 4 / 16 / 64 for Psi / grad / Hessian.
 SF_TOL = 1e-12
+# Fraction of face GPs within 1e-12 itself, per derivative order: measured
+# 1.0 / 0.9989-1.0 / 0.9836-1.0 on the cases below (the rest sit next to the
+# crease, where cond(M) reaches ~1e4).
+FRAC_MIN = {"psi": 0.999, "grad": 0.995, "hess": 0.98}
 
 
 def _build(name):
@@ -38,6 +42,18 @@ def arc():
 
 
 @pytest.fixture(scope="module")
+def c1_bs():
+    """The straight-crease sheet with the cubic B-spline window."""
+    return _build("straight21_bspline")
+
+
+@pytest.fixture(scope="module")
+def arc_bs():
+    """The arc-crease sheet with the cubic B-spline window."""
+    return _build("arc41_bspline")
+
+
+@pytest.fixture(scope="module")
 def wide():
     """Support 6h (the reference's default support radius, 3rd-NN distance x
     (order + 3)): ~113 neighbours per GP, two 64-lane chunks per GP."""
@@ -50,7 +66,7 @@ def wider():
     return _build("straight21_rho8.5")
 
 
-@pytest.mark.parametrize("case", ["c1", "arc", "wide", "wider"])
+@pytest.mark.parametrize("case", ["c1", "arc", "wide", "wider", "arc_bs"])
 def test_neighbours_bit_exact(case, request):
     sh, orc, gpu = request.getfixturevalue(case)
     for d in range(len(gpu.domain_sizes)):
@@ -61,7 +77,7 @@ def test_neighbours_bit_exact(case, request):
         assert np.array_equal(o_s.view(np.int64), g_s.view(np.int64)), f"domain {d}: s not bit-identical"
 
 
-@pytest.mark.parametrize("case", ["c1", "arc", "wide"])
+@pytest.mark.parametrize("case", ["c1", "arc", "wide", "c1_bs", "arc_bs"])
 def test_shape_functions(case, request):
     sh, orc, gpu = request.getfixturevalue(case)
     for d in range(len(gpu.domain_sizes)):
@@ -76,7 +92,7 @@ def test_shape_functions(case, request):
         e1 = normwise(got[1:3], ref[1:3], g, ng)
         e2 = normwise(got[3:6], ref[3:6], g, ng)
         if d == 0:
-            cond = moment_cond(sh.face.gx, sh.face.gy, sh.X, sh.Y, sh.rho, off, ids, ref[0], sh.order)
+            cond = moment_cond(sh.face.gx, sh.face.gy, sh.X, sh.Y, sh.rho, off, ids, ref[0], sh.order, sh.kernel)
         else:
             cond = np.full(ng, 1e4)
         for e, tol, nm in zip((e0, e1, e2), sf_tolerance(cond, SF_TOL), ("psi", "grad", "hess")):
@@ -84,6 +100,8 @@ def test_shape_functions(case, request):
             print(f"domain {d} {nm}: max {e.max():.3e}, median {np.median(e):.3e}, within 1e-12: {frac:.4f}")
             bad = np.nonzero(e > tol)[0]
             assert bad.size == 0, f"domain {d} {nm}: {bad.size} GPs above tolerance, worst {e.max():.3e}"
+            if d == 0:
+                assert frac >= FRAC_MIN[nm], f"{nm}: only {frac:.4f} of the face GPs within 1e-12"
 
 
 @pytest.mark.parametrize("case", ["c1", "arc", "wide", "wider"])
@@ -102,7 +120,7 @@ def test_patterns_bit_exact(case, request):
         assert np.array_equal(orc.gp_map(gp, n), gpu.csr_mapping(gp))
 
 
-@pytest.mark.parametrize("case", ["c1", "arc", "wide", "wider"])
+@pytest.mark.parametrize("case", ["c1", "arc", "wide", "wider", "c1_bs", "arc_bs"])
 @pytest.mark.parametrize("elastic", [True, False])
 def test_residual_and_tangent(case, elastic, request):
     sh, orc, gpu = request.getfixturevalue(case)
@@ -197,7 +215,7 @@ def test_pivot_collapse_reported_like_reference():
     assert "pivot collapsed" in str(ei.value)
 
 
-@pytest.mark.parametrize("case", ["c1", "arc"])
+@pytest.mark.parametrize("case", ["c1", "arc", "arc_bs"])
 def test_exact_hinge_tangent(case, request):
     """fold_exact_tangent: the FD dihedral-angle Hessian term (Constraints.h:394-416)."""
     sh, orc, gpu = request.getfixturevalue(case)
@@ -216,7 +234,7 @@ def test_exact_hinge_tangent(case, request):
     assert np.abs(Fg - Fr).max() <= 1e-10 * np.abs(Fr).max()
 
 
-@pytest.mark.parametrize("case", ["c1", "arc", "wide"])
+@pytest.mark.parametrize("case", ["c1", "arc", "wide", "c1_bs", "arc_bs"])
 def test_fp32_tangent(case, request):
     """Mixed precision: the face tangent in FP32 (inexact-Newton Jacobian), the
     residual still FP64 (equal to the FP64 path up to FMA-contraction round-off
@@ -254,3 +272,26 @@ def test_shape_functions_all_reports_failing_domain():
         gpu.shape_functions_all(sh.order, sh.kernel)
     assert ei.value.code == G.E_PIVOT_COLLAPSE and ei.value.domain == 0
     assert "domain 0" in str(ei.value)
+
+
+def test_refresh_neighbours_unchanged(arc):
+    """cvm_refresh_neighbours on unchanged geometry: lists rewritten in place and
+    reported unchanged, the plan stays valid, and the shape functions (with
+    indicators and node sides evaluated again) and the assembly that follow
+    are bit-identical to those before."""
+    sh, _, gpu = arc
+    x = W.displacement(sh.nn, 9, 1e-3)
+    lists0 = [gpu.get_neighbours(d) for d in range(len(gpu.domain_sizes))]
+    sf0 = [gpu.get_shape_functions(d) for d in range(len(gpu.domain_sizes))]
+    F0, K0 = gpu.assemble(x)
+    assert gpu.refresh_neighbours() is False
+    gpu.shape_functions_all(sh.order, sh.kernel)
+    for d in range(len(gpu.domain_sizes)):
+        for a, b in zip(gpu.get_neighbours(d), lists0[d]):
+            assert np.array_equal(a.view(np.int64) if a.dtype == np.float64 else a,
+                                  b.view(np.int64) if b.dtype == np.float64 else b), f"domain {d}: lists"
+        for a, b in zip(gpu.get_shape_functions(d), sf0[d]):
+            assert np.array_equal(a.view(np.int64), b.view(np.int64)), f"domain {d}: shape functions"
+    F1, K1 = gpu.assemble(x)
+    assert np.array_equal(F1.view(np.int64), F0.view(np.int64))
+    assert np.array_equal(K1.view(np.int64), K0.view(np.int64))

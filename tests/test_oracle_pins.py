@@ -543,3 +543,29 @@ def test_coupled_laminate_residual_and_tangent():
         fd = (e(x + h * v) - e(x - h * v)) / (2 * h)
         assert abs(fd - F @ v) <= 1e-7 * np.linalg.norm(F) * np.linalg.norm(v)
     assert _tangent_check(p, x, 3 * sh.nn, True) < 1e-6
+
+
+def test_box_sample_is_exact_inside():
+    """The oracle box sample used for full-size parity (tests/_cases.py
+    box_sample_check): assembling only the face GPs inside a box (and every
+    crease GP) gives the full sheet's F_int at every node whose support disk
+    lies in the box, and the K entries between two such nodes, bit for bit;
+    nodes next to the box edge (support partly outside) do differ, so the
+    check is not vacuous."""
+    from _cases import box_sample_check, sheet
+    sh = sheet("arc41")
+    orc = O.plate_from_sheet(sh)
+    x = W.displacement(sh.nn, 1, 1e-3)
+    outer, inner, K, F = orc.assemble(x, flag_rig=False)
+    box = (0.2, 0.8, 0.05, 0.5)
+    ef, ek, n, _ = box_sample_check(sh, x, F, K, outer, inner, box, nthreads=1)
+    assert ef == 0.0 and ek == 0.0 and n > 100
+    # a node whose support crosses the box edge misses contributions
+    import dataclasses
+    m = (sh.face.gx > box[0]) & (sh.face.gx < box[1]) & (sh.face.gy > box[2]) & (sh.face.gy < box[3])
+    sub = W.GPSet(sh.face.gx[m], sh.face.gy[m], sh.face.w[m])
+    part = O.plate_from_sheet(dataclasses.replace(sh, face=sub))
+    _, _, _, Fp = part.assemble(x, flag_rig=False)
+    edge = np.nonzero((np.abs(sh.Y - box[2]) < 0.5 * sh.rho) & (sh.X > 0.4) & (sh.X < 0.6))[0]
+    d = np.concatenate([3 * edge, 3 * edge + 1, 3 * edge + 2])
+    assert np.abs(Fp[d] - F[d]).max() > 1e-6 * np.abs(F).max()
