@@ -131,7 +131,6 @@ struct Domain {
     DBuf<int> cell_start, cell_nodes;
     DBuf<int> cnt, off_new;   // per GP counts and offsets of a refresh
     DBuf<unsigned char> scan_tmp;
-    DBuf<int> rf_flag;        // refresh change flags (offsets, lists)
     int grid_nn = -1, grid_ng = -1;
     // The same lists interleaved per block of RKPM_BLOCK GPs for the moment
     // pass, where each lane walks its own GP's pairs: the t-th pair of GP lane
@@ -289,6 +288,9 @@ struct Context {
     DBuf<double> d_cv;          // packed vertices x,y
     DBuf<int> d_cinfo;          // per crease: offset, count, ext_s, ext_e, closed
     DBuf<int8_t> d_node_sign;   // per (crease, node): +1 / -1
+    std::vector<int> h_crange;  // per sheet: first crease, crease count (node-sign launch)
+    DBuf<int> d_crange;
+    DBuf<int> rf_flags;         // neighbour refresh: per domain, offsets / lists changed
     bool creases_dirty = true;
     int node_sign_creases = -1;   // crease count the node signs were computed for (-1: stale)
     // sheets (batched contexts): nodes and creases per sheet, per-node sheet

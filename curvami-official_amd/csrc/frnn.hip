@@ -146,14 +146,15 @@ void launch_node_signs(Context& c) {
         for (const SheetSeg& s : c.sheets) crange.insert(crange.end(), {s.crease0, s.ncreases});
     else
         crange = {0, nc};
-    DBuf<int> d_cr;
-    d_cr.upload(crange, c.stream);
+    if (crange != c.h_crange) {
+        c.h_crange = crange;
+        c.d_crange.upload(c.h_crange, c.stream);
+    }
     const int slot = timing_begin(c, "node_sign");
     hipLaunchKernelGGL(k_node_sign, dim3((c.nn + 255) / 256), dim3(256), 0, c.stream, c.X.p, c.Y.p, c.nn,
-                       c.batched ? c.d_nsheet.p : nullptr, d_cr.p, c.d_cv.p, c.d_cinfo.p, c.d_node_sign.p);
+                       c.batched ? c.d_nsheet.p : nullptr, c.d_crange.p, c.d_cv.p, c.d_cinfo.p, c.d_node_sign.p);
     CVM_HIP(hipGetLastError());
     timing_end(c, slot);
-    CVM_HIP(hipStreamSynchronize(c.stream));
 }
 
 // The crease range (first crease, count) of every sheet in the domain's
@@ -198,19 +199,19 @@ void launch_indicator_gp(Context& c, Domain& d) {
 // reference does, including the wrap of an out-of-range x into the adjacent
 // row's key; keys outside [0, nkeys) are empty, like a failed map lookup.
 // ---------------------------------------------------------------------------
-// FILL: write the lists at off (else count per GP); CMP (with FILL): the lists
-// already hold a previous search's result, and any entry that changes sets
-// *changed.
-template <bool FILL, bool CMP = false>
+// FILL: write the lists at off (else count per GP). skip (may be null): the
+// fill does nothing while *skip is set (a re-search whose counts changed).
+template <bool FILL>
 __global__ void k_frnn(const double* __restrict__ X, const double* __restrict__ Y,
                        const double* __restrict__ gx, const double* __restrict__ gy, int ng,
                        const int* __restrict__ gsheet, const SheetDev* __restrict__ sheets,
                        const int* __restrict__ cstart, const int* __restrict__ cnodes,
                        int* __restrict__ count, const int* __restrict__ off,
-                       int* __restrict__ ids, double* __restrict__ s, int* __restrict__ changed = nullptr) {
+                       int* __restrict__ ids, double* __restrict__ s, const int* __restrict__ skip) {
 #pragma clang fp contract(off)
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= ng) return;
+    if (FILL && skip && *skip) return;
     const SheetDev& gr = sheets[gsheet ? gsheet[q] : 0];
     const double px = gx[q], py = gy[q];
     const int qx = static_cast<int>(floor((px - gr.minx) / gr.cell));
@@ -229,11 +230,8 @@ __global__ void k_frnn(const double* __restrict__ X, const double* __restrict__ 
                 const double d2 = ex * ex + ey * ey;
                 if (d2 <= rho2) {
                     if (FILL) {
-                        const double sv = sqrt(d2) / rho;
-                        if (CMP && (ids[o] != ni || __double_as_longlong(s[o]) != __double_as_longlong(sv)))
-                            atomicOr(changed, 1);
                         ids[o] = ni;
-                        s[o] = sv;
+                        s[o] = sqrt(d2) / rho;
                         ++o;
                     } else {
                         ++n;
@@ -257,19 +255,31 @@ __global__ void k_cmp_off(const int* __restrict__ a, const int* __restrict__ b, 
 __global__ void k_zero_int2(int* p) { p[0] = 0; p[1] = 0; }
 
 // Interleaved copy of the pair lists for the moment pass (layout on Domain::il_row).
+// CMP: the interleaved lists hold a previous search's result (same offsets);
+// an entry that changes sets flag[1]; nothing is done when flag[0] is set
+// (the offsets changed, so the in-place layout does not apply).
+template <bool CMP>
 __global__ __launch_bounds__(RKPM_BLOCK) void k_interleave(const int* __restrict__ off, const int* __restrict__ ids,
                                                            const double* __restrict__ s, int ng,
                                                            const int* __restrict__ row, int* __restrict__ il_ids,
-                                                           double* __restrict__ il_s, uint8_t* __restrict__ lgp) {
+                                                           double* __restrict__ il_s, uint8_t* __restrict__ lgp,
+                                                           int* __restrict__ flag) {
     const int g = blockIdx.x * RKPM_BLOCK + threadIdx.x;
     if (g >= ng) return;
+    if (CMP && flag[0]) return;
     const int b = off[g], n = off[g + 1] - b;
     const size_t base = static_cast<size_t>(row[blockIdx.x]) * RKPM_BLOCK + threadIdx.x;
+    bool diff = false;
     for (int t = 0; t < n; ++t) {
-        il_ids[base + static_cast<size_t>(t) * RKPM_BLOCK] = ids[b + t];
-        il_s[base + static_cast<size_t>(t) * RKPM_BLOCK] = s[b + t];
+        const size_t k = base + static_cast<size_t>(t) * RKPM_BLOCK;
+        const int id = ids[b + t];
+        const double sv = s[b + t];
+        if (CMP) diff |= (il_ids[k] != id) || (__double_as_longlong(il_s[k]) != __double_as_longlong(sv));
+        il_ids[k] = id;
+        il_s[k] = sv;
         lgp[b + t] = static_cast<uint8_t>(threadIdx.x);
     }
+    if (CMP && diff) atomicOr(flag + 1, 1);
 }
 
 // Per sheet: node cell grid with the bounds and column count of FRNN.h:30-42
@@ -395,8 +405,8 @@ void launch_frnn(Context& c, Domain& d, double rho) {
     d.il_s.alloc(slots);
     d.pair_lgp.alloc(std::max<int64_t>(d.npairs, 1));
     static_assert(RKPM_BLOCK <= 256, "GP-in-block index is a byte");
-    hipLaunchKernelGGL(k_interleave, dim3(nblk), dim3(RKPM_BLOCK), 0, c.stream, d.off.p, d.ids.p, d.s.p, d.ng,
-                       d.il_row.p, d.il_ids.p, d.il_s.p, d.pair_lgp.p);
+    hipLaunchKernelGGL(k_interleave<false>, dim3(nblk), dim3(RKPM_BLOCK), 0, c.stream, d.off.p, d.ids.p, d.s.p, d.ng,
+                       d.il_row.p, d.il_ids.p, d.il_s.p, d.pair_lgp.p, nullptr);
     CVM_HIP(hipGetLastError());
     d.h_ids.resize(d.npairs);
     if (d.npairs) CVM_HIP(hipMemcpyAsync(d.h_ids.data(), d.ids.p, d.npairs * sizeof(int), hipMemcpyDeviceToHost, c.stream));
@@ -405,16 +415,18 @@ void launch_frnn(Context& c, Domain& d, double rho) {
 }
 
 // The search again on the device with the grid of the last one, for every
-// searched domain of the context: counts and their scan, compared with the
-// current offsets; with equal offsets the lists are rewritten in place (any
-// changed entry flagged) and the interleaved copies redone. Two flag
-// read-backs for all domains together; no host loop over GPs or pairs. A
-// domain whose lists change takes the full path (host grid, mirrors,
-// interleave rows). Returns, per domain, whether its lists changed.
+// searched domain of the context, all enqueued before one read-back: counts
+// and their scan, compared with the current offsets (flag 0); with equal
+// offsets the lists are rewritten in place and the interleaved copies redone,
+// each entry compared with the previous interleaved copy, which holds the last
+// search's lists in coalesced order (flag 1); with other offsets both passes
+// do nothing (device-side test of flag 0) and the domain takes the full search
+// (host grid, mirrors, interleave rows). No host loop over GPs or pairs.
+// Returns, per domain, whether its lists changed.
 std::vector<char> refresh_frnn_all(Context& c) {
     const size_t nd = c.doms.size();
     std::vector<char> changed(nd, 0), live(nd, 0);
-    std::vector<int> fl(2 * nd, 0);
+    c.rf_flags.ensure(std::max<size_t>(2 * nd, 2));
     for (size_t k = 0; k < nd; ++k) {
         Domain& d = *c.doms[k];
         if (d.npairs < 0) continue;
@@ -426,17 +438,17 @@ std::vector<char> refresh_frnn_all(Context& c) {
         d.ind_creases = -1;             // the indicator is evaluated again, as in every RKPM solve
         if (d.ng == 0) continue;
         live[k] = 1;
+        int* flag = c.rf_flags.p + 2 * k;
         const int* gsh = c.batched ? d.d_gsheet.p : nullptr;
         const SheetDev* tab = reinterpret_cast<const SheetDev*>(d.d_sheets.p);
         const dim3 grid((d.ng + 255) / 256), blk(256);
         d.off_new.ensure(d.ng + 1);
-        d.rf_flag.ensure(2);
-        const int slot = timing_begin(c, "frnn_count");
+        int slot = timing_begin(c, "frnn_count");
         hipLaunchKernelGGL((k_frnn<false>), grid, blk, 0, c.stream, c.X.p, c.Y.p, d.gx.p, d.gy.p, d.ng, gsh, tab,
                            d.cell_start.p, d.cell_nodes.p, d.cnt.p, nullptr, nullptr, nullptr, nullptr);
         CVM_HIP(hipGetLastError());
         timing_end(c, slot);
-        hipLaunchKernelGGL(k_zero_int2, dim3(1), dim3(1), 0, c.stream, d.rf_flag.p);
+        hipLaunchKernelGGL(k_zero_int2, dim3(1), dim3(1), 0, c.stream, flag);
         hipLaunchKernelGGL(k_set_zero_int, dim3(1), dim3(1), 0, c.stream, d.off_new.p);
         size_t tmp_bytes = 0;
         CVM_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, d.cnt.p, d.off_new.p + 1, d.ng, c.stream));
@@ -444,47 +456,38 @@ std::vector<char> refresh_frnn_all(Context& c) {
         CVM_HIP(hipcub::DeviceScan::InclusiveSum(d.scan_tmp.p, tmp_bytes, d.cnt.p, d.off_new.p + 1, d.ng,
                                                  c.stream));
         hipLaunchKernelGGL(k_cmp_off, dim3((d.ng + 256) / 256), dim3(256), 0, c.stream, d.off_new.p, d.off.p,
-                           d.ng + 1, d.rf_flag.p);
+                           d.ng + 1, flag);
         CVM_HIP(hipGetLastError());
-        CVM_HIP(hipMemcpyAsync(&fl[2 * k], d.rf_flag.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        slot = timing_begin(c, "frnn_fill");
+        hipLaunchKernelGGL((k_frnn<true>), grid, blk, 0, c.stream, c.X.p, c.Y.p, d.gx.p, d.gy.p, d.ng, gsh, tab,
+                           d.cell_start.p, d.cell_nodes.p, nullptr, d.off.p, d.ids.p, d.s.p, flag);
+        CVM_HIP(hipGetLastError());
+        timing_end(c, slot);
+        const int nblk = (d.ng + RKPM_BLOCK - 1) / RKPM_BLOCK;
+        slot = timing_begin(c, "interleave");
+        hipLaunchKernelGGL(k_interleave<true>, dim3(nblk), dim3(RKPM_BLOCK), 0, c.stream, d.off.p, d.ids.p, d.s.p,
+                           d.ng, d.il_row.p, d.il_ids.p, d.il_s.p, d.pair_lgp.p, flag);
+        CVM_HIP(hipGetLastError());
+        timing_end(c, slot);
     }
+    std::vector<int> fl(2 * nd, 0);
+    CVM_HIP(hipMemcpyAsync(fl.data(), c.rf_flags.p, 2 * nd * sizeof(int), hipMemcpyDeviceToHost, c.stream));
     CVM_HIP(hipStreamSynchronize(c.stream));
     for (size_t k = 0; k < nd; ++k) {
         if (!live[k]) continue;
         Domain& d = *c.doms[k];
         if (fl[2 * k]) {                // other counts: the full search
-            live[k] = 0;
             launch_frnn(c, d, c.batched ? 0.0 : d.rho);
             changed[k] = 1;
-            continue;
+        } else if (fl[2 * k + 1]) {     // same counts, other nodes: the host mirror of the ids follows
+            if (d.npairs)
+                CVM_HIP(hipMemcpyAsync(d.h_ids.data(), d.ids.p, d.npairs * sizeof(int), hipMemcpyDeviceToHost,
+                                       c.stream));
+            CVM_HIP(hipStreamSynchronize(c.stream));
+            d.have_sf = false;
+            changed[k] = 1;
         }
-        const int* gsh = c.batched ? d.d_gsheet.p : nullptr;
-        const SheetDev* tab = reinterpret_cast<const SheetDev*>(d.d_sheets.p);
-        const dim3 grid((d.ng + 255) / 256), blk(256);
-        int slot = timing_begin(c, "frnn_fill");
-        hipLaunchKernelGGL((k_frnn<true, true>), grid, blk, 0, c.stream, c.X.p, c.Y.p, d.gx.p, d.gy.p, d.ng, gsh,
-                           tab, d.cell_start.p, d.cell_nodes.p, nullptr, d.off.p, d.ids.p, d.s.p, d.rf_flag.p + 1);
-        CVM_HIP(hipGetLastError());
-        timing_end(c, slot);
-        const int nblk = (d.ng + RKPM_BLOCK - 1) / RKPM_BLOCK;
-        slot = timing_begin(c, "interleave");
-        hipLaunchKernelGGL(k_interleave, dim3(nblk), dim3(RKPM_BLOCK), 0, c.stream, d.off.p, d.ids.p, d.s.p, d.ng,
-                           d.il_row.p, d.il_ids.p, d.il_s.p, d.pair_lgp.p);
-        CVM_HIP(hipGetLastError());
-        timing_end(c, slot);
-        CVM_HIP(hipMemcpyAsync(&fl[2 * k + 1], d.rf_flag.p + 1, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     }
-    CVM_HIP(hipStreamSynchronize(c.stream));
-    for (size_t k = 0; k < nd; ++k) {
-        if (!live[k] || !fl[2 * k + 1]) continue;
-        // same counts, other nodes: the host mirror of the ids follows
-        Domain& d = *c.doms[k];
-        if (d.npairs)
-            CVM_HIP(hipMemcpyAsync(d.h_ids.data(), d.ids.p, d.npairs * sizeof(int), hipMemcpyDeviceToHost, c.stream));
-        d.have_sf = false;
-        changed[k] = 1;
-    }
-    CVM_HIP(hipStreamSynchronize(c.stream));
     return changed;
 }
 
