@@ -244,6 +244,10 @@ def split_on_creases(face: GPSet, creases: List[Crease]):
     ng = face.n
     best_d = np.full(ng, np.inf)
     best_nx = np.zeros(ng); best_ny = np.zeros(ng)
+    # GPs sorted by x once: a segment's candidates (inside its box widened by
+    # 1e-6) are a slice found by binary search, then filtered in y
+    order = np.argsort(face.gx, kind="stable")
+    sx = face.gx[order]
     for cr in creases:
         for i in range(len(cr.vx) - 1):
             ax, ay, bx, by = cr.vx[i], cr.vy[i], cr.vx[i + 1], cr.vy[i + 1]
@@ -253,8 +257,8 @@ def split_on_creases(face: GPSet, creases: List[Crease]):
                 continue
             lo_x, hi_x = min(ax, bx) - 1e-6, max(ax, bx) + 1e-6
             lo_y, hi_y = min(ay, by) - 1e-6, max(ay, by) + 1e-6
-            cand = np.nonzero((face.gx >= lo_x) & (face.gx <= hi_x)
-                              & (face.gy >= lo_y) & (face.gy <= hi_y))[0]
+            cand = order[np.searchsorted(sx, lo_x, "left"):np.searchsorted(sx, hi_x, "right")]
+            cand = np.sort(cand[(face.gy[cand] >= lo_y) & (face.gy[cand] <= hi_y)])
             if cand.size == 0:
                 continue
             px, py = face.gx[cand], face.gy[cand]

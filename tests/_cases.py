@@ -97,40 +97,70 @@ def box_sample_check(sh, x, F, K, outer, inner, box, nthreads=16, label=""):
     # sh: a workloads.Sheet; x the displacement; F, K, outer, inner the device
     # results and pattern (with the elastic folds) of the full sheet
     """Compare a device residual and tangent of a full sheet with the oracle on
-    a bounded sample: the oracle assembles only the face GPs inside box =
-    (x0, x1, y0, y1) (plus every crease GP), in their order on the sheet. A
-    node whose support disk (radius rho) lies inside the box receives every
-    face contribution it has on the full sheet, so its F_int entries and the K
-    entries between two such nodes must agree with the device's full-sheet
-    values. Returns (F rel error, K rel error, number of nodes compared, the
-    largest moment-matrix condition number over the box's GPs)."""
+    a bounded sample: the oracle assembles only the face and crease GPs inside
+    box = (x0, x1, y0, y1), in their order on the sheet, over the nodes within
+    rho of the box (renumbered in ascending order) and the two corners of the
+    sheet's bounding box, which keep the neighbour search's cell grid and hence
+    every neighbour order as on the full sheet. A node whose support disk
+    (radius rho) lies inside the box receives every contribution it has on the
+    full sheet, so its F_int entries and the K entries between two such nodes
+    must agree with the device's full-sheet values. Returns (F rel error, K rel
+    error, number of nodes compared, the largest moment-matrix condition number
+    over the box's GPs)."""
+    import dataclasses
     import oracle as O
     x0, x1, y0, y1 = box
-    fx, fy = sh.face.gx, sh.face.gy
-    m = (fx > x0) & (fx < x1) & (fy > y0) & (fy < y1)
-    sub = W.GPSet(*(None if a is None else np.asarray(a)[m]
-                    for a in (sh.face.gx, sh.face.gy, sh.face.w, sh.face.nx, sh.face.ny, sh.face.tx, sh.face.ty)))
-    import dataclasses
-    sample = dataclasses.replace(sh, face=sub, boundaries=[])
+    r = sh.rho * (1.0 + 1e-6)
+    keep = (sh.X > x0 - r) & (sh.X < x1 + r) & (sh.Y > y0 - r) & (sh.Y < y1 + r)
+    keep[np.argmin(sh.X + sh.Y)] = keep[np.argmax(sh.X + sh.Y)] = True   # grid corners (min, max)
+    assert sh.X[np.argmin(sh.X + sh.Y)] == sh.X.min() and sh.Y[np.argmin(sh.X + sh.Y)] == sh.Y.min()
+    assert sh.X[np.argmax(sh.X + sh.Y)] == sh.X.max() and sh.Y[np.argmax(sh.X + sh.Y)] == sh.Y.max()
+    nodes = np.nonzero(keep)[0]
+    newid = np.full(sh.nn, -1)
+    newid[nodes] = np.arange(nodes.size)
+
+    def inside(g):
+        m = (g.gx > x0) & (g.gx < x1) & (g.gy > y0) & (g.gy < y1)
+        return W.GPSet(*(None if a is None else np.asarray(a)[m] for a in (g.gx, g.gy, g.w, g.nx, g.ny, g.tx, g.ty)))
+    sub = inside(sh.face)
+    folds = [dataclasses.replace(f, left=inside(f.left), right=inside(f.right)) for f in sh.folds]
+    sample = dataclasses.replace(sh, X=sh.X[nodes], Y=sh.Y[nodes], face=sub, folds=folds, boundaries=[])
+    xs = x.reshape(-1, 3)[nodes].ravel()
     orc = O.plate_from_sheet(sample, nthreads=nthreads)
-    o_outer, o_inner, o_K, o_F = orc.assemble(x, flag_rig=False, nthreads=1)
+    o_outer, o_inner, o_K, o_F = orc.assemble(xs, flag_rig=False, nthreads=1)
     o_off, o_ids, _ = orc.neighbours(0, sh.rho)
     psi = orc.get_shape_functions(0, int(o_off[-1]))[0]
-    cond = moment_cond(sub.gx, sub.gy, sh.X, sh.Y, sh.rho, o_off, o_ids, psi, sh.order, sh.kernel)
+    cond = moment_cond(sub.gx, sub.gy, sample.X, sample.Y, sh.rho, o_off, o_ids, psi, sh.order, sh.kernel)
     eps = 1e-9
     inner_nodes = np.nonzero((sh.X - sh.rho > x0 + eps) & (sh.X + sh.rho < x1 - eps) &
                              (sh.Y - sh.rho > y0 + eps) & (sh.Y + sh.rho < y1 - eps))[0]
     assert inner_nodes.size > 0, "box too small: no node has its support inside"
     dofs = (3 * inner_nodes[:, None] + np.arange(3)[None, :]).ravel()
-    n = 3 * sh.nn
-    # columns first (cheap on CSC), then rows
-    Kg = upper_csc(outer, inner, K, n)[:, dofs].tocsr()[dofs].toarray()
-    Ko = upper_csc(o_outer, o_inner, o_K, n)[:, dofs].tocsr()[dofs].toarray()
-    ef = np.abs(F[dofs] - o_F[dofs]).max() / np.abs(o_F[dofs]).max()
+    sdofs = (3 * newid[inner_nodes][:, None] + np.arange(3)[None, :]).ravel()
+    Kg = _csc_block(outer, inner, K, dofs)
+    Ko = _csc_block(o_outer, o_inner, o_K, sdofs)
+    ef = np.abs(F[dofs] - o_F[sdofs]).max() / np.abs(o_F[sdofs]).max()
     ek = np.abs(Kg - Ko).max() / np.abs(Ko).max()
-    print(f"{label} oracle box {box}: {m.sum()} GPs, {inner_nodes.size} nodes compared, "
+    print(f"{label} oracle box {box}: {sub.n} GPs, {nodes.size} nodes, {inner_nodes.size} compared, "
           f"F rel {ef:.3e}, K rel {ek:.3e}, max cond(M) {cond.max():.3e}", flush=True)
     return ef, ek, inner_nodes.size, float(cond.max())
+
+
+def _csc_block(outer, inner, K, dofs):
+    """Dense block K[dofs, dofs] of a stored CSC (rows and columns in the order
+    of dofs), read from the listed columns only."""
+    outer = np.asarray(outer, np.int64)
+    lo, hi = outer[dofs], outer[dofs + 1]
+    cnt = hi - lo
+    pos = np.repeat(lo - np.concatenate(([0], np.cumsum(cnt)[:-1])), cnt) + np.arange(cnt.sum())
+    col = np.repeat(np.arange(dofs.size), cnt)
+    rowmap = np.full(int(max(dofs.max(), np.asarray(inner).max(initial=0))) + 1, -1)
+    rowmap[dofs] = np.arange(dofs.size)
+    row = rowmap[np.asarray(inner)[pos]]
+    m = row >= 0
+    out = np.zeros((dofs.size, dofs.size))
+    out[row[m], col[m]] = np.asarray(K)[pos[m]]
+    return out
 
 
 def k_bar(cond_max, fp32=False):

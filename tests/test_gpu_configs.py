@@ -50,14 +50,22 @@ def _rigid(X, Y, angle, shift):
 
 
 def _sym_matvec(outer, inner, K, v):
-    """K v for the symmetric K whose upper triangle is stored (selfadjointView<Upper>:
-    the strictly lower entries of the stored diagonal blocks are ignored)."""
+    """K v for the symmetric K whose upper triangle is stored (selfadjointView<Upper>):
+    K = U + U^T - diag(U) - L - L^T, where L holds the strictly lower entries of
+    the stored diagonal node blocks, the last three entries of each column."""
     n = outer.size - 1
-    col = np.repeat(np.arange(n, dtype=np.int32), np.diff(outer))
-    Ku = np.where(inner <= col, K, 0.0)
-    del col
-    U = sp.csc_matrix((Ku, inner, outer), shape=(n, n))
-    return U @ v + U.T @ v - U.diagonal() * v
+    U = sp.csc_matrix((K, inner, outer), shape=(n, n))
+    y = U @ v + U.T @ v - U.diagonal() * v
+    col = np.arange(n)
+    node, c = col // 3, col % 3
+    for r in range(3):
+        m = r > c                                 # row 3 node + r below the diagonal
+        pos = outer[col[m] + 1] - 3 + r
+        assert np.all(inner[pos] == 3 * node[m] + r)
+        lv = K[pos]
+        np.subtract.at(y, 3 * node[m] + r, lv * v[col[m]])    # L v
+        np.subtract.at(y, col[m], lv * v[3 * node[m] + r])    # L^T v
+    return y
 
 
 # ---------------------------------------------------------------------------
@@ -191,10 +199,16 @@ def test_serving_size_classes_match_oracle(fp32):
 # ---------------------------------------------------------------------------
 def test_large_sheet_full_size():
     """(About 200 GB of HBM with the FP64 scratch: built here, closed at the end.)"""
+    import time
+    t0 = time.perf_counter()
+
+    def tick(msg):
+        print(f"[{time.perf_counter() - t0:6.1f} s] {msg}", flush=True)
     sh = W.config5(1001)
     sh = dataclasses.replace(sh, folds=[dataclasses.replace(sh.folds[0], rigid=True)] + list(sh.folds[1:]))
-    print(f"{sh.name}: {sh.nn} nodes, {sh.face.n} face GPs", flush=True)
+    tick(f"{sh.name}: {sh.nn} nodes, {sh.face.n} face GPs")
     p = G.plate_from_sheet(sh)
+    tick("device plate set up")
     # noise of 0.2 node spacings, as 1e-3 is on the 201 x 201 sheet: strains of
     # ~0.2, where central differences with step 1e-6 are accurate to ~1e-8
     hx = 1.0 / (sh.n_side - 1)
@@ -202,12 +216,13 @@ def test_large_sheet_full_size():
     F, K = p.assemble(x)
     assert np.all(np.isfinite(F)) and np.all(np.isfinite(K))
     outer, inner = p.pattern(with_folds=True)
-    print(f"K: {K.size} values, max {np.abs(K).max():.3e}; |F| {np.abs(F).max():.3e}", flush=True)
+    tick(f"K: {K.size} values, max {np.abs(K).max():.3e}; |F| {np.abs(F).max():.3e}")
     # the oracle box sample around the crossing of the straight creases, and on
     # the arc where the vertical crease meets it
     for box in ((0.47, 0.53, 0.47, 0.53), (0.47, 0.53, 0.77, 0.83)):
         ef, ek, _, cmax = box_sample_check(sh, x, F, K, outer, inner, box, label="1001x1001")
         assert ef <= 1e-10 and ek <= k_bar(cmax), f"box {box}: F {ef:.3e}, K {ek:.3e}"
+        tick("box checked")
     del K
     # tangent consistent with central differences of the residual (the exact,
     # FD-Hessian hinge tangent: the default hinge tangent is rank-one)
@@ -220,13 +235,14 @@ def test_large_sheet_full_size():
     Fm, _ = p.assemble(x - h * v, tangent=False, exact_hinge=True)
     fd = (Fp - Fm) / (2 * h)
     del Fp, Fm
+    tick("central differences")
     Kv = _sym_matvec(outer, inner, K, v)
     rel = np.linalg.norm(Kv - fd) / np.linalg.norm(fd)
-    print(f"|K v - FD| / |FD| = {rel:.3e}", flush=True)
+    tick(f"|K v - FD| / |FD| = {rel:.3e}")
     assert rel < 1e-6
     del K, Kv
     # rigid motion
     Fr, _ = p.assemble(_rigid(sh.X, sh.Y, 0.3, (0.2, -0.1, 0.05)), tangent=False)
-    print(f"rigid motion: |F| {np.abs(Fr).max():.3e} against {np.abs(F).max():.3e}", flush=True)
+    tick(f"rigid motion: |F| {np.abs(Fr).max():.3e} against {np.abs(F).max():.3e}")
     assert np.abs(Fr).max() <= 1e-9 * np.abs(F).max()
     p.close()
