@@ -168,6 +168,7 @@ int cvm_create(int device, cvm_context** out) {
     }
     ctx->c.ws.stream = ctx->c.stream;
     ctx->c.ws.timed = true;
+    cvm::noalloc_scratch_init(device);   // device address of this GPU's static scratch (device current)
     int lds = 0;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
         ctx->c.lds_limit = static_cast<size_t>(lds);
@@ -547,6 +548,19 @@ int cvm_assemble(cvm_context* ctx, const double* x, int flags, double* F_int, do
             CVM_HIP(hipMemcpyAsync(K_values, Kd, sizeof(double) * nnz, hipMemcpyDeviceToHost, c.stream));
         CVM_HIP(hipStreamSynchronize(c.stream));
     }
+    CVM_CATCH(ctx)
+}
+
+int cvm_assemble_noalloc(cvm_context* ctx, const double* x_dev, int flags, double* F_dev, double* K_dev) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    CVM_HIP(hipSetDevice(c.device));
+    cvm::Plan& P = c.plan;
+    if (!P.ready) throw Error(CVM_E_STATE, "assembly plan not initialised (cvm_init_assembly)");
+    for (cvm::Domain* d : c.doms)
+        if ((d->kind == CVM_DOMAIN_FACE) && !d->have_sf) throw Error(CVM_E_STATE, "face shape functions not computed");
+    if (!x_dev || !F_dev || ((flags & CVM_TANGENT) && !K_dev)) throw Error(CVM_E_INVALID, "null device pointer");
+    cvm::launch_assembly_noalloc(c, flags & ~CVM_DEVICE_PTRS, x_dev, F_dev, K_dev);
     CVM_CATCH(ctx)
 }
 

@@ -23,6 +23,8 @@
 // ascending tile order and then its fold contributions in crease-GP order.
 #include <algorithm>
 #include <initializer_list>
+#include <mutex>
+#include <string>
 
 #include "cvm_internal.h"
 #include "device_common.h"
@@ -842,7 +844,7 @@ void upload_fold_descriptors(Context& c) {
 // the shape functions, so evaluations on different workspaces may run
 // concurrently on different streams.
 void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, double* Fdev, double* Kdev,
-                     LaunchSig* sig) {
+                     LaunchSig* sig, const WsPtrs* ext) {
     Plan& P = c.plan;
     const bool dry = sig && sig->dry;
     if (!P.ready) throw Error(CVM_E_STATE, "assembly plan not initialised (cvm_init_assembly)");
@@ -860,11 +862,14 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
     std::memcpy(m.A, c.A, sizeof(m.A));
     std::memcpy(m.B, c.B, sizeof(m.B));
     std::memcpy(m.D, c.D, sizeof(m.D));
-    if (dry) {
+    if (ext) {
+        // caller-provided workspace: nothing is reserved
+    } else if (dry) {
         if (workspace_would_grow(c, ws, flags)) sig->grow = true;
     } else {
         reserve_workspace(c, ws, flags);
     }
+    const WsPtrs wp = ext ? *ext : WsPtrs{ws.scr.p, ws.fscr.p, ws.fold_gp.p, ws.fold_nd.p};
     const bool wide = P.max_nbr > 64;
     const size_t lds = tile_lds_bytes(P.rec_cap, P.max_union, fp32, wide);
     const int umax = tile_umax(P.max_union);
@@ -881,11 +886,11 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
         launch_k(sig, kern, dim3(grid), dim3(TILE_THREADS), lds, st, P.d_tile_gp.p, P.d_tile_uoff.p,
                            P.d_tile_unodes.p, P.d_tile_toff.p, P.d_tile_tpairs.p, P.d_tile_nd.p, P.d_slot_dst.p, fd.off.p,
                            P.d_pair_uloc.p, P.d_pair_urank.p, fd.sf.p, static_cast<int64_t>(fd.npairs), fd.d_w.p, xdev, m, P.rec_cap,
-                           umax, tangent ? 1 : 0, scr, ws.fscr.p, P.ntiles);
+                           umax, tangent ? 1 : 0, scr, wp.fscr, P.ntiles);
     };
-    float* scr32 = reinterpret_cast<float*>(ws.scr.p);
+    float* scr32 = reinterpret_cast<float*>(wp.scr);
     if (fp32) tile(tile_kernel<float>(lam, persist, wide), scr32);
-    else tile(tile_kernel<double>(lam, persist, wide), ws.scr.p);
+    else tile(tile_kernel<double>(lam, persist, wide), wp.scr);
     if (!dry) CVM_HIP(hipGetLastError());
     tend(slot);
     if (folds) {
@@ -893,13 +898,13 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
         if (P.nfgp > 0) launch_k(sig, k_fold, dim3(P.nfgp), dim3(64), 0, st, P.nfgp,
                            reinterpret_cast<const FoldDev*>(P.d_folddev.p), P.d_fgp_fold.p, P.d_fgp_local.p,
                            P.d_fgp_noff.p, P.d_fgp_lpo.p, P.d_fgp_rpo.p, P.d_fgp_lpos.p, P.d_fgp_rpos.p, xdev,
-                           exact ? 1 : 0, ws.fold_gp.p, ws.fold_nd.p);
+                           exact ? 1 : 0, wp.fold_gp, wp.fold_nd);
         if (!dry) CVM_HIP(hipGetLastError());
         tend(slot);
     }
     slot = tbeg("reduce_F");
     launch_k(sig, k_reduce_F, dim3((c.nn + 255) / 256), dim3(256), 0, st, c.nn, P.d_fc_off.p, P.d_fc_idx.p,
-                       ws.fscr.p, P.d_ff_off.p, P.d_ff_idx.p, P.d_fnd_gp.p, ws.fold_gp.p, ws.fold_nd.p,
+                       wp.fscr, P.d_ff_off.p, P.d_ff_idx.p, P.d_fnd_gp.p, wp.fold_gp, wp.fold_nd,
                        folds ? 1 : 0, Fdev);
     if (!dry) CVM_HIP(hipGetLastError());
     tend(slot);
@@ -910,7 +915,7 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
             launch_k(sig, kern, dim3(static_cast<unsigned>((P.nblocks + RK_THREADS - 1) / RK_THREADS)),
                                dim3(RK_THREADS), 0, st,
                                P.nblocks, P.d_kc_off.p, scr, P.d_kf_off.p, P.d_kf_idx.p, P.d_fnd_gp.p,
-                               ws.fold_gp.p, ws.fold_nd.p, folds ? 1 : 0, exact ? 1 : 0,
+                               wp.fold_gp, wp.fold_nd, folds ? 1 : 0, exact ? 1 : 0,
                                uni ? P.d_blk_base_u.p : P.d_blk_base_f.p,
                                uni ? P.d_blk_stride_u.p : P.d_blk_stride_f.p, Kdev);
         };
@@ -918,10 +923,88 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
         // of ~4 h), the direct walk beyond
         const bool staged = P.nscratch <= 24 * P.nblocks;
         if (fp32) red(staged ? k_reduce_K<float> : k_reduce_K_rows<float>, scr32);
-        else red(staged ? k_reduce_K<double> : k_reduce_K_rows<double>, ws.scr.p);
+        else red(staged ? k_reduce_K<double> : k_reduce_K_rows<double>, wp.scr);
         if (!dry) CVM_HIP(hipGetLastError());
         tend(slot);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free evaluation. Its workspace is this file-scope device array:
+// one copy per GPU, loaded with the code object and held until the process
+// ends. In host code `scratch` names a host-side stand-in, not the device
+// memory, so the device address of each GPU's copy is taken once with
+// hipGetSymbolAddress (cvm_create calls noalloc_scratch_init with the device
+// current) and only that address is passed to kernels. Every call on one GPU
+// shares the array: calls that overlap on one GPU race. Nothing in it needs
+// clearing between calls: each kernel writes every scratch element that a
+// later kernel of the same evaluation reads.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = static_cast<size_t>(256) << 20;
+static_assert(kNoAllocScratchBytes == static_cast<size_t>(CVM_NOALLOC_SCRATCH_BYTES), "scratch size");
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+static hipError_t g_scratch_err[kMaxDevices] = {};
+
+// Called with `device` current; a failure is kept and reported by the entry point.
+void noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && p) g_scratch_ptr[device] = p;
+    else g_scratch_err[device] = (e == hipSuccess) ? hipErrorInvalidSymbol : e;
+}
+
+// Byte offsets of the four workspace regions in the scratch (256-byte aligned)
+// and the total.
+static size_t noalloc_layout(const Plan& P, int flags, size_t (&off)[4]) {
+    size_t n[4];
+    workspace_counts(P, flags, n);
+    size_t at = 0;
+    for (int k = 0; k < 4; ++k) {
+        off[k] = at;
+        at += (n[k] * sizeof(double) + 255) / 256 * 256;
+    }
+    return at;
+}
+
+void launch_assembly_noalloc(Context& c, int flags, const double* xdev, double* Fdev, double* Kdev) {
+    void* base = nullptr;
+    hipError_t err = hipSuccess;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (c.device >= 0 && c.device < kMaxDevices) { base = g_scratch_ptr[c.device]; err = g_scratch_err[c.device]; }
+    }
+    if (!base)
+        throw Error(CVM_E_HIP, std::string("static scratch of this device unavailable (hipGetSymbolAddress: ") +
+                               hipGetErrorString(err) + ")");
+    size_t off[4];
+    const size_t need = noalloc_layout(c.plan, flags, off);
+    if (need > kNoAllocScratchBytes)
+        throw Error(CVM_E_BUDGET, "cvm_assemble_noalloc: this sheet needs " + std::to_string(need) +
+                                  " bytes of workspace; the static scratch holds " +
+                                  std::to_string(kNoAllocScratchBytes) + " (use cvm_assemble)");
+    unsigned char* b = static_cast<unsigned char*>(base);
+    WsPtrs wp;
+    wp.fscr = reinterpret_cast<double*>(b + off[0]);
+    wp.scr = reinterpret_cast<double*>(b + off[1]);
+    wp.fold_gp = reinterpret_cast<double*>(b + off[2]);
+    wp.fold_nd = reinterpret_cast<double*>(b + off[3]);
+    const bool timed = c.ws.timed;
+    c.ws.timed = false;          // timing events are created lazily: none here
+    try {
+        launch_assembly(c, c.ws, flags, xdev, Fdev, Kdev, nullptr, &wp);
+    } catch (...) {
+        c.ws.timed = timed;
+        throw;
+    }
+    c.ws.timed = timed;
 }
 
 }  // namespace cvm

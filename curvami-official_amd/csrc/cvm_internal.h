@@ -349,8 +349,20 @@ void launch_frnn(Context& c, Domain& d, double rho);
 std::vector<char> refresh_frnn_all(Context& c);
 void launch_rkpm(Context& c, Domain& d, double rho, int order, int kernel, int* code, int* nbad, int* first);
 void launch_rkpm_all(Context& c, int order, int kernel, int* dom, int* code, int* nbad, int* first);
+// Workspace buffers given by address (the allocation-free entry point's static
+// scratch) instead of a Workspace's own buffers.
+struct WsPtrs {
+    double* scr = nullptr;
+    double* fscr = nullptr;
+    double* fold_gp = nullptr;
+    double* fold_nd = nullptr;
+};
 void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, double* Fdev, double* Kdev,
-                     LaunchSig* sig = nullptr);
+                     LaunchSig* sig = nullptr, const WsPtrs* ext = nullptr);
+// The allocation-free evaluation (cvm_assemble_noalloc): the workspace lies in
+// the library's static device scratch of the context's GPU.
+void noalloc_scratch_init(int device);
+void launch_assembly_noalloc(Context& c, int flags, const double* xdev, double* Fdev, double* Kdev);
 size_t assembly_workspace_bytes(const Context& c, int flags);
 void reserve_workspace(const Context& c, Workspace& ws, int flags);
 // true if reserve_workspace(c, ws, flags) would reallocate a buffer

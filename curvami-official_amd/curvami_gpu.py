@@ -51,6 +51,7 @@ SIGNATURES = {
     "cvm_get_pattern": (C.c_int, [C.c_void_p, C.c_int, _i, _i]),
     "cvm_csr_mapping": (C.c_int, [C.c_void_p, C.c_int, _i, C.c_int, _i]),
     "cvm_assemble": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "cvm_assemble_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "cvm_set_domain_bc": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _d, _d, C.c_double]),
     "cvm_constraint_layout": (C.c_int, [C.c_void_p, C.c_int, _i64, _i64]),
     "cvm_get_constraint_pattern": (C.c_int, [C.c_void_p, _i, _i]),
@@ -303,6 +304,12 @@ class Plate:
         """Device-pointer variant (inputs already resident in HBM)."""
         self._check(self._L.cvm_assemble(self._h, C.c_void_p(x_ptr), flags | DEVICE_PTRS,
                                          C.c_void_p(F_ptr), C.c_void_p(K_ptr) if K_ptr else None))
+
+    def assemble_noalloc(self, x_ptr, F_ptr, K_ptr, flags):
+        """cvm_assemble_noalloc: device pointers, no device allocation (the
+        library's static scratch; CVM_E_BUDGET when the sheet needs more)."""
+        self._check(self._L.cvm_assemble_noalloc(self._h, C.c_void_p(x_ptr), flags, C.c_void_p(F_ptr),
+                                                 C.c_void_p(K_ptr) if K_ptr else None))
 
     # ---- constraints and loads ----------------------------------------------
     def set_bc(self, dom, active=0, val=(0.0, 0.0, 0.0, 0.0), force=(0.0, 0.0, 0.0), moment=0.0):

@@ -187,6 +187,24 @@ CVM_API int cvm_csr_mapping(cvm_context* ctx, int gp, int* out, int cap, int* n)
  * follows cvm_get_pattern(with_folds = flags & CVM_ELASTIC_FOLDS). */
 CVM_API int cvm_assemble(cvm_context* ctx, const double* x, int flags, double* F_int, double* K_values);
 
+/* Bytes of the static device scratch of cvm_assemble_noalloc, per GPU. */
+#define CVM_NOALLOC_SCRATCH_BYTES (256LL << 20)
+
+/* cvm_assemble with device pointers (x_dev, F_dev, K_dev; CVM_DEVICE_PTRS is
+ * implied) that allocates nothing: the workspace (tile scratch rows, residual
+ * rows, hinge records) lies in a static device array of CVM_NOALLOC_SCRATCH_BYTES
+ * that the library holds on each GPU until the process ends, and the call makes
+ * no hipMalloc, hipFree or other allocating call. No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). CVM_E_BUDGET, with the
+ * size needed in cvm_last_error, when the sheet's workspace exceeds the array
+ * (up to about 48 x 48 nodes with the FP64 tangent, 66 x 66 with the FP32
+ * tangent, at a 3.75 h support).
+ * Every call on one GPU shares the array, which keeps the last call's contents:
+ * calls that overlap on one GPU (other contexts or streams included) race.
+ * Results equal cvm_assemble's bit for bit. */
+CVM_API int cvm_assemble_noalloc(cvm_context* ctx, const double* x_dev, int flags, double* F_dev, double* K_dev);
+
 /* ---- constraints and loads (the rest of PlateProblem, plate/PlateProblem.h:52-91) ---- */
 
 /* BoundaryCondition of a domain (plate/PlateData.h:36-50): active bit 0 u, 1 v,
