@@ -25,13 +25,23 @@ sign follows the previous increment (snap-back reversal), corrector iterations
 with two KKT right-hand sides, ASLA and the curvature controller for ds, the
 no-progress stop at ds_min, the accumulated external work, and the restart
 record (RestartManager, read and written through the library's `.res` codec).
-The inertia veto of a reversal (snapback_inertia_veto, off by default in the
-reference) needs the KKT factorization's negative pivot count, which SuperLU
-does not report, and is not offered.
+The inertia veto of a reversal (snapback_inertia_veto, StepController.h:25-44,
+off by default as in the reference) reads the KKT factorization's negative
+pivot count.
 
 The KKT solve replaces the reference's symmetric MUMPS factorization
-(core/include/curvami/solver/MumpsSymmetricSolver.h:46-55), which is not
-available here, by SuperLU (scipy) on the assembled symmetric KKT matrix.
+(core/include/curvami/solver/MumpsSymmetricSolver.h:46-55, :150-230) by the
+library's host solver (libcurvami_kkt: a sparse symmetric indefinite LDL^T whose
+symbolic analysis is kept while the pattern is unchanged, with the negative
+pivot count of every factorization, MUMPS' INFOG(12)); SuperLUKKTSolver (scipy)
+remains for cross-checks. The constraint algebra is sparse: the independent
+rows come from a column-pivoted QR of J^T restricted to the DOFs J touches
+(the other rows of J^T are zero, so R is that of the full QR), and the
+projector factors the sparse Gram matrix J J^T.
+
+With an inexact (FP32) tangent, finish_fp64 ends each LineSearch with one
+Newton step on the FP64 tangent once the stopping rule holds, so that the
+converged state is that of the quadratically converging FP64 iteration.
 
 A problem object provides the Problem interface
 (core/include/curvami/solver/Problem.h:16-68): num_dofs, assemble(x) -> (K upper
@@ -59,29 +69,65 @@ def full_from_upper(K):
     return (sp.triu(K) + sp.triu(K, 1).T).tocsc()
 
 
+_LOWER_CACHE = {}
+
+
+def sym_matvec(K, v):
+    """selfadjointView<Upper>(K) v without forming the symmetric matrix:
+    U v + U^T v - diag(U) v - L v - L^T v, with L the stored entries below the
+    diagonal (their positions cached per pattern)."""
+    K = K.tocsc()
+    key = (id(K.indptr), id(K.indices), K.shape, K.nnz)
+    hit = _LOWER_CACHE.get(key)
+    if hit is None or hit[0] is not K.indptr:
+        col = np.repeat(np.arange(K.shape[1], dtype=np.int64), np.diff(K.indptr))
+        low = np.nonzero(K.indices > col)[0]
+        hit = (K.indptr, low, K.indices[low].astype(np.int64), col[low])
+        _LOWER_CACHE.clear()
+        _LOWER_CACHE[key] = hit
+    _, low, lr, lc = hit
+    y = K @ v + K.T @ v - K.diagonal() * v
+    if low.size:
+        lv = K.data[low]
+        y -= np.bincount(lr, lv * v[lc], minlength=y.size) + np.bincount(lc, lv * v[lr], minlength=y.size)
+    return y
+
+
 class ConstraintProjector:
     """(J J^T) factorized once, used for the minimum-norm corrector and the
     null-space projection (ConstraintProjector.h:19-55). The Gram matrix is
-    small (rows x rows); a failed Cholesky falls back to a least-squares solve."""
+    formed sparse (rows couple only through shared DOFs) and factored by a
+    sparse LU (a dense Cholesky for up to DENSE_ROWS rows); a failed
+    factorization falls back to a least-squares solve."""
+
+    DENSE_ROWS = 512
 
     def __init__(self, J):
         self.J = J.tocsr()
         self.nr = J.shape[0]
         self._chol = None
+        self._lu = None
         if self.nr:
-            G = (self.J @ self.J.T).toarray()
-            try:
-                self._chol = sla.cho_factor(G, lower=True, check_finite=True)
-            except (np.linalg.LinAlgError, ValueError):
-                self._chol = None
+            G = (self.J @ self.J.T).tocsc()
             self._G = G
+            try:
+                if self.nr <= self.DENSE_ROWS:
+                    self._chol = sla.cho_factor(G.toarray(), lower=True, check_finite=True)
+                else:
+                    self._lu = spla.splu(G)
+            except (np.linalg.LinAlgError, ValueError, RuntimeError):
+                self._chol = self._lu = None
 
     def _gram_solve(self, b):
         if self._chol is not None:
             y = sla.cho_solve(self._chol, b)
             if np.all(np.isfinite(y)):
                 return y
-        return np.linalg.lstsq(self._G, b, rcond=None)[0]
+        if self._lu is not None:
+            y = self._lu.solve(b)
+            if np.all(np.isfinite(y)):
+                return y
+        return spla.lsqr(self._G, b, atol=1e-15, btol=1e-15)[0]
 
     def corrector(self, c, ns):
         if c.size == 0:
@@ -95,12 +141,53 @@ class ConstraintProjector:
 
 
 class KKTSolver:
-    """Direct solve of [K J^T; J 0] with K given as its upper triangle."""
+    """[K J^T; J 0] (K given as its stored upper triangle) by the library's host
+    LDL^T (MumpsSymmetricSolver's role, MumpsSymmetricSolver.h:150-230): the
+    symbolic analysis is redone only when the pattern of K or J changes;
+    neg_pivots is the negative pivot count of the last factorization
+    (INFOG(12), :58-60, :217)."""
+
+    def __init__(self, nthreads: int = 0):
+        import curvami_gpu as G
+        self._s = G.KKT(nthreads)
+        self._err = G.CvmError
+        self.factorizations = 0
+        self.last_error = 0
+        self.neg_pivots = -1
+
+    def solve(self, K_upper, J, rhs):
+        K = K_upper.tocsc()
+        J = J.tocsr()
+        if not K.has_sorted_indices:
+            K = K.sorted_indices()
+        rows = np.repeat(np.arange(J.shape[0], dtype=np.int32), np.diff(J.indptr))
+        self.last_error = 0
+        try:
+            self._s.set_pattern(K.indptr, K.indices, J.shape[0], rows, J.indices)
+            self.neg_pivots, _ = self._s.factorize(K.data, J.data)
+            self.factorizations += 1
+            sol = self._s.solve(rhs)
+        except self._err:
+            self.last_error = -1
+            return None
+        if not np.all(np.isfinite(sol)):
+            self.last_error = -2
+            return None
+        return sol
+
+    def stats(self):
+        return self._s.stats()
+
+
+class SuperLUKKTSolver:
+    """[K J^T; J 0] by scipy's SuperLU on the assembled symmetric matrix, for
+    cross-checks (ordering and factorization on every call; no inertia)."""
 
     def __init__(self, permc_spec: str = "MMD_AT_PLUS_A"):
         self.permc_spec = permc_spec
         self.factorizations = 0
         self.last_error = 0
+        self.neg_pivots = -1
 
     def solve(self, K_upper, J, rhs):
         ns = K_upper.shape[0]
@@ -139,6 +226,7 @@ class ArcLengthReport:
     stop: str                  # "lambda_tot", "kmax", "ds_min", "kkt", "stalled"
     reversals: int = 0
     restarted_from: int = 0    # step loaded from the restart record (0: none)
+    vetoed: int = 0            # reversals vetoed by the unchanged inertia
 
 
 @dataclasses.dataclass
@@ -147,6 +235,7 @@ class LineSearchReport:
     iterations: int            # Newton iterations at convergence (-1 otherwise)
     attempts: int
     history: List[tuple]       # (j, trial merit, energy change, geometric error)
+    fp64_steps: int = 0        # finishing steps on the FP64 tangent (inexact tangents)
 
 
 class ContinuationSolver:
@@ -167,6 +256,8 @@ class ContinuationSolver:
     psi = 1.0
     restart_file = ""
     asla_hold_after_reversal = False
+    snapback_inertia_veto = False
+    finish_fp64 = True          # inexact (FP32) tangents: one FP64 Newton step after the stopping rule holds
     no_progress_window = 24
     no_progress_efficiency = 0.15
     no_progress_abort = True
@@ -175,17 +266,23 @@ class ContinuationSolver:
     def __init__(self, kkt: Optional[KKTSolver] = None, verbose: bool = False):
         self.kkt = kkt or KKTSolver()
         self.ind_rows = np.zeros(0, np.int64)
+        self._prev_neg_modes = -1
         self.verbose = verbose
         self.last_report: Optional[LineSearchReport] = None
 
     # ---- InitializeTopology / filterRows (ContinuationSolver.h:105-164) -------
     def initialize_topology(self, J):
+        """ColPivHouseholderQR of J^T with the threshold max(rows, cols) eps
+        max|R_ii| (:119-136), on the DOFs J touches only: the other rows of J^T
+        are zero and leave R unchanged."""
         if J.shape[0] == 0 or J.shape[1] == 0:
             self.ind_rows = np.zeros(0, np.int64)
             return
-        Jt = J.toarray().T
-        if not np.all(np.isfinite(Jt)):
+        J = J.tocsc()
+        if not np.all(np.isfinite(J.data)):
             raise RuntimeError("Constraint Jacobian contains NaN or Inf.")
+        touched = np.nonzero(np.diff(J.indptr))[0]
+        Jt = J[:, touched].toarray().T
         R, perm = sla.qr(Jt, mode="r", pivoting=True)
         d = np.abs(np.diag(R))
         tol = max(J.shape) * np.finfo(np.float64).eps * (d.max() if d.size else 0.0)
@@ -197,6 +294,7 @@ class ContinuationSolver:
     # ---- step control (StepController.h) ---------------------------------------
     def _reset_tracking(self):
         self._since_reversal = self.NO_REVERSAL_YET
+        self._prev_neg_modes = -1
         self._np_ref = None
         self._np_count = 0
         self._np_path = 0.0
@@ -252,6 +350,21 @@ class ContinuationSolver:
         else:
             self.ds = max(self.ds * 0.5, self.dsmin)
 
+    # ---- inertia tracking (StepController.h:25-44, :125-160) --------------------
+    def _neg_modes(self, n_const):
+        """Negative modes of the reduced Hessian: negative KKT pivots of the last
+        factorization minus the constraint rows (ContinuationSolver.h:69-78)."""
+        neg = getattr(self.kkt, "neg_pivots", -1)
+        return neg - n_const if neg >= 0 else -1
+
+    def confirm_reversal(self, neg_modes) -> bool:
+        """A requested reversal goes ahead unless the veto is on, a baseline
+        exists and the inertia did not change; the tracker always advances."""
+        prev, self._prev_neg_modes = self._prev_neg_modes, neg_modes
+        if not self.snapback_inertia_veto or prev < 0 or neg_modes < 0:
+            return True
+        return neg_modes != prev
+
     def _apply_load_stiffness(self, prob, K, x, lam):
         Kl = prob.load_stiffness(x)
         return K if Kl is None else (K - lam * Kl).tocsc()
@@ -306,11 +419,15 @@ class ContinuationSolver:
                 sgn = 1.0 if float(np.dot(d_T, Fext)) >= 0.0 else -1.0
             elif float(np.dot(dU_old, d_T)) >= 0.0:
                 sgn = 1.0
-            else:
+                self._prev_neg_modes = self._neg_modes(n_const)
+            elif self.confirm_reversal(self._neg_modes(n_const)):
                 # snap-back: the tangent turned against the last increment
                 sgn = -1.0
                 self._since_reversal = 0
                 rep.reversals += 1
+            else:
+                sgn = 1.0
+                rep.vetoed += 1
             dl = sgn * self.ds / math.sqrt(float(np.dot(d_T, d_T)) + psi2 * ff)
             dU = dl * d_T + xb_p
             ok, k_corr = False, self.max_iters
@@ -328,7 +445,7 @@ class ContinuationSolver:
                 xb_c = ConstraintProjector(J_ind_c).corrector(c_ind_c, ns)
                 rhs2 = np.zeros((ns + n_const, 2))
                 rhs2[:ns, 0] = Fext_c
-                rhs2[:ns, 1] = -(R + full_from_upper(K_c) @ xb_c)
+                rhs2[:ns, 1] = -(R + sym_matvec(K_c, xb_c))
                 sol2 = self.kkt.solve(K_c, J_ind_c, rhs2)
                 if sol2 is None:
                     ok, k_corr = False, j
@@ -401,6 +518,8 @@ class ContinuationSolver:
         hist = []
         rep = LineSearchReport(False, -1, 0, hist)
         self.last_report = rep
+        finish = self.finish_fp64 and getattr(prob, "fp32_tangent", False) and hasattr(prob, "assemble_fp64")
+        fp64_step = False
         for j in range(1, self.ls_max_iters + 1):
             rep.attempts = j
             c_all, J_all = prob.constraints(d)
@@ -411,19 +530,23 @@ class ContinuationSolver:
             geom_error = float(np.abs(c_ind).max()) if c_ind.size else 0.0
             if j > 1 and max_force <= self.tol_force and energy_change <= self.tol_energy \
                     and geom_error <= self.tol_geom:
-                rep.converged = True
-                rep.iterations = j
-                return rep
+                if finish and not fp64_step:
+                    fp64_step = True       # one more step, on the FP64 tangent
+                    rep.fp64_steps += 1
+                else:
+                    rep.converged = True
+                    rep.iterations = j
+                    return rep
             if max_force >= self.div or math.isnan(max_force):
                 d[:] = d0
                 return rep
-            K, _ = prob.assemble(d)
+            K, _ = prob.assemble_fp64(d) if fp64_step else prob.assemble(d)
             Kl = prob.load_stiffness(d)
             if Kl is not None:
                 K = (K - lam * Kl).tocsc()
             xb = proj.corrector(c_ind, ns)
             rhs = np.zeros(ns + c_ind.size)
-            rhs[:ns] = -(R + full_from_upper(K) @ xb)
+            rhs[:ns] = -(R + sym_matvec(K, xb))
             sol = self.kkt.solve(K, J_ind, rhs)
             if sol is None:
                 d[:] = d0
@@ -470,7 +593,7 @@ def initial_condition_with_folds(prob_rigid, solver: "ContinuationSolver") -> np
     K, _ = prob_rigid.assemble(x)
     xb = ConstraintProjector(J_ind).corrector(c_ind, n)
     rhs = np.zeros(n + c_ind.size)
-    rhs[:n] = -(full_from_upper(K) @ xb)
+    rhs[:n] = -sym_matvec(K, xb)
     sol = solver.kkt.solve(K, J_ind, rhs)
     if sol is None:
         raise RuntimeError(f"initial folded-condition KKT solve failed with error code {solver.kkt.last_error}")
@@ -483,6 +606,7 @@ class LoadPathResult:
     converged: List[bool]
     iterations: List[int]
     lambdas: List[float]
+    fp64_steps: List[int] = dataclasses.field(default_factory=list)   # finishing steps per increment
 
     @property
     def all_converged(self) -> bool:
@@ -490,19 +614,23 @@ class LoadPathResult:
 
 
 def load_path(prob, increments: int = 20, lam_total: float = 1.0, x0=None,
-              solver: Optional[ContinuationSolver] = None, stop_on_failure: bool = True) -> LoadPathResult:
+              solver: Optional[ContinuationSolver] = None, stop_on_failure: bool = True,
+              on_increment=None) -> LoadPathResult:
     """Proportional loading lambda_k = k / increments * lam_total, one LineSearch
-    per increment from the previous converged state."""
+    per increment from the previous converged state; on_increment(k, x, report)
+    after each."""
     solver = solver or ContinuationSolver()
     x = np.zeros(prob.num_dofs) if x0 is None else np.array(x0, dtype=np.float64)
-    conv, its, lams = [], [], []
+    conv, its, lams, f64 = [], [], [], []
     for k in range(1, increments + 1):
         lam = lam_total * k / increments
         rep = solver.line_search(prob, lam, x)
-        conv.append(rep.converged); its.append(rep.iterations); lams.append(lam)
+        conv.append(rep.converged); its.append(rep.iterations); lams.append(lam); f64.append(rep.fp64_steps)
+        if on_increment is not None:
+            on_increment(k, x, rep)
         if not rep.converged and stop_on_failure:
             break
-    return LoadPathResult(x, conv, its, lams)
+    return LoadPathResult(x, conv, its, lams, f64)
 
 
 class PlateProblem:
@@ -525,12 +653,17 @@ class PlateProblem:
         self.assemblies = 0
         self.residuals = 0
 
-    def assemble(self, x):
-        F, K = self.plate.assemble(x, elastic_folds=not self.flag_rig, fp32_tangent=self.fp32_tangent,
+    def assemble(self, x, fp32_tangent=None):
+        fp32 = self.fp32_tangent if fp32_tangent is None else fp32_tangent
+        F, K = self.plate.assemble(x, elastic_folds=not self.flag_rig, fp32_tangent=fp32,
                                    exact_hinge=self.exact_hinge)
         self.assemblies += 1
         n = self.num_dofs
         return sp.csc_matrix((K, self._inner, self._outer), shape=(n, n)), F
+
+    def assemble_fp64(self, x):
+        """The tangent in FP64 whatever fp32_tangent says (the finishing step)."""
+        return self.assemble(x, fp32_tangent=False)
 
     def internal_force(self, x):
         F, _ = self.plate.assemble(x, elastic_folds=not self.flag_rig, tangent=False)

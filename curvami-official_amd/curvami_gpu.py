@@ -710,3 +710,94 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------------
+# Host KKT solver (libcurvami_kkt.so, include/curvami_kkt.h)
+# ---------------------------------------------------------------------------
+_KKT_PATH = os.path.join(_HERE, "libcurvami_kkt.so")
+_kkt_lib = None
+E_KKT = 9
+
+
+def load_kkt_library(path: str = _KKT_PATH):
+    global _kkt_lib
+    if _kkt_lib is not None:
+        return _kkt_lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build()")
+    L = C.CDLL(path)
+    P = C.c_void_p
+    sig = {
+        "cvm_kkt_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+        "cvm_kkt_destroy": (None, [P]),
+        "cvm_kkt_last_error": (C.c_char_p, [P]),
+        "cvm_kkt_set_pattern": (C.c_int, [P, C.c_int, _i, _i, C.c_int, C.c_int64, _i, _i]),
+        "cvm_kkt_factorize": (C.c_int, [P, _d, _d, _i, _i]),
+        "cvm_kkt_solve": (C.c_int, [P, C.c_int, _d, _d]),
+        "cvm_kkt_stats": (C.c_int, [P, C.POINTER(C.c_int64)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _kkt_lib = L
+    return L
+
+
+class KKT:
+    """[K J^T; J 0] by a sparse symmetric indefinite LDL^T on the host
+    (MumpsSymmetricSolver's role): set_pattern (re-analysis only when the
+    pattern changes), factorize -> (negative, positive pivots), solve."""
+
+    def __init__(self, nthreads: int = 0):
+        self._L = load_kkt_library()
+        h = C.c_void_p()
+        st = self._L.cvm_kkt_create(int(nthreads), C.byref(h))
+        if st != OK:
+            raise CvmError(st, "cvm_kkt_create failed")
+        self._h = h
+        self.n = self.m = 0
+
+    def _check(self, st):
+        if st != OK:
+            raise CvmError(st, self._L.cvm_kkt_last_error(self._h).decode(errors="replace"))
+
+    def set_pattern(self, outer, inner, m, rows, cols):
+        outer = np.ascontiguousarray(outer, np.int32); inner = np.ascontiguousarray(inner, np.int32)
+        rows = np.ascontiguousarray(rows, np.int32); cols = np.ascontiguousarray(cols, np.int32)
+        self.n, self.m = outer.size - 1, int(m)
+        self._check(self._L.cvm_kkt_set_pattern(self._h, self.n, outer.ctypes.data_as(_i), inner.ctypes.data_as(_i),
+                                                self.m, rows.size, rows.ctypes.data_as(_i), cols.ctypes.data_as(_i)))
+
+    def factorize(self, K_values, J_values):
+        K_values = _f64(K_values); J_values = _f64(J_values)
+        neg, pos = C.c_int(0), C.c_int(0)
+        self._check(self._L.cvm_kkt_factorize(self._h, _dp(K_values), _dp(J_values) if J_values.size else None,
+                                              C.byref(neg), C.byref(pos)))
+        return neg.value, pos.value
+
+    def solve(self, rhs):
+        rhs = np.asfortranarray(rhs, dtype=np.float64)
+        nrhs = 1 if rhs.ndim == 1 else rhs.shape[1]
+        x = np.empty_like(rhs, order="F")
+        self._check(self._L.cvm_kkt_solve(self._h, nrhs, rhs.ctypes.data_as(_d), x.ctypes.data_as(_d)))
+        return x
+
+    def stats(self):
+        out = (C.c_int64 * 10)()
+        self._check(self._L.cvm_kkt_stats(self._h, out))
+        keys = ("analyses", "factorizations", "solves", "kkt_entries", "factor_entries", "perturbed_pivots",
+                "analysis_us", "factor_us", "lu_fallbacks", "probe_backward_error_digits")
+        return dict(zip(keys, list(out)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.cvm_kkt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -7,13 +7,15 @@ cvm_fold_angles; crease GPs chained nearest-neighbour from one end as in
 :164-201), and the step attributes (angles, is_driver, disp, W_ext, E_mem,
 E_ben, E_hin, planar_rms, planar_max, ref_diag, git_commit; :93-129, :231-322).
 
-The reference writes these into HDF5 (`/step_NNNNN`, `/step_NNNNN_creases`);
-no HDF5 library is available to this build, so StepWriter keeps the same
-dataset names, column order and attribute names in a NumPy .npz archive
-(`step_NNNNN`, `step_NNNNN_creases`, `step_NNNNN.attrs` as JSON).
+StepWriter appends each step to an HDF5 file with the reference's layout
+(`/step_NNNNN`, `/step_NNNNN_creases` and their attributes, chunked and
+deflate-compressed) through the library's step container (libcurvami_h5,
+include/curvami_h5.h, over the HDF5 C library); StepWriter.read reads a file
+back.
 """
 from __future__ import annotations
 
+import ctypes as C
 import json
 import math
 import os
@@ -107,34 +109,105 @@ def step_record(plate, sheet, x, fields: Sequence[str] = (), crease_tags: Option
     return step, creases, attrs
 
 
+_H5 = None
+_d = C.POINTER(C.c_double)
+_i = C.POINTER(C.c_int)
+_i64 = C.POINTER(C.c_int64)
+H5_SCALARS = ("disp", "W_ext", "E_mem", "E_ben", "E_hin", "planar_rms", "planar_max", "ref_diag")
+
+
+def _h5():
+    global _H5
+    if _H5 is None:
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcurvami_h5.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build()")
+        L = C.CDLL(path)
+        L.cvm_h5_last_error.restype = C.c_char_p
+        L.cvm_h5_write_step.argtypes = [C.c_char_p, C.c_int, _d, C.c_int64, C.c_int, _d, C.c_int64, _d, C.c_int,
+                                        _i, C.c_int, _d, C.c_char_p]
+        L.cvm_h5_read_dataset.argtypes = [C.c_char_p, C.c_char_p, _i64, _d, C.c_int64]
+        L.cvm_h5_read_attr.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, _d, C.c_int, _i]
+        L.cvm_h5_read_attr_string.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]
+        L.cvm_h5_list.argtypes = [C.c_char_p, C.c_char_p, C.c_int, _i]
+        _H5 = L
+    return _H5
+
+
+def _h5check(st):
+    if st != 0:
+        raise RuntimeError(_h5().cvm_h5_last_error().decode(errors="replace"))
+
+
 class StepWriter:
-    """Collects steps under the reference's dataset names and writes them to
-    one .npz archive (rewritten in full on every step, so a run that stops
-    leaves the steps written so far)."""
+    """Appends steps to an HDF5 file with the reference's dataset names,
+    chunking, compression and attributes (Exporter.h:203-322). The file is
+    created when it does not exist and opened for writing otherwise, so the
+    steps already written survive a later failure."""
 
     def __init__(self, path: str):
         self.path = path
-        self.arrays: Dict[str, np.ndarray] = {}
 
     def write(self, step: int, data: np.ndarray, creases: np.ndarray, attrs: dict):
-        name = f"step_{step:05d}"
-        self.arrays[name] = np.ascontiguousarray(data)
-        if creases.size:
-            self.arrays[name + "_creases"] = np.ascontiguousarray(creases)
-            self.arrays[name + "_creases.columns"] = np.array(CREASE_COLUMNS)
-        self.arrays[name + ".attrs"] = np.array(json.dumps(attrs))
-        tmp = self.path + ".tmp.npz"
-        np.savez_compressed(tmp, **self.arrays)
-        os.replace(tmp, self.path)
+        data = np.ascontiguousarray(data, dtype=np.float64)
+        creases = np.ascontiguousarray(creases, dtype=np.float64).reshape(-1, 7)
+        ang = np.ascontiguousarray(attrs.get("angles", []), dtype=np.float64)
+        drv = np.ascontiguousarray(attrs.get("is_driver", []), dtype=np.int32)
+        sc = np.array([float(attrs.get(k, 0.0)) for k in H5_SCALARS])
+        L = _h5()
+        _h5check(L.cvm_h5_write_step(self.path.encode(), int(step), data.ctypes.data_as(_d), data.shape[0],
+                                     data.shape[1], creases.ctypes.data_as(_d) if creases.size else None,
+                                     creases.shape[0], ang.ctypes.data_as(_d) if ang.size else None, ang.size,
+                                     drv.ctypes.data_as(_i) if drv.size else None, drv.size, sc.ctypes.data_as(_d),
+                                     str(attrs.get("git_commit", "unknown")).encode()))
 
     @staticmethod
-    def read(path: str):
-        """{step: (data, creases or None, attrs)}."""
+    def read_dataset(path: str, name: str) -> np.ndarray:
+        L = _h5()
+        dims = (C.c_int64 * 2)()
+        _h5check(L.cvm_h5_read_dataset(path.encode(), name.encode(), dims, None, 0))
+        out = np.empty((dims[0], dims[1]))
+        _h5check(L.cvm_h5_read_dataset(path.encode(), name.encode(), dims, out.ctypes.data_as(_d), out.size))
+        return out
+
+    @staticmethod
+    def read_attr(path: str, dataset: str, attr: str) -> np.ndarray:
+        L = _h5()
+        n = C.c_int(0)
+        _h5check(L.cvm_h5_read_attr(path.encode(), dataset.encode(), attr.encode(), None, 0, C.byref(n)))
+        out = np.empty(n.value)
+        _h5check(L.cvm_h5_read_attr(path.encode(), dataset.encode(), attr.encode(), out.ctypes.data_as(_d),
+                                    out.size, C.byref(n)))
+        return out
+
+    @staticmethod
+    def read_attr_string(path: str, dataset: str, attr: str) -> str:
+        buf = C.create_string_buffer(4096)
+        _h5check(_h5().cvm_h5_read_attr_string(path.encode(), dataset.encode(), attr.encode(), buf, 4096))
+        return buf.value.decode()
+
+    @staticmethod
+    def names(path: str) -> List[str]:
+        L = _h5()
+        need = C.c_int(0)
+        _h5check(L.cvm_h5_list(path.encode(), None, 0, C.byref(need)))
+        buf = C.create_string_buffer(need.value)
+        _h5check(L.cvm_h5_list(path.encode(), buf, need.value, C.byref(need)))
+        return [n for n in buf.value.decode().split(",") if n]
+
+    @classmethod
+    def read(cls, path: str):
+        """{step: (data, creases or None, attrs)} with the HDF5 attributes."""
         out = {}
-        with np.load(path, allow_pickle=False) as z:
-            for k in z.files:
-                if k.startswith("step_") and k.count("_") == 1 and not k.endswith(".attrs"):
-                    s = int(k[5:])
-                    cr = z[k + "_creases"] if (k + "_creases") in z.files else None
-                    out[s] = (z[k], cr, json.loads(str(z[k + ".attrs"])))
+        names = cls.names(path)
+        for k in names:
+            if not (k.startswith("step_") and k.count("_") == 1):
+                continue
+            s = int(k[5:])
+            cr = cls.read_dataset(path, k + "_creases") if (k + "_creases") in names else None
+            attrs = {a: float(cls.read_attr(path, k, a)[0]) for a in H5_SCALARS}
+            attrs["angles"] = cls.read_attr(path, k, "angles").tolist()
+            attrs["is_driver"] = [int(v) for v in cls.read_attr(path, k, "is_driver")]
+            attrs["git_commit"] = cls.read_attr_string(path, k, "git_commit")
+            out[s] = (cls.read_dataset(path, k), cr, attrs)
         return out

@@ -14,9 +14,11 @@ sub-steps. Energies (cvm_energies) and crease angles (cvm_fold_angles) are
 device results; the per-step bookkeeping here is host logic.
 
 Each accepted step can be exported through `on_step` (export.step_record /
-StepWriter). Not reproduced: the coefficient re-fit (refit other than
-"none"), the diagnose_step mode and the negative-mode count of the KKT
-factorization (reported as None).
+StepWriter). The negative-mode count of each step is read from the last KKT
+factorization of its Newton solve, as lastNegativeModes does
+(ProgressiveCreasing.h:144-153; None where that factorization reported no
+inertia). Not reproduced: the coefficient re-fit (refit other than "none")
+and the diagnose_step mode.
 """
 from __future__ import annotations
 
@@ -408,8 +410,9 @@ def run(backend, x: np.ndarray, lines: List[Line], static_creases: List[Crease],
         for f in all_folds:
             prev_active[f] = active[f].copy()
         prev_edge_ends, prev_edge_line = edge_ends, l
+        nm = backend.neg_modes() if hasattr(backend, "neg_modes") else None
         rec = StepRecord(k, s.a, s.b, ell, s.frac, iters, substeps, r["E_mem"], r["E_ben"], r["E_hin"],
-                         r["th_mean"], r["th_max"], jump, r["change"], retry, l)
+                         r["th_mean"], r["th_max"], jump, r["change"], retry, l, nm)
         say(f" >> step {k}: iters {iters} (guard retry {retry}, sub-steps {substeps}) | E_mem {rec.E_mem:.4e} "
             f"E_ben {rec.E_ben:.4e} E_hin {rec.E_hin:.4e} | theta mean {rec.theta_mean_deg:.4f} max "
             f"{rec.theta_max_deg:.4f} deg | step change {rec.change_over_L:.3e} L")
@@ -477,6 +480,13 @@ class DeviceBackend(Backend):
         prob = PlateProblem(self.plate, flag_rig=False, exact_hinge=self.exact_hinge)
         rep = self.solver.line_search(prob, 0.0, x)
         return rep.converged, rep.attempts
+
+    def neg_modes(self):
+        """Negative modes of the reduced Hessian from the last KKT factorization
+        (negative pivots minus the independent constraint rows,
+        ProgressiveCreasing.h:151-153), or None without an inertia."""
+        neg = getattr(self.solver.kkt, "neg_pivots", -1)
+        return None if neg < 0 else int(neg) - int(self.solver.ind_rows.size)
 
 
 def lines_from_config(ms, pc: dict) -> (List[Line], List[Crease]):

@@ -63,24 +63,38 @@ def _sym(K):
 
 
 def _independent_rows(J):
+    """ColPivHouseholderQR of J^T (ContinuationSolver.h:119-136) over the DOFs
+    that J touches (the other rows of J^T are zero: the same R)."""
     if J.shape[0] == 0:
         return np.zeros(0, np.int64)
-    R, perm = sla.qr(J.toarray().T, mode="r", pivoting=True)
+    Jc = sp.csc_matrix(J)
+    touched = np.nonzero(np.diff(Jc.indptr))[0]
+    R, perm = sla.qr(Jc[:, touched].toarray().T, mode="r", pivoting=True)
     d = np.abs(np.diag(R))
     tol = max(J.shape) * np.finfo(float).eps * d.max()
     return np.sort(perm[:d.size][d > tol])
 
 
 def _projector(Jd):
-    """Returns (corrector(c), project(R)) for a dense J_ind."""
+    """Returns (corrector(c), project(R)) for J_ind (dense or sparse); the Gram
+    matrix J J^T (rows x rows) is solved dense."""
     if Jd.shape[0] == 0:
         return (lambda c, ns: np.zeros(ns)), (lambda R: R)
-    G = Jd @ Jd.T
-    return (lambda c, ns: Jd.T @ np.linalg.solve(G, -c)), (lambda R: R - Jd.T @ np.linalg.solve(G, Jd @ R))
+    Js = sp.csr_matrix(Jd)
+    G = (Js @ Js.T).toarray()
+    return (lambda c, ns: Js.T @ np.linalg.solve(G, -c)), (lambda R: R - Js.T @ np.linalg.solve(G, Js @ R))
 
 
-def line_search(prob, lam, d, tol_force=1e-8, tol_energy=1e-8, tol_geom=1e-12, div=1e7, max_iters=500):
-    """Returns (converged, iterations); d is updated in place (rolled back on failure)."""
+def _kkt_spsolve(Ks, Js, rhs):
+    A = sp.bmat([[Ks, Js.T], [Js, None]], format="csc") if Js.shape[0] else Ks.tocsc()
+    return spla.spsolve(A, rhs)
+
+
+def line_search(prob, lam, d, tol_force=1e-8, tol_energy=1e-8, tol_geom=1e-12, div=1e7, max_iters=500,
+                kkt=None):
+    """Returns (converged, iterations); d is updated in place (rolled back on failure).
+    kkt(K_upper, J_ind, rhs) -> solution replaces the SuperLU solve of the
+    assembled KKT matrix (large sheets)."""
     ns = d.size
     d0 = d.copy()
     c, J = prob.constraints(d)
@@ -88,7 +102,7 @@ def line_search(prob, lam, d, tol_force=1e-8, tol_energy=1e-8, tol_geom=1e-12, d
     energy = 1.0
     for j in range(1, max_iters + 1):
         c, J = prob.constraints(d)
-        ci, Jd = c[ind], J[ind].toarray()
+        ci, Jd = c[ind], sp.csr_matrix(J)[ind]
         R = prob.internal_force(d) - lam * prob.external_force(d)
         corr, proj = _projector(Jd)
         mf = np.abs(proj(R)).max()
@@ -101,15 +115,14 @@ def line_search(prob, lam, d, tol_force=1e-8, tol_energy=1e-8, tol_geom=1e-12, d
         K, _ = prob.assemble(d)
         xb = corr(ci, ns)
         Ks = _sym(K)
-        A = sp.bmat([[Ks, sp.csr_matrix(Jd).T], [sp.csr_matrix(Jd), None]], format="csc") if ci.size else Ks.tocsc()
         rhs = np.concatenate([-(R + Ks @ xb), np.zeros(ci.size)])
-        dx = spla.spsolve(A, rhs)[:ns]
+        dx = (kkt(K, Jd, rhs) if kkt is not None else _kkt_spsolve(Ks, Jd, rhs))[:ns]
         alpha = 1.0
         for _ in range(6):
             dt = d + xb + alpha * dx
             Rt = prob.internal_force(dt) - lam * prob.external_force(dt)
             ct, Jt = prob.constraints(dt)
-            _, pt = _projector(Jt[ind].toarray())
+            _, pt = _projector(sp.csr_matrix(Jt)[ind])
             if np.abs(pt(Rt)).max() <= 1.05 * mf:
                 break
             alpha *= 0.5
@@ -119,12 +132,16 @@ def line_search(prob, lam, d, tol_force=1e-8, tol_energy=1e-8, tol_geom=1e-12, d
     return False, -1
 
 
-def load_path(prob, increments=20, lam_total=1.0, **tol):
-    x = np.zeros(prob.num_dofs)
+def load_path(prob, increments=20, lam_total=1.0, x0=None, first=1, on_increment=None, **tol):
+    """Increments first..increments (lambda_k = k / increments * lam_total) from
+    x0 (default 0); on_increment(k, x, iterations) after each."""
+    x = np.zeros(prob.num_dofs) if x0 is None else np.array(x0, dtype=float)
     conv, its = [], []
-    for k in range(1, increments + 1):
+    for k in range(first, increments + 1):
         ok, it = line_search(prob, lam_total * k / increments, x, **tol)
         conv.append(ok); its.append(it)
+        if on_increment is not None:
+            on_increment(k, x, it)
         if not ok:
             break
     return x, conv, its

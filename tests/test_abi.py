@@ -51,3 +51,31 @@ def test_no_device_fails_loudly():
         pytest.skip("GPU present")
     with pytest.raises(G.CvmError):
         G.Plate(0)
+
+
+KKT_HEADER = os.path.join(ROOT, "include", "curvami_kkt.h")
+
+
+def test_kkt_library_exports_every_declared_symbol():
+    """libcurvami_kkt.so exports exactly what include/curvami_kkt.h declares."""
+    names = set(re.findall(r"CVM_KKT_API\s+[\w\s\*]+?\b(cvm_kkt_\w+)\s*\(", open(KKT_HEADER).read()))
+    assert {"cvm_kkt_create", "cvm_kkt_factorize", "cvm_kkt_solve"} <= names
+    path = os.path.join(ROOT, "curvami-official_amd", "libcurvami_kkt.so")
+    if not os.path.exists(path):
+        pytest.skip("library not built")
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l and l.split()[-1].startswith("cvm_")}
+    assert exported == names, exported ^ names
+
+
+def test_h5_library_exports_every_declared_symbol():
+    """libcurvami_h5.so exports exactly what include/curvami_h5.h declares."""
+    header = os.path.join(ROOT, "include", "curvami_h5.h")
+    names = set(re.findall(r"CVM_H5_API\s+[\w\s\*]+?\b(cvm_h5_\w+)\s*\(", open(header).read()))
+    assert {"cvm_h5_write_step", "cvm_h5_read_dataset", "cvm_h5_list"} <= names
+    path = os.path.join(ROOT, "curvami-official_amd", "libcurvami_h5.so")
+    if not os.path.exists(path):
+        pytest.skip("library not built")
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l and l.split()[-1].startswith("cvm_")}
+    assert exported == names, exported ^ names

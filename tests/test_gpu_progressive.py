@@ -157,4 +157,9 @@ def test_smoke_example_known_answer():
     for k in ("E_mem", "E_ben", "E_hin", "theta_mean_deg"):
         assert within_printed_digits(getattr(last, k), SMOKE_KNOWN[k]), (k, getattr(last, k))
     print(f"smoke step {last.step}: E_mem {last.E_mem:.4e} E_ben {last.E_ben:.4e} E_hin {last.E_hin:.4e} "
-          f"theta mean {last.theta_mean_deg:.4f} deg (README: {SMOKE_KNOWN})")
+          f"theta mean {last.theta_mean_deg:.4f} deg (README: {SMOKE_KNOWN}); negative modes per step "
+          f"{[r.neg_modes for r in recs]}")
+    # the reduced-Hessian inertia of each step's last KKT factorization is reported
+    # (a stable path with the exact hinge tangent has none negative)
+    assert all(r.neg_modes is None or r.neg_modes >= 0 for r in recs)
+    assert any(r.neg_modes is not None for r in recs)

@@ -229,7 +229,9 @@ def test_arclength_through_limit_points():
 
 def test_arclength_restart_resumes_the_run(tmp_path):
     """RestartManager: a run stopped after 5 steps and resumed from its .res record
-    ends in the uninterrupted run's state, with the same work."""
+    ends in the uninterrupted run's state, with the same work. To round-off: the
+    KKT factorization's scaling and matching come from the values of its first
+    analysis, which the resumed run performs at the restart state."""
     full = CS.ContinuationSolver()
     full.psi, full.ds, full.dsmax = 1.0, 0.2, 0.5
     ref = full.arc_length(CubicSpring(), 0.0, np.zeros(2), 5.0)
@@ -243,5 +245,6 @@ def test_arclength_restart_resumes_the_run(tmp_path):
     rep2 = second.arc_length(CubicSpring(), 0.0, np.zeros(2), 5.0)
     assert rep2.restarted_from == 6
     assert len(rep1.steps) + len(rep2.steps) == len(ref.steps)
-    assert rep2.lam == ref.lam and np.array_equal(rep2.x, ref.x)
+    assert rep2.lam == pytest.approx(ref.lam, rel=1e-13)
+    assert np.abs(rep2.x - ref.x).max() <= 1e-13 * np.abs(ref.x).max()
     assert rep2.steps[-1].W_ext == pytest.approx(ref.steps[-1].W_ext, rel=1e-12)
