@@ -39,9 +39,15 @@ rows come from a column-pivoted QR of J^T restricted to the DOFs J touches
 (the other rows of J^T are zero, so R is that of the full QR), and the
 projector factors the sparse Gram matrix J J^T.
 
-With an inexact (FP32) tangent, finish_fp64 ends each LineSearch with one
-Newton step on the FP64 tangent once the stopping rule holds, so that the
-converged state is that of the quadratically converging FP64 iteration.
+With an inexact (FP32) tangent the iteration is safeguarded: an FP32-tangent
+step that reduces the merit by less than fp32_stall_ratio switches the rest of
+the increment to the FP64 tangent, and after fp32_stall_limit such increments
+the solver keeps the FP64 tangent (on large sheets folding is nearly rigid for
+the facets, so the face tangent's action along the fold cancels down to the
+hinge's small share, below the FP32 rounding of the face blocks, and the
+inexact iteration stalls); finish_fp64 ends each LineSearch with one Newton
+step on the FP64 tangent once the stopping rule holds, so that the converged
+state is that of the quadratically converging FP64 iteration.
 
 A problem object provides the Problem interface
 (core/include/curvami/solver/Problem.h:16-68): num_dofs, assemble(x) -> (K upper
@@ -235,7 +241,8 @@ class LineSearchReport:
     iterations: int            # Newton iterations at convergence (-1 otherwise)
     attempts: int
     history: List[tuple]       # (j, trial merit, energy change, geometric error)
-    fp64_steps: int = 0        # finishing steps on the FP64 tangent (inexact tangents)
+    fp64_steps: int = 0        # FP64 tangents used by an inexact-tangent LineSearch (finishing step, stall)
+    fp32_stalled: bool = False  # the inexact iteration stalled and switched to the FP64 tangent
 
 
 class ContinuationSolver:
@@ -258,6 +265,8 @@ class ContinuationSolver:
     asla_hold_after_reversal = False
     snapback_inertia_veto = False
     finish_fp64 = True          # inexact (FP32) tangents: one FP64 Newton step after the stopping rule holds
+    fp32_stall_ratio = 0.9      # inexact tangents: a step keeping more of the merit switches to FP64
+    fp32_stall_limit = 2        # after this many stalled increments the FP32 tangent is given up
     no_progress_window = 24
     no_progress_efficiency = 0.15
     no_progress_abort = True
@@ -267,6 +276,7 @@ class ContinuationSolver:
         self.kkt = kkt or KKTSolver()
         self.ind_rows = np.zeros(0, np.int64)
         self._prev_neg_modes = -1
+        self.fp32_stalls = 0         # increments whose FP32-tangent iteration stalled
         self.verbose = verbose
         self.last_report: Optional[LineSearchReport] = None
 
@@ -518,8 +528,12 @@ class ContinuationSolver:
         hist = []
         rep = LineSearchReport(False, -1, 0, hist)
         self.last_report = rep
-        finish = self.finish_fp64 and getattr(prob, "fp32_tangent", False) and hasattr(prob, "assemble_fp64")
-        fp64_step = False
+        inexact = getattr(prob, "fp32_tangent", False) and hasattr(prob, "assemble_fp64")
+        finish = self.finish_fp64 and inexact
+        fp64_step = False           # the next tangent is FP64 (finishing step)
+        # every further tangent of this increment is FP64: after a stall, or from
+        # the start once the FP32 tangent has stalled fp32_stall_limit times
+        fp64_rest = inexact and self.fp32_stalls >= self.fp32_stall_limit
         for j in range(1, self.ls_max_iters + 1):
             rep.attempts = j
             c_all, J_all = prob.constraints(d)
@@ -530,9 +544,8 @@ class ContinuationSolver:
             geom_error = float(np.abs(c_ind).max()) if c_ind.size else 0.0
             if j > 1 and max_force <= self.tol_force and energy_change <= self.tol_energy \
                     and geom_error <= self.tol_geom:
-                if finish and not fp64_step:
+                if finish and not fp64_step and not fp64_rest:
                     fp64_step = True       # one more step, on the FP64 tangent
-                    rep.fp64_steps += 1
                 else:
                     rep.converged = True
                     rep.iterations = j
@@ -540,7 +553,10 @@ class ContinuationSolver:
             if max_force >= self.div or math.isnan(max_force):
                 d[:] = d0
                 return rep
-            K, _ = prob.assemble_fp64(d) if fp64_step else prob.assemble(d)
+            use64 = fp64_step or fp64_rest
+            K, _ = prob.assemble_fp64(d) if use64 else prob.assemble(d)
+            if use64 and inexact:
+                rep.fp64_steps += 1
             Kl = prob.load_stiffness(d)
             if Kl is not None:
                 K = (K - lam * Kl).tocsc()
@@ -570,6 +586,10 @@ class ContinuationSolver:
                 d[:] = d0
                 return rep
             energy_change = abs(float(np.dot(xb + alpha * dx, R)))
+            if inexact and not use64 and trial > self.fp32_stall_ratio * max_force:
+                fp64_rest = True
+                rep.fp32_stalled = True
+                self.fp32_stalls += 1
             d[:] = d_trial
             hist.append((j, trial, energy_change, geom_error))
             if self.verbose:

@@ -100,17 +100,18 @@ def test_newton_mixed_precision_tangent(c1m):
     (max|P R| <= tol_force), where the quadratically converging FP64 iteration
     overshoots it by orders of magnitude; one Newton step on the FP64 tangent
     once the rule holds (ContinuationSolver.finish_fp64) brings the state to
-    the FP64 one. Bar: 1e-8 relative to the FP64 oracle's path; the iteration
-    counts (the finishing steps included) and their change are reported."""
+    the FP64 one, and a stalled FP32 step switches the rest of the increment to
+    the FP64 tangent. Bar: 1e-8 relative to the FP64 oracle's path; the
+    iteration counts (FP32 and FP64 tangents) and their change are reported."""
     sh, _, gpu = c1m
     ref, res, prob, dt = _run_path(gpu, fp32=True)
     err = np.abs(res.x - ref["x"]).max() / np.abs(ref["x"]).max()
     d_it = int(np.sum(res.iterations) - np.sum(ref["iterations"]))
-    print(f"mixed path: iterations {res.iterations} (sum {sum(res.iterations)}, of which {sum(res.fp64_steps)} "
-          f"FP64 finishing steps; oracle FP64 sum {int(np.sum(ref['iterations']))}, delta {d_it:+d}), "
-          f"|dx|/|x| {err:.3e}, {dt:.1f} s")
+    print(f"mixed path: iterations {res.iterations} (sum {sum(res.iterations)}: {sum(res.iterations) - sum(res.fp64_steps)} "
+          f"on the FP32 tangent, {sum(res.fp64_steps)} on the FP64 one; oracle FP64 sum "
+          f"{int(np.sum(ref['iterations']))}, delta {d_it:+d}), |dx|/|x| {err:.3e}, {dt:.1f} s")
     assert res.all_converged and len(res.converged) == W.NEWTON_INCREMENTS
-    assert sum(res.fp64_steps) == W.NEWTON_INCREMENTS
+    assert all(f >= 1 for f in res.fp64_steps)      # every increment ends on the FP64 tangent
     assert err <= 1e-8
 
 

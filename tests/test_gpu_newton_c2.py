@@ -9,9 +9,10 @@ make_c2_newton_fixture.py).
 
 Bars: every increment converged, the same Newton iteration count per increment
 as the oracle's path, the displacement within 1e-8 (inf-norm, relative) after
-increments 1, 10 and 20. The mixed path (FP32 face tangent, one FP64
-finishing step per increment) converges to the same states within 1e-8 and
-reports its iteration counts.
+increments 1, 10 and 20. The mixed path (FP32 face tangent; a stalled FP32
+step switches the increment to the FP64 tangent, and every increment ends with
+an FP64 step) converges to the same states within 1e-8 and reports its
+iteration counts.
 
 The first two increments run by default; the whole path (about 100
 factorizations of the KKT system, minutes on the host) runs with
@@ -59,7 +60,8 @@ def _path(gpu, ref, fp32, increments):
         f64.append(rep.fp64_steps)
         if k in (1, 10):
             states[k] = x.copy()
-        print(f"increment {k}: {rep.iterations} iterations, {time.perf_counter() - t0:.1f} s", flush=True)
+        print(f"increment {k}: {rep.iterations} iterations ({rep.fp64_steps} on the FP64 tangent), "
+              f"{time.perf_counter() - t0:.1f} s", flush=True)
     states["last"] = x.copy()
     return its, f64, states, solver.kkt.stats(), time.perf_counter() - t0
 
@@ -96,6 +98,7 @@ def test_c2_mixed_precision_path(c2):
     its, f64, states, st, dt = _path(gpu, ref, fp32=True, increments=W.NEWTON_INCREMENTS)
     err = _rel(states["last"], ref["x_last"])
     d_it = int(sum(its) - int(np.sum(ref["iterations"])))
-    print(f"C2 mixed path: iterations {its} (sum {sum(its)}, {sum(f64)} FP64 finishing steps; oracle FP64 sum "
-          f"{int(np.sum(ref['iterations']))}, delta {d_it:+d}), x rel after 20 {err:.3e}, {dt:.1f} s, KKT {st}")
+    print(f"C2 mixed path: iterations {its} (sum {sum(its)}: {sum(its) - sum(f64)} on the FP32 tangent, {sum(f64)} on "
+          f"the FP64 one; oracle FP64 sum {int(np.sum(ref['iterations']))}, delta {d_it:+d}), x rel after 20 "
+          f"{err:.3e}, {dt:.1f} s, KKT {st}")
     assert err <= 1e-8
