@@ -83,6 +83,10 @@ class Hip:
         if self.L.hipMemcpy(arr.ctypes.data_as(C.c_void_p), C.c_void_p(src), arr.nbytes, 2) != 0:
             raise RuntimeError("hipMemcpy D2H failed")
 
+    def d2d(self, dst, src, nbytes):
+        if self.L.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), C.c_size_t(nbytes), 3) != 0:
+            raise RuntimeError("hipMemcpy D2D failed")
+
     def free(self, p):
         self.L.hipFree(C.c_void_p(p))
 
@@ -418,7 +422,21 @@ def main():
                     help="skip the leg at the reference's defaults (cubic basis, rho = 6h)")
     ap.add_argument("--batch-sheets", type=int, default=W.BATCH_SHEETS)
     ap.add_argument("--serving-requests", type=int, default=200)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device three times (the known-size dispatch that "
+                         "calibrates the FETCH_SIZE / WRITE_SIZE counters), then exit")
     args = ap.parse_args()
+    if args.copy_probe > 0:
+        hip = Hip()
+        nb = args.copy_probe << 20
+        src, dst = hip.malloc(nb), hip.malloc(nb)
+        for _ in range(3):
+            hip.d2d(dst, src, nb)
+        hip.sync()
+        hip.free(src)
+        hip.free(dst)
+        log(f"copy probe: 3 copies of {args.copy_probe} MiB")
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only: sharding a sheet or a batch over "
                                    "several GPUs is designed, not built or measured", "n_gpus": args.gpus}))

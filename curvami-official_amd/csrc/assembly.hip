@@ -282,13 +282,11 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
             for (int k = 0; k < 9; ++k) { acc[k] = TT(0); acc2[k] = TT(0); }
             // Per coupling GP the block is K_IJ = T0 a1^T + T1 a2^T + x b^T + b y^T + W geo I
             // with the weight folded into T0, T1, x, y (W S, W Tu);
-            // terms() forms those factors, add() accumulates them into the slot's block.
-            // Two coupling GPs per iteration, the first GP's terms added first (ascending
-            // GP order).
+            // terms() forms those factors, add() accumulates them into the slot's block,
+            // coupling GPs in ascending order.
             struct Terms { TT T0[3], T1[3], xv[3], yv[3], a1[3], a2[3], bv[3], wgeo; };
-            auto terms = [&](int wv, int ubb, Terms& t) {
-                const int ri = loc[wv * U + ua];
-                const int rj = loc[wv * U + ubb];
+            // ri, rj: the records of the slot's two nodes at coupling GP wv
+            auto terms = [&](int wv, int ri, int rj, Terms& t) {
                 const TT W = Gp(G_W, wv);
 #pragma unroll
                 for (int r = 0; r < 3; ++r) {
@@ -365,33 +363,39 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
                         a[3 * c + r] = v;
                     }
             };
+            // The record indices of the next coupling GP are read one iteration
+            // ahead, so that an iteration waits for one LDS round trip (its
+            // records), not two (indices, then records).
             if (paired) {
                 // one coupling GP per iteration, both blocks
+                int w0 = mask ? __ffs(mask) - 1 : 0;
+                int li = loc[w0 * U + ua], lj = loc[w0 * U + ub1], lk = loc[w0 * U + ub2];
                 while (mask) {
-                    const int w0 = __ffs(mask) - 1;
+                    const int wc = w0, ci = li, cj = lj, ck = lk;
                     mask &= mask - 1;
+                    if (mask) {
+                        w0 = __ffs(mask) - 1;
+                        li = loc[w0 * U + ua]; lj = loc[w0 * U + ub1]; lk = loc[w0 * U + ub2];
+                    }
                     Terms t0, t1;
-                    terms(w0, ub1, t0);
-                    terms(w0, ub2, t1);
+                    terms(wc, ci, cj, t0);
+                    terms(wc, ci, ck, t1);
                     add(acc, t0);
                     add(acc2, t1);
                 }
             } else {
+                int w0 = mask ? __ffs(mask) - 1 : 0;
+                int li = loc[w0 * U + ua], lj = loc[w0 * U + ub1];
                 while (mask) {
-                    const int w0 = __ffs(mask) - 1;
+                    const int wc = w0, ci = li, cj = lj;
                     mask &= mask - 1;
-                    Terms t0;
-                    terms(w0, ub1, t0);
                     if (mask) {
-                        const int w1 = __ffs(mask) - 1;
-                        mask &= mask - 1;
-                        Terms t1;
-                        terms(w1, ub1, t1);
-                        add(acc, t0);
-                        add(acc, t1);
-                    } else {
-                        add(acc, t0);
+                        w0 = __ffs(mask) - 1;
+                        li = loc[w0 * U + ua]; lj = loc[w0 * U + ub1];
                     }
+                    Terms t0;
+                    terms(wc, ci, cj, t0);
+                    add(acc, t0);
                 }
             }
             TT* dst = scr + 9 * static_cast<int64_t>(slot_dst[tb + sl]);

@@ -129,7 +129,7 @@ struct Domain {
     // the node cell grid of the last search (FRNN.h:30-51), kept for
     // cvm_refresh_neighbours; valid for grid_nn nodes and grid_ng GPs
     DBuf<int> cell_start, cell_nodes;
-    DBuf<int> cnt, off_new;   // per GP counts and offsets of a refresh
+    DBuf<int> cnt;            // per GP neighbour counts of the search
     DBuf<unsigned char> scan_tmp;
     int grid_nn = -1, grid_ng = -1;
     // The same lists interleaved per block of RKPM_BLOCK GPs for the moment
@@ -286,11 +286,15 @@ struct Context {
     // creases (host) + device packed vertices
     std::vector<Crease> creases;
     DBuf<double> d_cv;          // packed vertices x,y
-    DBuf<int> d_cinfo;          // per crease: offset, count, ext_s, ext_e, closed
+    DBuf<int> d_cinfo;          // per crease: offset, count, ext_s, ext_e, closed, first group, groups
+    DBuf<double> d_cgrp;        // per group of 16 segments: bounding box and first vertex
+    DBuf<double> d_cmax;        // per crease: largest |vertex coordinate|
     DBuf<int8_t> d_node_sign;   // per (crease, node): +1 / -1
     std::vector<int> h_crange;  // per sheet: first crease, crease count (node-sign launch)
     DBuf<int> d_crange;
-    DBuf<int> rf_flags;         // neighbour refresh: per domain, offsets / lists changed
+    DBuf<int> rf_flags;         // neighbour refresh: per domain, lists changed
+    DBuf<unsigned char> rf_tab, ind_tab;      // refresh and indicator domain tables
+    std::vector<unsigned char> rf_tab_host, ind_tab_host;
     bool creases_dirty = true;
     int node_sign_creases = -1;   // crease count the node signs were computed for (-1: stale)
     // sheets (batched contexts): nodes and creases per sheet, per-node sheet
@@ -340,7 +344,7 @@ struct Context {
 };
 
 // kernel launchers (implemented in the .hip files)
-void launch_indicator_gp(Context& c, Domain& d);
+void launch_indicators(Context& c, const std::vector<Domain*>& ds);
 void launch_node_signs(Context& c);
 void launch_frnn(Context& c, Domain& d, double rho);
 // Repeats the search of every searched domain on the device (same nodes, GPs

@@ -579,11 +579,10 @@ static void check_args(int order, int kernel) {
         throw Error(CVM_E_INVALID, "unknown window kernel");
 }
 
-// Host checks and per-domain preparation (indicator values, shape-function
-// buffer) before a domain joins a launch.
+// Host checks and the shape-function buffer of a domain before it joins a
+// launch (its indicator values are evaluated before, for the whole group).
 static void prepare_rkpm(Context& c, Domain& d) {
     if (d.npairs < 0) throw Error(CVM_E_STATE, "neighbours not computed for this domain");
-    launch_indicator_gp(c, d);
     // the kernels read ind[(lc * 6 + k) * ng + g] and sign[lc * nn + I] for
     // lc < its sheet's crease count: both buffers must hold that many creases
     const size_t mc = static_cast<size_t>(c.max_sheet_creases());
@@ -605,6 +604,9 @@ static void prepare_rkpm(Context& c, Domain& d) {
 static void enqueue_group(Context& c, const std::vector<Domain*>& ds, int order, int kernel, int* st, int cache,
                           const char* stage) {
     if (ds.empty()) return;
+    for (Domain* d : ds)
+        if (d->npairs < 0) throw Error(CVM_E_STATE, "neighbours not computed for this domain");
+    launch_indicators(c, ds);
     for (Domain* d : ds) prepare_rkpm(c, *d);
     const size_t ncj = 6 * static_cast<size_t>((order + 1) * (order + 2) / 2);   // RK<order>::NCJ
     std::vector<RkDom> tab;

@@ -140,10 +140,12 @@ CVM_API int cvm_neighbours(cvm_context* ctx, int domain, double rho, int64_t* np
 /* The neighbour search of every searched domain again, on the device, with
  * the radius and node grid of its last cvm_neighbours call: the FRNN of
  * SetupHelper::recomputeShapeFunctions (plate/SetupHelper.h:112-138), which
- * searches again before every RKPM solve. The lists are rewritten in place;
+ * searches again before every RKPM solve. Every list is computed again and
+ * compared with the resident one; a domain whose lists all match keeps them.
  * *changed = 1 if any list differs from the previous search (nodes or GPs
- * moved), in which case the host mirrors follow, the shape functions must be
- * recomputed and a changed face or fold domain invalidates the assembly plan.
+ * moved): such a domain takes the full search (lists, offsets, host mirrors),
+ * its shape functions must be recomputed and a changed face or fold domain
+ * invalidates the assembly plan.
  * The crease indicators and node sides are evaluated again by the next
  * shape-function call, as GlobalRKPM::solve evaluates them on every call
  * (GlobalRKPM.h:86-112). */
