@@ -421,7 +421,7 @@ def main():
     ap.add_argument("--no-defaults", action="store_true",
                     help="skip the leg at the reference's defaults (cubic basis, rho = 6h)")
     ap.add_argument("--batch-sheets", type=int, default=W.BATCH_SHEETS)
-    ap.add_argument("--serving-requests", type=int, default=200)
+    ap.add_argument("--serving-requests", type=int, default=2000)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device three times (the known-size dispatch that "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters), then exit")

@@ -270,12 +270,31 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
         // their register budgets (six waves per SIMD; 128 VGPRs) without spilling or
         // losing a resident workgroup.
         const int nd = (sizeof(TT) == 8 && LAM != LAM_COUPLED) ? tile_nd[t] : 0;
+        // A work item's slot pairs and scratch rows, loaded one item ahead: the
+        // loads of the next item are issued before this item's scratch stores, so
+        // waiting for them never waits for those stores to complete.
+        struct SlotMeta { uint32_t pr, d1, d2; };
+        auto meta = [&](int it) {
+            SlotMeta m{0u, 0u, 0u};
+            if (it < T - nd) {
+                const bool pd = it < nd;
+                const int sl = pd ? 2 * it : nd + it;
+                m.pr = tpairs[tb + sl];
+                m.d1 = slot_dst[tb + sl];
+                if (pd) {
+                    m.pr |= static_cast<uint32_t>(tpairs[tb + sl + 1]) << 16;
+                    m.d2 = slot_dst[tb + sl + 1];
+                }
+            }
+            return m;
+        };
+        SlotMeta cur = meta(tid);
         for (int it = tid; it < T - nd; it += TILE_THREADS) {
+            const SlotMeta nxt = meta(it + TILE_THREADS);
             const bool paired = it < nd;
-            const int sl = paired ? 2 * it : nd + it;
-            const uint16_t pr = tpairs[tb + sl];
+            const uint32_t pr = cur.pr & 0xffffu;
             const int ua = pr >> 8, ub1 = pr & 0xff;
-            const int ub2 = paired ? (tpairs[tb + sl + 1] & 0xff) : ub1;
+            const int ub2 = paired ? static_cast<int>((cur.pr >> 16) & 0xff) : ub1;
             uint32_t mask = pm[ua] & pm[ub1];
             TT acc[9], acc2[9];
 #pragma unroll
@@ -398,14 +417,15 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
                     add(acc, t0);
                 }
             }
-            TT* dst = scr + 9 * static_cast<int64_t>(slot_dst[tb + sl]);
+            TT* dst = scr + 9 * static_cast<int64_t>(cur.d1);
 #pragma unroll
             for (int k = 0; k < 9; ++k) dst[k] = acc[k];
             if (paired) {
-                TT* dst2 = scr + 9 * static_cast<int64_t>(slot_dst[tb + sl + 1]);
+                TT* dst2 = scr + 9 * static_cast<int64_t>(cur.d2);
 #pragma unroll
                 for (int k = 0; k < 9; ++k) dst2[k] = acc2[k];
             }
+            cur = nxt;
         }
     }
     // ---- residual per union node, GPs in order ---------------------------------------
@@ -783,7 +803,7 @@ size_t assembly_workspace_bytes(const Context& c, int flags) {
 
 // Element counts of the four workspace buffers: fscr, scr (0 without the
 // tangent), fold_gp, fold_nd.
-static void workspace_counts(const Plan& P, int flags, size_t (&n)[4]) {
+void workspace_counts(const Plan& P, int flags, size_t (&n)[4]) {
     n[0] = static_cast<size_t>(std::max<int64_t>(3 * P.nfscratch, 1));
     n[1] = 0;
     if (flags & CVM_TANGENT) {

@@ -371,6 +371,9 @@ size_t assembly_workspace_bytes(const Context& c, int flags);
 void reserve_workspace(const Context& c, Workspace& ws, int flags);
 // true if reserve_workspace(c, ws, flags) would reallocate a buffer
 bool workspace_would_grow(const Context& c, const Workspace& ws, int flags);
+// element counts (doubles) of the workspace buffers fscr, scr (0 without the
+// tangent), fold_gp, fold_nd for an evaluation with these flags
+void workspace_counts(const Plan& P, int flags, size_t (&n)[4]);
 void upload_fold_descriptors(Context& c);
 size_t tile_lds_bytes(int rec_cap, int max_union, bool fp32, bool wide);
 int tile_umax(int max_union);

@@ -4,10 +4,16 @@
 //  * Sheets are prepared contexts (nodes, neighbour lists, shape functions and
 //    the assembly plan stay in HBM); a request names a sheet and a displacement.
 //  * Slots: up to max_inflight evaluations are in flight at once, each on its own
-//    stream with its own workspace (scratch rows, x, F, optionally K). Requests
-//    that arrive while others run are admitted as soon as a slot is free and the
-//    workspace bytes of the admitted requests stay within the HBM budget, so
-//    their kernels overlap on the device (continuous batching).
+//    stream with its own workspace (scratch rows, x, F, K, pinned staging).
+//    Workspaces are provisioned while the server is idle (at the first request
+//    after sheets were registered): slot by slot, each for the largest requests
+//    that still fit the HBM budget left (the first slots for every registered
+//    sheet, later ones for the smaller sheets); the dispatcher never allocates
+//    or frees device memory (hipFree would synchronise the device under
+//    requests in flight).
+//    Requests that arrive while others run are admitted as soon as a slot that
+//    holds them is free, so their kernels overlap on the device (continuous
+//    batching).
 //  * A request's arithmetic does not depend on what else is in flight: each
 //    evaluation reads only its sheet's plan and writes only its own workspace,
 //    so results are bit-identical alone and in any mix.
@@ -59,7 +65,10 @@ struct Slot {
     size_t hcap = 0;
     hipEvent_t ev = nullptr;
     int64_t req = -1;
-    size_t cap_bytes = 0;           // workspace bytes currently allocated
+
+    size_t bytes() const {
+        return sizeof(double) * (ws.scr.n + ws.fscr.n + ws.fold_gp.n + ws.fold_nd.n + x.n + F.n + K.n);
+    }
 };
 
 }  // namespace
@@ -92,8 +101,83 @@ struct cvm_server {
     // allocated workspace bytes over all slots
     size_t allocated() const {
         size_t t = 0;
-        for (const Slot& s : slots) t += s.cap_bytes;
+        for (const Slot& s : slots) t += s.bytes();
         return t;
+    }
+
+    // flags of a sheet's largest request: FP64 tangent with the hinge, K in the slot
+    static constexpr int kFullFlags = CVM_RESIDUAL | CVM_TANGENT | CVM_ELASTIC_FOLDS;
+
+    // true if slot sl holds a request of sheet sh with these flags without growing
+    static bool fits(const Slot& sl, const SheetRef& sh, int flags, bool own_K) {
+        const size_t n3 = 3 * static_cast<size_t>(sh.nn);
+        const int64_t nnz = (flags & CVM_ELASTIC_FOLDS) ? sh.nnz_union : sh.nnz_face;
+        if (cvm::workspace_would_grow(sh.ctx->c, sl.ws, flags)) return false;
+        if (sl.x.n < n3 || sl.F.n < n3 || sl.hcap < n3) return false;
+        if ((flags & CVM_TANGENT) && own_K && sl.K.n < static_cast<size_t>(std::max<int64_t>(nnz, 1))) return false;
+        return true;
+    }
+
+    // element counts of a slot's buffers: fscr, scr, fold_gp, fold_nd, x (and F), K
+    struct SlotSize {
+        size_t n[6] = {0, 0, 0, 0, 0, 0};
+        size_t bytes() const { return sizeof(double) * (n[0] + n[1] + n[2] + n[3] + 2 * n[4] + n[5]); }
+        void cover(const SheetRef& sh) {
+            size_t w[4];
+            cvm::workspace_counts(sh.ctx->c.plan, kFullFlags, w);
+            for (int k = 0; k < 4; ++k) n[k] = std::max(n[k], w[k]);
+            n[4] = std::max(n[4], 3 * static_cast<size_t>(sh.nn));
+            n[5] = std::max(n[5], static_cast<size_t>(std::max<int64_t>(sh.nnz_union, 1)));
+        }
+    };
+
+    bool planned = true;             // false after a sheet is registered, until the slots are provisioned
+
+    // Provisions the slots (server idle): the sheets by their largest request,
+    // descending; slot after slot takes the buffers that hold every sheet from
+    // the largest one that still fits the budget left down to the smallest.
+    void plan_slots() {
+        (void)hipSetDevice(device);
+        for (Slot& sl : slots) {
+            sl.ws.scr.release(); sl.ws.fscr.release(); sl.ws.fold_gp.release(); sl.ws.fold_nd.release();
+            sl.x.release(); sl.F.release(); sl.K.release();
+        }
+        std::vector<int> order(sheets.size());
+        for (size_t k = 0; k < order.size(); ++k) order[k] = static_cast<int>(k);
+        std::vector<size_t> need(sheets.size());
+        for (size_t k = 0; k < sheets.size(); ++k) need[k] = request_bytes(sheets[k], kFullFlags, true);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return need[a] > need[b]; });
+        // suffix covers: buffers holding the sheets order[i..]
+        std::vector<SlotSize> cover(order.size() + 1);
+        for (size_t i = order.size(); i-- > 0;) {
+            cover[i] = cover[i + 1];
+            cover[i].cover(sheets[order[i]]);
+        }
+        size_t left = static_cast<size_t>(budget);
+        for (Slot& sl : slots) {
+            size_t i = 0;
+            while (i < order.size() && cover[i].bytes() > left) ++i;
+            if (i == order.size()) break;
+            const SlotSize& z = cover[i];
+            sl.ws.fscr.alloc(z.n[0]);
+            sl.ws.scr.alloc(z.n[1]);
+            sl.ws.fold_gp.alloc(z.n[2]);
+            sl.ws.fold_nd.alloc(z.n[3]);
+            sl.x.alloc(z.n[4]);
+            sl.F.alloc(z.n[4]);
+            sl.K.alloc(z.n[5]);
+            if (sl.hcap < z.n[4]) {
+                if (sl.hx) (void)hipHostFree(sl.hx);
+                if (sl.hF) (void)hipHostFree(sl.hF);
+                sl.hx = sl.hF = nullptr;
+                sl.hcap = 0;
+                CVM_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.hx), z.n[4] * sizeof(double), hipHostMallocDefault));
+                CVM_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.hF), z.n[4] * sizeof(double), hipHostMallocDefault));
+                sl.hcap = z.n[4];
+            }
+            left -= z.bytes();
+        }
+        planned = true;
     }
 
     void launch(int64_t id, Request& r, int si) {
@@ -102,21 +186,7 @@ struct cvm_server {
         cvm::Context& c = sh.ctx->c;
         const size_t n3 = 3 * static_cast<size_t>(sh.nn);
         const bool own_K = r.K_dev == nullptr;
-        const int64_t nnz = (r.flags & CVM_ELASTIC_FOLDS) ? sh.nnz_union : sh.nnz_face;
-        cvm::reserve_workspace(c, sl.ws, r.flags);
-        sl.x.ensure(n3);
-        sl.F.ensure(n3);
-        if ((r.flags & CVM_TANGENT) && own_K) sl.K.ensure(std::max<int64_t>(nnz, 1));
-        if (sl.hcap < n3) {
-            if (sl.hx) (void)hipHostFree(sl.hx);
-            if (sl.hF) (void)hipHostFree(sl.hF);
-            sl.hx = sl.hF = nullptr;
-            sl.hcap = 0;
-            CVM_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.hx), n3 * sizeof(double), hipHostMallocDefault));
-            CVM_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.hF), n3 * sizeof(double), hipHostMallocDefault));
-            sl.hcap = n3;
-        }
-        sl.cap_bytes = std::max(sl.cap_bytes, r.bytes);
+        if (!fits(sl, sh, r.flags, own_K)) throw cvm::Error(CVM_E_STATE, "serving slot not provisioned for this request");
         std::memcpy(sl.hx, r.x.data(), n3 * sizeof(double));
         CVM_HIP(hipMemcpyAsync(sl.x.p, sl.hx, n3 * sizeof(double), hipMemcpyHostToDevice, sl.ws.stream));
         double* Kd = (r.flags & CVM_TANGENT) ? (own_K ? sl.K.p : r.K_dev) : nullptr;
@@ -177,22 +247,14 @@ struct cvm_server {
             while (!pending.empty()) {
                 const int64_t id = pending.front();
                 Request& r = reqs[id];
-                int free_fit = -1, free_any = -1;
+                // the smallest free slot that holds the request
+                const SheetRef& sh = sheets[r.sheet];
+                int use = -1;
                 for (size_t si = 0; si < slots.size(); ++si) {
-                    if (slots[si].req >= 0) continue;
-                    if (slots[si].cap_bytes >= r.bytes &&
-                        (free_fit < 0 || slots[si].cap_bytes < slots[free_fit].cap_bytes))
-                        free_fit = static_cast<int>(si);
-                    if (free_any < 0 || slots[si].cap_bytes > slots[free_any].cap_bytes) free_any = static_cast<int>(si);
-                }
-                int use = free_fit;
-                if (use < 0 && free_any >= 0) {
-                    // growing a slot must keep the allocated workspaces within the budget
-                    const size_t grown = allocated() - slots[free_any].cap_bytes + r.bytes;
-                    if (grown <= static_cast<size_t>(budget) || inflight == 0) use = free_any;
+                    if (slots[si].req >= 0 || !fits(slots[si], sh, r.flags, r.K_dev == nullptr)) continue;
+                    if (use < 0 || slots[si].bytes() < slots[use].bytes()) use = static_cast<int>(si);
                 }
                 if (use < 0) break;
-                if (inflight > 0 && in_use + static_cast<int64_t>(r.bytes) > budget) break;
                 pending.pop_front();
                 try {
                     launch(id, r, use);
@@ -273,12 +335,17 @@ int cvm_server_add_sheet(cvm_server* s, cvm_context* ctx, int* sheet_id) {
     cvm::Context& c = ctx->c;
     if (c.device != s->device) { s->err = "sheet context is on another device"; return CVM_E_INVALID; }
     if (!c.plan.ready) { s->err = "sheet has no assembly plan (cvm_init_assembly)"; return CVM_E_STATE; }
+    if (s->inflight > 0 || !s->pending.empty()) {
+        s->err = "sheets are registered while no request is pending or in flight";
+        return CVM_E_STATE;
+    }
     SheetRef r;
     r.ctx = ctx;
     r.nn = c.nn;
     r.nnz_union = static_cast<int64_t>(c.plan.uni_inner.size());
     r.nnz_face = static_cast<int64_t>(c.plan.face_inner.size());
     s->sheets.push_back(r);
+    s->planned = false;
     if (sheet_id) *sheet_id = static_cast<int>(s->sheets.size()) - 1;
     return CVM_OK;
 }
@@ -304,8 +371,20 @@ int cvm_server_submit(cvm_server* s, int sheet, const double* x, int flags, doub
     r.F_host = F_host;
     r.K_dev = K_dev;
     r.bytes = s->request_bytes(sh, r.flags, K_dev == nullptr);
-    if (static_cast<int64_t>(r.bytes) > s->budget) {
-        s->err = "request workspace (" + std::to_string(r.bytes) + " bytes) exceeds the HBM budget";
+    if (!s->planned) {
+        if (s->inflight > 0 || !s->pending.empty()) { s->err = "server not idle while unplanned"; return CVM_E_STATE; }
+        try {
+            s->plan_slots();
+        } catch (const cvm::Error& e) {
+            s->err = e.what();
+            return e.code;
+        }
+    }
+    bool held = false;
+    for (const Slot& sl : s->slots) held = held || cvm_server::fits(sl, sh, r.flags, K_dev == nullptr);
+    if (static_cast<int64_t>(r.bytes) > s->budget || !held) {
+        s->err = "request workspace (" + std::to_string(r.bytes) + " bytes) exceeds the HBM budget: no slot "
+                 "provisioned within the budget holds it";
         return CVM_E_BUDGET;
     }
     r.x.assign(x, x + 3 * static_cast<size_t>(sh.nn));

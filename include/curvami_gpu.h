@@ -259,10 +259,15 @@ CVM_API int cvm_synchronize(cvm_context* ctx);
  *
  * A server owns max_inflight slots (HIP stream + workspace). Sheets are prepared
  * contexts (shape functions and assembly plan resident in HBM) registered with
- * the server; they must not be modified while registered. Requests are admitted
- * in arrival order as soon as a slot is free and the workspace bytes of the
- * requests in flight stay within hbm_budget; admitted requests run concurrently.
- * A request's result does not depend on what else is in flight. */
+ * the server while no request is pending or in flight (CVM_E_STATE otherwise);
+ * they must not be modified while registered. The slots' workspaces are
+ * allocated at the first request after a registration, within hbm_budget: the
+ * first slots hold every registered sheet, later ones the smaller sheets that
+ * the budget left still covers; evaluations never allocate device memory.
+ * Requests are admitted in arrival order as soon as a free slot holds them;
+ * admitted requests run concurrently. A request that no slot holds is refused
+ * at submit with CVM_E_BUDGET. A request's result does not depend on what else
+ * is in flight. */
 typedef struct cvm_server cvm_server;
 CVM_API int cvm_server_create(int device, int64_t hbm_budget, int max_inflight, cvm_server** out);
 CVM_API void cvm_server_destroy(cvm_server* s);
