@@ -211,6 +211,19 @@ int cvm_set_laminate(cvm_context* ctx, int nply, const double* plies) {
     CVM_CATCH(ctx)
 }
 
+int cvm_set_natural_curvature(cvm_context* ctx, int active, double x0, double width, double theta0, double sign) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    if (active && !(width > 0.0)) throw Error(CVM_E_INVALID, "natural_curvature.width must be > 0");
+    if (active && sign != 1.0 && sign != -1.0) throw Error(CVM_E_INVALID, "natural curvature sign must be +1 or -1");
+    CVM_HIP(hipSetDevice(c.device));
+    c.nat.active = active != 0;
+    c.nat.x0 = x0; c.nat.width = width; c.nat.theta0 = theta0; c.nat.sign = sign;
+    c.kbar_ng = -1;
+    if (c.plan.ready) (void)cvm::natural_kappa(c, true);
+    CVM_CATCH(ctx)
+}
+
 int cvm_get_abd(const cvm_context* ctx, double* A, double* B, double* D) {
     if (!ctx) return CVM_E_INVALID;
     for (int k = 0; k < 9; ++k) {
@@ -470,6 +483,8 @@ int cvm_init_assembly(cvm_context* ctx, int64_t* nnz_face, int64_t* nnz_union) {
     CVM_HIP(hipSetDevice(c.device));
     if (!c.have_mat) throw Error(CVM_E_STATE, "laminate not set");
     cvm::build_plan(c);
+    c.kbar_ng = -1;
+    (void)cvm::natural_kappa(c, true);
     if (nnz_face) *nnz_face = static_cast<int64_t>(c.plan.face_inner.size());
     if (nnz_union) *nnz_union = static_cast<int64_t>(c.plan.uni_inner.size());
     CVM_CATCH(ctx)

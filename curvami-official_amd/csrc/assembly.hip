@@ -97,8 +97,8 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
     const uint32_t* __restrict__ slot_dst,
     const int* __restrict__ off, const uint8_t* __restrict__ uloc, const uint8_t* __restrict__ urank,
     const double* __restrict__ sf, int64_t npairs, const double* __restrict__ w,
-    const double* __restrict__ xu, MatC mat, int rec_cap, int umax, int with_tangent,
-    TT* __restrict__ scr, double* __restrict__ fscr, int ntiles) {
+    const double* __restrict__ xu, const double* __restrict__ kbar, MatC mat, int rec_cap, int umax,
+    int with_tangent, TT* __restrict__ scr, double* __restrict__ fscr, int ntiles) {
     extern __shared__ double smem[];
     const int tid = threadIdx.x;
     // one tile per workgroup, or (PERSIST) a grid striding over the tiles
@@ -188,9 +188,11 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
             const double J = sqrt(nx * nx + ny * ny + nz * nz);
             const double bx = nx / J, by = ny / J, bz = nz / J;
             // strains (Voigt, engineering shear) and resultants (Constitutive.h:57-70)
+            // on the elastic bending strain E1_el = E1 + kappa_bar (Kinematics.h:114-115)
             const double e0[3] = {0.5 * (C00 - 1.0), 0.5 * (C11 - 1.0), C01};
-            const double e1[3] = {-(rxx0 * bx + rxx1 * by + rxx2 * bz), -(ryy0 * bx + ryy1 * by + ryy2 * bz),
-                                  -2.0 * (rxy0 * bx + rxy1 * by + rxy2 * bz)};
+            double e1[3] = {-(rxx0 * bx + rxx1 * by + rxx2 * bz), -(ryy0 * bx + ryy1 * by + ryy2 * bz),
+                            -2.0 * (rxy0 * bx + rxy1 * by + rxy2 * bz)};
+            if (kbar) e1[0] += kbar[g];
             double N[3], M[3];
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
@@ -609,15 +611,22 @@ __device__ __forceinline__ void finish_K_block(int64_t blk, int64_t bs, const TT
                 const double* H = rec + 2;
                 const double bl[4] = {Gl[3], Gl[4], Gl[5], Gl[6]};
                 const double bh[4] = {Gh[3], Gh[4], Gh[5], Gh[6]};
+                // one column c of B(lo)^T H B(hi) per iteration (not unrolled, so that
+                // the 144 values of H are not all held at once)
+#pragma unroll 1
                 for (int c = 0; c < 3; ++c) {
-                    const int qc[4] = {c, c + 3, 6 + c, 9 + c};
-                    for (int r = 0; r < 3; ++r) {
-                        const int pr[4] = {r, r + 3, 6 + r, 9 + r};
-                        double acc = 0.0;
-                        for (int u = 0; u < 4; ++u)
-                            for (int v = 0; v < 4; ++v) acc += bl[u] * H[12 * pr[u] + qc[v]] * bh[v];
-                        f[3 * c + r] += wm * acc;
-                    }
+                    // hv[p] = sum_v H(p, qc[v]) bh[v] with qc = {c, c + 3, 6 + c, 9 + c}
+                    double hv[12];
+#pragma unroll
+                    for (int p = 0; p < 12; ++p)
+                        hv[p] = H[12 * p + c] * bh[0] + H[12 * p + c + 3] * bh[1] + H[12 * p + 6 + c] * bh[2]
+                                + H[12 * p + 9 + c] * bh[3];
+                    double col[3];
+#pragma unroll
+                    for (int r = 0; r < 3; ++r)
+                        col[r] = wm * (bl[0] * hv[r] + bl[1] * hv[r + 3] + bl[2] * hv[6 + r] + bl[3] * hv[9 + r]);
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) f[k] += (k / 3 == c) ? col[k % 3] : 0.0;
                 }
             }
         }
@@ -847,6 +856,21 @@ static void launch_k(LaunchSig* sig, K kern, dim3 grid, dim3 block, size_t lds, 
     hipLaunchKernelGGL(kern, grid, block, lds, st, args...);
 }
 
+const double* natural_kappa(Context& c, bool allow_alloc) {
+    if (!c.nat.active) return nullptr;
+    Domain* fd = nullptr;
+    for (Domain* d : c.doms) if (d->kind == CVM_DOMAIN_FACE) fd = d;
+    if (!fd) throw Error(CVM_E_STATE, "natural curvature needs a face domain");
+    if (c.kbar_ng == fd->ng && c.d_kbar.p) return c.d_kbar.p;
+    if (!allow_alloc) throw Error(CVM_E_STATE, "natural curvature field not formed (cvm_init_assembly after the face GPs change)");
+    std::vector<double> k(std::max(fd->ng, 1), 0.0);
+    for (int g = 0; g < fd->ng; ++g) k[g] = c.nat.kappa00(fd->hx[g]);
+    c.d_kbar.upload(k, c.stream);
+    CVM_HIP(hipStreamSynchronize(c.stream));
+    c.kbar_ng = fd->ng;
+    return c.d_kbar.p;
+}
+
 void upload_fold_descriptors(Context& c) {
     Plan& P = c.plan;
     std::vector<FoldDev> fdv(std::max<size_t>(c.folds.size(), 1));
@@ -896,6 +920,9 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
     const WsPtrs wp = ext ? *ext : WsPtrs{ws.scr.p, ws.fscr.p, ws.fold_gp.p, ws.fold_nd.p};
     const bool wide = P.max_nbr > 64;
     const size_t lds = tile_lds_bytes(P.rec_cap, P.max_union, fp32, wide);
+    // the field is formed when the plan is built or the band is set, so that
+    // evaluations (the allocation-free one and graph captures among them) only read it
+    const double* kbar = natural_kappa(c, false);
     const int umax = tile_umax(P.max_union);
     int slot = tbeg(!tangent ? "tile_residual" : (fp32 ? "tile_residual_tangent_fp32" : "tile_residual_tangent"));
     auto zero = [](const double* a, std::initializer_list<int> ks) {
@@ -909,8 +936,8 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
     auto tile = [&](auto kern, auto* scr) {
         launch_k(sig, kern, dim3(grid), dim3(TILE_THREADS), lds, st, P.d_tile_gp.p, P.d_tile_uoff.p,
                            P.d_tile_unodes.p, P.d_tile_toff.p, P.d_tile_tpairs.p, P.d_tile_nd.p, P.d_slot_dst.p, fd.off.p,
-                           P.d_pair_uloc.p, P.d_pair_urank.p, fd.sf.p, static_cast<int64_t>(fd.npairs), fd.d_w.p, xdev, m, P.rec_cap,
-                           umax, tangent ? 1 : 0, scr, wp.fscr, P.ntiles);
+                           P.d_pair_uloc.p, P.d_pair_urank.p, fd.sf.p, static_cast<int64_t>(fd.npairs), fd.d_w.p, xdev,
+                           kbar, m, P.rec_cap, umax, tangent ? 1 : 0, scr, wp.fscr, P.ntiles);
     };
     float* scr32 = reinterpret_cast<float*>(wp.scr);
     if (fp32) tile(tile_kernel<float>(lam, persist, wide), scr32);

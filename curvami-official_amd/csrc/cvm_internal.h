@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <stdexcept>
@@ -149,6 +150,20 @@ struct Domain {
     int ind_creases = -1;
 };
 
+// Natural curvature band (Kinematics::NaturalCurvature, plate/Kinematics.h:9-45):
+// kappa_bar(0, 0) = sign * theta0 * exp(-((X - x0) / width)^2) / (sqrt(pi) width)
+// at a face GP's reference position, added to E1(0, 0) in the resultants and
+// the bending energy. Inactive by default.
+struct NatCurv {
+    bool active = false;
+    double x0 = 0.0, width = 1.0, theta0 = 0.0, sign = -1.0;
+    double kappa00(double X) const {
+        if (!active) return 0.0;
+        const double t = (X - x0) / width;
+        return sign * theta0 * (std::exp(-t * t) / (1.7724538509055160273 * width));
+    }
+};
+
 struct Fold {
     int left = -1, right = -1;
     double Kt = 0.0, Ang0 = 0.0;
@@ -283,6 +298,10 @@ struct Context {
     // material
     double A[9] = {0}, B[9] = {0}, D[9] = {0};
     bool have_mat = false;
+    // natural curvature and its per-face-GP kappa_bar(0, 0) on the device
+    NatCurv nat;
+    DBuf<double> d_kbar;
+    int kbar_ng = -1;             // face GP count d_kbar was formed for (-1: stale)
     // creases (host) + device packed vertices
     std::vector<Crease> creases;
     DBuf<double> d_cv;          // packed vertices x,y
@@ -375,6 +394,10 @@ bool workspace_would_grow(const Context& c, const Workspace& ws, int flags);
 // tangent), fold_gp, fold_nd for an evaluation with these flags
 void workspace_counts(const Plan& P, int flags, size_t (&n)[4]);
 void upload_fold_descriptors(Context& c);
+// Device pointer of kappa_bar(0, 0) per face GP, or null with the natural
+// curvature inactive; forms and uploads it when stale (allocates) unless
+// allow_alloc is false, which then refuses with CVM_E_STATE.
+const double* natural_kappa(Context& c, bool allow_alloc);
 size_t tile_lds_bytes(int rec_cap, int max_union, bool fp32, bool wide);
 int tile_umax(int max_union);
 constexpr int kTileGps = 16;    // GPs per assembly tile

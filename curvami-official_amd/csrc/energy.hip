@@ -28,7 +28,8 @@ __global__ __launch_bounds__(EN_THREADS) void k_face_energy(int ng, const int* _
                                                              const int* __restrict__ ids,
                                                              const double* __restrict__ sf, int64_t npairs,
                                                              const double* __restrict__ w,
-                                                             const double* __restrict__ xu, MatC mat,
+                                                             const double* __restrict__ xu,
+                                                             const double* __restrict__ kbar, MatC mat,
                                                              double* __restrict__ em, double* __restrict__ eb) {
     const int lane = threadIdx.x & 63;
     const int g = blockIdx.x * (EN_THREADS / 64) + (threadIdx.x >> 6);
@@ -59,9 +60,10 @@ __global__ __launch_bounds__(EN_THREADS) void k_face_energy(int ng, const int* _
     // Voigt strains: e0 = (E0_11, E0_22, 2 E0_12), E0 = (C0 - I) / 2; e1 from
     // E1 = -r0,ab . b (Kinematics.h:99-107, Constitutive.h:57-70)
     const double e0[3] = {0.5 * (C00 - 1.0), 0.5 * (C11 - 1.0), 2.0 * (0.5 * C01)};
-    const double e1[3] = {-(rxx[0] * b[0] + rxx[1] * b[1] + rxx[2] * b[2]),
-                          -(ryy[0] * b[0] + ryy[1] * b[1] + ryy[2] * b[2]),
-                          2.0 * -(rxy[0] * b[0] + rxy[1] * b[1] + rxy[2] * b[2])};
+    double e1[3] = {-(rxx[0] * b[0] + rxx[1] * b[1] + rxx[2] * b[2]),
+                    -(ryy[0] * b[0] + ryy[1] * b[1] + ryy[2] * b[2]),
+                    2.0 * -(rxy[0] * b[0] + rxy[1] * b[1] + rxy[2] * b[2])};
+    if (kbar) e1[0] += kbar[g];   // E1_el = E1 + kappa_bar (Meshless.h:212)
     double N[3], M[3];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -220,7 +222,8 @@ void launch_energies(Context& c, const double* xd, double* out_dev) {
         const unsigned grid = static_cast<unsigned>((ng + EN_THREADS / 64 - 1) / (EN_THREADS / 64));
         auto kern = has_b ? k_face_energy<true> : k_face_energy<false>;
         hipLaunchKernelGGL(kern, dim3(grid), dim3(EN_THREADS), 0, st, static_cast<int>(ng), face->off.p,
-                           face->ids.p, face->sf.p, static_cast<int64_t>(face->npairs), face->d_w.p, xd, m, em, eb);
+                           face->ids.p, face->sf.p, static_cast<int64_t>(face->npairs), face->d_w.p, xd,
+                           natural_kappa(c, true), m, em, eb);
         CVM_HIP(hipGetLastError());
     }
     int64_t o = 0;

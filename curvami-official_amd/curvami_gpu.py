@@ -34,6 +34,7 @@ SIGNATURES = {
     "cvm_last_error": (C.c_char_p, [C.c_void_p]),
     "cvm_set_nodes": (C.c_int, [C.c_void_p, C.c_int, _d, _d]),
     "cvm_set_laminate": (C.c_int, [C.c_void_p, C.c_int, _d]),
+    "cvm_set_natural_curvature": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]),
     "cvm_add_sheet": (C.c_int, [C.c_void_p, C.c_int, _d, _d, C.c_double, _i, _i]),
     "cvm_get_abd": (C.c_int, [C.c_void_p, _d, _d, _d]),
     "cvm_add_crease": (C.c_int, [C.c_void_p, C.c_int, _d, _d, C.c_int, C.c_int, C.c_int]),
@@ -168,6 +169,12 @@ class Plate:
     def set_plies(self, plies):
         plies = _f64(plies)
         self._check(self._L.cvm_set_laminate(self._h, plies.shape[0], _dp(plies)))
+
+    def set_natural_curvature(self, active=True, x0=0.0, width=1.0, theta0=0.0, sign=-1.0):
+        """Kinematics::naturalCurvature() (plate/Kinematics.h:20-45): the band
+        sign * theta0 * g(X) in E1_el(0, 0); sign -1 is a mountain, +1 a valley."""
+        self._check(self._L.cvm_set_natural_curvature(self._h, int(active), float(x0), float(width), float(theta0),
+                                                      float(sign)))
 
     def abd(self):
         A, B, D = np.empty(9), np.empty(9), np.empty(9)
@@ -439,6 +446,8 @@ def plate_from_sheet(sh, device: int = 0, compute: bool = True, order=None, kern
     p = Plate(device)
     p.set_nodes(sh.X, sh.Y)
     p.set_plies(sh.plies)
+    if getattr(sh, "natural_curvature", None) is not None:
+        p.set_natural_curvature(True, *sh.natural_curvature)
     for cr in sh.creases:
         p.add_crease(cr.vx, cr.vy, cr.ext_s, cr.ext_e, cr.closed)
     p.face_dom = p.add_domain(sh.face, FACE)

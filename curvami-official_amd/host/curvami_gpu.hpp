@@ -93,6 +93,10 @@ public:
     void setLaminate(const std::vector<double>& plies) {
         ctx_.check(cvm_set_laminate(ctx_.get(), static_cast<int>(plies.size() / 6), plies.data()));
     }
+    // Kinematics::naturalCurvature() (plate/Kinematics.h:20-45): sign -1 mountain, +1 valley
+    void setNaturalCurvature(bool active, double x0, double width, double theta0, double sign) {
+        ctx_.check(cvm_set_natural_curvature(ctx_.get(), active ? 1 : 0, x0, width, theta0, sign));
+    }
     void addCrease(const std::vector<double>& vx, const std::vector<double>& vy, bool ext_s, bool ext_e,
                    bool closed = false) {
         ctx_.check(cvm_add_crease(ctx_.get(), static_cast<int>(vx.size()), vx.data(), vy.data(), ext_s, ext_e, closed));

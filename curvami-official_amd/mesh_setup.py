@@ -123,6 +123,27 @@ def _apply_bc(bd: Boundary, cfg: dict, edge: bool):
         bd.moment = float(cfg["moment"])
 
 
+def natural_curvature_from_config(config: dict):
+    """The configuration's "natural_curvature" band (main.cpp:80-97) as
+    (x0, width, theta0 [rad], sign), sign -1 for a mountain and +1 for a valley;
+    None without one. Raises ValueError with the reference's messages."""
+    if "natural_curvature" not in config:
+        return None
+    nc = config["natural_curvature"]
+    if nc.get("type", "band") != "band":
+        raise ValueError("natural_curvature.type must be band")
+    x0 = float(nc["x0"])
+    width = float(nc["width"])
+    theta0 = float(nc["angle_deg"]) * math.pi / 180.0
+    sense = nc.get("sense", "mountain")
+    if sense not in ("mountain", "valley"):
+        raise ValueError("natural_curvature.sense must be mountain or valley")
+    sign = -1.0 if sense == "mountain" else 1.0
+    if not (width > 0.0):
+        raise ValueError("natural_curvature.width must be > 0")
+    return (x0, width, theta0, sign)
+
+
 def sheet_from_config(config, base_dir: Optional[str] = None, order: int = 3, kernel: int = 0,
                       mesh_file: Optional[str] = None) -> MeshSheet:
     """Sheet for a reference configuration (a dict or a JSON path). mesh_file
@@ -137,8 +158,7 @@ def sheet_from_config(config, base_dir: Optional[str] = None, order: int = 3, ke
     path = mesh_file or config["mesh_file"]
     if mesh_file is None and not os.path.isabs(path):
         path = os.path.join(base_dir, path)
-    if "natural_curvature" in config:
-        raise NotImplementedError("natural_curvature bands are not supported by the device kinematics")
+    natural = natural_curvature_from_config(config)
     mat = config["material"]
     inp = load_mesh_and_quadrature(path)
     rho = G.support_radius(inp.X, inp.Y, 3)            # main.cpp:78
@@ -183,5 +203,5 @@ def sheet_from_config(config, base_dir: Optional[str] = None, order: int = 3, ke
     name = os.path.splitext(os.path.basename(path))[0]
     sheet = Sheet(name, 0, inp.X, inp.Y, face, creases, folds, rho, order, kernel,
                   E=float(mat["E"]), nu=float(mat["nu"]), t=float(mat["h"]), n_split=n_split,
-                  boundaries=boundaries)
+                  boundaries=boundaries, natural_curvature=natural)
     return MeshSheet(sheet, inp, info, config)

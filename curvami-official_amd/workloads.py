@@ -27,7 +27,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
-from typing import List, Optional
+from typing import List, Optional, Tuple
 
 import numpy as np
 
@@ -113,6 +113,8 @@ class Sheet:
     t: float = 0.01
     n_split: int = 0
     boundaries: List[Boundary] = dataclasses.field(default_factory=list)
+    # natural curvature band (x0, width, theta0 [rad], sign: -1 mountain, +1 valley), or None
+    natural_curvature: Optional[Tuple[float, float, float, float]] = None
 
     @property
     def nn(self) -> int:

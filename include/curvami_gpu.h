@@ -86,6 +86,17 @@ CVM_API int cvm_set_laminate(cvm_context* ctx, int nply, const double* plies);
 /* Read back A, B, D (row-major 3x3 each). */
 CVM_API int cvm_get_abd(const cvm_context* ctx, double* A, double* B, double* D);
 
+/* Natural curvature (Kinematics::naturalCurvature(), plate/Kinematics.h:20-45;
+ * set from the configuration's "natural_curvature" band, main.cpp:80-97): with
+ * active != 0, every face GP's elastic bending strain is E1_el = E1 +
+ * kappa_bar with kappa_bar(0, 0) = sign * theta0 * exp(-((X - x0) / width)^2) /
+ * (sqrt(pi) width) at its reference X (theta0 in radians; sign -1 mountain, +1
+ * valley); the residual, the tangent (through the resultants) and the bending
+ * energy use E1_el. The field is formed per face GP here or at
+ * cvm_init_assembly. CVM_E_INVALID for width <= 0 or sign other than +-1. */
+CVM_API int cvm_set_natural_curvature(cvm_context* ctx, int active, double x0, double width, double theta0,
+                                      double sign);
+
 /* Batched contexts: many independent sheets in one context, so that each stage
  * (neighbour search, shape functions, assembly) runs as one launch over all of
  * them. No reference counterpart (the reference holds one sheet per Meshless).

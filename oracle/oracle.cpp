@@ -562,7 +562,33 @@ static ABD computeABD(const std::vector<Ply>& plies) {
 }
 
 // ---------------------------------------------------------------------------
-// Kirchhoff-Love kinematics: plate/Kinematics.h:61-120 (natural curvature off)
+// Natural curvature: plate/Kinematics.h:9-45. A band along Y centred at x0,
+// kappa_bar(X) = sign * theta0 * g(X) eX eX^T with g(X) = exp(-((X - x0) / w)^2)
+// / (sqrt(pi) w); it enters the elastic bending strain E1_el = E1 + kappa_bar
+// (:114-115), which the resultants (Constitutive.h:64-65) and the bending
+// energy (Meshless.h:212) use. Inactive by default.
+// ---------------------------------------------------------------------------
+struct NatCurv {
+    bool active = false;
+    double x0 = 0.0, width = 1.0, theta0 = 0.0, sign = -1.0;
+    double g(double X) const {                                        // :25-28
+        const double t = (X - x0) / width;
+        return std::exp(-t * t) / (1.7724538509055160273 * width);
+    }
+    double kappa00(double X) const { return active ? sign * theta0 * g(X) : 0.0; }   // :29-33
+};
+
+// per GP of a domain: kappa_bar(0, 0) at the GP's reference position
+static std::vector<double> natural_kappa(const NatCurv& nc, const std::vector<V2>& X) {
+    std::vector<double> k(X.size(), 0.0);
+    if (nc.active)
+        for (size_t g = 0; g < X.size(); ++g) k[g] = nc.kappa00(X[g].x);
+    return k;
+}
+
+// ---------------------------------------------------------------------------
+// Kirchhoff-Love kinematics: plate/Kinematics.h:61-120 (the natural curvature
+// is added to E1 by the callers that form E1_el)
 // ---------------------------------------------------------------------------
 struct Kin {
     V3 a1, a2;            // F0 columns (r0,X, r0,Y)
@@ -671,7 +697,7 @@ struct Mat3x2 { V3 c0, c1; };
 static void update_stiffness(const NbrList& nl, const Shape& sh, const std::vector<double>& w,
                              const ABD& mat, const Pattern& pat, const double* u,
                              std::vector<double>& F, std::vector<double>& Kv, int nthreads,
-                             bool with_tangent) {
+                             bool with_tangent, const std::vector<double>& kbar) {
     const int num_dofs = 3 * pat.nn;
     const int nnz = static_cast<int>(pat.inner.size());
     int T = 1;
@@ -708,9 +734,9 @@ static void update_stiffness(const NbrList& nl, const Shape& sh, const std::vect
             const int* ids = nl.ids.data() + b;
             const Kin kin = kinematics(ids, n, sh.dpx.data() + b, sh.dpy.data() + b, sh.dxx.data() + b,
                                        sh.dyy.data() + b, sh.dxy.data() + b, u);
-            // Constitutive::evaluate (Constitutive.h:57-70)
+            // Constitutive::evaluate (Constitutive.h:57-70) on E1_el = E1 + kappa_bar
             const V3 e0 = v3(kin.E0[0][0], kin.E0[1][1], 2.0 * kin.E0[0][1]);
-            const V3 e1 = v3(kin.E1[0][0], kin.E1[1][1], 2.0 * kin.E1[0][1]);
+            const V3 e1 = v3(kin.E1[0][0] + kbar[g], kin.E1[1][1], 2.0 * kin.E1[0][1]);
             const V3 Nr = add(mv(A, e0), mv(Bm_, e1));
             const V3 Mr = add(mv(mT(Bm_), e0), mv(D, e1));
             const double W = w[g];
@@ -1052,6 +1078,7 @@ struct Plate {
     std::vector<FoldDef> folds;
     std::vector<FoldLine> creases;
     ABD mat;
+    NatCurv nat;
     double rho = 0.0;
     int order = 2, kernel = 0;
     Pattern pat;
@@ -1076,6 +1103,13 @@ extern "C" {
 
 Plate* orc_create() { return new Plate(); }
 void orc_destroy(Plate* p) { delete p; }
+
+// Kinematics::naturalCurvature() (plate/Kinematics.h:42-45), set from the
+// configuration's "natural_curvature" band (main.cpp:80-97).
+void orc_set_natural_curvature(Plate* p, int active, double x0, double width, double theta0, double sign) {
+    p->nat.active = active != 0;
+    p->nat.x0 = x0; p->nat.width = width; p->nat.theta0 = theta0; p->nat.sign = sign;
+}
 const char* orc_last_error(Plate* p) { return p->err.c_str(); }
 
 void orc_set_nodes(Plate* p, int nn, const double* X, const double* Y) {
@@ -1213,7 +1247,7 @@ long orc_assemble(Plate* p, const double* x, int flag_rig, int exact_tangent, in
 #ifdef _OPENMP
     const double t0 = omp_get_wtime();
 #endif
-    update_stiffness(fd.nl, fd.sh, fd.w, p->mat, p->pat, x, p->F, Kv, nthreads, true);
+    update_stiffness(fd.nl, fd.sh, fd.w, p->mat, p->pat, x, p->F, Kv, nthreads, true, natural_kappa(p->nat, fd.X));
     const int nn = p->pat.nn;
     if (flag_rig || p->folds.empty()) {
         p->K_outer = p->pat.outer; p->K_inner = p->pat.inner; p->K_val = Kv;
@@ -1254,7 +1288,8 @@ long orc_assemble(Plate* p, const double* x, int flag_rig, int exact_tangent, in
 void orc_internal_force(Plate* p, const double* x, int flag_rig, int with_tangent, int nthreads, double* F_out) {
     Domain& fd = p->doms[p->face];
     std::vector<double> Kv, F;
-    update_stiffness(fd.nl, fd.sh, fd.w, p->mat, p->pat, x, F, Kv, nthreads, with_tangent != 0);
+    update_stiffness(fd.nl, fd.sh, fd.w, p->mat, p->pat, x, F, Kv, nthreads, with_tangent != 0,
+                     natural_kappa(p->nat, fd.X));
     if (!flag_rig)
         for (const auto& f : p->folds)
             if (!f.rigid) elastic_fold(p->doms[f.left], p->doms[f.right], f, x, F, nullptr, false);
@@ -1298,13 +1333,14 @@ void orc_kinematics(Plate* p, int dom, const double* x, double* out) {
 void orc_energies(Plate* p, const double* x, double* out) {
     const Domain& d = p->doms[p->face];
     const ABD& m = p->mat;
+    const std::vector<double> kbar = natural_kappa(p->nat, d.X);
     double Em = 0.0, Eb = 0.0;
     for (size_t g = 0; g + 1 < d.nl.off.size(); ++g) {
         const int b = d.nl.off[g], n = d.nl.off[g + 1] - b;
         const Kin k = kinematics(d.nl.ids.data() + b, n, d.sh.dpx.data() + b, d.sh.dpy.data() + b,
                                  d.sh.dxx.data() + b, d.sh.dyy.data() + b, d.sh.dxy.data() + b, x);
         const V3 e0 = v3(k.E0[0][0], k.E0[1][1], 2.0 * k.E0[0][1]);
-        const V3 e1 = v3(k.E1[0][0], k.E1[1][1], 2.0 * k.E1[0][1]);
+        const V3 e1 = v3(k.E1[0][0] + kbar[g], k.E1[1][1], 2.0 * k.E1[0][1]);   // E1_el (Meshless.h:212)
         const V3 N = add(mv(m.A, e0), mv(m.B, e1));                       // Constitutive.h:67-68
         const V3 M = add(mv(mT(m.B), e0), mv(m.D, e1));
         Em += 0.5 * d.w[g] * dot(e0, N);

@@ -67,6 +67,7 @@ def lib():
             "orc_get_constraints": (None, [P, _d, _i, _i, _d]),
             "orc_external_force": (None, [P, _d]),
             "orc_energies": (None, [P, _d, _d]),
+            "orc_set_natural_curvature": (None, [P, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]),
             "orc_fold_angles": (None, [P, C.c_int, _d, _d]),
             "orc_clear_creases": (None, [P]),
         }
@@ -133,6 +134,11 @@ class OraclePlate:
     def set_plies(self, plies):
         plies = _f64(plies)
         self._L.orc_set_material_plies(self._p, plies.shape[0], _dp(plies))
+
+    def set_natural_curvature(self, active=True, x0=0.0, width=1.0, theta0=0.0, sign=-1.0):
+        """Kinematics::naturalCurvature() (plate/Kinematics.h:20-45): the band
+        sign * theta0 * g(X) in E1_el(0, 0); sign -1 is a mountain, +1 a valley."""
+        self._L.orc_set_natural_curvature(self._p, int(active), float(x0), float(width), float(theta0), float(sign))
 
     def abd(self):
         A = np.empty(9); B = np.empty(9); D = np.empty(9)
@@ -301,6 +307,8 @@ def plate_from_sheet(sh, nthreads: int = 1, compute: bool = True):
     p = OraclePlate()
     p.set_nodes(sh.X, sh.Y)
     p.set_plies(sh.plies)
+    if getattr(sh, "natural_curvature", None) is not None:
+        p.set_natural_curvature(True, *sh.natural_curvature)
     for cr in sh.creases:
         p.add_crease(cr.vx, cr.vy, cr.ext_s, cr.ext_e, cr.closed)
     face = p.add_domain(sh.face, is_face=True)

@@ -287,3 +287,20 @@ def test_sheet_from_config_mirrors_driver_setup():
     ms2 = MS.sheet_from_config(os.path.join(INPUTS, "ls_miura.json"), mesh_file=mpath("Miura"))
     assert [fl.rigid for fl in ms2.sheet.folds] == [True] * 4 and len(ms2.sheet.creases) == 2
     assert abs(ms2.sheet.folds[0].Ang0 - (-15.26158022631557 * math.pi / 180.0)) < 1e-15
+
+
+def test_natural_curvature_config():
+    """The "natural_curvature" band of a configuration (main.cpp:80-97): degrees to
+    radians, mountain -1 / valley +1 (mountain by default), the reference's errors."""
+    import mesh_setup as MS
+    cfg = {"natural_curvature": {"type": "band", "x0": 0.25, "width": 0.1, "angle_deg": 45.0}}
+    assert MS.natural_curvature_from_config(cfg) == (0.25, 0.1, math.pi / 4, -1.0)
+    cfg["natural_curvature"]["sense"] = "valley"
+    assert MS.natural_curvature_from_config(cfg)[3] == 1.0
+    assert MS.natural_curvature_from_config({}) is None
+    for bad, msg in (({"type": "disk"}, "natural_curvature.type must be band"),
+                     ({"sense": "ridge"}, "natural_curvature.sense must be mountain or valley"),
+                     ({"width": 0.0}, "natural_curvature.width must be > 0")):
+        c = {"natural_curvature": {"type": "band", "x0": 0.0, "width": 0.1, "angle_deg": 10.0, **bad}}
+        with pytest.raises(ValueError, match=msg):
+            MS.natural_curvature_from_config(c)

@@ -569,3 +569,79 @@ def test_box_sample_is_exact_inside():
     edge = np.nonzero((np.abs(sh.Y - box[2]) < 0.5 * sh.rho) & (sh.X > 0.4) & (sh.X < 0.6))[0]
     d = np.concatenate([3 * edge, 3 * edge + 1, 3 * edge + 2])
     assert np.abs(Fp[d] - F[d]).max() > 1e-6 * np.abs(F).max()
+
+
+# ---- natural curvature (Kinematics.h:9-45, :114-115; Meshless.h:212) -----------------
+def _band(x, x0, width, theta0, sign):
+    """kappa_bar(0, 0) of the band, written from its definition: sign theta0 g(X),
+    g(X) = exp(-((X - x0) / w)^2) / (sqrt(pi) w), whose integral over X is 1."""
+    return sign * theta0 * np.exp(-((x - x0) / width) ** 2) / (math.sqrt(math.pi) * width)
+
+
+@pytest.fixture(scope="module")
+def natural():
+    sh = W.config1(rho_factor=RHO_FACTOR, n_side=21)
+    band = (0.37, 0.12, math.radians(30.0), -1.0)          # a mountain band off the crease
+    p = O.plate_from_sheet(sh)
+    p.set_natural_curvature(True, *band)
+    return sh, p, band
+
+
+def test_natural_curvature_energy_at_rest(natural):
+    """At u = 0 the geometric E1 vanishes, so E1_el = kappa_bar eX eX and the
+    bending energy is sum_g 1/2 W_g D11 kappa_bar(X_g)^2 (D11 = E t^3 / (12 (1 - nu^2))
+    for the one isotropic ply); the membrane energy stays zero. The band's
+    integral over X is theta0 (g integrates to 1)."""
+    sh, p, band = natural
+    Em, Eb, Eh = p.energies(np.zeros(3 * sh.nn))
+    D11 = sh.E * sh.t ** 3 / (12.0 * (1.0 - sh.nu ** 2))
+    kb = _band(sh.face.gx, *band)
+    expect = float(np.sum(0.5 * sh.face.w * D11 * kb ** 2))
+    assert Eb == pytest.approx(expect, rel=1e-12)
+    assert Em == 0.0 and Eh == 0.0
+    xs = np.linspace(-2.0, 3.0, 200001)
+    assert np.trapezoid(_band(xs, *band), xs) == pytest.approx(band[3] * band[2], rel=1e-9)
+
+
+def test_natural_curvature_mountain_sign(natural):
+    """Mountain (sign -1): kappa_bar < 0, and E1_el = E1 + kappa_bar is relieved by
+    w_xx < 0 (E1 = -w_xx to first order): the bending energy falls along w = c x^2/2
+    for c < 0 and rises for c > 0, so the sheet ridges towards +z. A valley does
+    the opposite."""
+    sh, p, band = natural
+    def Eb(c):
+        x = np.zeros(3 * sh.nn)
+        x[2::3] = 0.5 * c * (sh.X - band[0]) ** 2
+        return p.energies(x)[1]
+    e0 = Eb(0.0)
+    assert Eb(-0.05) < e0 < Eb(0.05)
+    p.set_natural_curvature(True, band[0], band[1], band[2], +1.0)
+    try:
+        e0v = Eb(0.0)
+        assert e0v == pytest.approx(e0, rel=1e-14)
+        assert Eb(0.05) < e0v < Eb(-0.05)
+    finally:
+        p.set_natural_curvature(True, *band)
+
+
+def test_natural_curvature_residual_and_tangent(natural):
+    """With the band on, F_int is still dE/dx of E_mem + E_ben + E_hin (so the
+    residual uses E1_el exactly where the energy does), the face tangent stays
+    consistent with it, and switching the band off gives the plain plate back."""
+    sh, p, band = natural
+    x = W.displacement(sh.nn, 11, 2e-3)
+    F = p.internal_force(x, flag_rig=False)
+    h = 1e-7
+    for seed in (1, 2):
+        v = W.displacement(sh.nn, 200 + seed, 1.0)
+        fd = (sum(p.energies(x + h * v)) - sum(p.energies(x - h * v))) / (2 * h)
+        assert abs(fd - F @ v) <= 1e-8 * np.linalg.norm(F) * np.linalg.norm(v)
+    assert _tangent_check(p, x, 3 * sh.nn, True) < 1e-6
+    plain = O.plate_from_sheet(sh)
+    F0 = plain.internal_force(x, flag_rig=False)
+    assert np.abs(F - F0).max() > 1e-4 * np.abs(F0).max()      # the band changes the residual (~6e-4 here)
+    p.set_natural_curvature(False)
+    try:
+        assert np.array_equal(p.internal_force(x, flag_rig=False), F0)
+    finally:
+        p.set_natural_curvature(True, *band)
