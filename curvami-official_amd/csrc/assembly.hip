@@ -90,19 +90,17 @@ __device__ __forceinline__ int xcd_tile(int b, int nwg) {
 // B != 0. The orthotropic class skips the products with those zero entries.
 enum { LAM_ORTHO = 0, LAM_SYM = 1, LAM_COUPLED = 2 };
 
-template <int LAM, typename TT, bool PERSIST, bool WIDE>
-__global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((sizeof(TT) == 4 && LAM != LAM_COUPLED && !WIDE && !PERSIST) ? 6 : 1))) void k_tile(
+// One tile (the workgroup's threads; LDS at smem).
+template <int LAM, typename TT, bool WIDE>
+__device__ __forceinline__ void tile_body(
     const int* __restrict__ tile_gp, const int* __restrict__ tile_uoff, const int* __restrict__ tile_unodes,
     const int64_t* __restrict__ tile_toff, const uint16_t* __restrict__ tpairs, const int* __restrict__ tile_nd,
     const uint32_t* __restrict__ slot_dst,
     const int* __restrict__ off, const uint8_t* __restrict__ uloc, const uint8_t* __restrict__ urank,
     const double* __restrict__ sf, int64_t npairs, const double* __restrict__ w,
     const double* __restrict__ xu, const double* __restrict__ kbar, MatC mat, int rec_cap, int umax,
-    int with_tangent, TT* __restrict__ scr, double* __restrict__ fscr, int ntiles) {
-    extern __shared__ double smem[];
+    int with_tangent, TT* __restrict__ scr, double* __restrict__ fscr, const int t, double* smem) {
     const int tid = threadIdx.x;
-    // one tile per workgroup, or (PERSIST) a grid striding over the tiles
-    auto body = [&](const int t) {
     const int g0 = tile_gp[t], ngp = tile_gp[t + 1] - g0;
     const int ub = tile_uoff[t], U = tile_uoff[t + 1] - ub;
     const int64_t tb = tile_toff[t];
@@ -441,15 +439,54 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
         double* dst = fscr + 3 * static_cast<int64_t>(ub + tid);
         dst[0] = f0; dst[1] = f1; dst[2] = f2;
     }
-    };
+}
+
+template <int LAM, typename TT, bool PERSIST, bool WIDE>
+__global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((sizeof(TT) == 4 && LAM != LAM_COUPLED && !WIDE && !PERSIST) ? 6 : 1))) void k_tile(
+    const int* __restrict__ tile_gp, const int* __restrict__ tile_uoff, const int* __restrict__ tile_unodes,
+    const int64_t* __restrict__ tile_toff, const uint16_t* __restrict__ tpairs, const int* __restrict__ tile_nd,
+    const uint32_t* __restrict__ slot_dst,
+    const int* __restrict__ off, const uint8_t* __restrict__ uloc, const uint8_t* __restrict__ urank,
+    const double* __restrict__ sf, int64_t npairs, const double* __restrict__ w,
+    const double* __restrict__ xu, const double* __restrict__ kbar, MatC mat, int rec_cap, int umax,
+    int with_tangent, TT* __restrict__ scr, double* __restrict__ fscr, int ntiles) {
+    extern __shared__ double smem[];
+    // one tile per workgroup, or (PERSIST) a grid striding over the tiles
     if constexpr (PERSIST) {
         for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-            body(t);
+            tile_body<LAM, TT, WIDE>(tile_gp, tile_uoff, tile_unodes, tile_toff, tpairs, tile_nd, slot_dst, off, uloc, urank, sf, npairs, w, xu,
+                             kbar, mat, rec_cap, umax, with_tangent, scr, fscr, t, smem);
             __syncthreads();   // the next tile reuses the LDS
         }
     } else {
-        body(xcd_tile(blockIdx.x, gridDim.x));
+        tile_body<LAM, TT, WIDE>(tile_gp, tile_uoff, tile_unodes, tile_toff, tpairs, tile_nd, slot_dst, off, uloc, urank, sf, npairs, w, xu,
+                             kbar, mat, rec_cap, umax, with_tangent, scr, fscr, xcd_tile(blockIdx.x, gridDim.x), smem);
     }
+}
+
+// Tiles of several evaluations in one launch (serving: the requests admitted
+// together). Each job is one evaluation's plan, inputs and workspace, its tiles
+// numbered from tile0 on; the jobs share the laminate class, the tangent type
+// and the neighbour width, and the launch's LDS is the largest job's.
+struct TileJob {
+    const int* tile_gp; const int* tile_uoff; const int* tile_unodes; const int64_t* tile_toff;
+    const uint16_t* tpairs; const int* tile_nd; const uint32_t* slot_dst;
+    const int* off; const uint8_t* uloc; const uint8_t* urank; const double* sf; int64_t npairs;
+    const double* w; const double* xu; const double* kbar; MatC mat;
+    int rec_cap, umax, with_tangent, tile0;
+    void* scr; double* fscr;
+};
+
+template <int LAM, typename TT, bool WIDE>
+__global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((sizeof(TT) == 4 && LAM != LAM_COUPLED && !WIDE) ? 6 : 1))) void k_tile_multi(const TileJob* __restrict__ jobs, int njobs) {
+    extern __shared__ double smem[];
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    int j = 0;
+    while (j + 1 < njobs && jobs[j + 1].tile0 <= b) ++j;   // njobs is small (the serving slots)
+    const TileJob& J = jobs[j];
+    tile_body<LAM, TT, WIDE>(J.tile_gp, J.tile_uoff, J.tile_unodes, J.tile_toff, J.tpairs, J.tile_nd, J.slot_dst,
+                             J.off, J.uloc, J.urank, J.sf, J.npairs, J.w, J.xu, J.kbar, J.mat, J.rec_cap, J.umax,
+                             J.with_tangent, static_cast<TT*>(J.scr), J.fscr, b - J.tile0, smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -871,6 +908,70 @@ const double* natural_kappa(Context& c, bool allow_alloc) {
     return c.d_kbar.p;
 }
 
+static int laminate_class(const double* A, const double* B, const double* D) {
+    auto zero = [](const double* a, std::initializer_list<int> ks) {
+        for (int k : ks) if (a[k] != 0.0) return false;
+        return true;
+    };
+    return !zero(B, {0, 1, 2, 3, 4, 5, 6, 7, 8}) ? LAM_COUPLED
+         : zero(A, {2, 5, 6, 7}) && zero(D, {2, 5, 6, 7}) ? LAM_ORTHO : LAM_SYM;
+}
+
+size_t tile_job_bytes() { return sizeof(TileJob); }
+
+int tile_group_key(const Context& c, int flags) {
+    const Plan& P = c.plan;
+    if (!P.ready || (c.tile_grid > 0 && c.tile_grid < P.ntiles)) return -1;
+    return ((flags & CVM_FP32_TANGENT) ? 1 : 0) | (laminate_class(c.A, c.B, c.D) << 1) | ((P.max_nbr > 64 ? 1 : 0) << 3);
+}
+
+template <typename TT, bool WIDE>
+static auto tile_multi_kernel(int lam) {
+    return lam == LAM_COUPLED ? k_tile_multi<LAM_COUPLED, TT, WIDE>
+         : lam == LAM_SYM     ? k_tile_multi<LAM_SYM, TT, WIDE>
+                              : k_tile_multi<LAM_ORTHO, TT, WIDE>;
+}
+
+void launch_tile_group(const TileGroupItem* items, int n, void* host_jobs, void* dev_jobs, hipStream_t st) {
+    if (n <= 0) return;
+    TileJob* H = static_cast<TileJob*>(host_jobs);
+    const int key = tile_group_key(*items[0].c, items[0].flags);
+    if (key < 0) throw Error(CVM_E_STATE, "evaluation cannot share a tile launch");
+    size_t lds = 0;
+    int tiles = 0;
+    for (int i = 0; i < n; ++i) {
+        Context& c = *items[i].c;
+        if (tile_group_key(c, items[i].flags) != key) throw Error(CVM_E_STATE, "grouped evaluations differ in kind");
+        const Plan& P = c.plan;
+        Domain* fdp = nullptr;
+        for (Domain* d : c.doms) if (d->kind == CVM_DOMAIN_FACE) fdp = d;
+        const Domain& fd = *fdp;
+        TileJob& J = H[i];
+        std::memset(&J, 0, sizeof(J));
+        J.tile_gp = P.d_tile_gp.p; J.tile_uoff = P.d_tile_uoff.p; J.tile_unodes = P.d_tile_unodes.p;
+        J.tile_toff = P.d_tile_toff.p; J.tpairs = P.d_tile_tpairs.p; J.tile_nd = P.d_tile_nd.p;
+        J.slot_dst = P.d_slot_dst.p; J.off = fd.off.p; J.uloc = P.d_pair_uloc.p; J.urank = P.d_pair_urank.p;
+        J.sf = fd.sf.p; J.npairs = static_cast<int64_t>(fd.npairs); J.w = fd.d_w.p; J.xu = items[i].x;
+        J.kbar = natural_kappa(c, false);
+        std::memcpy(J.mat.A, c.A, sizeof(J.mat.A));
+        std::memcpy(J.mat.B, c.B, sizeof(J.mat.B));
+        std::memcpy(J.mat.D, c.D, sizeof(J.mat.D));
+        J.rec_cap = P.rec_cap; J.umax = tile_umax(P.max_union);
+        J.with_tangent = (items[i].flags & CVM_TANGENT) ? 1 : 0;
+        J.tile0 = tiles;
+        J.scr = items[i].scr; J.fscr = items[i].fscr;
+        tiles += P.ntiles;
+        lds = std::max(lds, tile_lds_bytes(P.rec_cap, P.max_union, (key & 1) != 0, (key >> 3) != 0));
+    }
+    CVM_HIP(hipMemcpyAsync(dev_jobs, host_jobs, sizeof(TileJob) * n, hipMemcpyHostToDevice, st));
+    const int lam = (key >> 1) & 3;
+    const bool fp32 = (key & 1) != 0, wide = (key >> 3) != 0;
+    auto kern = fp32 ? (wide ? tile_multi_kernel<float, true>(lam) : tile_multi_kernel<float, false>(lam))
+                     : (wide ? tile_multi_kernel<double, true>(lam) : tile_multi_kernel<double, false>(lam));
+    hipLaunchKernelGGL(kern, dim3(tiles), dim3(TILE_THREADS), lds, st, static_cast<const TileJob*>(dev_jobs), n);
+    CVM_HIP(hipGetLastError());
+}
+
 void upload_fold_descriptors(Context& c) {
     Plan& P = c.plan;
     std::vector<FoldDev> fdv(std::max<size_t>(c.folds.size(), 1));
@@ -892,7 +993,7 @@ void upload_fold_descriptors(Context& c) {
 // the shape functions, so evaluations on different workspaces may run
 // concurrently on different streams.
 void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, double* Fdev, double* Kdev,
-                     LaunchSig* sig, const WsPtrs* ext) {
+                     LaunchSig* sig, const WsPtrs* ext, bool tile_done) {
     Plan& P = c.plan;
     const bool dry = sig && sig->dry;
     if (!P.ready) throw Error(CVM_E_STATE, "assembly plan not initialised (cvm_init_assembly)");
@@ -924,13 +1025,8 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
     // evaluations (the allocation-free one and graph captures among them) only read it
     const double* kbar = natural_kappa(c, false);
     const int umax = tile_umax(P.max_union);
-    int slot = tbeg(!tangent ? "tile_residual" : (fp32 ? "tile_residual_tangent_fp32" : "tile_residual_tangent"));
-    auto zero = [](const double* a, std::initializer_list<int> ks) {
-        for (int k : ks) if (a[k] != 0.0) return false;
-        return true;
-    };
-    const int lam = !zero(m.B, {0, 1, 2, 3, 4, 5, 6, 7, 8}) ? LAM_COUPLED
-                  : zero(m.A, {2, 5, 6, 7}) && zero(m.D, {2, 5, 6, 7}) ? LAM_ORTHO : LAM_SYM;
+    int slot = -1;
+    const int lam = laminate_class(m.A, m.B, m.D);
     const bool persist = c.tile_grid > 0 && c.tile_grid < P.ntiles;
     const int grid = persist ? c.tile_grid : P.ntiles;
     auto tile = [&](auto kern, auto* scr) {
@@ -940,10 +1036,13 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
                            kbar, m, P.rec_cap, umax, tangent ? 1 : 0, scr, wp.fscr, P.ntiles);
     };
     float* scr32 = reinterpret_cast<float*>(wp.scr);
-    if (fp32) tile(tile_kernel<float>(lam, persist, wide), scr32);
-    else tile(tile_kernel<double>(lam, persist, wide), wp.scr);
-    if (!dry) CVM_HIP(hipGetLastError());
-    tend(slot);
+    if (!tile_done) {    // (tile_done: the tiles ran in a grouped launch, launch_tile_group)
+        slot = tbeg(!tangent ? "tile_residual" : (fp32 ? "tile_residual_tangent_fp32" : "tile_residual_tangent"));
+        if (fp32) tile(tile_kernel<float>(lam, persist, wide), scr32);
+        else tile(tile_kernel<double>(lam, persist, wide), wp.scr);
+        if (!dry) CVM_HIP(hipGetLastError());
+        tend(slot);
+    }
     if (folds) {
         slot = tbeg("fold_hinge");
         if (P.nfgp > 0) launch_k(sig, k_fold, dim3(P.nfgp), dim3(64), 0, st, P.nfgp,

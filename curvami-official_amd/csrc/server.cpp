@@ -14,9 +14,16 @@
 //    Requests that arrive while others run are admitted as soon as a slot that
 //    holds them is free, so their kernels overlap on the device (continuous
 //    batching).
+//  * Requests admitted in the same dispatcher pass (several slots free at
+//    once, as when a group of requests completes under load) share their tile
+//    launch: the Gauss-point tiles of all their sheets run as one k_tile_multi
+//    launch over a job table, then each request's hinge, F and K reductions
+//    follow on the same stream. Requests of different tangent types, laminate
+//    classes or neighbour widths go into separate groups.
 //  * A request's arithmetic does not depend on what else is in flight: each
 //    evaluation reads only its sheet's plan and writes only its own workspace,
-//    so results are bit-identical alone and in any mix.
+//    and a tile computes the same in a shared launch as in its own, so results
+//    are bit-identical alone and in any mix.
 //  * One dispatcher thread admits, launches and retires requests (event polling);
 //    submit and wait may be called from any host thread.
 #include <chrono>
@@ -83,6 +90,10 @@ struct cvm_server {
     std::deque<int64_t> pending;
     std::unordered_map<int64_t, Request> reqs;
     int64_t next_id = 1;
+    bool batching = true;            // requests admitted together share their tile launch
+    int64_t groups = 0, grouped = 0; // shared tile launches and the requests in them
+    void* job_host = nullptr;        // per slot: max_inflight tile-job records (pinned host, device)
+    void* job_dev = nullptr;
     int64_t in_use = 0;              // bytes charged by in-flight requests
     int64_t completed = 0;
     int inflight = 0;
@@ -198,6 +209,97 @@ struct cvm_server {
         r.t_start = Clock::now();
     }
 
+    // Requests admitted together (slots reserved): groups of equal tile keys of two
+    // or more share one tile launch on the first request's stream; the rest run
+    // alone on their own slots' streams.
+    void launch_batch(const std::vector<std::pair<int64_t, int>>& batch) {
+        std::vector<int> key(batch.size());
+        for (size_t i = 0; i < batch.size(); ++i) {
+            const Request& r = reqs[batch[i].first];
+            key[i] = batching ? cvm::tile_group_key(sheets[r.sheet].ctx->c, r.flags & ~CVM_DEVICE_PTRS) : -1;
+        }
+        std::vector<char> done(batch.size(), 0);
+        for (size_t i = 0; i < batch.size(); ++i) {
+            if (done[i]) continue;
+            std::vector<size_t> grp{i};
+            if (key[i] >= 0)
+                for (size_t k = i + 1; k < batch.size(); ++k)
+                    if (!done[k] && key[k] == key[i]) grp.push_back(k);
+            for (size_t k : grp) done[k] = 1;
+            if (grp.size() == 1) {
+                launch_one(batch[i].first, batch[i].second);
+            } else {
+                std::vector<std::pair<int64_t, int>> g;
+                for (size_t k : grp) g.push_back(batch[k]);
+                launch_group(g);
+            }
+        }
+    }
+
+    void launch_one(int64_t id, int si) {
+        Request& r = reqs[id];
+        try {
+            launch(id, r, si);
+        } catch (const cvm::Error& e) {
+            unreserve(id, si, e);
+        } catch (const std::exception& e) {
+            unreserve(id, si, cvm::Error(CVM_E_NO_MEMORY, e.what()));
+        }
+    }
+
+    void launch_group(const std::vector<std::pair<int64_t, int>>& g) {
+        Slot& lead = slots[g[0].second];
+        const hipStream_t st = lead.ws.stream;
+        try {
+            std::vector<cvm::TileGroupItem> items(g.size());
+            for (size_t k = 0; k < g.size(); ++k) {
+                Request& r = reqs[g[k].first];
+                Slot& sl = slots[g[k].second];
+                const SheetRef& sh = sheets[r.sheet];
+                if (!fits(sl, sh, r.flags, r.K_dev == nullptr))
+                    throw cvm::Error(CVM_E_STATE, "serving slot not provisioned for this request");
+                const size_t n3 = 3 * static_cast<size_t>(sh.nn);
+                std::memcpy(sl.hx, r.x.data(), n3 * sizeof(double));
+                CVM_HIP(hipMemcpyAsync(sl.x.p, sl.hx, n3 * sizeof(double), hipMemcpyHostToDevice, st));
+                items[k] = cvm::TileGroupItem{&sh.ctx->c, r.flags & ~CVM_DEVICE_PTRS, sl.x.p, sl.ws.scr.p, sl.ws.fscr.p};
+            }
+            const size_t jb = cvm::tile_job_bytes() * slots.size();
+            const size_t lo = static_cast<size_t>(g[0].second) * jb;
+            cvm::launch_tile_group(items.data(), static_cast<int>(items.size()),
+                                   static_cast<unsigned char*>(job_host) + lo, static_cast<unsigned char*>(job_dev) + lo, st);
+            for (size_t k = 0; k < g.size(); ++k) {
+                Request& r = reqs[g[k].first];
+                Slot& sl = slots[g[k].second];
+                const SheetRef& sh = sheets[r.sheet];
+                const size_t n3 = 3 * static_cast<size_t>(sh.nn);
+                const cvm::WsPtrs wp{sl.ws.scr.p, sl.ws.fscr.p, sl.ws.fold_gp.p, sl.ws.fold_nd.p};
+                double* Kd = (r.flags & CVM_TANGENT) ? (r.K_dev ? r.K_dev : sl.K.p) : nullptr;
+                cvm::launch_assembly(sh.ctx->c, lead.ws, r.flags & ~CVM_DEVICE_PTRS, sl.x.p, sl.F.p, Kd, nullptr, &wp, true);
+                CVM_HIP(hipMemcpyAsync(sl.hF, sl.F.p, n3 * sizeof(double), hipMemcpyDeviceToHost, st));
+                CVM_HIP(hipEventRecord(sl.ev, st));
+                r.t_start = Clock::now();
+            }
+            ++groups;
+            grouped += static_cast<int64_t>(g.size());
+        } catch (const std::exception& e) {
+            // nothing of the group is reported: wait for what was queued, then fail all of it
+            (void)hipStreamSynchronize(st);
+            const cvm::Error* ce = dynamic_cast<const cvm::Error*>(&e);
+            for (const auto& q : g) unreserve(q.first, q.second, ce ? *ce : cvm::Error(CVM_E_NO_MEMORY, e.what()));
+        }
+    }
+
+    // a reserved request that could not be launched: fail it and free its slot
+    void unreserve(int64_t id, int si, const cvm::Error& e) {
+        Request& r = reqs[id];
+        slots[si].req = -1;
+        r.slot = -1;
+        in_use -= static_cast<int64_t>(r.bytes);
+        --inflight;
+        fail(id, r, e);
+        cv_done.notify_all();
+    }
+
     void retire(int si) {
         Slot& sl = slots[si];
         Request& r = reqs[sl.req];
@@ -223,6 +325,7 @@ struct cvm_server {
     void run() {
         (void)hipSetDevice(device);
         std::unique_lock<std::mutex> lk(mu);
+        std::vector<std::pair<int64_t, int>> batch;
         while (true) {
             // retire finished evaluations
             for (size_t si = 0; si < slots.size(); ++si) {
@@ -256,17 +359,15 @@ struct cvm_server {
                 }
                 if (use < 0) break;
                 pending.pop_front();
-                try {
-                    launch(id, r, use);
-                    in_use += static_cast<int64_t>(r.bytes);
-                    ++inflight;
-                } catch (const cvm::Error& e) {
-                    fail(id, r, e);
-                    cv_done.notify_all();
-                } catch (const std::exception& e) {
-                    fail(id, r, cvm::Error(CVM_E_NO_MEMORY, e.what()));
-                    cv_done.notify_all();
-                }
+                slots[use].req = id;             // reserved; launched with the pass's batch
+                r.slot = use;
+                in_use += static_cast<int64_t>(r.bytes);
+                ++inflight;
+                batch.emplace_back(id, use);
+            }
+            if (!batch.empty()) {
+                launch_batch(batch);
+                batch.clear();
             }
             if (stop && inflight == 0 && pending.empty()) break;
             if (inflight == 0) {
@@ -301,6 +402,16 @@ int cvm_server_create(int device, int64_t hbm_budget, int max_inflight, cvm_serv
             return CVM_E_HIP;
         }
     }
+    const size_t jt = cvm::tile_job_bytes() * static_cast<size_t>(max_inflight) * static_cast<size_t>(max_inflight);
+    if (hipHostMalloc(&s->job_host, jt, hipHostMallocDefault) != hipSuccess || hipMalloc(&s->job_dev, jt) != hipSuccess) {
+        if (s->job_host) (void)hipHostFree(s->job_host);
+        for (Slot& t : s->slots) {
+            (void)hipStreamDestroy(t.ws.stream);
+            (void)hipEventDestroy(t.ev);
+        }
+        delete s;
+        return CVM_E_NO_MEMORY;
+    }
     s->worker = std::thread([s] { s->run(); });
     *out = s;
     return CVM_OK;
@@ -324,6 +435,8 @@ void cvm_server_destroy(cvm_server* s) {
         sl.x.release(); sl.F.release(); sl.K.release();
         if (sl.ws.stream) (void)hipStreamDestroy(sl.ws.stream);
     }
+    if (s->job_host) (void)hipHostFree(s->job_host);
+    if (s->job_dev) (void)hipFree(s->job_dev);
     delete s;
 }
 
@@ -426,6 +539,21 @@ int cvm_server_poll(cvm_server* s, int64_t id, int* done) {
     auto it = s->reqs.find(id);
     if (it == s->reqs.end()) { s->err = "unknown request id"; return CVM_E_INVALID; }
     *done = it->second.done ? 1 : 0;
+    return CVM_OK;
+}
+
+int cvm_server_set_batching(cvm_server* s, int on) {
+    if (!s) return CVM_E_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->batching = on != 0;
+    return CVM_OK;
+}
+
+int cvm_server_batching_stats(cvm_server* s, int64_t* out2) {
+    if (!s || !out2) return CVM_E_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    out2[0] = s->groups;
+    out2[1] = s->grouped;
     return CVM_OK;
 }
 

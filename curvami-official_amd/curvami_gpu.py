@@ -80,6 +80,8 @@ SIGNATURES = {
     "cvm_server_poll": (C.c_int, [C.c_void_p, C.c_int64, _i]),
     "cvm_server_wait": (C.c_int, [C.c_void_p, C.c_int64, _d]),
     "cvm_server_stats": (C.c_int, [C.c_void_p, _i64]),
+    "cvm_server_set_batching": (C.c_int, [C.c_void_p, C.c_int]),
+    "cvm_server_batching_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "cvm_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "cvm_kernel_count": (C.c_int, [C.c_void_p]),
     "cvm_kernel_stats": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int, _d, _i64]),
@@ -708,6 +710,14 @@ class Server:
         out = (C.c_int64 * 4)()
         self._check(self._L.cvm_server_stats(self._h, out))
         return {"completed": out[0], "inflight": out[1], "pending": out[2], "workspace_bytes": out[3]}
+
+    def set_batching(self, on=True):
+        self._check(self._L.cvm_server_set_batching(self._h, int(on)))
+
+    def batching_stats(self):
+        out = (C.c_int64 * 2)()
+        self._check(self._L.cvm_server_batching_stats(self._h, out))
+        return {"shared_tile_launches": out[0], "requests_in_shared_launches": out[1]}
 
     def close(self):
         if getattr(self, "_h", None):

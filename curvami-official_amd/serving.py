@@ -51,6 +51,7 @@ def run_rate(server, sheet_ids, reqs, xs, rate, seed, flags):
         done_t[0] = time.perf_counter()
 
     th = threading.Thread(target=collect)
+    b0 = server.batching_stats()
     th.start()
     t0 = time.perf_counter()
     for i in range(n):
@@ -61,10 +62,13 @@ def run_rate(server, sheet_ids, reqs, xs, rate, seed, flags):
         ready.release()
     th.join()
     span = done_t[0] - t0
+    b1 = server.batching_stats()
     return {"rate": rate, "requests": n, "p50_ms": float(np.percentile(lat, 50)),
             "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
             "queue_p50_ms": float(np.percentile(queue, 50)), "device_p50_ms": float(np.percentile(dev, 50)),
-            "throughput_rps": n / span, "offered_rps": n / arrivals[-1]}
+            "throughput_rps": n / span, "offered_rps": n / arrivals[-1],
+            "shared_tile_launches": b1["shared_tile_launches"] - b0["shared_tile_launches"],
+            "requests_in_shared_launches": b1["requests_in_shared_launches"] - b0["requests_in_shared_launches"]}
 
 
 def size_class_latency(server, sheet_ids, plates, n, x, flags, reps=20):

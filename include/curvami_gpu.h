@@ -299,6 +299,13 @@ CVM_API int cvm_server_poll(cvm_server* s, int64_t request_id, int* done);
 CVM_API int cvm_server_wait(cvm_server* s, int64_t request_id, double* times);
 /* completed, in flight, pending, workspace bytes allocated */
 CVM_API int cvm_server_stats(cvm_server* s, int64_t* out4);
+/* Requests admitted in the same dispatcher pass share one tile launch (the
+ * Gauss-point tiles of all their sheets in one kernel over a job table) when
+ * they agree in tangent type, laminate class and neighbour width; on by
+ * default. Results are the same bits either way. */
+CVM_API int cvm_server_set_batching(cvm_server* s, int on);
+/* out2 = shared tile launches so far, requests evaluated in them. */
+CVM_API int cvm_server_batching_stats(cvm_server* s, int64_t* out2);
 
 /* ---- energies and fold angles --------------------------------------------- */
 

@@ -61,7 +61,40 @@ def test_alone_equals_direct_and_mix(fleet):
         assert np.array_equal(Ks[j].get().view(np.int64), alone[i][1].view(np.int64)), f"request {i}: K differs"
     st = srv.stats()
     assert st["completed"] == 3 * len(reqs) and st["inflight"] == 0 and st["pending"] == 0
+    # requests admitted together shared tile launches (FP64 and FP32 in separate groups)
+    bs = srv.batching_stats()
+    print(bs)
+    assert bs["shared_tile_launches"] >= 1 and bs["requests_in_shared_launches"] >= 2
     srv.close()
+
+
+def test_shared_tile_launches_bit_identical(fleet):
+    """The same burst with shared tile launches and with every request on its own:
+    identical bits, and the shared run did share."""
+    sides, sheets, plates, reqs, xs = fleet
+    flags = [FLAGS | (G.FP32_TANGENT if i % 3 == 1 else 0) for i in range(len(reqs))]
+    out = {}
+    for on in (True, False):
+        srv = G.Server(hbm_budget_bytes=16 << 30, max_inflight=8)
+        srv.set_batching(on)
+        sid = {n: srv.add_sheet(plates[n]) for n in sides}
+        Fs, Ks, ids = [], [], []
+        for i, (n, _, _) in enumerate(reqs):
+            K = G.DeviceArray(_nnz(plates[n], flags[i]))
+            F = np.empty(3 * plates[n].nn)
+            ids.append(srv.submit(sid[n], xs[i], F, flags[i], K))
+            Fs.append(F); Ks.append(K)
+        for r in ids:
+            srv.wait(r)
+        out[on] = ([F.copy() for F in Fs], [K.get() for K in Ks], srv.batching_stats())
+        for K in Ks:
+            K.free()
+        srv.close()
+    print(out[True][2], out[False][2])
+    assert out[True][2]["shared_tile_launches"] >= 1 and out[False][2]["shared_tile_launches"] == 0
+    for i in range(len(reqs)):
+        assert np.array_equal(out[True][0][i].view(np.int64), out[False][0][i].view(np.int64))
+        assert np.array_equal(out[True][1][i].view(np.int64), out[False][1][i].view(np.int64))
 
 
 def test_budget_admission(fleet):
