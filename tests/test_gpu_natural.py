@@ -76,3 +76,35 @@ def test_set_after_plan_and_off_again():
     assert ei.value.code == G.E_INVALID
     for a in (xd, Fd, Kd):
         a.free()
+
+
+def test_shared_serving_launch_with_bands():
+    """Sheets with different bands (and one without) evaluated in one shared tile
+    launch of the serving loop: each result is the context's own assembly, bit for bit."""
+    specs = [("straight21", BANDS["straight21"]), ("arc41", BANDS["arc41"]), ("straight21", None)]
+    plates, xs = [], []
+    for k, (name, band) in enumerate(specs):
+        sh = dataclasses.replace(sheet(name), natural_curvature=band)
+        plates.append(G.plate_from_sheet(sh))
+        xs.append(W.displacement(sh.nn, 40 + k, 1e-3))
+    flags = G.RESIDUAL | G.TANGENT | G.ELASTIC_FOLDS
+    srv = G.Server(hbm_budget_bytes=8 << 30, max_inflight=4)
+    sid = [srv.add_sheet(p) for p in plates]
+    outs, Ks, ids, which = [], [], [], []
+    for rep in range(4):                      # a burst: requests queue up and are admitted together
+        for k, p in enumerate(plates):
+            F = np.empty(3 * p.nn)
+            K = G.DeviceArray(p.nnz_union)
+            ids.append(srv.submit(sid[k], xs[k], F, flags, K))
+            outs.append(F); Ks.append(K); which.append(k)
+    for r in ids:
+        srv.wait(r)
+    bs = srv.batching_stats()
+    print(bs)
+    assert bs["requests_in_shared_launches"] >= 2
+    ref = [p.assemble(xs[k]) for k, p in enumerate(plates)]
+    for j, k in enumerate(which):
+        assert np.array_equal(outs[j].view(np.int64), ref[k][0].view(np.int64))
+        assert np.array_equal(Ks[j].get().view(np.int64), ref[k][1].view(np.int64))
+        Ks[j].free()
+    srv.close()

@@ -65,3 +65,21 @@ def test_noalloc_refuses_large_sheet():
     for a in (x, F, K):
         a.free()
     p.close()
+
+
+def test_noalloc_with_natural_curvature():
+    """The natural-curvature field formed at cvm_init_assembly is read, not
+    formed, by the allocation-free evaluation: same bits as cvm_assemble."""
+    import dataclasses
+    import math
+    base = sheet("straight21")
+    sh = dataclasses.replace(base, natural_curvature=(0.37, 0.12, math.radians(30.0), -1.0))
+    p = G.plate_from_sheet(sh)
+    xv = W.displacement(sh.nn, 63, 1e-3)
+    ref = _eval(p, sh, xv, FLAGS, noalloc=False)
+    got = _eval(p, sh, xv, FLAGS, noalloc=True)
+    assert np.array_equal(got[0].view(np.int64), ref[0].view(np.int64))
+    assert np.array_equal(got[1].view(np.int64), ref[1].view(np.int64))
+    plain = G.plate_from_sheet(base)
+    assert not np.array_equal(_eval(plain, base, xv, FLAGS, noalloc=True)[0], got[0])
+    p.close(); plain.close()
