@@ -388,10 +388,18 @@ __global__ __launch_bounds__(256) void k_frnn_verify(const double* __restrict__ 
     const double* __restrict__ ls = D.il_s + base;
     const int* __restrict__ cstart = D.cstart;
     const int* __restrict__ cnodes = D.cnodes;
-    // the expected entry of the next hit, loaded a hit ahead; the candidate's
-    // node index one candidate ahead
-    int eid = old > 0 ? lid[0] : -1;
-    long long es = old > 0 ? __double_as_longlong(ls[0]) : 0;
+    // the expected entries of the next four hits in a shift register (the entry
+    // four hits ahead is loaded as one is consumed, so a hit does not wait for a
+    // load issued at the previous one); the candidate's node index one candidate
+    // ahead
+    constexpr int LA = 4;
+    int eid[LA];
+    long long es[LA];
+#pragma unroll
+    for (int j = 0; j < LA; ++j) {
+        eid[j] = j < old ? lid[static_cast<size_t>(j) * RKPM_BLOCK] : -1;
+        es[j] = j < old ? __double_as_longlong(ls[static_cast<size_t>(j) * RKPM_BLOCK]) : 0;
+    }
     int n = 0;
     bool diff = false;
     for (int dx = -1; dx <= 1; ++dx) {
@@ -408,11 +416,13 @@ __global__ __launch_bounds__(256) void k_frnn_verify(const double* __restrict__ 
                 if (d2 <= rho2) {
                     if (n < old) {      // slot n of this lane exists: n < old <= the block's row count
                         const double sv = sqrt(d2) / rho;
-                        diff |= (eid != ni) || (es != __double_as_longlong(sv));
-                        if (n + 1 < old) {
-                            const size_t k2 = static_cast<size_t>(n + 1) * RKPM_BLOCK;
-                            eid = lid[k2];
-                            es = __double_as_longlong(ls[k2]);
+                        diff |= (eid[0] != ni) || (es[0] != __double_as_longlong(sv));
+#pragma unroll
+                        for (int j = 0; j + 1 < LA; ++j) { eid[j] = eid[j + 1]; es[j] = es[j + 1]; }
+                        if (n + LA < old) {
+                            const size_t k2 = static_cast<size_t>(n + LA) * RKPM_BLOCK;
+                            eid[LA - 1] = lid[k2];
+                            es[LA - 1] = __double_as_longlong(ls[k2]);
                         }
                     }
                     ++n;
