@@ -467,7 +467,12 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((s
 // Tiles of several evaluations in one launch (serving: the requests admitted
 // together). Each job is one evaluation's plan, inputs and workspace, its tiles
 // numbered from tile0 on; the jobs share the laminate class, the tangent type
-// and the neighbour width, and the launch's LDS is the largest job's.
+// and the neighbour width, and the launch's LDS is the largest job's. The job
+// table is a kernel argument (passed by value, about 3 KB): its fields are read
+// like k_tile's arguments, so the kernel needs no more registers than k_tile
+// (127 VGPRs FP64, 79 FP32, no spills; read from a device table they took 135,
+// i.e. three waves per SIMD instead of four, and FP32 spilled) and a launch
+// needs no table copy.
 struct TileJob {
     const int* tile_gp; const int* tile_uoff; const int* tile_unodes; const int64_t* tile_toff;
     const uint16_t* tpairs; const int* tile_nd; const uint32_t* slot_dst;
@@ -476,14 +481,18 @@ struct TileJob {
     int rec_cap, umax, with_tangent, tile0;
     void* scr; double* fscr;
 };
+struct TileJobTable {
+    TileJob job[kTileJobsMax];
+    int n;
+};
 
 template <int LAM, typename TT, bool WIDE>
-__global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((sizeof(TT) == 4 && LAM != LAM_COUPLED && !WIDE) ? 6 : 1))) void k_tile_multi(const TileJob* __restrict__ jobs, int njobs) {
+__global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu((sizeof(TT) == 4 && LAM != LAM_COUPLED && !WIDE) ? 6 : 1))) void k_tile_multi(const TileJobTable tab) {
     extern __shared__ double smem[];
     const int b = xcd_tile(blockIdx.x, gridDim.x);
     int j = 0;
-    while (j + 1 < njobs && jobs[j + 1].tile0 <= b) ++j;   // njobs is small (the serving slots)
-    const TileJob& J = jobs[j];
+    while (j + 1 < tab.n && tab.job[j + 1].tile0 <= b) ++j;
+    const TileJob& J = tab.job[j];
     tile_body<LAM, TT, WIDE>(J.tile_gp, J.tile_uoff, J.tile_unodes, J.tile_toff, J.tpairs, J.tile_nd, J.slot_dst,
                              J.off, J.uloc, J.urank, J.sf, J.npairs, J.w, J.xu, J.kbar, J.mat, J.rec_cap, J.umax,
                              J.with_tangent, static_cast<TT*>(J.scr), J.fscr, b - J.tile0, smem);
@@ -917,8 +926,6 @@ static int laminate_class(const double* A, const double* B, const double* D) {
          : zero(A, {2, 5, 6, 7}) && zero(D, {2, 5, 6, 7}) ? LAM_ORTHO : LAM_SYM;
 }
 
-size_t tile_job_bytes() { return sizeof(TileJob); }
-
 int tile_group_key(const Context& c, int flags) {
     const Plan& P = c.plan;
     if (!P.ready || (c.tile_grid > 0 && c.tile_grid < P.ntiles)) return -1;
@@ -932,9 +939,13 @@ static auto tile_multi_kernel(int lam) {
                               : k_tile_multi<LAM_ORTHO, TT, WIDE>;
 }
 
-void launch_tile_group(const TileGroupItem* items, int n, void* host_jobs, void* dev_jobs, hipStream_t st) {
+void launch_tile_group(const TileGroupItem* items, int n, hipStream_t st) {
     if (n <= 0) return;
-    TileJob* H = static_cast<TileJob*>(host_jobs);
+    if (n > kTileJobsMax) throw Error(CVM_E_INVALID, "more evaluations than one tile launch takes");
+    TileJobTable tab;
+    std::memset(&tab, 0, sizeof(tab));
+    tab.n = n;
+    TileJob* H = tab.job;
     const int key = tile_group_key(*items[0].c, items[0].flags);
     if (key < 0) throw Error(CVM_E_STATE, "evaluation cannot share a tile launch");
     size_t lds = 0;
@@ -947,7 +958,6 @@ void launch_tile_group(const TileGroupItem* items, int n, void* host_jobs, void*
         for (Domain* d : c.doms) if (d->kind == CVM_DOMAIN_FACE) fdp = d;
         const Domain& fd = *fdp;
         TileJob& J = H[i];
-        std::memset(&J, 0, sizeof(J));
         J.tile_gp = P.d_tile_gp.p; J.tile_uoff = P.d_tile_uoff.p; J.tile_unodes = P.d_tile_unodes.p;
         J.tile_toff = P.d_tile_toff.p; J.tpairs = P.d_tile_tpairs.p; J.tile_nd = P.d_tile_nd.p;
         J.slot_dst = P.d_slot_dst.p; J.off = fd.off.p; J.uloc = P.d_pair_uloc.p; J.urank = P.d_pair_urank.p;
@@ -963,12 +973,11 @@ void launch_tile_group(const TileGroupItem* items, int n, void* host_jobs, void*
         tiles += P.ntiles;
         lds = std::max(lds, tile_lds_bytes(P.rec_cap, P.max_union, (key & 1) != 0, (key >> 3) != 0));
     }
-    CVM_HIP(hipMemcpyAsync(dev_jobs, host_jobs, sizeof(TileJob) * n, hipMemcpyHostToDevice, st));
     const int lam = (key >> 1) & 3;
     const bool fp32 = (key & 1) != 0, wide = (key >> 3) != 0;
     auto kern = fp32 ? (wide ? tile_multi_kernel<float, true>(lam) : tile_multi_kernel<float, false>(lam))
                      : (wide ? tile_multi_kernel<double, true>(lam) : tile_multi_kernel<double, false>(lam));
-    hipLaunchKernelGGL(kern, dim3(tiles), dim3(TILE_THREADS), lds, st, static_cast<const TileJob*>(dev_jobs), n);
+    hipLaunchKernelGGL(kern, dim3(tiles), dim3(TILE_THREADS), lds, st, tab);
     CVM_HIP(hipGetLastError());
 }
 

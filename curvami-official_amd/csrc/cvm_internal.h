@@ -385,11 +385,10 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
 // Tiles of several evaluations (contexts) in one launch, k_tile_multi: the
 // serving loop's requests admitted together. Evaluations share a launch when
 // their keys are equal (tangent type, laminate class, neighbour width; -1: a
-// persistent tile grid, never shared). host_jobs / dev_jobs hold n job
-// records of tile_job_bytes() each (pinned host, device); the records are
-// copied on stream st, so host_jobs must stay untouched until st has passed
-// the copy. The evaluations' remaining stages are launch_assembly(...,
-// tile_done = true) on the same stream.
+// persistent tile grid, never shared); a launch takes at most kTileJobsMax
+// of them (the job table is a kernel argument). The evaluations' remaining
+// stages are launch_assembly(..., tile_done = true) on the same stream.
+constexpr int kTileJobsMax = 8;
 struct TileGroupItem {
     Context* c;
     int flags;
@@ -397,9 +396,8 @@ struct TileGroupItem {
     double* scr;
     double* fscr;
 };
-size_t tile_job_bytes();
 int tile_group_key(const Context& c, int flags);
-void launch_tile_group(const TileGroupItem* items, int n, void* host_jobs, void* dev_jobs, hipStream_t st);
+void launch_tile_group(const TileGroupItem* items, int n, hipStream_t st);
 // The allocation-free evaluation (cvm_assemble_noalloc): the workspace lies in
 // the library's static device scratch of the context's GPU.
 void noalloc_scratch_init(int device);

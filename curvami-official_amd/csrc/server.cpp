@@ -17,9 +17,10 @@
 //  * Requests admitted in the same dispatcher pass (several slots free at
 //    once, as when a group of requests completes under load) share their tile
 //    launch: the Gauss-point tiles of all their sheets run as one k_tile_multi
-//    launch over a job table, then each request's hinge, F and K reductions
-//    follow on the same stream. Requests of different tangent types, laminate
-//    classes or neighbour widths go into separate groups.
+//    launch over a job table (a kernel argument; at most 8 requests per
+//    launch), then each request's hinge, F and K reductions follow on the
+//    same stream. Requests of different tangent types, laminate classes or
+//    neighbour widths go into separate groups.
 //  * A request's arithmetic does not depend on what else is in flight: each
 //    evaluation reads only its sheet's plan and writes only its own workspace,
 //    and a tile computes the same in a shared launch as in its own, so results
@@ -92,8 +93,6 @@ struct cvm_server {
     int64_t next_id = 1;
     bool batching = true;            // requests admitted together share their tile launch
     int64_t groups = 0, grouped = 0; // shared tile launches and the requests in them
-    void* job_host = nullptr;        // per slot: max_inflight tile-job records (pinned host, device)
-    void* job_dev = nullptr;
     int64_t in_use = 0;              // bytes charged by in-flight requests
     int64_t completed = 0;
     int inflight = 0;
@@ -210,8 +209,8 @@ struct cvm_server {
     }
 
     // Requests admitted together (slots reserved): groups of equal tile keys of two
-    // or more share one tile launch on the first request's stream; the rest run
-    // alone on their own slots' streams.
+    // or more (at most kTileJobsMax per launch) share one tile launch on the first
+    // request's stream; the rest run alone on their own slots' streams.
     void launch_batch(const std::vector<std::pair<int64_t, int>>& batch) {
         std::vector<int> key(batch.size());
         for (size_t i = 0; i < batch.size(); ++i) {
@@ -223,7 +222,7 @@ struct cvm_server {
             if (done[i]) continue;
             std::vector<size_t> grp{i};
             if (key[i] >= 0)
-                for (size_t k = i + 1; k < batch.size(); ++k)
+                for (size_t k = i + 1; k < batch.size() && grp.size() < static_cast<size_t>(cvm::kTileJobsMax); ++k)
                     if (!done[k] && key[k] == key[i]) grp.push_back(k);
             for (size_t k : grp) done[k] = 1;
             if (grp.size() == 1) {
@@ -263,10 +262,7 @@ struct cvm_server {
                 CVM_HIP(hipMemcpyAsync(sl.x.p, sl.hx, n3 * sizeof(double), hipMemcpyHostToDevice, st));
                 items[k] = cvm::TileGroupItem{&sh.ctx->c, r.flags & ~CVM_DEVICE_PTRS, sl.x.p, sl.ws.scr.p, sl.ws.fscr.p};
             }
-            const size_t jb = cvm::tile_job_bytes() * slots.size();
-            const size_t lo = static_cast<size_t>(g[0].second) * jb;
-            cvm::launch_tile_group(items.data(), static_cast<int>(items.size()),
-                                   static_cast<unsigned char*>(job_host) + lo, static_cast<unsigned char*>(job_dev) + lo, st);
+            cvm::launch_tile_group(items.data(), static_cast<int>(items.size()), st);
             for (size_t k = 0; k < g.size(); ++k) {
                 Request& r = reqs[g[k].first];
                 Slot& sl = slots[g[k].second];
@@ -402,16 +398,6 @@ int cvm_server_create(int device, int64_t hbm_budget, int max_inflight, cvm_serv
             return CVM_E_HIP;
         }
     }
-    const size_t jt = cvm::tile_job_bytes() * static_cast<size_t>(max_inflight) * static_cast<size_t>(max_inflight);
-    if (hipHostMalloc(&s->job_host, jt, hipHostMallocDefault) != hipSuccess || hipMalloc(&s->job_dev, jt) != hipSuccess) {
-        if (s->job_host) (void)hipHostFree(s->job_host);
-        for (Slot& t : s->slots) {
-            (void)hipStreamDestroy(t.ws.stream);
-            (void)hipEventDestroy(t.ev);
-        }
-        delete s;
-        return CVM_E_NO_MEMORY;
-    }
     s->worker = std::thread([s] { s->run(); });
     *out = s;
     return CVM_OK;
@@ -435,8 +421,6 @@ void cvm_server_destroy(cvm_server* s) {
         sl.x.release(); sl.F.release(); sl.K.release();
         if (sl.ws.stream) (void)hipStreamDestroy(sl.ws.stream);
     }
-    if (s->job_host) (void)hipHostFree(s->job_host);
-    if (s->job_dev) (void)hipFree(s->job_dev);
     delete s;
 }
 
