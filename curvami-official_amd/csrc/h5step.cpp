@@ -48,17 +48,18 @@ bool file_exists(const char* path) {
     return true;
 }
 
-// rows x cols doubles, chunked (min(rows, 1000), cols), deflate 6 (Exporter.h:222-232)
+// rows x cols doubles, chunked (min(rows, 1000), cols), deflate 6 (Exporter.h:222-232);
+// a table without rows is stored contiguous (a chunk needs at least one row)
 bool write_table(hid_t file, const char* name, const double* v, hsize_t rows, hsize_t cols, hid_t* out) {
     const hsize_t dims[2] = {rows, cols};
     Id space(H5Screate_simple(2, dims, nullptr));
     Id plist(H5Pcreate(H5P_DATASET_CREATE));
     const hsize_t chunk[2] = {std::min<hsize_t>(rows, 1000), cols};
-    if (!space.ok() || !plist.ok() || H5Pset_chunk(plist.id, 2, chunk) < 0 || H5Pset_deflate(plist.id, 6) < 0)
-        return false;
+    if (!space.ok() || !plist.ok()) return false;
+    if (rows > 0 && (H5Pset_chunk(plist.id, 2, chunk) < 0 || H5Pset_deflate(plist.id, 6) < 0)) return false;
     const hid_t ds = H5Dcreate2(file, name, H5T_NATIVE_DOUBLE, space.id, H5P_DEFAULT, plist.id, H5P_DEFAULT);
     if (ds < 0) return false;
-    if (H5Dwrite(ds, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, v) < 0) {
+    if (rows > 0 && H5Dwrite(ds, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, v) < 0) {
         H5Dclose(ds);
         return false;
     }

@@ -219,7 +219,8 @@ int cvm_kkt_set_pattern(cvm_kkt* s, int n, const int* outer, const int* inner, i
                         const int* cols) {
     if (!s) return 1;
     Kkt& k = s->k;
-    if (n <= 0 || m < 0 || nt < 0 || !outer || (outer[n] > 0 && !inner) || (nt > 0 && (!rows || !cols))) {
+    if (n <= 0 || m < 0 || nt < 0 || !outer || outer[0] != 0 || (outer[n] > 0 && !inner) ||
+        (nt > 0 && (!rows || !cols))) {
         k.err = "bad pattern arrays";
         return 1;
     }

@@ -116,6 +116,10 @@ def test_bad_input_refused(system):
     with pytest.raises(G.CvmError) as e:
         s.solve(np.zeros(outer.size - 1 + m))
     assert e.value.code == 7
+    # column pointers must start at zero
+    with pytest.raises(G.CvmError) as e:
+        s.set_pattern(outer - 1, inner, m, r, col)
+    assert e.value.code == 1 and "bad pattern" in str(e.value)
 
 
 def test_bit_reproducible_across_threads(system):

@@ -148,3 +148,7 @@ def test_hdf5_step_container_layout(tmp_path):
     assert a["angles"] == [0.5, -1.25] and a["is_driver"] == [1, 0] and a["git_commit"] == "abc123"
     assert all(a[k] == attrs[k] for k in ("disp", "W_ext", "E_mem", "E_ben", "E_hin", "planar_rms"))
     assert got[4][1] is None and got[4][2]["angles"] == [0.0] and got[4][2]["is_driver"] == [0]
+    # a step without Gauss points: an empty table, stored contiguous
+    w.write(5, np.zeros((0, 10)), np.zeros((0, 7)), attrs)
+    d5, c5, a5 = EX.StepWriter.read(path)[5]
+    assert d5.shape == (0, 10) and c5 is None and a5["disp"] == attrs["disp"]
