@@ -501,10 +501,15 @@ int cvm_server_submit(cvm_server* s, int sheet, const double* x, int flags, doub
 int cvm_server_wait(cvm_server* s, int64_t id, double* times) {
     if (!s) return CVM_E_INVALID;
     std::unique_lock<std::mutex> lk(s->mu);
-    auto it = s->reqs.find(id);
-    if (it == s->reqs.end()) { s->err = "unknown request id"; return CVM_E_INVALID; }
-    s->cv_done.wait(lk, [&] { return s->reqs[id].done; });
-    Request& r = s->reqs[id];
+    if (s->reqs.find(id) == s->reqs.end()) { s->err = "unknown request id"; return CVM_E_INVALID; }
+    // another thread's wait on the same id may forget the request meanwhile
+    s->cv_done.wait(lk, [&] {
+        const auto f = s->reqs.find(id);
+        return f == s->reqs.end() || f->second.done;
+    });
+    const auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) { s->err = "request already waited for"; return CVM_E_INVALID; }
+    Request& r = it->second;
     if (times) {
         const bool started = r.slot >= 0;
         times[0] = started ? ms_between(r.t_submit, r.t_start) : ms_between(r.t_submit, r.t_done);

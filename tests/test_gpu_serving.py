@@ -148,3 +148,35 @@ def test_poisson_run_smoke(fleet):
     print(r)
     assert r["requests"] == len(reqs) and 0 < r["p50_ms"] <= r["p99_ms"]
     srv.close()
+
+
+def test_waits_on_a_forgotten_request(fleet):
+    """A request is forgotten by the wait that returns it: two threads waiting
+    on one request get one result and one refusal (no hang), and a later wait
+    on the id is refused."""
+    sides, sheets, plates, reqs, xs = fleet
+    n = sides[-1]
+    srv = G.Server(hbm_budget_bytes=4 << 30, max_inflight=2)
+    sid = srv.add_sheet(plates[n])
+    F = np.empty(3 * plates[n].nn)
+    rid = srv.submit(sid, xs[2], F, FLAGS)
+    codes = []
+
+    def waiter():
+        try:
+            srv.wait(rid)
+            codes.append(G.OK)
+        except G.CvmError as e:
+            codes.append(e.code)
+
+    th = [threading.Thread(target=waiter) for _ in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=60)
+    assert not any(t.is_alive() for t in th)
+    assert sorted(codes) == sorted([G.OK, G.E_INVALID])
+    with pytest.raises(G.CvmError) as ei:
+        srv.wait(rid)
+    assert ei.value.code == G.E_INVALID
+    srv.close()
