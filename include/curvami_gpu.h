@@ -295,7 +295,9 @@ CVM_API int cvm_server_request_bytes(cvm_server* s, int sheet, int flags, int wi
 CVM_API int cvm_server_submit(cvm_server* s, int sheet, const double* x, int flags, double* F_host, double* K_dev,
                               int64_t* request_id);
 CVM_API int cvm_server_poll(cvm_server* s, int64_t request_id, int* done);
-/* Wait for a request; times[3] (may be NULL): queue ms, device ms, total ms. */
+/* Wait for a request; times[3] (may be NULL): queue ms, device ms, total ms.
+ * The wait that returns a request forgets it: another wait on the same id
+ * (later, or in another thread at the same time) returns CVM_E_INVALID. */
 CVM_API int cvm_server_wait(cvm_server* s, int64_t request_id, double* times);
 /* completed, in flight, pending, workspace bytes allocated */
 CVM_API int cvm_server_stats(cvm_server* s, int64_t* out4);
