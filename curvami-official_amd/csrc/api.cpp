@@ -17,10 +17,23 @@ Context::~Context() {
     }
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (side) (void)hipStreamSynchronize(side);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
-// ---- per-kernel event timing on the context's stream ------------------------
+hipStream_t side_stream(Context& c) {
+    if (!c.side) {
+        CVM_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+        CVM_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
+        CVM_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
+    }
+    return c.side;
+}
+
+// ---- per-kernel event timing (the context's stream, or its side stream) ---------
 static int kernel_id(Timing& t, const char* name) {
     for (size_t i = 0; i < t.names.size(); ++i)
         if (t.names[i] == name) return static_cast<int>(i);
@@ -34,6 +47,7 @@ static void timing_collect(Context& c) {
     Timing& t = c.timing;
     if (t.pending.empty()) return;
     CVM_HIP(hipStreamSynchronize(c.stream));
+    if (c.side) CVM_HIP(hipStreamSynchronize(c.side));   // kernels timed on the side stream
     for (size_t i = 0; i < t.pending.size(); ++i) {
         float ms = 0.0f;
         CVM_HIP(hipEventElapsedTime(&ms, t.ev0[i], t.ev1[i]));
@@ -43,7 +57,7 @@ static void timing_collect(Context& c) {
     t.pending.clear();
 }
 
-int timing_begin(Context& c, const char* name) {
+int timing_begin(Context& c, const char* name, hipStream_t st) {
     Timing& t = c.timing;
     if (!t.on) return -1;
     const int id = kernel_id(t, name);
@@ -58,13 +72,13 @@ int timing_begin(Context& c, const char* name) {
         t.ev1.push_back(b);
     }
     t.pending.push_back(id);
-    CVM_HIP(hipEventRecord(t.ev0[s2], c.stream));
+    CVM_HIP(hipEventRecord(t.ev0[s2], st ? st : c.stream));
     return static_cast<int>(s2);
 }
 
-void timing_end(Context& c, int slot) {
+void timing_end(Context& c, int slot, hipStream_t st) {
     if (slot < 0) return;
-    CVM_HIP(hipEventRecord(c.timing.ev1[slot], c.stream));
+    CVM_HIP(hipEventRecord(c.timing.ev1[slot], st ? st : c.stream));
 }
 
 void sync_nodes(Context& c) {
@@ -400,7 +414,6 @@ int cvm_refresh_neighbours(cvm_context* ctx, int* changed) {
             pattern = true;
     }
     if (pattern) c.plan.ready = false;
-    c.node_sign_creases = -1;          // node sides are evaluated again with the next shape functions
     if (changed) *changed = any ? 1 : 0;
     CVM_CATCH(ctx)
 }

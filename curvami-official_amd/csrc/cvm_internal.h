@@ -346,6 +346,11 @@ struct Context {
     DBuf<double> d_F;           // 3 nn
     DBuf<double> d_K;           // nnz (union)
     Workspace ws;               // the context's own assembly workspace (its stream)
+    // A second stream (default priority) for work that runs beside the
+    // context's stream: the crease indicators and node sides of a neighbour
+    // refresh, beside its verification pass. fork / join order the two.
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     size_t lds_limit = 65536;
     Timing timing;
     int tile_gps = 16;          // GPs per assembly tile (at most kTileGps)
@@ -363,8 +368,13 @@ struct Context {
 };
 
 // kernel launchers (implemented in the .hip files)
-void launch_indicators(Context& c, const std::vector<Domain*>& ds);
-void launch_node_signs(Context& c);
+// Indicator values and node sides (when stale). Uploads go on the context's
+// stream; the kernel runs on kst (default: the context's stream) after the
+// uploads, and the caller orders later work of the context's stream after it.
+void launch_indicators(Context& c, const std::vector<Domain*>& ds, hipStream_t kst = nullptr);
+void launch_node_signs(Context& c, hipStream_t kst = nullptr);
+// The context's side stream and its fork / join events (created on first use).
+hipStream_t side_stream(Context& c);
 void launch_frnn(Context& c, Domain& d, double rho);
 // Repeats the search of every searched domain on the device (same nodes, GPs
 // and radius): the lists are rewritten in place; per domain, whether its lists
@@ -420,8 +430,11 @@ constexpr int kTileGps = 16;    // GPs per assembly tile
 constexpr int kTileUnion = 255; // union nodes per assembly tile (uint8 union indices)
 
 // timing helpers
-int timing_begin(Context& c, const char* name);
-void timing_end(Context& c, int slot);
+// per-kernel event timing on stream st (default: the context's stream); a
+// kernel timed on another stream is joined into the context's stream before
+// the timings are read
+int timing_begin(Context& c, const char* name, hipStream_t st = nullptr);
+void timing_end(Context& c, int slot, hipStream_t st = nullptr);
 
 void build_plan(Context& c);
 void sync_nodes(Context& c);

@@ -211,7 +211,16 @@ static void upload_creases(Context& c) {
     c.creases_dirty = false;
 }
 
-void launch_node_signs(Context& c) {
+// Orders kernel stream kst after the work enqueued so far on the context's
+// stream (nothing to do when kst is that stream).
+static hipStream_t fork_to(Context& c, hipStream_t kst) {
+    if (!kst || kst == c.stream) return c.stream;
+    CVM_HIP(hipEventRecord(c.ev_fork, c.stream));
+    CVM_HIP(hipStreamWaitEvent(kst, c.ev_fork, 0));
+    return kst;
+}
+
+void launch_node_signs(Context& c, hipStream_t kst) {
     sync_nodes(c);
     upload_creases(c);
     const int nc = static_cast<int>(c.creases.size());
@@ -229,12 +238,13 @@ void launch_node_signs(Context& c) {
         c.h_crange = crange;
         c.d_crange.upload(c.h_crange, c.stream);
     }
-    const int slot = timing_begin(c, "node_sign");
-    hipLaunchKernelGGL(k_node_sign, dim3((c.nn + 255) / 256), dim3(256), 0, c.stream, c.X.p, c.Y.p, c.nn,
+    const hipStream_t ks = fork_to(c, kst);
+    const int slot = timing_begin(c, "node_sign", ks);
+    hipLaunchKernelGGL(k_node_sign, dim3((c.nn + 255) / 256), dim3(256), 0, ks, c.X.p, c.Y.p, c.nn,
                        c.batched ? c.d_nsheet.p : nullptr, c.d_crange.p, c.d_cv.p, c.d_cinfo.p, c.d_cgrp.p,
                        c.d_cmax.p, c.d_node_sign.p);
     CVM_HIP(hipGetLastError());
-    timing_end(c, slot);
+    timing_end(c, slot, ks);
 }
 
 // The crease range (first crease, count) of every sheet in the domain's
@@ -255,7 +265,7 @@ static void sync_sheet_creases(Context& c, Domain& d) {
 }
 
 // Indicator values of the domains whose values are stale, one launch.
-void launch_indicators(Context& c, const std::vector<Domain*>& ds) {
+void launch_indicators(Context& c, const std::vector<Domain*>& ds, hipStream_t kst) {
     upload_creases(c);
     const int nc = static_cast<int>(c.creases.size());
     const int mc = c.max_sheet_creases();
@@ -283,12 +293,13 @@ void launch_indicators(Context& c, const std::vector<Domain*>& ds) {
                               reinterpret_cast<const unsigned char*>(tab.data()) + bytes);
         c.ind_tab.upload(c.ind_tab_host, c.stream);
     }
-    const int slot = timing_begin(c, "indicator_gp");
-    hipLaunchKernelGGL(k_indicator_gp, dim3(static_cast<unsigned>(nblocks)), dim3(256), 0, c.stream,
+    const hipStream_t ks = fork_to(c, kst);
+    const int slot = timing_begin(c, "indicator_gp", ks);
+    hipLaunchKernelGGL(k_indicator_gp, dim3(static_cast<unsigned>(nblocks)), dim3(256), 0, ks,
                        reinterpret_cast<const IndDom*>(c.ind_tab.p), static_cast<int>(tab.size()), c.d_cv.p,
                        c.d_cinfo.p, c.d_cgrp.p, c.d_cmax.p);
     CVM_HIP(hipGetLastError());
-    timing_end(c, slot);
+    timing_end(c, slot, ks);
 }
 
 // ---------------------------------------------------------------------------
@@ -580,6 +591,7 @@ void launch_frnn(Context& c, Domain& d, double rho) {
 std::vector<char> refresh_frnn_all(Context& c) {
     const size_t nd = c.doms.size();
     std::vector<char> changed(nd, 0), live(nd, 0);
+    c.node_sign_creases = -1;          // node sides are evaluated again, here or with the next shape functions
     c.rf_flags.ensure(std::max<size_t>(nd, 1));
     std::vector<VerDom> tab;
     int64_t nblocks = 0;
@@ -613,6 +625,18 @@ std::vector<char> refresh_frnn_all(Context& c) {
                              reinterpret_cast<const unsigned char*>(tab.data()) + bytes);
         c.rf_tab.upload(c.rf_tab_host, c.stream);
     }
+    // The node sides and the verified domains' crease indicators, which depend
+    // on neither the lists nor their verification, run on the side stream
+    // beside the verification pass; the context's stream joins them before the
+    // flags are read. (A domain whose lists then differ takes the full search,
+    // which marks its indicators stale again.)
+    const hipStream_t sd = side_stream(c);
+    launch_node_signs(c, sd);
+    std::vector<Domain*> ind_ds;
+    for (size_t k = 0; k < nd; ++k)
+        if (live[k]) ind_ds.push_back(c.doms[k]);
+    launch_indicators(c, ind_ds, sd);
+    CVM_HIP(hipEventRecord(c.ev_join, sd));
     hipLaunchKernelGGL(k_zero_flags, dim3((static_cast<unsigned>(nd) + 255) / 256), dim3(256), 0, c.stream,
                        c.rf_flags.p, static_cast<int>(nd));
     CVM_HIP(hipGetLastError());
@@ -623,6 +647,7 @@ std::vector<char> refresh_frnn_all(Context& c) {
     timing_end(c, slot);
     std::vector<int> fl(nd, 0);
     CVM_HIP(hipMemcpyAsync(fl.data(), c.rf_flags.p, nd * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    CVM_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));
     CVM_HIP(hipStreamSynchronize(c.stream));
     for (size_t k = 0; k < nd; ++k) {
         if (!live[k] || !fl[k]) continue;
