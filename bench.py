@@ -153,12 +153,12 @@ def cpu_baseline(sh, threads, reps=5):
 
 def _step_time(plate, ndom, order, kernel, xd, Fd, Kd, flags, steps, warmup, graph=False):
     """Seconds per step: the neighbour search of every domain again on the
-    device, shape functions of every domain (queued back to back, one status
-    read-back), then one evaluation (eager, or the captured graph)."""
+    device and the shape functions of every domain (one call: the shape
+    functions formed beside the search's verification, one status read-back),
+    then one evaluation (eager, or the captured graph)."""
     def step():
-        if plate.refresh_neighbours():
+        if plate.recompute_shape_functions(order, kernel):
             raise RuntimeError("neighbour lists changed on unchanged geometry")
-        plate.shape_functions_all(order, kernel)
         if graph:
             plate.graph_launch()
         else:
@@ -353,17 +353,15 @@ def defaults_leg(n_side, warmup, steps):
     for name, fl in (("fp64", base), ("mixed", base | G.FP32_TANGENT)):
         log(f"defaults: timing the {name} step")
         for _ in range(warmup):
-            p.refresh_neighbours()
-            p.shape_functions_all(sh.order, sh.kernel)
+            p.recompute_shape_functions(sh.order, sh.kernel)
             p.assemble_device(x.ptr, F.ptr, K.ptr, fl)
         p.synchronize()
         p.set_profiling(True)
         p.reset_profiling()
         t0 = time.perf_counter()
         for _ in range(steps):
-            if p.refresh_neighbours():
+            if p.recompute_shape_functions(sh.order, sh.kernel):
                 raise RuntimeError("neighbour lists changed on unchanged geometry")
-            p.shape_functions_all(sh.order, sh.kernel)
             p.assemble_device(x.ptr, F.ptr, K.ptr, fl)
         p.synchronize()
         dt = (time.perf_counter() - t0) / steps
@@ -468,9 +466,13 @@ def main():
     flags = G.RESIDUAL | G.TANGENT | G.ELASTIC_FOLDS
 
     def step(extra=0, search=True):
-        if search and p.refresh_neighbours():
-            raise RuntimeError("neighbour lists changed on unchanged geometry")
-        p.shape_functions_all(sh.order, sh.kernel)
+        # the search and the shape functions of every domain in one call
+        # (cvm_recompute_shape_functions), or the shape functions alone
+        if search:
+            if p.recompute_shape_functions(sh.order, sh.kernel):
+                raise RuntimeError("neighbour lists changed on unchanged geometry")
+        else:
+            p.shape_functions_all(sh.order, sh.kernel)
         p.assemble_device(xd, Fd, Kd, flags | extra)
 
     def timed(extra, search=True):

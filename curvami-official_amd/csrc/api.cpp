@@ -475,6 +475,46 @@ int cvm_shape_functions_all(cvm_context* ctx, int order, int kernel, int* failed
     CVM_CATCH(ctx)
 }
 
+int cvm_recompute_shape_functions(cvm_context* ctx, int order, int kernel, int* changed, int* failed_domain,
+                                  int* nbad, int* first_bad) {
+    CVM_TRY(ctx)
+    Context& c = ctx->c;
+    CVM_HIP(hipSetDevice(c.device));
+    if (order < 1 || order > 3) throw Error(CVM_E_INVALID, "order must be 1, 2 or 3");
+    if (kernel != CVM_KERNEL_QUARTIC && kernel != CVM_KERNEL_CUBIC_BSPLINE)
+        throw Error(CVM_E_INVALID, "unknown window kernel");
+    for (cvm::Domain* d : c.doms)
+        if (d->npairs < 0) throw Error(CVM_E_STATE, "neighbours not computed for this domain");
+    // the shape functions of every domain beside the verification, from the
+    // resident lists; formed again from the new lists if any list changed
+    bool enq = false;
+    cvm::RkpmBatch rb;
+    const std::vector<char> ch = cvm::refresh_frnn_all(c, [&](hipStream_t sd) {
+        rb = cvm::enqueue_rkpm_all(c, order, kernel, sd);
+        enq = true;
+    });
+    bool any = false, pattern = false;
+    for (size_t k = 0; k < c.doms.size(); ++k) {
+        cvm::Domain* d = c.doms[k];
+        if (!ch[k]) continue;
+        any = true;
+        ++d->gen;
+        if (d->kind == CVM_DOMAIN_FACE || d->kind == CVM_DOMAIN_FOLD_LEFT || d->kind == CVM_DOMAIN_FOLD_RIGHT)
+            pattern = true;
+    }
+    if (pattern) c.plan.ready = false;
+    int dom = -1, code = 0, nb = 0, fb = 0;
+    if (enq && !any) cvm::finish_rkpm_all(c, rb, &dom, &code, &nb, &fb);
+    else cvm::launch_rkpm_all(c, order, kernel, &dom, &code, &nb, &fb);
+    for (cvm::Domain* d : c.doms) ++d->gen;
+    if (changed) *changed = any ? 1 : 0;
+    if (failed_domain) *failed_domain = dom;
+    if (nbad) *nbad = nb;
+    if (first_bad) *first_bad = fb;
+    if (code != CVM_OK) rkpm_error(*c.doms[dom], code, nb, fb, "domain " + std::to_string(dom) + ": ");
+    CVM_CATCH(ctx)
+}
+
 int cvm_get_shape_functions(cvm_context* ctx, int domain, double* psi, double* dpx, double* dpy, double* dxx,
                             double* dyy, double* dxy) {
     CVM_TRY(ctx)

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -379,9 +380,26 @@ void launch_frnn(Context& c, Domain& d, double rho);
 // Repeats the search of every searched domain on the device (same nodes, GPs
 // and radius): the lists are rewritten in place; per domain, whether its lists
 // changed (lists, mirrors and interleaved copies then replaced).
-std::vector<char> refresh_frnn_all(Context& c);
+// beside (optional) enqueues more work on the side stream after the crease
+// indicators, beside the verification pass; it is called only when the
+// verification runs (some domain keeps its grid), and its work is complete
+// when the call returns.
+std::vector<char> refresh_frnn_all(Context& c, const std::function<void(hipStream_t)>& beside = nullptr);
 void launch_rkpm(Context& c, Domain& d, double rho, int order, int kernel, int* code, int* nbad, int* first);
 void launch_rkpm_all(Context& c, int order, int kernel, int* dom, int* code, int* nbad, int* first);
+// The shape functions of every domain enqueued with their kernels on stream
+// kst (default: the context's stream), statuses in the context's status
+// words; then, after they completed, the statuses read back and the first
+// failing domain (in domain order) and its status.
+struct RkpmBatch {
+    std::vector<size_t> order_of;   // domain of each launch slot
+    int order = 0, kernel = 0;
+};
+RkpmBatch enqueue_rkpm_all(Context& c, int order, int kernel, hipStream_t kst = nullptr);
+void finish_rkpm_all(Context& c, const RkpmBatch& b, int* dom, int* code, int* nbad, int* first);
+// Orders stream kst after the work enqueued so far on the context's stream;
+// returns the stream to launch on (the context's stream when kst is null).
+hipStream_t fork_to(Context& c, hipStream_t kst);
 // Workspace buffers given by address (the allocation-free entry point's static
 // scratch) instead of a Workspace's own buffers.
 struct WsPtrs {

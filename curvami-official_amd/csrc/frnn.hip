@@ -211,9 +211,7 @@ static void upload_creases(Context& c) {
     c.creases_dirty = false;
 }
 
-// Orders kernel stream kst after the work enqueued so far on the context's
-// stream (nothing to do when kst is that stream).
-static hipStream_t fork_to(Context& c, hipStream_t kst) {
+hipStream_t fork_to(Context& c, hipStream_t kst) {
     if (!kst || kst == c.stream) return c.stream;
     CVM_HIP(hipEventRecord(c.ev_fork, c.stream));
     CVM_HIP(hipStreamWaitEvent(kst, c.ev_fork, 0));
@@ -588,7 +586,7 @@ void launch_frnn(Context& c, Domain& d, double rho) {
 // result; a domain with any difference takes the full search (host grid,
 // counts, offsets, lists, interleaved copies and mirrors). No host loop over
 // GPs or pairs. Returns, per domain, whether its lists changed.
-std::vector<char> refresh_frnn_all(Context& c) {
+std::vector<char> refresh_frnn_all(Context& c, const std::function<void(hipStream_t)>& beside) {
     const size_t nd = c.doms.size();
     std::vector<char> changed(nd, 0), live(nd, 0);
     c.node_sign_creases = -1;          // node sides are evaluated again, here or with the next shape functions
@@ -625,17 +623,27 @@ std::vector<char> refresh_frnn_all(Context& c) {
                              reinterpret_cast<const unsigned char*>(tab.data()) + bytes);
         c.rf_tab.upload(c.rf_tab_host, c.stream);
     }
-    // The node sides and the verified domains' crease indicators, which depend
-    // on neither the lists nor their verification, run on the side stream
-    // beside the verification pass; the context's stream joins them before the
-    // flags are read. (A domain whose lists then differ takes the full search,
-    // which marks its indicators stale again.)
+    // The verified domains' crease indicators, which depend on neither the
+    // lists nor their verification, run on the side stream beside the
+    // verification pass, and the node sides on the context's stream ahead of
+    // it; side work enqueued by `beside` follows the indicators and (through
+    // its fork) the node sides. The context's stream joins the side stream
+    // before the flags are read. (A domain whose lists then differ takes the
+    // full search, which marks its indicators stale again.)
     const hipStream_t sd = side_stream(c);
-    launch_node_signs(c, sd);
     std::vector<Domain*> ind_ds;
     for (size_t k = 0; k < nd; ++k)
         if (live[k]) ind_ds.push_back(c.doms[k]);
     launch_indicators(c, ind_ds, sd);
+    launch_node_signs(c);
+    if (beside) {
+        try {
+            beside(sd);
+        } catch (...) {
+            (void)hipStreamSynchronize(sd);   // nothing of the side stream outlives the call
+            throw;
+        }
+    }
     CVM_HIP(hipEventRecord(c.ev_join, sd));
     hipLaunchKernelGGL(k_zero_flags, dim3((static_cast<unsigned>(nd) + 255) / 256), dim3(256), 0, c.stream,
                        c.rf_flags.p, static_cast<int>(nd));

@@ -564,11 +564,11 @@ __global__ void k_status_init(int* s, int n) {
 #endif
 
 template <int P, int KER>
-static void run(Context& c, const RkDom* tab, int nd, int nblocks) {
-    hipLaunchKernelGGL((k_rkpm_coef<P, KER, RKPM_SPLIT(P)>), dim3(nblocks), dim3(RKPM_THREADS), 0, c.stream, c.X.p,
+static void run(Context& c, hipStream_t st, const RkDom* tab, int nd, int nblocks) {
+    hipLaunchKernelGGL((k_rkpm_coef<P, KER, RKPM_SPLIT(P)>), dim3(nblocks), dim3(RKPM_THREADS), 0, st, c.X.p,
                        c.Y.p, c.nn, c.d_node_sign.p, tab, nd);
     CVM_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_rkpm_sf<P, KER>), dim3(nblocks), dim3(RKSF_THREADS), 0, c.stream, c.X.p, c.Y.p, c.nn,
+    hipLaunchKernelGGL((k_rkpm_sf<P, KER>), dim3(nblocks), dim3(RKSF_THREADS), 0, st, c.X.p, c.Y.p, c.nn,
                        c.d_node_sign.p, tab, nd);
     CVM_HIP(hipGetLastError());
 }
@@ -601,12 +601,14 @@ static void prepare_rkpm(Context& c, Domain& d) {
 // Enqueues the shape functions of a group of domains in one coefficient and one
 // shape-function launch; domain k's status goes to st[4 k .. 4 k + 3]. `cache`
 // selects the context's descriptor table (uploaded only when it changes).
+// Uploads go on the context's stream, the kernels on kst (default: the
+// context's stream) after them.
 static void enqueue_group(Context& c, const std::vector<Domain*>& ds, int order, int kernel, int* st, int cache,
-                          const char* stage) {
+                          const char* stage, hipStream_t kst) {
     if (ds.empty()) return;
     for (Domain* d : ds)
         if (d->npairs < 0) throw Error(CVM_E_STATE, "neighbours not computed for this domain");
-    launch_indicators(c, ds);
+    launch_indicators(c, ds, kst);
     for (Domain* d : ds) prepare_rkpm(c, *d);
     const size_t ncj = 6 * static_cast<size_t>((order + 1) * (order + 2) / 2);   // RK<order>::NCJ
     std::vector<RkDom> tab;
@@ -630,32 +632,35 @@ static void enqueue_group(Context& c, const std::vector<Domain*>& ds, int order,
         if (nblocks > INT32_MAX) throw Error(CVM_E_INVALID, "too many Gauss points for one shape-function launch");
         tab.push_back(r);
     }
-    hipLaunchKernelGGL(k_status_init, dim3(1), dim3(256), 0, c.stream, st, static_cast<int>(ds.size()));
+    if (!tab.empty()) {
+        c.rk_cj.ensure(std::max<size_t>(cj_total, 1));
+        for (size_t k = 0; k < tab.size(); ++k) tab[k].cj = c.rk_cj.p + cj_off[k];
+        auto& host = c.rk_tab_host[cache];
+        auto& dev = c.rk_tab[cache];
+        const size_t bytes = tab.size() * sizeof(RkDom);
+        if (host.size() != bytes || std::memcmp(host.data(), tab.data(), bytes) != 0) {
+            host.assign(reinterpret_cast<const unsigned char*>(tab.data()),
+                        reinterpret_cast<const unsigned char*>(tab.data()) + bytes);
+            dev.upload(host, c.stream);
+        }
+    }
+    const hipStream_t ks = fork_to(c, kst);
+    hipLaunchKernelGGL(k_status_init, dim3(1), dim3(256), 0, ks, st, static_cast<int>(ds.size()));
     CVM_HIP(hipGetLastError());
     if (tab.empty()) return;
-    c.rk_cj.ensure(std::max<size_t>(cj_total, 1));
-    for (size_t k = 0; k < tab.size(); ++k) tab[k].cj = c.rk_cj.p + cj_off[k];
-    auto& host = c.rk_tab_host[cache];
-    auto& dev = c.rk_tab[cache];
-    const size_t bytes = tab.size() * sizeof(RkDom);
-    if (host.size() != bytes || std::memcmp(host.data(), tab.data(), bytes) != 0) {
-        host.assign(reinterpret_cast<const unsigned char*>(tab.data()),
-                    reinterpret_cast<const unsigned char*>(tab.data()) + bytes);
-        dev.upload(host, c.stream);
-    }
-    const RkDom* dtab = reinterpret_cast<const RkDom*>(dev.p);
+    const RkDom* dtab = reinterpret_cast<const RkDom*>(c.rk_tab[cache].p);
     const int nd = static_cast<int>(tab.size()), nb = static_cast<int>(nblocks);
-    const int slot = timing_begin(c, stage);
+    const int slot = timing_begin(c, stage, ks);
     if (kernel == CVM_KERNEL_QUARTIC) {
-        if (order == 1) run<1, CVM_KERNEL_QUARTIC>(c, dtab, nd, nb);
-        else if (order == 2) run<2, CVM_KERNEL_QUARTIC>(c, dtab, nd, nb);
-        else run<3, CVM_KERNEL_QUARTIC>(c, dtab, nd, nb);
+        if (order == 1) run<1, CVM_KERNEL_QUARTIC>(c, ks, dtab, nd, nb);
+        else if (order == 2) run<2, CVM_KERNEL_QUARTIC>(c, ks, dtab, nd, nb);
+        else run<3, CVM_KERNEL_QUARTIC>(c, ks, dtab, nd, nb);
     } else {
-        if (order == 1) run<1, CVM_KERNEL_CUBIC_BSPLINE>(c, dtab, nd, nb);
-        else if (order == 2) run<2, CVM_KERNEL_CUBIC_BSPLINE>(c, dtab, nd, nb);
-        else run<3, CVM_KERNEL_CUBIC_BSPLINE>(c, dtab, nd, nb);
+        if (order == 1) run<1, CVM_KERNEL_CUBIC_BSPLINE>(c, ks, dtab, nd, nb);
+        else if (order == 2) run<2, CVM_KERNEL_CUBIC_BSPLINE>(c, ks, dtab, nd, nb);
+        else run<3, CVM_KERNEL_CUBIC_BSPLINE>(c, ks, dtab, nd, nb);
     }
-    timing_end(c, slot);
+    timing_end(c, slot, ks);
 }
 
 static void finish_rkpm(Domain& d, int order, int kernel, const int* hs, int* code, int* nbad, int* first) {
@@ -672,7 +677,7 @@ void launch_rkpm(Context& c, Domain& d, double /*rho*/, int order, int kernel, i
     check_args(order, kernel);
     launch_node_signs(c);
     c.d_status.ensure(4);
-    enqueue_group(c, {&d}, order, kernel, c.d_status.p, 1, stage_of(d));
+    enqueue_group(c, {&d}, order, kernel, c.d_status.p, 1, stage_of(d), nullptr);
     int hs[4];
     CVM_HIP(hipMemcpyAsync(hs, c.d_status.p, sizeof(hs), hipMemcpyDeviceToHost, c.stream));
     CVM_HIP(hipStreamSynchronize(c.stream));
@@ -683,30 +688,41 @@ void launch_rkpm(Context& c, Domain& d, double /*rho*/, int order, int kernel, i
 // their blocks are dispatched first and finish under the face domain's; one
 // status read-back at the end. On failure: the first failing domain (in domain
 // order) and its status.
-void launch_rkpm_all(Context& c, int order, int kernel, int* dom, int* code, int* nbad, int* first) {
+RkpmBatch enqueue_rkpm_all(Context& c, int order, int kernel, hipStream_t kst) {
     check_args(order, kernel);
-    launch_node_signs(c);
+    launch_node_signs(c, kst);
     const size_t nd = c.doms.size();
     c.d_status.ensure(std::max<size_t>(4 * nd, 4));
     // launch order: non-face domains, then face domains; status slot k of the
     // launch order, mapped back to domain order after the read-back
+    RkpmBatch b;
+    b.order = order; b.kernel = kernel;
     std::vector<Domain*> group;
-    std::vector<size_t> order_of;
     for (int pass = 0; pass < 2; ++pass)
         for (size_t i = 0; i < nd; ++i)
-            if ((c.doms[i]->kind == CVM_DOMAIN_FACE) == (pass == 1)) { group.push_back(c.doms[i]); order_of.push_back(i); }
-    enqueue_group(c, group, order, kernel, c.d_status.p, 0, "shape_functions");
+            if ((c.doms[i]->kind == CVM_DOMAIN_FACE) == (pass == 1)) { group.push_back(c.doms[i]); b.order_of.push_back(i); }
+    enqueue_group(c, group, order, kernel, c.d_status.p, 0, "shape_functions", kst);
+    return b;
+}
+
+void finish_rkpm_all(Context& c, const RkpmBatch& b, int* dom, int* code, int* nbad, int* first) {
+    const size_t nd = c.doms.size();
     std::vector<int> hs(4 * std::max<size_t>(nd, 1));
     if (nd) CVM_HIP(hipMemcpyAsync(hs.data(), c.d_status.p, 4 * nd * sizeof(int), hipMemcpyDeviceToHost, c.stream));
     CVM_HIP(hipStreamSynchronize(c.stream));
     std::vector<int> slot_of(nd);
-    for (size_t k = 0; k < order_of.size(); ++k) slot_of[order_of[k]] = static_cast<int>(k);
+    for (size_t k = 0; k < b.order_of.size(); ++k) slot_of[b.order_of[k]] = static_cast<int>(k);
     *dom = -1; *code = CVM_OK; *nbad = 0; *first = 0;
     for (size_t i = 0; i < nd; ++i) {
         int cd, nb, fb;
-        finish_rkpm(*c.doms[i], order, kernel, hs.data() + 4 * slot_of[i], &cd, &nb, &fb);
+        finish_rkpm(*c.doms[i], b.order, b.kernel, hs.data() + 4 * slot_of[i], &cd, &nb, &fb);
         if (cd != CVM_OK && *dom < 0) { *dom = static_cast<int>(i); *code = cd; *nbad = nb; *first = fb; }
     }
+}
+
+void launch_rkpm_all(Context& c, int order, int kernel, int* dom, int* code, int* nbad, int* first) {
+    const RkpmBatch b = enqueue_rkpm_all(c, order, kernel, nullptr);
+    finish_rkpm_all(c, b, dom, code, nbad, first);
 }
 
 }  // namespace cvm

@@ -48,6 +48,7 @@ SIGNATURES = {
     "cvm_shape_functions": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _i, _i]),
     "cvm_get_shape_functions": (C.c_int, [C.c_void_p, C.c_int, _d, _d, _d, _d, _d, _d]),
     "cvm_shape_functions_all": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _i, _i, _i]),
+    "cvm_recompute_shape_functions": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _i, _i, _i, _i]),
     "cvm_init_assembly": (C.c_int, [C.c_void_p, _i64, _i64]),
     "cvm_get_pattern": (C.c_int, [C.c_void_p, C.c_int, _i, _i]),
     "cvm_csr_mapping": (C.c_int, [C.c_void_p, C.c_int, _i, C.c_int, _i]),
@@ -270,6 +271,26 @@ class Plate:
             err.domain, err.nbad, err.first = dm.value, nb.value, fb.value
             raise err
         self._check(st)
+
+    def recompute_shape_functions(self, order=2, kernel=0):
+        """The neighbour search and the shape functions of every domain in one
+        call (refresh_neighbours + shape_functions_all); returns True if any
+        list changed, raises as shape_functions_all."""
+        ch, dm, nb, fb = C.c_int(0), C.c_int(-1), C.c_int(0), C.c_int(0)
+        st = self._L.cvm_recompute_shape_functions(self._h, order, kernel, C.byref(ch), C.byref(dm), C.byref(nb),
+                                                   C.byref(fb))
+        if st in (E_SINGULAR_MOMENT, E_PIVOT_COLLAPSE):
+            err = CvmError(st, self._L.cvm_last_error(self._h).decode())
+            err.domain, err.nbad, err.first = dm.value, nb.value, fb.value
+            raise err
+        self._check(st)
+        if ch.value:
+            for d in range(len(self.domain_sizes)):
+                if d in self.npairs:
+                    off = np.empty(self.domain_sizes[d] + 1, np.int32)
+                    self._check(self._L.cvm_get_neighbours(self._h, d, off.ctypes.data_as(_i), None, None))
+                    self.npairs[d] = int(off[-1])
+        return bool(ch.value)
 
     def get_shape_functions(self, dom):
         n = max(self.npairs[dom], 1)

@@ -157,9 +157,10 @@ CVM_API int cvm_neighbours(cvm_context* ctx, int domain, double rho, int64_t* np
  * moved): such a domain takes the full search (lists, offsets, host mirrors),
  * its shape functions must be recomputed and a changed face or fold domain
  * invalidates the assembly plan.
- * The crease indicators and node sides are evaluated again by the next
- * shape-function call, as GlobalRKPM::solve evaluates them on every call
- * (GlobalRKPM.h:86-112). */
+ * The crease indicators and node sides are evaluated again, as
+ * GlobalRKPM::solve evaluates them on every call (GlobalRKPM.h:86-112): here,
+ * beside the verification (on the context's side stream), or by the next
+ * shape-function call for a domain that took the full search. */
 CVM_API int cvm_refresh_neighbours(cvm_context* ctx, int* changed);
 /* Copy the lists out: off[ng+1], ids[npairs], s[npairs] (any may be NULL). */
 CVM_API int cvm_get_neighbours(cvm_context* ctx, int domain, int* off, int* ids, double* s);
@@ -176,6 +177,16 @@ CVM_API int cvm_shape_functions(cvm_context* ctx, int domain, int order, int ker
  * failing domain and the status is as above. */
 CVM_API int cvm_shape_functions_all(cvm_context* ctx, int order, int kernel, int* failed_domain, int* nbad,
                                     int* first_bad);
+/* SetupHelper::recomputeShapeFunctions (plate/SetupHelper.h:112-138) in one
+ * call: the neighbour search of every domain again (cvm_refresh_neighbours:
+ * *changed as there) and the shape functions of every domain
+ * (cvm_shape_functions_all: *failed_domain, *nbad, *first_bad and the status
+ * as there), with the same results as the two calls. While the lists are
+ * verified, the shape functions are formed from the resident lists beside
+ * the verification (on the context's side stream); if any list changed they
+ * are formed again from the new lists. */
+CVM_API int cvm_recompute_shape_functions(cvm_context* ctx, int order, int kernel, int* changed, int* failed_domain,
+                                          int* nbad, int* first_bad);
 /* Copy Psi, dPsi/dX, dPsi/dY, d2Psi/dX2, d2Psi/dY2, d2Psi/dXdY (npairs each,
  * FRNN order; GaussPointData::Psi/dPsi/ddPsi). Any pointer may be NULL. */
 CVM_API int cvm_get_shape_functions(cvm_context* ctx, int domain, double* psi, double* dpx, double* dpy,

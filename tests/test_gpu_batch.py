@@ -68,3 +68,11 @@ def test_refresh_neighbours_batched(batch):
     F1, K1 = bp.assemble(xs)
     assert np.array_equal(F1.view(np.int64), F0.view(np.int64))
     assert np.array_equal(K1.view(np.int64), K0.view(np.int64))
+    # the search and the shape functions in one call: the same bits
+    sf1 = bp.get_shape_functions(bp.face_dom)
+    assert bp.recompute_shape_functions(sheets[0].order, sheets[0].kernel) is False
+    for a, b in zip(bp.get_shape_functions(bp.face_dom), sf1):
+        assert np.array_equal(a.view(np.int64), b.view(np.int64))
+    F2, K2 = bp.assemble(xs)
+    assert np.array_equal(F2.view(np.int64), F0.view(np.int64))
+    assert np.array_equal(K2.view(np.int64), K0.view(np.int64))

@@ -272,6 +272,37 @@ def test_shape_functions_all_reports_failing_domain():
         gpu.shape_functions_all(sh.order, sh.kernel)
     assert ei.value.code == G.E_PIVOT_COLLAPSE and ei.value.domain == 0
     assert "domain 0" in str(ei.value)
+    # the same failure through the search and shape functions in one call
+    with pytest.raises(G.CvmError) as ei:
+        gpu.recompute_shape_functions(sh.order, sh.kernel)
+    assert ei.value.code == G.E_PIVOT_COLLAPSE and ei.value.domain == 0
+    assert "domain 0" in str(ei.value)
+
+
+def test_recompute_shape_functions_matches_separate_calls(arc):
+    """cvm_recompute_shape_functions (the search and the shape functions of
+    every domain in one call, the shape functions formed beside the search's
+    verification) gives the lists, shape functions and assembly of
+    cvm_refresh_neighbours + cvm_shape_functions_all bit for bit, and reports
+    the lists unchanged."""
+    sh, _, gpu = arc
+    x = W.displacement(sh.nn, 11, 1e-3)
+    assert gpu.refresh_neighbours() is False
+    gpu.shape_functions_all(sh.order, sh.kernel)
+    lists0 = [gpu.get_neighbours(d) for d in range(len(gpu.domain_sizes))]
+    sf0 = [gpu.get_shape_functions(d) for d in range(len(gpu.domain_sizes))]
+    F0, K0 = gpu.assemble(x)
+    for _ in range(2):
+        assert gpu.recompute_shape_functions(sh.order, sh.kernel) is False
+        for d in range(len(gpu.domain_sizes)):
+            for a, b in zip(gpu.get_neighbours(d), lists0[d]):
+                assert np.array_equal(a.view(np.int64) if a.dtype == np.float64 else a,
+                                      b.view(np.int64) if b.dtype == np.float64 else b), f"domain {d}: lists"
+            for a, b in zip(gpu.get_shape_functions(d), sf0[d]):
+                assert np.array_equal(a.view(np.int64), b.view(np.int64)), f"domain {d}: shape functions"
+        F1, K1 = gpu.assemble(x)
+        assert np.array_equal(F1.view(np.int64), F0.view(np.int64))
+        assert np.array_equal(K1.view(np.int64), K0.view(np.int64))
 
 
 def test_refresh_neighbours_unchanged(arc):
