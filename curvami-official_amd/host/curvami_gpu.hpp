@@ -119,9 +119,17 @@ public:
         return id;
     }
     // SetupHelper::recomputeShapeFunctions (SetupHelper.h:112-138): FRNN + RKPM
-    // on every domain.
+    // on every domain. The first call (and any after a domain was added or the
+    // radius changed) searches and solves domain by domain; the others search
+    // again on the device and solve every domain in one call
+    // (cvm_recompute_shape_functions), with the same results.
     void recomputeShapeFunctions() {
         cons_ready_ = false;
+        if (searched_ > 0 && searched_ == static_cast<int>(sizes_.size()) && searched_rho_ == rho) {
+            int changed = 0, failed = -1, nbad = 0, first = 0;
+            ctx_.check(cvm_recompute_shape_functions(ctx_.get(), order, kernel, &changed, &failed, &nbad, &first));
+            return;
+        }
         for (int d = 0; d < static_cast<int>(sizes_.size()); ++d) {
             GlobalFRNN frnn(rho);
             std::vector<int> off(static_cast<size_t>(sizes_[d]) + 1), ids;
@@ -129,6 +137,8 @@ public:
             frnn.execute(ctx_, d, off, ids, s);
             GlobalRKPM::solve(ctx_, d, order, kernel);
         }
+        searched_ = static_cast<int>(sizes_.size());
+        searched_rho_ = rho;
     }
     // Meshless::initAssembly -> NonLinearAssembler::initSparsityPattern
     void initAssembly() { ctx_.check(cvm_init_assembly(ctx_.get(), &nnz_face_, &nnz_union_)); }
@@ -199,6 +209,8 @@ private:
     int nn_ = 0;
     int64_t nnz_face_ = 0, nnz_union_ = 0;
     std::vector<int> sizes_;
+    int searched_ = 0;            // domains searched by the last full recomputeShapeFunctions
+    double searched_rho_ = 0.0;   // and its radius
     bool cons_ready_ = false, cons_rigid_ = false;
     int cnrows_ = 0;
     std::vector<int> crow_, ccol_;

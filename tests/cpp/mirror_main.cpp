@@ -62,6 +62,7 @@ int main(int argc, char** argv) {
             plate.setBoundaryCondition(dom, static_cast<int>(bm[1]), &bm[2], &bm[6], bm[9]);
         }
         plate.recomputeShapeFunctions();
+        plate.recomputeShapeFunctions();   // again: the device search and every domain in one call
         plate.initAssembly();
         curvami_gpu::CSC K;
         std::vector<double> F;
