@@ -489,10 +489,16 @@ int cvm_recompute_shape_functions(cvm_context* ctx, int order, int kernel, int* 
     // resident lists; formed again from the new lists if any list changed
     bool enq = false;
     cvm::RkpmBatch rb;
-    const std::vector<char> ch = cvm::refresh_frnn_all(c, [&](hipStream_t sd) {
-        rb = cvm::enqueue_rkpm_all(c, order, kernel, sd);
-        enq = true;
-    });
+    std::vector<int> hs(4 * std::max<size_t>(c.doms.size(), 1));
+    const std::vector<char> ch = cvm::refresh_frnn_all(
+        c,
+        [&](hipStream_t sd) {
+            rb = cvm::enqueue_rkpm_all(c, order, kernel, sd);
+            enq = true;
+        },
+        [&] {
+            if (enq) cvm::read_rkpm_status(c, hs.data());   // read back with the verification flags
+        });
     bool any = false, pattern = false;
     for (size_t k = 0; k < c.doms.size(); ++k) {
         cvm::Domain* d = c.doms[k];
@@ -504,7 +510,7 @@ int cvm_recompute_shape_functions(cvm_context* ctx, int order, int kernel, int* 
     }
     if (pattern) c.plan.ready = false;
     int dom = -1, code = 0, nb = 0, fb = 0;
-    if (enq && !any) cvm::finish_rkpm_all(c, rb, &dom, &code, &nb, &fb);
+    if (enq && !any) cvm::finish_rkpm_all(c, rb, hs.data(), &dom, &code, &nb, &fb);
     else cvm::launch_rkpm_all(c, order, kernel, &dom, &code, &nb, &fb);
     for (cvm::Domain* d : c.doms) ++d->gen;
     if (changed) *changed = any ? 1 : 0;

@@ -384,7 +384,10 @@ void launch_frnn(Context& c, Domain& d, double rho);
 // indicators, beside the verification pass; it is called only when the
 // verification runs (some domain keeps its grid), and its work is complete
 // when the call returns.
-std::vector<char> refresh_frnn_all(Context& c, const std::function<void(hipStream_t)>& beside = nullptr);
+// joined (optional) enqueues work on the context's stream after the side
+// stream's work, before the one synchronisation of the call (read-backs).
+std::vector<char> refresh_frnn_all(Context& c, const std::function<void(hipStream_t)>& beside = nullptr,
+                                   const std::function<void()>& joined = nullptr);
 void launch_rkpm(Context& c, Domain& d, double rho, int order, int kernel, int* code, int* nbad, int* first);
 void launch_rkpm_all(Context& c, int order, int kernel, int* dom, int* code, int* nbad, int* first);
 // The shape functions of every domain enqueued with their kernels on stream
@@ -397,6 +400,10 @@ struct RkpmBatch {
 };
 RkpmBatch enqueue_rkpm_all(Context& c, int order, int kernel, hipStream_t kst = nullptr);
 void finish_rkpm_all(Context& c, const RkpmBatch& b, int* dom, int* code, int* nbad, int* first);
+// The status words of a batch into host memory hs (4 per domain) on the
+// context's stream (asynchronous), and the statuses once read.
+void read_rkpm_status(Context& c, int* hs);
+void finish_rkpm_all(Context& c, const RkpmBatch& b, const int* hs, int* dom, int* code, int* nbad, int* first);
 // Orders stream kst after the work enqueued so far on the context's stream;
 // returns the stream to launch on (the context's stream when kst is null).
 hipStream_t fork_to(Context& c, hipStream_t kst);

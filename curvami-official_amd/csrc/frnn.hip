@@ -586,7 +586,8 @@ void launch_frnn(Context& c, Domain& d, double rho) {
 // result; a domain with any difference takes the full search (host grid,
 // counts, offsets, lists, interleaved copies and mirrors). No host loop over
 // GPs or pairs. Returns, per domain, whether its lists changed.
-std::vector<char> refresh_frnn_all(Context& c, const std::function<void(hipStream_t)>& beside) {
+std::vector<char> refresh_frnn_all(Context& c, const std::function<void(hipStream_t)>& beside,
+                                   const std::function<void()>& joined) {
     const size_t nd = c.doms.size();
     std::vector<char> changed(nd, 0), live(nd, 0);
     c.node_sign_creases = -1;          // node sides are evaluated again, here or with the next shape functions
@@ -656,6 +657,7 @@ std::vector<char> refresh_frnn_all(Context& c, const std::function<void(hipStrea
     std::vector<int> fl(nd, 0);
     CVM_HIP(hipMemcpyAsync(fl.data(), c.rf_flags.p, nd * sizeof(int), hipMemcpyDeviceToHost, c.stream));
     CVM_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));
+    if (joined) joined();
     CVM_HIP(hipStreamSynchronize(c.stream));
     for (size_t k = 0; k < nd; ++k) {
         if (!live[k] || !fl[k]) continue;

@@ -705,17 +705,26 @@ RkpmBatch enqueue_rkpm_all(Context& c, int order, int kernel, hipStream_t kst) {
     return b;
 }
 
-void finish_rkpm_all(Context& c, const RkpmBatch& b, int* dom, int* code, int* nbad, int* first) {
+void read_rkpm_status(Context& c, int* hs) {
     const size_t nd = c.doms.size();
-    std::vector<int> hs(4 * std::max<size_t>(nd, 1));
-    if (nd) CVM_HIP(hipMemcpyAsync(hs.data(), c.d_status.p, 4 * nd * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    if (nd) CVM_HIP(hipMemcpyAsync(hs, c.d_status.p, 4 * nd * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+}
+
+void finish_rkpm_all(Context& c, const RkpmBatch& b, int* dom, int* code, int* nbad, int* first) {
+    std::vector<int> hs(4 * std::max<size_t>(c.doms.size(), 1));
+    read_rkpm_status(c, hs.data());
     CVM_HIP(hipStreamSynchronize(c.stream));
+    finish_rkpm_all(c, b, hs.data(), dom, code, nbad, first);
+}
+
+void finish_rkpm_all(Context& c, const RkpmBatch& b, const int* hs, int* dom, int* code, int* nbad, int* first) {
+    const size_t nd = c.doms.size();
     std::vector<int> slot_of(nd);
     for (size_t k = 0; k < b.order_of.size(); ++k) slot_of[b.order_of[k]] = static_cast<int>(k);
     *dom = -1; *code = CVM_OK; *nbad = 0; *first = 0;
     for (size_t i = 0; i < nd; ++i) {
         int cd, nb, fb;
-        finish_rkpm(*c.doms[i], b.order, b.kernel, hs.data() + 4 * slot_of[i], &cd, &nb, &fb);
+        finish_rkpm(*c.doms[i], b.order, b.kernel, hs + 4 * slot_of[i], &cd, &nb, &fb);
         if (cd != CVM_OK && *dom < 0) { *dom = static_cast<int>(i); *code = cd; *nbad = nb; *first = fb; }
     }
 }
