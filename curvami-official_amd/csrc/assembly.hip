@@ -116,6 +116,7 @@ __device__ __forceinline__ void tile_body(
     TT* gs = rect + static_cast<size_t>(rec_cap) * RS;          // TILE_GPS records of GSP
     uint32_t* pm = reinterpret_cast<uint32_t*>(gs + TILE_GPS * GSP);  // umax presence masks
     uint16_t* loc = reinterpret_cast<uint16_t*>(pm + umax);     // ngp rows of U record indices
+    int* chunk_next = reinterpret_cast<int*>(loc + TILE_GPS * umax);  // next unclaimed item chunk
     auto RF = [&](int f, int r) -> double& { return recf[static_cast<size_t>(f) * rec_cap + r]; };
     auto R = [&](int f, int r) -> TT& { return rect[static_cast<size_t>(r) * RS + f]; };  // record-major
     auto Gp = [&](int f, int wv) -> TT& { return gs[wv * GSP + f]; };
@@ -125,6 +126,7 @@ __device__ __forceinline__ void tile_body(
         uc[3 * tid] = xu[3 * v]; uc[3 * tid + 1] = xu[3 * v + 1]; uc[3 * tid + 2] = xu[3 * v + 2];
         pm[tid] = 0u;
     }
+    if (tid == 0) *chunk_next = TILE_THREADS / 64;
     for (int k = tid; k < ngp * U; k += TILE_THREADS) loc[k] = NO_REC;
     __syncthreads();
 
@@ -288,9 +290,8 @@ __device__ __forceinline__ void tile_body(
             }
             return m;
         };
-        SlotMeta cur = meta(tid);
-        for (int it = tid; it < T - nd; it += TILE_THREADS) {
-            const SlotMeta nxt = meta(it + TILE_THREADS);
+        // One work item: the slot(s) of item it, whose slot pairs and scratch rows are cur.
+        auto item = [&](const int it, const SlotMeta& cur) {
             const bool paired = it < nd;
             const uint32_t pr = cur.pr & 0xffffu;
             const int ua = pr >> 8, ub1 = pr & 0xff;
@@ -425,7 +426,42 @@ __device__ __forceinline__ void tile_body(
 #pragma unroll
                 for (int k = 0; k < 9; ++k) dst2[k] = acc2[k];
             }
-            cur = nxt;
+        };
+        const int nitems = T - nd;
+        if constexpr (sizeof(TT) == 8 && LAM != LAM_COUPLED) {
+            // Work items go to the waves in chunks of 64 (one item per lane). The plan
+            // sorts them by decreasing cost, so dealing the chunks round-robin would
+            // leave the waves that drew the cheap tail idle while the others finish:
+            // wave w takes chunk w, then claims the next unclaimed chunk from an LDS
+            // counter, one chunk before it needs it. Each item is still summed by one
+            // lane in GP order, so the blocks do not depend on which wave takes which
+            // chunk. (The FP32 tangent and the coupled laminate, whose items are all
+            // single slots, keep the round-robin deal: the claim loop would take the
+            // FP32 kernel past its 80-register budget and the coupled one past 128.)
+            const int nchunks = (nitems + 63) >> 6;
+            const int lane = tid & 63;
+            auto claim = [&]() {
+                int k = 0;
+                if (lane == 0) k = atomicAdd(chunk_next, 1);
+                return __builtin_amdgcn_readfirstlane(k);   // every lane of the wave is active here
+            };
+            int ch = __builtin_amdgcn_readfirstlane(tid >> 6);
+            SlotMeta cur = meta(ch * 64 + lane);
+            int chn = ch < nchunks ? claim() : nchunks;
+            while (ch < nchunks) {
+                const SlotMeta nxt = meta(chn * 64 + lane);
+                const int it = ch * 64 + lane;
+                if (it < nitems) item(it, cur);
+                ch = chn; cur = nxt;
+                if (ch < nchunks) chn = claim();
+            }
+        } else {
+            SlotMeta cur = meta(tid);
+            for (int it = tid; it < nitems; it += TILE_THREADS) {
+                const SlotMeta nxt = meta(it + TILE_THREADS);
+                item(it, cur);
+                cur = nxt;
+            }
         }
     }
     // ---- residual per union node, GPs in order ---------------------------------------
@@ -842,7 +878,7 @@ size_t tile_lds_bytes(int rec_cap, int max_union, bool fp32, bool wide) {
          + tt * (static_cast<size_t>(rec_cap) * (fp32 ? (wide ? rec_stride<float, false>() : rec_stride<float, true>())
                                                       : (wide ? rec_stride<double, false>() : rec_stride<double, true>())) +
                  TILE_GPS * (fp32 ? gp_stride<float>() : gp_stride<double>())) + sizeof(uint32_t) * um
-         + sizeof(uint16_t) * TILE_GPS * um;
+         + sizeof(uint16_t) * TILE_GPS * um + sizeof(int);
 }
 
 size_t assembly_workspace_bytes(const Context& c, int flags) {
