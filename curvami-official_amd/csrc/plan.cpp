@@ -135,16 +135,23 @@ void build_plan(Context& c) {
             std::vector<uint16_t> sgl;
             tl[ti].clear();
             tl[ti].reserve(tlist.size());
+            // A run of slots with the same mask and lower node pairs its first half
+            // with its second half (slot i with slot i + L/2): the lanes that take
+            // consecutive pairs then read the upper nodes' records one record apart
+            // (18 dwords: distinct LDS banks for 16 lanes) instead of two apart (36
+            // dwords: two lanes per bank). An odd run leaves its last slot single.
             for (size_t k = 0; k < tlist.size();) {
-                if (k + 1 < tlist.size() && tlist[k].first == tlist[k + 1].first &&
-                    (tlist[k].second >> 8) == (tlist[k + 1].second >> 8)) {
-                    tl[ti].push_back(tlist[k].second);
-                    tl[ti].push_back(tlist[k + 1].second);
-                    k += 2;
-                } else {
-                    sgl.push_back(tlist[k].second);
-                    ++k;
+                size_t e = k + 1;
+                while (e < tlist.size() && tlist[e].first == tlist[k].first &&
+                       (tlist[e].second >> 8) == (tlist[k].second >> 8))
+                    ++e;
+                const size_t half = (e - k) / 2;
+                for (size_t i = 0; i < half; ++i) {
+                    tl[ti].push_back(tlist[k + i].second);
+                    tl[ti].push_back(tlist[k + half + i].second);
                 }
+                if ((e - k) & 1) sgl.push_back(tlist[e - 1].second);
+                k = e;
             }
             tile_nd[ti] = static_cast<int>(tl[ti].size() / 2);
             tl[ti].insert(tl[ti].end(), sgl.begin(), sgl.end());
