@@ -305,8 +305,30 @@ __device__ __forceinline__ void tile_body(
             // terms() forms those factors, add() accumulates them into the slot's block,
             // coupling GPs in ascending order.
             struct Terms { TT T0[3], T1[3], xv[3], yv[3], a1[3], a2[3], bv[3], wgeo; };
-            // ri, rj: the records of the slot's two nodes at coupling GP wv
-            auto terms = [&](int wv, int ri, int rj, Terms& t) {
+            // Factors of the lower node alone at coupling GP wv, which a paired item
+            // forms once for its two blocks: Q = W Psi_i^T A (2 x 3), so that
+            // W S = Q Psi_j, and n = N grad Psi_i scaled by W, so that W geo = n . grad Psi_j
+            // (8 + 2 operations per block instead of 18 + 6).
+            struct ISide { TT q00, q01, q02, q10, q11, q12, n0, n1; };
+            auto iside = [&](int wv, int ri) {
+                ISide s;
+                const TT W = Gp(G_W, wv);
+                const TT pxi = R(R_PX, ri), pyi = R(R_PY, ri);
+                const TT wpx = W * pxi, wpy = W * pyi;
+                // Q[0][c] = W (pxi A[0][c] + pyi A[2][c]), Q[1][c] = W (pyi A[1][c] + pxi A[2][c])
+                s.q00 = wpx * Ac[0]; s.q01 = wpx * Ac[1]; s.q02 = wpy * Ac[8];
+                s.q10 = wpy * Ac[3]; s.q11 = wpy * Ac[4]; s.q12 = wpx * Ac[8];
+                if constexpr (LAM != LAM_ORTHO) {
+                    s.q00 += wpy * Ac[6]; s.q01 += wpy * Ac[7]; s.q02 += wpx * Ac[2];
+                    s.q10 += wpx * Ac[6]; s.q11 += wpx * Ac[7]; s.q12 += wpy * Ac[5];
+                }
+                s.n0 = pxi * Gp(G_WN, wv) + pyi * Gp(G_WN + 2, wv);
+                s.n1 = pxi * Gp(G_WN + 2, wv) + pyi * Gp(G_WN + 1, wv);
+                return s;
+            };
+            // ri, rj: the records of the slot's two nodes at coupling GP wv; is: the
+            // lower node's factors at wv, or null to form the block from the records alone
+            auto terms = [&](int wv, int ri, int rj, Terms& t, const ISide* is) {
                 const TT W = Gp(G_W, wv);
 #pragma unroll
                 for (int r = 0; r < 3; ++r) {
@@ -331,15 +353,22 @@ __device__ __forceinline__ void tile_body(
                         gj[r] = t.a1[r] * cj0 + t.a2[r] * cj1;
                     }
                 }
-                // A Psi_j (3x2) and W S = W Psi_i^T A Psi_j
-                TT ap00 = Ac[0] * pxj, ap01 = Ac[1] * pyj, ap10 = Ac[3] * pxj, ap11 = Ac[4] * pyj;
-                TT ap20 = Ac[8] * pyj, ap21 = Ac[8] * pxj;
-                if constexpr (LAM != LAM_ORTHO) {
-                    ap00 += Ac[2] * pyj; ap01 += Ac[2] * pxj; ap10 += Ac[5] * pyj; ap11 += Ac[5] * pxj;
-                    ap20 += Ac[6] * pxj; ap21 += Ac[7] * pyj;
+                // W S = W Psi_i^T A Psi_j
+                TT S00, S01, S10, S11;
+                if (is) {
+                    S00 = is->q00 * pxj + is->q02 * pyj; S01 = is->q01 * pyj + is->q02 * pxj;
+                    S10 = is->q10 * pxj + is->q12 * pyj; S11 = is->q11 * pyj + is->q12 * pxj;
+                } else {
+                    // A Psi_j (3x2)
+                    TT ap00 = Ac[0] * pxj, ap01 = Ac[1] * pyj, ap10 = Ac[3] * pxj, ap11 = Ac[4] * pyj;
+                    TT ap20 = Ac[8] * pyj, ap21 = Ac[8] * pxj;
+                    if constexpr (LAM != LAM_ORTHO) {
+                        ap00 += Ac[2] * pyj; ap01 += Ac[2] * pxj; ap10 += Ac[5] * pyj; ap11 += Ac[5] * pxj;
+                        ap20 += Ac[6] * pxj; ap21 += Ac[7] * pyj;
+                    }
+                    S00 = W * (pxi * ap00 + pyi * ap20); S01 = W * (pxi * ap01 + pyi * ap21);
+                    S10 = W * (pyi * ap10 + pxi * ap20); S11 = W * (pyi * ap11 + pxi * ap21);
                 }
-                const TT S00 = W * (pxi * ap00 + pyi * ap20), S01 = W * (pxi * ap01 + pyi * ap21);
-                const TT S10 = W * (pyi * ap10 + pxi * ap20), S11 = W * (pyi * ap11 + pxi * ap21);
                 // z = beta_i . D beta_j + (b.v) g_i . g_j
                 TT db0 = Dc[0] * bj0 + Dc[1] * bj1, db1 = Dc[3] * bj0 + Dc[4] * bj1, db2 = Dc[8] * bj2;
                 if constexpr (LAM != LAM_ORTHO) {
@@ -347,8 +376,9 @@ __device__ __forceinline__ void tile_body(
                 }
                 const TT Wz = W * (bi0 * db0 + bi1 * db1 + bi2 * db2
                                    + Gp(G_BDV, wv) * (gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2]));
-                t.wgeo = pxi * (Gp(G_WN, wv) * pxj + Gp(G_WN + 2, wv) * pyj)
-                         + pyi * (Gp(G_WN + 2, wv) * pxj + Gp(G_WN + 1, wv) * pyj);
+                if (is) t.wgeo = is->n0 * pxj + is->n1 * pyj;
+                else t.wgeo = pxi * (Gp(G_WN, wv) * pxj + Gp(G_WN + 2, wv) * pyj)
+                              + pyi * (Gp(G_WN + 2, wv) * pxj + Gp(G_WN + 1, wv) * pyj);
 #pragma unroll
                 for (int r = 0; r < 3; ++r) {
                     t.xv[r] = -(tui * gj[r]);
@@ -398,8 +428,9 @@ __device__ __forceinline__ void tile_body(
                         li = loc[w0 * U + ua]; lj = loc[w0 * U + ub1]; lk = loc[w0 * U + ub2];
                     }
                     Terms t0, t1;
-                    terms(wc, ci, cj, t0);
-                    terms(wc, ci, ck, t1);
+                    const ISide is = iside(wc, ci);
+                    terms(wc, ci, cj, t0, &is);
+                    terms(wc, ci, ck, t1, &is);
                     add(acc, t0);
                     add(acc2, t1);
                 }
@@ -414,7 +445,7 @@ __device__ __forceinline__ void tile_body(
                         li = loc[w0 * U + ua]; lj = loc[w0 * U + ub1];
                     }
                     Terms t0;
-                    terms(wc, ci, cj, t0);
+                    terms(wc, ci, cj, t0, nullptr);
                     add(acc, t0);
                 }
             }
