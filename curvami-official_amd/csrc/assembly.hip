@@ -903,13 +903,23 @@ static auto tile_kernel(int lam, bool persist, bool wide) {
 // The FP32 tangent over wide tiles whose LDS leaves room for one workgroup per
 // CU: 1024 threads (four waves per SIMD) instead of 512 (two). Its 98 VGPRs fit
 // four waves; the FP64 kernel at 1024 threads would spill past 128 VGPRs. At the
-// reference defaults (cubic basis, rho = 6h) the FP32 tile takes 15.6 ms instead
+// reference defaults (cubic basis, rho = 6h) the FP32 tile takes 15.9 ms instead
 // of 17.9 ms with bit-identical results.
 static auto tile_kernel_fp32_wide_big(int lam) {
     constexpr int NT = 2 * TILE_THREADS;
     return lam == LAM_COUPLED ? k_tile<LAM_COUPLED, float, false, true, NT>
          : lam == LAM_SYM     ? k_tile<LAM_SYM, float, false, true, NT>
                               : k_tile<LAM_ORTHO, float, false, true, NT>;
+}
+
+// The FP64 tangent (and the residual alone) in the same case: 768 threads, three
+// waves per SIMD, within the 170 VGPRs that three waves leave each (22.4 ms
+// instead of 23.5 ms at the reference defaults, bit-identical).
+static auto tile_kernel_fp64_wide_big(int lam) {
+    constexpr int NT = 3 * TILE_THREADS / 2;
+    return lam == LAM_COUPLED ? k_tile<LAM_COUPLED, double, false, true, NT>
+         : lam == LAM_SYM     ? k_tile<LAM_SYM, double, false, true, NT>
+                              : k_tile<LAM_ORTHO, double, false, true, NT>;
 }
 
 int tile_umax(int max_union) { return std::min(UMAX, (std::max(max_union, 1) + 7) / 8 * 8); }
@@ -1117,9 +1127,9 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
     const int lam = laminate_class(m.A, m.B, m.D);
     const bool persist = c.tile_grid > 0 && c.tile_grid < P.ntiles;
     const int grid = persist ? c.tile_grid : P.ntiles;
-    // one workgroup per CU by LDS: the FP32 wide tile runs 1024 threads
-    const bool big = fp32 && wide && !persist && 2 * lds > c.lds_limit;
-    const int nt = big ? 2 * TILE_THREADS : TILE_THREADS;
+    // one workgroup per CU by LDS: the wide tile runs 1024 threads (FP32) or 768 (FP64)
+    const bool big = wide && !persist && 2 * lds > c.lds_limit;
+    const int nt = !big ? TILE_THREADS : fp32 ? 2 * TILE_THREADS : 3 * TILE_THREADS / 2;
     auto tile = [&](auto kern, auto* scr) {
         launch_k(sig, kern, dim3(grid), dim3(nt), lds, st, P.d_tile_gp.p, P.d_tile_uoff.p,
                            P.d_tile_unodes.p, P.d_tile_toff.p, P.d_tile_tpairs.p, P.d_tile_nd.p, P.d_slot_dst.p, fd.off.p,
@@ -1129,7 +1139,8 @@ void launch_assembly(Context& c, Workspace& ws, int flags, const double* xdev, d
     float* scr32 = reinterpret_cast<float*>(wp.scr);
     if (!tile_done) {    // (tile_done: the tiles ran in a grouped launch, launch_tile_group)
         slot = tbeg(!tangent ? "tile_residual" : (fp32 ? "tile_residual_tangent_fp32" : "tile_residual_tangent"));
-        if (big) tile(tile_kernel_fp32_wide_big(lam), scr32);
+        if (big && fp32) tile(tile_kernel_fp32_wide_big(lam), scr32);
+        else if (big) tile(tile_kernel_fp64_wide_big(lam), wp.scr);
         else if (fp32) tile(tile_kernel<float>(lam, persist, wide), scr32);
         else tile(tile_kernel<double>(lam, persist, wide), wp.scr);
         if (!dry) CVM_HIP(hipGetLastError());
