@@ -32,7 +32,7 @@
 
 namespace cvm {
 
-constexpr int TILE_THREADS = 512;    // 8 waves; wave w prepares GPs w and w + 8 of the tile
+constexpr int TILE_THREADS = 512;    // default workgroup, 8 waves: wave w prepares GPs w and w + 8 of the tile
 constexpr int TILE_GPS = 16;         // GPs per tile (presence masks are 16-bit)
 constexpr int UMAX = 256;            // union nodes per tile (uint8 union indices, at most 255 used)
 constexpr int GS = 17;               // GP record fields (tangent type)
