@@ -179,13 +179,19 @@ def test_laminates(name, lam):
     assert e32 <= 1e-5
 
 
-def test_deterministic(c1):
-    sh, _, gpu = c1
+@pytest.mark.parametrize("case", ["c1", "arc", "wide"])
+def test_deterministic(case, request):
+    """Bit-identical results run to run, in both tangent types: the FP64 tile's
+    waves claim their item chunks in whatever order they finish, and the wide
+    case runs the larger workgroups, but every block is still summed by one lane
+    in GP order."""
+    sh, _, gpu = request.getfixturevalue(case)
     x = W.displacement(sh.nn, 5, 1e-3)
-    F1, K1 = gpu.assemble(x)
-    F2, K2 = gpu.assemble(x)
-    assert np.array_equal(F1.view(np.int64), F2.view(np.int64))
-    assert np.array_equal(K1.view(np.int64), K2.view(np.int64))
+    for fp32 in (False, True):
+        runs = [gpu.assemble(x, fp32_tangent=fp32) for _ in range(3)]
+        for F, K in runs[1:]:
+            assert np.array_equal(runs[0][0].view(np.int64), F.view(np.int64))
+            assert np.array_equal(runs[0][1].view(np.int64), K.view(np.int64))
 
 
 def test_residual_only_matches(c1):
